@@ -1,0 +1,1553 @@
+/*
+ * oracle.c -- CPU restatement of MDDEM's single-process per-step particle path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h). Built with -ffp-contract=off so
+ * that every a*b+c is two roundings unless the reference writes mul_add,
+ * which maps to fma() here. File:line citations are into SueHeir/MDDEM
+ * v0.1.3.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_MAX_MAT 64
+#define ORC_MAX_WALL 16
+/* dem_atom/src/lib.rs:63 */
+#define SQRT_5_3 0.9128709291752768
+/* dem_granular/src/lib.rs:93-108 */
+#define TANGENTIAL_EPSILON 1e-30
+#define LARGE_OVERLAP_WARN_THRESHOLD 0.90
+#define MAX_OVERLAP_WARNINGS 500
+
+typedef struct {
+    uint32_t tag;
+    double s[3];
+    int active;
+} hist_entry;
+
+typedef struct {
+    hist_entry* e;
+    int n, cap;
+} hist_list;
+
+typedef struct {
+    uint64_t key;
+    double s[3];
+    int used, active;
+} cslot;
+
+typedef struct {
+    cslot* slots;
+    uint64_t cap, count;
+} cmap;
+
+typedef struct {
+    int* send;
+    int nsend, capsend;
+    int recv_start, recv_count;
+    int to_self;     /* SingleProcessComm periodic self-send */
+    double pbc_shift[3];
+    int dim, sign;
+} swap_t;
+
+typedef struct {
+    double p[3], n[3], blo[3], bhi[3], v[3];
+    int mat;
+} wall_t;
+
+struct orc_sys {
+    int model;
+    double lo[3], hi[3], size[3];
+    int periodic[3];
+    double dt;
+    /* [neighbor] config (neighbor.rs:45-82) */
+    double skin_fraction, bin_min_size;
+    int every, check, sort_every, newton;
+    /* neighbor state (neighbor.rs:87-159) */
+    double max_cutoff, ghost_cutoff, bin_size[3], bin_origin[3];
+    int bin_count[3], total_cells;
+    int* stencil_fwd;
+    int n_fwd, has_self;
+    int have_uniform;
+    double uniform_cutoff_sq, cached_min_skin;
+    double (*last_build_pos)[3];
+    int n_last_build; /* -1: cleared (Vec::clear) */
+    int steps_since_build, sort_counter, state;
+    int setup_done;
+    uint32_t* offsets;
+    uint32_t* indices;
+    int64_t nidx, capidx;
+    int cap_offsets;
+    uint32_t *atom_cell, *bin_cnt, *bin_start, *sorted_atoms, *atom_sorted_idx;
+    double (*sorted_pos)[3];
+    int cap_bins, cap_cells;
+    /* atoms: locals then ghosts (atom.rs:310-335 + DemAtom lib.rs:530-557) */
+    int nlocal, nghost, cap;
+    uint32_t *tag, *type;
+    int32_t* origin;
+    int8_t (*gshift)[3];
+    double (*pos)[3], (*vel)[3], (*force)[3];
+    double *cutoff_radius, *mass, *inv_mass;
+    int32_t (*image)[3];
+    double *radius, *inv_inertia;
+    double (*quat)[4], (*omega)[3], (*torque)[3];
+    hist_list* hist;
+    int history_mode;
+    cmap cm;
+    swap_t swaps[6];
+    int nswaps;
+    /* materials (dem_atom lib.rs:389-515) */
+    int nmat;
+    double youngs[ORC_MAX_MAT], poisson[ORC_MAX_MAT], restitution[ORC_MAX_MAT],
+        friction[ORC_MAX_MAT];
+    double* beta_ij;
+    double* friction_ij;
+    double* e_eff_ij;
+    double* g_eff_ij;
+    int nwalls;
+    wall_t walls[ORC_MAX_WALL];
+    int gravity_on;
+    double gravity[3];
+    /* LJ (md_lj lib.rs:230-239) */
+    double lj1, lj2, lj_cut2;
+    /* Nose-Hoover (md_thermostat lib.rs:128-139) */
+    int nh_on;
+    double nh_T, nh_tau, nh_ndof, nh_Q, p_xi;
+    int vv_on, rot_on;
+    int64_t overlap_total, rebuilds, step;
+    char err[256];
+};
+
+/* ---------------------------------------------------------------------- */
+
+static void* xrealloc(void* p, size_t n) {
+    void* q = realloc(p, n ? n : 1);
+    if (!q) {
+        fprintf(stderr, "oracle: out of memory\n");
+        abort();
+    }
+    return q;
+}
+
+static void set_err(orc_sys* s, const char* msg) {
+    snprintf(s->err, sizeof(s->err), "%s", msg);
+}
+
+static void ensure_cap(orc_sys* s, int n) {
+    if (n <= s->cap) return;
+    int c = s->cap ? s->cap : 64;
+    while (c < n) c *= 2;
+    s->tag = xrealloc(s->tag, sizeof(uint32_t) * c);
+    s->type = xrealloc(s->type, sizeof(uint32_t) * c);
+    s->origin = xrealloc(s->origin, sizeof(int32_t) * c);
+    s->gshift = xrealloc(s->gshift, sizeof(*s->gshift) * c);
+    s->pos = xrealloc(s->pos, sizeof(*s->pos) * c);
+    s->vel = xrealloc(s->vel, sizeof(*s->vel) * c);
+    s->force = xrealloc(s->force, sizeof(*s->force) * c);
+    s->cutoff_radius = xrealloc(s->cutoff_radius, sizeof(double) * c);
+    s->mass = xrealloc(s->mass, sizeof(double) * c);
+    s->inv_mass = xrealloc(s->inv_mass, sizeof(double) * c);
+    s->image = xrealloc(s->image, sizeof(*s->image) * c);
+    s->radius = xrealloc(s->radius, sizeof(double) * c);
+    s->inv_inertia = xrealloc(s->inv_inertia, sizeof(double) * c);
+    s->quat = xrealloc(s->quat, sizeof(*s->quat) * c);
+    s->omega = xrealloc(s->omega, sizeof(*s->omega) * c);
+    s->torque = xrealloc(s->torque, sizeof(*s->torque) * c);
+    s->hist = xrealloc(s->hist, sizeof(hist_list) * c);
+    for (int i = s->cap; i < c; i++) {
+        s->hist[i].e = NULL;
+        s->hist[i].n = s->hist[i].cap = 0;
+    }
+    s->cap = c;
+}
+
+static void hist_copy(hist_list* dst, const hist_list* src) {
+    if (dst->cap < src->n) {
+        dst->e = xrealloc(dst->e, sizeof(hist_entry) * src->n);
+        dst->cap = src->n;
+    }
+    if (src->n) memcpy(dst->e, src->e, sizeof(hist_entry) * src->n);
+    dst->n = src->n;
+}
+
+static void hist_push(hist_list* h, uint32_t tag, const double s[3], int active) {
+    if (h->n == h->cap) {
+        h->cap = h->cap ? 2 * h->cap : 4;
+        h->e = xrealloc(h->e, sizeof(hist_entry) * h->cap);
+    }
+    h->e[h->n].tag = tag;
+    h->e[h->n].s[0] = s[0];
+    h->e[h->n].s[1] = s[1];
+    h->e[h->n].s[2] = s[2];
+    h->e[h->n].active = active;
+    h->n++;
+}
+
+/* copy every per-atom field of atom src into slot dst (registry pack/unpack
+ * semantics of comm.rs:256-299 without the f64 round trip of integers) */
+static void copy_atom(orc_sys* s, int dst, int src) {
+    s->tag[dst] = s->tag[src];
+    s->type[dst] = s->type[src];
+    s->origin[dst] = s->origin[src];
+    memcpy(s->gshift[dst], s->gshift[src], sizeof(s->gshift[0]));
+    memcpy(s->pos[dst], s->pos[src], sizeof(s->pos[0]));
+    memcpy(s->vel[dst], s->vel[src], sizeof(s->vel[0]));
+    memcpy(s->force[dst], s->force[src], sizeof(s->force[0]));
+    s->cutoff_radius[dst] = s->cutoff_radius[src];
+    s->mass[dst] = s->mass[src];
+    s->inv_mass[dst] = s->inv_mass[src];
+    memcpy(s->image[dst], s->image[src], sizeof(s->image[0]));
+    s->radius[dst] = s->radius[src];
+    s->inv_inertia[dst] = s->inv_inertia[src];
+    memcpy(s->quat[dst], s->quat[src], sizeof(s->quat[0]));
+    memcpy(s->omega[dst], s->omega[src], sizeof(s->omega[0]));
+    memcpy(s->torque[dst], s->torque[src], sizeof(s->torque[0]));
+    hist_copy(&s->hist[dst], &s->hist[src]);
+}
+
+/* ---- canonical pair-key history map ----------------------------------- */
+
+static uint64_t mix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+static void cmap_init(cmap* m, uint64_t cap) {
+    uint64_t c = 16;
+    while (c < cap) c <<= 1;
+    m->slots = calloc(c, sizeof(cslot));
+    if (!m->slots) abort();
+    m->cap = c;
+    m->count = 0;
+}
+
+static cslot* cmap_find(cmap* m, uint64_t key, int insert) {
+    if (insert && 2 * (m->count + 1) > m->cap) {
+        cmap old = *m;
+        cmap_init(m, old.cap * 2);
+        for (uint64_t i = 0; i < old.cap; i++)
+            if (old.slots[i].used) {
+                cslot* d = cmap_find(m, old.slots[i].key, 1);
+                *d = old.slots[i];
+            }
+        free(old.slots);
+    }
+    uint64_t h = mix64(key) & (m->cap - 1);
+    for (;;) {
+        cslot* sl = &m->slots[h];
+        if (!sl->used) {
+            if (!insert) return NULL;
+            sl->used = 1;
+            sl->key = key;
+            sl->s[0] = sl->s[1] = sl->s[2] = 0.0;
+            sl->active = 0;
+            m->count++;
+            return sl;
+        }
+        if (sl->key == key) return sl;
+        h = (h + 1) & (m->cap - 1);
+    }
+}
+
+static void cmap_prune(cmap* m) {
+    cmap old = *m;
+    cmap_init(m, old.cap);
+    for (uint64_t i = 0; i < old.cap; i++)
+        if (old.slots[i].used && old.slots[i].active) {
+            cslot* d = cmap_find(m, old.slots[i].key, 1);
+            *d = old.slots[i];
+        }
+    free(old.slots);
+}
+
+/* ---------------------------------------------------------------------- */
+
+orc_sys* orc_create(int model) {
+    orc_sys* s = calloc(1, sizeof(orc_sys));
+    if (!s) return NULL;
+    s->model = model;
+    for (int d = 0; d < 3; d++) {
+        s->lo[d] = 0.0;
+        s->hi[d] = 1.0;
+        s->size[d] = 1.0;
+        s->periodic[d] = 1;
+    }
+    s->dt = 1.0;
+    s->skin_fraction = 1.0;
+    s->bin_min_size = 1.0;
+    s->every = 0;
+    s->check = 1;
+    s->sort_every = 1000;
+    s->newton = 1;
+    s->state = ORC_STATE_FULL_REBUILD;
+    s->n_last_build = -1;
+    s->cached_min_skin = 1.7976931348623157e308;
+    s->vv_on = (model == ORC_MODEL_DEM);
+    s->rot_on = (model == ORC_MODEL_DEM);
+    s->history_mode = ORC_HIST_REFERENCE;
+    cmap_init(&s->cm, 1024);
+    /* LJForcePlugin default table (md_lj lib.rs:313-315) */
+    orc_set_lj(s, 1.0, 1.0, 2.5);
+    return s;
+}
+
+void orc_destroy(orc_sys* s) {
+    if (!s) return;
+    for (int i = 0; i < s->cap; i++) free(s->hist[i].e);
+    free(s->hist);
+    free(s->tag); free(s->type); free(s->origin); free(s->gshift);
+    free(s->pos); free(s->vel); free(s->force); free(s->cutoff_radius);
+    free(s->mass); free(s->inv_mass); free(s->image); free(s->radius);
+    free(s->inv_inertia); free(s->quat); free(s->omega); free(s->torque);
+    free(s->stencil_fwd); free(s->last_build_pos); free(s->offsets);
+    free(s->indices); free(s->atom_cell); free(s->bin_cnt); free(s->bin_start);
+    free(s->sorted_atoms); free(s->atom_sorted_idx); free(s->sorted_pos);
+    for (int k = 0; k < 6; k++) free(s->swaps[k].send);
+    free(s->beta_ij); free(s->friction_ij); free(s->e_eff_ij); free(s->g_eff_ij);
+    free(s->cm.slots);
+    free(s);
+}
+
+const char* orc_error(const orc_sys* s) { return s->err; }
+
+void orc_set_box(orc_sys* s, const double lo[3], const double hi[3], const int periodic[3]) {
+    for (int d = 0; d < 3; d++) {
+        s->lo[d] = lo[d];
+        s->hi[d] = hi[d];
+        /* Domain::update_derived (domain.rs) */
+        s->size[d] = hi[d] - lo[d];
+        s->periodic[d] = periodic[d] != 0;
+    }
+}
+
+void orc_set_neighbor(orc_sys* s, double skin_fraction, double bin_size, int every, int check,
+                      int sort_every, int newton) {
+    /* neighbor_read_input (neighbor.rs:449-458) */
+    s->skin_fraction = skin_fraction;
+    s->bin_min_size = bin_size;
+    s->every = every;
+    s->check = check;
+    s->sort_every = sort_every;
+    s->newton = newton;
+}
+
+void orc_set_dt(orc_sys* s, double dt) { s->dt = dt; }
+
+int orc_set_materials(orc_sys* s, int n, const double* E, const double* nu, const double* e,
+                      const double* mu) {
+    if (n <= 0 || n > ORC_MAX_MAT) {
+        set_err(s, "material count out of range");
+        return -1;
+    }
+    s->nmat = n;
+    for (int i = 0; i < n; i++) {
+        s->youngs[i] = E[i];
+        s->poisson[i] = nu[i];
+        s->restitution[i] = e[i];
+        s->friction[i] = mu[i];
+    }
+    free(s->beta_ij); free(s->friction_ij); free(s->e_eff_ij); free(s->g_eff_ij);
+    s->beta_ij = xrealloc(NULL, sizeof(double) * n * n);
+    s->friction_ij = xrealloc(NULL, sizeof(double) * n * n);
+    s->e_eff_ij = xrealloc(NULL, sizeof(double) * n * n);
+    s->g_eff_ij = xrealloc(NULL, sizeof(double) * n * n);
+    /* MaterialTable::build_pair_tables (dem_atom/src/lib.rs:389-515) */
+    const double PI = 3.141592653589793;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            double e_ij = sqrt(e[i] * e[j]);
+            double log_e = log(e_ij);
+            s->beta_ij[i * n + j] = -log_e / sqrt(PI * PI + log_e * log_e);
+            s->friction_ij[i * n + j] = sqrt(mu[i] * mu[j]);
+            double nu_i = nu[i], nu_j = nu[j];
+            s->e_eff_ij[i * n + j] =
+                1.0 / ((1.0 - nu_i * nu_i) / E[i] + (1.0 - nu_j * nu_j) / E[j]);
+            s->g_eff_ij[i * n + j] = 1.0 / (2.0 * (2.0 - nu_i) * (1.0 + nu_i) / E[i] +
+                                            2.0 * (2.0 - nu_j) * (1.0 + nu_j) / E[j]);
+        }
+    return 0;
+}
+
+int orc_get_material_tables(const orc_sys* s, double* beta, double* friction, double* e_eff,
+                            double* g_eff) {
+    int nn = s->nmat * s->nmat;
+    if (beta) memcpy(beta, s->beta_ij, sizeof(double) * nn);
+    if (friction) memcpy(friction, s->friction_ij, sizeof(double) * nn);
+    if (e_eff) memcpy(e_eff, s->e_eff_ij, sizeof(double) * nn);
+    if (g_eff) memcpy(g_eff, s->g_eff_ij, sizeof(double) * nn);
+    return s->nmat;
+}
+
+void orc_set_lj(orc_sys* s, double epsilon, double sigma, double cutoff) {
+    /* LJPairCoeffs::from_params (md_lj/src/lib.rs:230-239); powi(3) is
+     * (x*x)*x and powi(2) is x*x */
+    double s2 = sigma * sigma;
+    double sigma6 = (s2 * s2) * s2;
+    s->lj1 = 48.0 * epsilon * sigma6 * sigma6;
+    s->lj2 = 24.0 * epsilon * sigma6;
+    double c = cutoff * sigma;
+    s->lj_cut2 = c * c;
+}
+
+void orc_get_lj(const orc_sys* s, double out[3]) {
+    out[0] = s->lj1;
+    out[1] = s->lj2;
+    out[2] = s->lj_cut2;
+}
+
+void orc_set_nose_hoover(orc_sys* s, int enabled, double temperature, double coupling) {
+    s->nh_on = enabled;
+    s->nh_T = temperature;
+    s->nh_tau = coupling;
+}
+
+void orc_set_integrators(orc_sys* s, int velocity_verlet, int rotation) {
+    s->vv_on = velocity_verlet;
+    s->rot_on = rotation;
+}
+
+void orc_set_gravity(orc_sys* s, int enabled, const double g[3]) {
+    s->gravity_on = enabled;
+    for (int d = 0; d < 3; d++) s->gravity[d] = g[d];
+}
+
+int orc_add_wall(orc_sys* s, const double point[3], const double normal[3], int material,
+                 const double blo[3], const double bhi[3]) {
+    if (s->nwalls >= ORC_MAX_WALL) return -1;
+    wall_t* w = &s->walls[s->nwalls++];
+    for (int d = 0; d < 3; d++) {
+        w->p[d] = point[d];
+        w->n[d] = normal[d];
+        w->blo[d] = blo[d];
+        w->bhi[d] = bhi[d];
+        w->v[d] = 0.0;
+    }
+    w->mat = material;
+    return 0;
+}
+
+void orc_set_history_mode(orc_sys* s, int mode) { s->history_mode = mode; }
+
+double orc_dem_mass(double density, double radius) {
+    /* insert_single_particle (dem_atom/src/insert.rs:340) */
+    const double PI = 3.141592653589793;
+    return density * 4.0 / 3.0 * PI * ((radius * radius) * radius);
+}
+
+double orc_dem_inv_inertia(double mass, double radius) {
+    /* insert.rs:349 */
+    return 1.0 / (0.4 * mass * radius * radius);
+}
+
+double orc_rayleigh_dt(double youngs, double poisson, double density, double radius) {
+    /* calculate_delta_time (insert.rs:1411-1427), per particle, before x0.15 */
+    const double PI = 3.141592653589793;
+    double g = youngs / (2.0 * (1.0 + poisson));
+    double alpha = 0.1631 * poisson + 0.876605;
+    return PI * radius / alpha * sqrt(density / g);
+}
+
+int orc_set_atoms(orc_sys* s, int n, const uint32_t* tag, const uint32_t* type,
+                  const double* pos, const double* vel, const double* omega,
+                  const double* radius, const double* cutoff_radius, const double* mass) {
+    ensure_cap(s, n);
+    for (int i = 0; i < s->nlocal + s->nghost; i++) s->hist[i].n = 0;
+    s->nlocal = n;
+    s->nghost = 0;
+    for (int i = 0; i < n; i++) {
+        s->tag[i] = tag[i];
+        s->type[i] = type ? type[i] : 0;
+        s->origin[i] = 0;
+        s->gshift[i][0] = s->gshift[i][1] = s->gshift[i][2] = 0;
+        for (int d = 0; d < 3; d++) {
+            s->pos[i][d] = pos[3 * i + d];
+            s->vel[i][d] = vel ? vel[3 * i + d] : 0.0;
+            s->force[i][d] = 0.0;
+            s->image[i][d] = 0;
+            s->omega[i][d] = omega ? omega[3 * i + d] : 0.0;
+            s->torque[i][d] = 0.0;
+        }
+        s->cutoff_radius[i] = cutoff_radius[i];
+        s->mass[i] = mass[i];
+        s->inv_mass[i] = 1.0 / mass[i];
+        s->radius[i] = radius ? radius[i] : cutoff_radius[i];
+        s->inv_inertia[i] = radius ? orc_dem_inv_inertia(mass[i], radius[i]) : 0.0;
+        s->quat[i][0] = 1.0;
+        s->quat[i][1] = s->quat[i][2] = s->quat[i][3] = 0.0;
+        s->hist[i].n = 0;
+    }
+    s->state = ORC_STATE_FULL_REBUILD;
+    s->n_last_build = -1;
+    s->steps_since_build = 0;
+    s->sort_counter = 0;
+    s->setup_done = 0;
+    s->nidx = 0;
+    s->step = 0;
+    free(s->cm.slots);
+    cmap_init(&s->cm, 1024);
+    return 0;
+}
+
+int orc_nlocal(const orc_sys* s) { return s->nlocal; }
+int orc_nghost(const orc_sys* s) { return s->nghost; }
+
+void orc_get_atoms(const orc_sys* s, uint32_t* tag, double* pos, double* vel, double* force,
+                   double* omega, double* torque, double* quat, int32_t* image) {
+    for (int i = 0; i < s->nlocal; i++) {
+        if (tag) tag[i] = s->tag[i];
+        for (int d = 0; d < 3; d++) {
+            if (pos) pos[3 * i + d] = s->pos[i][d];
+            if (vel) vel[3 * i + d] = s->vel[i][d];
+            if (force) force[3 * i + d] = s->force[i][d];
+            if (omega) omega[3 * i + d] = s->omega[i][d];
+            if (torque) torque[3 * i + d] = s->torque[i][d];
+            if (image) image[3 * i + d] = s->image[i][d];
+        }
+        if (quat)
+            for (int d = 0; d < 4; d++) quat[4 * i + d] = s->quat[i][d];
+    }
+}
+
+void orc_set_forces(orc_sys* s, const double* force, const double* torque) {
+    for (int i = 0; i < s->nlocal; i++)
+        for (int d = 0; d < 3; d++) {
+            if (force) s->force[i][d] = force[3 * i + d];
+            if (torque) s->torque[i][d] = torque[3 * i + d];
+        }
+}
+
+/* ---- neighbor setup: compute_bin_grid + neighbor_setup ------------------ */
+
+static void compute_bin_grid(orc_sys* s) {
+    /* compute_bin_grid (neighbor.rs:290-371), comm_size = 1 */
+    double max_cutoff = s->max_cutoff;
+    double required_bin = max_cutoff;
+    double min_bin = s->bin_min_size > required_bin ? s->bin_min_size : required_bin;
+    int n_int[3];
+    for (int d = 0; d < 3; d++) {
+        double q = floor(s->size[d] / min_bin);
+        if (q < 1.0) q = 1.0;
+        n_int[d] = (int)q;
+        s->bin_size[d] = s->size[d] / (double)n_int[d];
+    }
+    double ghost_cut = s->ghost_cutoff > max_cutoff ? s->ghost_cutoff : max_cutoff;
+    int sl[3];
+    for (int d = 0; d < 3; d++) {
+        int range = (int)ceil(max_cutoff / s->bin_size[d]);
+        sl[d] = (int)ceil(ghost_cut / s->bin_size[d]) + range;
+        s->bin_count[d] = n_int[d] + 2 * sl[d];
+        s->bin_origin[d] = s->lo[d] - s->bin_size[d] * (double)sl[d];
+    }
+    int nx = s->bin_count[0], ny = s->bin_count[1], nz = s->bin_count[2];
+    s->total_cells = nx * ny * nz;
+    double cutoff_sq = max_cutoff * max_cutoff;
+    free(s->stencil_fwd);
+    s->stencil_fwd = xrealloc(NULL, sizeof(int) * (2 * sl[0] + 1) * (2 * sl[1] + 1) *
+                                        (2 * sl[2] + 1));
+    s->n_fwd = 0;
+    s->has_self = 0;
+    for (int dx = -sl[0]; dx <= sl[0]; dx++)
+        for (int dy = -sl[1]; dy <= sl[1]; dy++)
+            for (int dz = -sl[2]; dz <= sl[2]; dz++) {
+                int ax = abs(dx) - 1, ay = abs(dy) - 1, az = abs(dz) - 1;
+                double mdx = (double)(ax > 0 ? ax : 0) * s->bin_size[0];
+                double mdy = (double)(ay > 0 ? ay : 0) * s->bin_size[1];
+                double mdz = (double)(az > 0 ? az : 0) * s->bin_size[2];
+                if (mdx * mdx + mdy * mdy + mdz * mdz < cutoff_sq) {
+                    int off = dx * ny * nz + dy * nz + dz;
+                    if (off > 0)
+                        s->stencil_fwd[s->n_fwd++] = off;
+                    else if (off == 0)
+                        s->has_self = 1;
+                }
+            }
+}
+
+void orc_neighbor_setup(orc_sys* s) {
+    /* neighbor_setup (neighbor.rs:480-541) */
+    double max_skin = 0.0, min_skin = 1.7976931348623157e308;
+    for (int i = 0; i < s->nlocal; i++) {
+        if (s->cutoff_radius[i] > max_skin) max_skin = s->cutoff_radius[i];
+        if (s->cutoff_radius[i] < min_skin) min_skin = s->cutoff_radius[i];
+    }
+    double max_cutoff = max_skin > 0.0 ? 2.0 * max_skin * s->skin_fraction : s->bin_min_size;
+    double buffer = min_skin < 1.7976931348623157e308 * 0.5
+                        ? (s->skin_fraction - 1.0) * min_skin
+                        : 0.0;
+    double ghost_cut = max_cutoff + 2.0 * buffer;
+    if (ghost_cut < 0.0) ghost_cut = 0.0; /* .max(domain.ghost_cutoff = 0) */
+    s->ghost_cutoff = ghost_cut;
+    s->max_cutoff = max_cutoff;
+    if (s->bin_min_size < max_cutoff) s->bin_min_size = max_cutoff;
+    compute_bin_grid(s);
+    s->setup_done = 1;
+}
+
+int orc_bin_grid(const orc_sys* s, double* out) {
+    /* out: max_cutoff, ghost_cutoff, bin_size[3], bin_origin[3], bin_count[3],
+     * n_forward, has_self */
+    out[0] = s->max_cutoff;
+    out[1] = s->ghost_cutoff;
+    for (int d = 0; d < 3; d++) {
+        out[2 + d] = s->bin_size[d];
+        out[5 + d] = s->bin_origin[d];
+        out[8 + d] = (double)s->bin_count[d];
+    }
+    out[11] = (double)s->n_fwd;
+    out[12] = (double)s->has_self;
+    return s->n_fwd;
+}
+
+/* ---- rebuild decision -------------------------------------------------- */
+
+static void save_build_positions(orc_sys* s) {
+    /* save_build_positions (neighbor.rs:545-561) */
+    int n = s->nlocal;
+    s->last_build_pos = xrealloc(s->last_build_pos, sizeof(*s->last_build_pos) * (n ? n : 1));
+    memcpy(s->last_build_pos, s->pos, sizeof(*s->pos) * n);
+    s->n_last_build = n;
+    s->steps_since_build = 0;
+    double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
+    for (int i = 0; i < n; i++) {
+        double c = s->cutoff_radius[i];
+        if (c < mn) mn = c;
+        if (c > mx) mx = c;
+    }
+    s->cached_min_skin = mn;
+    if (fabs(mx - mn) < 1e-15) {
+        double cutoff = 2.0 * mn * s->skin_fraction;
+        s->have_uniform = 1;
+        s->uniform_cutoff_sq = cutoff * cutoff;
+    } else {
+        s->have_uniform = 0;
+    }
+}
+
+static int displacement_exceeded(const orc_sys* s) {
+    /* displacement_exceeded (neighbor.rs:564-601) */
+    double per_atom_skin = (s->skin_fraction - 1.0) * s->cached_min_skin;
+    double threshold_sq = per_atom_skin * per_atom_skin;
+    double bx = s->size[0], by = s->size[1], bz = s->size[2];
+    double hbx = bx * 0.5, hby = by * 0.5, hbz = bz * 0.5;
+    for (int i = 0; i < s->n_last_build; i++) {
+        double dx = s->pos[i][0] - s->last_build_pos[i][0];
+        double dy = s->pos[i][1] - s->last_build_pos[i][1];
+        double dz = s->pos[i][2] - s->last_build_pos[i][2];
+        if (s->periodic[0]) {
+            if (dx > hbx) dx -= bx; else if (dx < -hbx) dx += bx;
+        }
+        if (s->periodic[1]) {
+            if (dy > hby) dy -= by; else if (dy < -hby) dy += by;
+        }
+        if (s->periodic[2]) {
+            if (dz > hbz) dz -= bz; else if (dz < -hbz) dz += bz;
+        }
+        if (dx * dx + dy * dy + dz * dz > threshold_sq) return 1;
+    }
+    return 0;
+}
+
+static int needs_rebuild(const orc_sys* s) {
+    /* needs_rebuild (neighbor.rs:605-627) */
+    if (s->n_last_build != s->nlocal || s->nlocal == 0) return 1;
+    if (s->state == ORC_STATE_FULL_REBUILD) return 1;
+    if (s->every == 0) return displacement_exceeded(s);
+    if (s->check) return s->steps_since_build >= s->every || displacement_exceeded(s);
+    return s->steps_since_build >= s->every;
+}
+
+void orc_decide_rebuild(orc_sys* s) {
+    /* decide_rebuild (neighbor.rs:639-674) */
+    int need = needs_rebuild(s);
+    if (!need) {
+        for (int i = 0; i < s->nlocal && !need; i++)
+            for (int d = 0; d < 3; d++)
+                if (s->periodic[d] && (s->pos[i][d] < s->lo[d] || s->pos[i][d] >= s->hi[d])) {
+                    need = 1;
+                    break;
+                }
+    }
+    if (need)
+        s->state = ORC_STATE_FULL_REBUILD;
+    else
+        s->steps_since_build += 1;
+}
+
+/* ---- atoms / domain ---------------------------------------------------- */
+
+void orc_zero_all_forces(orc_sys* s) {
+    /* zero_all_forces (atom.rs:529-533) incl. DemAtom #[zero] torque */
+    int n = s->nlocal + s->nghost;
+    memset(s->force, 0, sizeof(*s->force) * n);
+    memset(s->torque, 0, sizeof(*s->torque) * n);
+}
+
+void orc_remove_ghost_atoms(orc_sys* s) {
+    /* remove_ghost_atoms (atom.rs:523-527) */
+    for (int i = s->nlocal; i < s->nlocal + s->nghost; i++) s->hist[i].n = 0;
+    s->nghost = 0;
+}
+
+static void swap_remove(orc_sys* s, int i) {
+    int last = s->nlocal - 1;
+    if (i != last) copy_atom(s, i, last);
+    s->hist[last].n = 0;
+    s->nlocal -= 1;
+}
+
+void orc_pbc(orc_sys* s) {
+    /* pbc (domain.rs:413-459) with wrap_periodic (396-407) */
+    int all = s->periodic[0] && s->periodic[1] && s->periodic[2];
+    if (all) {
+        for (int i = 0; i < s->nlocal; i++)
+            for (int d = 0; d < 3; d++) {
+                double high = s->lo[d] + s->size[d];
+                if (s->pos[i][d] < s->lo[d]) {
+                    s->pos[i][d] += s->size[d];
+                    s->image[i][d] += -1;
+                } else if (s->pos[i][d] >= high) {
+                    s->pos[i][d] -= s->size[d];
+                    s->image[i][d] += 1;
+                }
+            }
+        return;
+    }
+    int removed = 0;
+    for (int i = s->nlocal - 1; i >= 0; i--) {
+        for (int d = 0; d < 3; d++) {
+            if (s->periodic[d]) {
+                double high = s->lo[d] + s->size[d];
+                if (s->pos[i][d] < s->lo[d]) {
+                    s->pos[i][d] += s->size[d];
+                    s->image[i][d] += -1;
+                } else if (s->pos[i][d] >= high) {
+                    s->pos[i][d] -= s->size[d];
+                    s->image[i][d] += 1;
+                }
+            } else if (s->pos[i][d] < s->lo[d] || s->pos[i][d] >= s->hi[d]) {
+                swap_remove(s, i);
+                removed++;
+                break;
+            }
+        }
+    }
+    if (removed > 0) s->state = ORC_STATE_FULL_REBUILD;
+}
+
+/* ---- ghosts (SingleProcessComm self-send) -------------------------------- */
+
+void orc_borders(orc_sys* s) {
+    /* borders (comm.rs:449-570): maxneed = 1 per dim for one process; swap 0
+     * sends atoms near the low face with +L, swap 1 atoms near the high face
+     * with -L (periodic_swap from comm_setup, comm.rs:221-251). */
+    if (s->nghost > 0) orc_remove_ghost_atoms(s);
+    s->nswaps = 0;
+    int scan_end = s->nlocal;
+    double gc = s->ghost_cutoff;
+    for (int dim = 0; dim < 3; dim++) {
+        int dim_scan_end = scan_end;
+        for (int sw = 0; sw < 2; sw++) {
+            swap_t* w = &s->swaps[s->nswaps++];
+            w->nsend = 0;
+            w->dim = dim;
+            w->sign = sw == 0 ? 1 : -1;
+            w->to_self = s->periodic[dim];
+            double offset = s->periodic[dim] ? (sw == 0 ? 1.0 : -1.0) : 0.0;
+            w->pbc_shift[0] = w->pbc_shift[1] = w->pbc_shift[2] = 0.0;
+            w->pbc_shift[dim] = offset * s->size[dim];
+            w->recv_start = s->nlocal + s->nghost;
+            w->recv_count = 0;
+            for (int i = 0; i < dim_scan_end; i++) {
+                double p = s->pos[i][dim];
+                double cut = gc > 0.0 ? gc : s->cutoff_radius[i] * 4.0;
+                int in_skin = sw == 0 ? (p < s->lo[dim] + cut) : (p >= s->hi[dim] - cut);
+                if (!in_skin) continue;
+                if (w->nsend == w->capsend) {
+                    w->capsend = w->capsend ? 2 * w->capsend : 256;
+                    w->send = xrealloc(w->send, sizeof(int) * w->capsend);
+                }
+                w->send[w->nsend++] = i;
+                if (!w->to_self) continue; /* non-periodic dim: nobody receives */
+                int g = s->nlocal + s->nghost;
+                ensure_cap(s, g + 1);
+                copy_atom(s, g, i);
+                s->origin[g] = i;
+                /* pack_atom_inner: pos + change_pos on every component */
+                for (int d = 0; d < 3; d++) s->pos[g][d] = s->pos[i][d] + w->pbc_shift[d];
+                s->gshift[g][dim] = (int8_t)(s->gshift[i][dim] + (sw == 0 ? 1 : -1));
+                s->inv_mass[g] = 1.0 / s->mass[g];
+                s->nghost++;
+                w->recv_count++;
+            }
+        }
+        scan_end = s->nlocal + s->nghost;
+    }
+}
+
+void orc_forward_comm(orc_sys* s) {
+    /* forward_comm (comm.rs:367-424) with compute_per_atom_offset (318-353) */
+    for (int k = 0; k < s->nswaps; k++) {
+        swap_t* w = &s->swaps[k];
+        if (!w->to_self) continue;
+        for (int m = 0; m < w->nsend; m++) {
+            int idx = w->send[m];
+            double offset[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < 3; d++) {
+                if (w->pbc_shift[d] != 0.0) {
+                    double sign = w->pbc_shift[d] > 0.0 ? 1.0 : -1.0;
+                    double ghost_pos = s->pos[idx][d] + sign * s->size[d];
+                    if (ghost_pos > s->lo[d] && ghost_pos < s->hi[d])
+                        offset[d] = -sign * s->size[d];
+                    else
+                        offset[d] = sign * s->size[d];
+                }
+            }
+            int g = w->recv_start + m;
+            for (int d = 0; d < 3; d++) {
+                s->pos[g][d] = s->pos[idx][d] + offset[d];
+                s->vel[g][d] = s->vel[idx][d];
+                s->omega[g][d] = s->omega[idx][d]; /* DemAtom #[forward] omega */
+            }
+        }
+    }
+}
+
+void orc_reverse_send_force(orc_sys* s) {
+    /* reverse_send_force (comm.rs:579-656), self-send branch */
+    for (int k = s->nswaps - 1; k >= 0; k--) {
+        swap_t* w = &s->swaps[k];
+        if (!w->to_self) continue;
+        for (int m = 0; m < w->recv_count; m++) {
+            int g = w->recv_start + m;
+            int o = s->origin[g];
+            for (int d = 0; d < 3; d++) {
+                s->force[o][d] += s->force[g][d];
+                s->torque[o][d] += s->torque[g][d]; /* DemAtom #[reverse] torque */
+            }
+        }
+    }
+}
+
+/* ---- spatial sort -------------------------------------------------------- */
+
+static uint32_t cell_of(const orc_sys* s, const double p[3], double ibx, double iby,
+                        double ibz) {
+    int nx = s->bin_count[0], ny = s->bin_count[1], nz = s->bin_count[2];
+    int cx = (int)floor((p[0] - s->bin_origin[0]) * ibx);
+    int cy = (int)floor((p[1] - s->bin_origin[1]) * iby);
+    int cz = (int)floor((p[2] - s->bin_origin[2]) * ibz);
+    cx = cx < 0 ? 0 : (cx > nx - 1 ? nx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy > ny - 1 ? ny - 1 : cy);
+    cz = cz < 0 ? 0 : (cz > nz - 1 ? nz - 1 : cz);
+    return (uint32_t)(cx * ny * nz + cy * nz + cz);
+}
+
+typedef struct {
+    uint32_t key;
+    int idx;
+} keyidx;
+
+static int cmp_keyidx(const void* a, const void* b) {
+    const keyidx* x = a;
+    const keyidx* y = b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+void orc_sort_atoms_by_bin(orc_sys* s) {
+    /* sort_atoms_by_bin (neighbor.rs:686-794); stable order among equal keys */
+    s->sort_counter += 1;
+    int periodic_sort = s->sort_every > 0 && s->sort_counter >= s->sort_every;
+    if (periodic_sort) s->sort_counter = 0;
+    int n = s->nlocal;
+    if (n == 0 || s->total_cells == 0) {
+        if (periodic_sort) { /* all_reduce(0) = 0: nothing */ }
+        return;
+    }
+    int rebuild_needed = needs_rebuild(s);
+    if (!periodic_sort && !rebuild_needed) return;
+    double ibx = 1.0 / s->bin_size[0], iby = 1.0 / s->bin_size[1], ibz = 1.0 / s->bin_size[2];
+    keyidx* ki = xrealloc(NULL, sizeof(keyidx) * n);
+    for (int i = 0; i < n; i++) {
+        ki[i].key = cell_of(s, s->pos[i], ibx, iby, ibz);
+        ki[i].idx = i;
+    }
+    qsort(ki, n, sizeof(keyidx), cmp_keyidx);
+    int already = 1;
+    for (int i = 0; i < n; i++)
+        if (ki[i].idx != i) {
+            already = 0;
+            break;
+        }
+    if (!already) {
+        /* Atom::apply_permutation + registry.apply_permutation_all (atom.rs:383-394):
+         * gather through scratch copies placed after the ghosts */
+        int base = s->nlocal + s->nghost;
+        ensure_cap(s, base + n);
+        for (int i = 0; i < n; i++) copy_atom(s, base + i, ki[i].idx);
+        for (int i = 0; i < n; i++) copy_atom(s, i, base + i);
+        for (int i = 0; i < n; i++) s->hist[base + i].n = 0;
+        if (s->n_last_build >= n) {
+            double (*old)[3] = xrealloc(NULL, sizeof(*old) * n);
+            memcpy(old, s->last_build_pos, sizeof(*old) * n);
+            for (int i = 0; i < n; i++) memcpy(s->last_build_pos[i], old[ki[i].idx], sizeof(*old));
+            free(old);
+        }
+        if (s->nghost > 0) {
+            int* inv = xrealloc(NULL, sizeof(int) * n);
+            for (int i = 0; i < n; i++) inv[ki[i].idx] = i;
+            for (int g = n; g < n + s->nghost; g++)
+                if (s->origin[g] < n) s->origin[g] = inv[s->origin[g]];
+            /* sendlists hold pre-sort indices; the reference forces a full
+             * borders rebuild on a periodic sort (neighbor.rs:781-793) */
+            free(inv);
+        }
+    }
+    free(ki);
+    if (periodic_sort && !already) {
+        s->n_last_build = -1;
+        s->state = ORC_STATE_FULL_REBUILD;
+    }
+}
+
+/* ---- half neighbor list ---------------------------------------------------- */
+
+#define PUSH_INDEX(val)                                                          \
+    do {                                                                         \
+        if (nidx >= s->capidx) {                                                 \
+            s->capidx = s->capidx ? s->capidx * 2 : 1024;                        \
+            s->indices = xrealloc(s->indices, sizeof(uint32_t) * s->capidx);     \
+        }                                                                        \
+        s->indices[nidx++] = (val);                                              \
+    } while (0)
+
+void orc_bin_neighbor_list(orc_sys* s) {
+    /* bin_neighbor_list (neighbor.rs:810-1080), newton = true */
+    int nlocal = s->nlocal;
+    int total = s->nlocal + s->nghost;
+    s->state = ORC_STATE_COMMUNICATE_ONLY;
+    save_build_positions(s);
+    s->rebuilds++;
+    int nx = s->bin_count[0], ny = s->bin_count[1], nz = s->bin_count[2];
+    int cells = s->total_cells;
+    double ibx = 1.0 / s->bin_size[0], iby = 1.0 / s->bin_size[1], ibz = 1.0 / s->bin_size[2];
+    (void)nx; (void)ny; (void)nz;
+    if (s->cap_bins < total) {
+        s->cap_bins = total;
+        s->atom_cell = xrealloc(s->atom_cell, sizeof(uint32_t) * total);
+        s->sorted_atoms = xrealloc(s->sorted_atoms, sizeof(uint32_t) * total);
+        s->atom_sorted_idx = xrealloc(s->atom_sorted_idx, sizeof(uint32_t) * total);
+        s->sorted_pos = xrealloc(s->sorted_pos, sizeof(*s->sorted_pos) * total);
+    }
+    if (s->cap_cells < cells + 1) {
+        s->cap_cells = cells + 1;
+        s->bin_cnt = xrealloc(s->bin_cnt, sizeof(uint32_t) * (cells + 1));
+        s->bin_start = xrealloc(s->bin_start, sizeof(uint32_t) * (cells + 1));
+    }
+    memset(s->bin_cnt, 0, sizeof(uint32_t) * cells);
+    for (int i = 0; i < total; i++) {
+        uint32_t c = cell_of(s, s->pos[i], ibx, iby, ibz);
+        s->atom_cell[i] = c;
+        s->bin_cnt[c] += 1;
+    }
+    s->bin_start[0] = 0;
+    for (int c = 0; c < cells; c++) s->bin_start[c + 1] = s->bin_start[c] + s->bin_cnt[c];
+    memcpy(s->bin_cnt, s->bin_start, sizeof(uint32_t) * cells);
+    for (int i = 0; i < total; i++) {
+        uint32_t c = s->atom_cell[i];
+        uint32_t p = s->bin_cnt[c];
+        s->sorted_atoms[p] = (uint32_t)i;
+        s->atom_sorted_idx[i] = p;
+        s->bin_cnt[c] = p + 1;
+    }
+    for (int m = 0; m < total; m++) memcpy(s->sorted_pos[m], s->pos[s->sorted_atoms[m]], sizeof(double) * 3);
+
+    if (s->cap_offsets < nlocal + 1) {
+        s->cap_offsets = nlocal + 1;
+        s->offsets = xrealloc(s->offsets, sizeof(uint32_t) * (nlocal + 1));
+    }
+    int64_t nidx = 0;
+    if (s->have_uniform) {
+        double cutoff_sq = s->uniform_cutoff_sq;
+        for (int i = 0; i < nlocal; i++) {
+            s->offsets[i] = (uint32_t)nidx;
+            uint32_t my_cell = s->atom_cell[i];
+            const double* pi = s->pos[i];
+            if (s->has_self) {
+                uint32_t end = s->bin_start[my_cell + 1];
+                for (uint32_t m = s->atom_sorted_idx[i] + 1; m < end; m++) {
+                    const double* pj = s->sorted_pos[m];
+                    double dx = pj[0] - pi[0], dy = pj[1] - pi[1], dz = pj[2] - pi[2];
+                    double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    if (r2 < cutoff_sq) PUSH_INDEX(s->sorted_atoms[m]);
+                }
+            }
+            for (int k = 0; k < s->n_fwd; k++) {
+                int c = (int)my_cell + s->stencil_fwd[k];
+                for (uint32_t m = s->bin_start[c]; m < s->bin_start[c + 1]; m++) {
+                    const double* pj = s->sorted_pos[m];
+                    double dx = pj[0] - pi[0], dy = pj[1] - pi[1], dz = pj[2] - pi[2];
+                    double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    if (r2 < cutoff_sq) PUSH_INDEX(s->sorted_atoms[m]);
+                }
+            }
+        }
+    } else {
+        double sf2 = s->skin_fraction * s->skin_fraction;
+        for (int i = 0; i < nlocal; i++) {
+            s->offsets[i] = (uint32_t)nidx;
+            uint32_t my_cell = s->atom_cell[i];
+            const double* pi = s->pos[i];
+            double si = s->cutoff_radius[i];
+            if (s->has_self) {
+                uint32_t end = s->bin_start[my_cell + 1];
+                for (uint32_t m = s->atom_sorted_idx[i] + 1; m < end; m++) {
+                    uint32_t j = s->sorted_atoms[m];
+                    const double* pj = s->sorted_pos[m];
+                    double dx = pj[0] - pi[0], dy = pj[1] - pi[1], dz = pj[2] - pi[2];
+                    double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    double sum = si + s->cutoff_radius[j];
+                    if (r2 < sum * sum * sf2) PUSH_INDEX(j);
+                }
+            }
+            for (int k = 0; k < s->n_fwd; k++) {
+                int c = (int)my_cell + s->stencil_fwd[k];
+                for (uint32_t m = s->bin_start[c]; m < s->bin_start[c + 1]; m++) {
+                    uint32_t j = s->sorted_atoms[m];
+                    const double* pj = s->sorted_pos[m];
+                    double dx = pj[0] - pi[0], dy = pj[1] - pi[1], dz = pj[2] - pi[2];
+                    double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    double sum = si + s->cutoff_radius[j];
+                    if (r2 < sum * sum * sf2) PUSH_INDEX(j);
+                }
+            }
+        }
+    }
+    s->offsets[nlocal] = (uint32_t)nidx;
+    s->nidx = nidx;
+}
+
+int orc_set_half_list(orc_sys* s, int nlocal, const uint32_t* offsets, const uint32_t* indices) {
+    if (nlocal != s->nlocal) {
+        set_err(s, "half list row count differs from nlocal");
+        return -1;
+    }
+    if (s->cap_offsets < nlocal + 1) {
+        s->cap_offsets = nlocal + 1;
+        s->offsets = xrealloc(s->offsets, sizeof(uint32_t) * (nlocal + 1));
+    }
+    memcpy(s->offsets, offsets, sizeof(uint32_t) * (nlocal + 1));
+    int64_t n = offsets[nlocal];
+    if (s->capidx < n) {
+        s->capidx = n;
+        s->indices = xrealloc(s->indices, sizeof(uint32_t) * (n ? n : 1));
+    }
+    if (n) memcpy(s->indices, indices, sizeof(uint32_t) * n);
+    s->nidx = n;
+    return 0;
+}
+
+int64_t orc_npairs(const orc_sys* s) { return s->nidx; }
+
+int64_t orc_get_pairs(const orc_sys* s, uint32_t* owner_tag, uint32_t* partner_tag,
+                      uint8_t* code) {
+    int64_t k = 0;
+    for (int i = 0; i < s->nlocal; i++)
+        for (uint32_t m = s->offsets[i]; m < s->offsets[i + 1]; m++, k++) {
+            uint32_t j = s->indices[m];
+            if (owner_tag) owner_tag[k] = s->tag[i];
+            if (partner_tag) partner_tag[k] = s->tag[j];
+            if (code)
+                code[k] = (uint8_t)((s->gshift[j][0] + 1) * 9 + (s->gshift[j][1] + 1) * 3 +
+                                    (s->gshift[j][2] + 1));
+        }
+    return k;
+}
+
+/* ---- forces ---------------------------------------------------------------- */
+
+static double fmax0(double a) { return a > 0.0 ? a : 0.0; }
+
+int orc_hertz_mindlin_contact_force(orc_sys* s) {
+    /* hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615),
+     * surface_energy = cohesion = mu_r = mu_tw = 0, newton = true */
+    int nlocal = s->nlocal;
+    int nm = s->nmat;
+    double dt = s->dt;
+    int warnings = 0;
+    if (s->history_mode == ORC_HIST_REFERENCE) {
+        for (int i = 0; i < nlocal; i++)
+            for (int k = 0; k < s->hist[i].n; k++) s->hist[i].e[k].active = 0;
+    } else {
+        for (uint64_t k = 0; k < s->cm.cap; k++) s->cm.slots[k].active = 0;
+    }
+    for (int i = 0; i < nlocal; i++) {
+        for (uint32_t m = s->offsets[i]; m < s->offsets[i + 1]; m++) {
+            int j = (int)s->indices[m];
+            double r1 = s->radius[i], r2 = s->radius[j];
+            double dx = s->pos[j][0] - s->pos[i][0];
+            double dy = s->pos[j][1] - s->pos[i][1];
+            double dz = s->pos[j][2] - s->pos[i][2];
+            double dist_sq = dx * dx + dy * dy + dz * dz;
+            double sum_r = r1 + r2;
+            int mi = (int)s->type[i], mj = (int)s->type[j];
+            double r_eff = (r1 * r2) / sum_r;
+            double e_eff = s->e_eff_ij[mi * nm + mj];
+            double interaction_r = sum_r + 0.0;
+            if (dist_sq >= interaction_r * interaction_r) continue;
+            double distance = sqrt(dist_sq);
+            if (distance == 0.0) continue;
+            double r_min = r1 < r2 ? r1 : r2;
+            double delta = sum_r - distance;
+            if (0.5 * r_min < delta) delta = 0.5 * r_min;
+            if (delta > 0.0 && distance / sum_r < LARGE_OVERLAP_WARN_THRESHOLD) {
+                warnings++;
+                s->overlap_total++;
+                if (warnings > MAX_OVERLAP_WARNINGS) {
+                    set_err(s, "Over 500 excessive overlaps this step -- aborting");
+                    return -2;
+                }
+            }
+            if (delta <= 0.0) continue;
+            double inv_dist = 1.0 / distance;
+            double nx = dx * inv_dist, ny = dy * inv_dist, nz = dz * inv_dist;
+            double g_eff = s->g_eff_ij[mi * nm + mj];
+            double inv_m_i = s->inv_mass[i] > 0.0 ? s->inv_mass[i] : 1.0 / s->mass[i];
+            double inv_m_j = s->inv_mass[j] > 0.0 ? s->inv_mass[j] : 1.0 / s->mass[j];
+            double m_r = 1.0 / (inv_m_i + inv_m_j);
+            double beta = s->beta_ij[mi * nm + mj];
+            double mu = s->friction_ij[mi * nm + mj];
+            double sdr = sqrt(delta * r_eff);
+            double s_n = 2.0 * e_eff * sdr;
+            double k_n = 4.0 / 3.0 * e_eff * sdr;
+            double k_t = 8.0 * g_eff * sdr;
+            double oix = s->omega[i][0], oiy = s->omega[i][1], oiz = s->omega[i][2];
+            double ojx = s->omega[j][0], ojy = s->omega[j][1], ojz = s->omega[j][2];
+            double r1nx = r1 * nx, r1ny = r1 * ny, r1nz = r1 * nz;
+            double vcix = s->vel[i][0] + (oiy * r1nz - oiz * r1ny);
+            double vciy = s->vel[i][1] + (oiz * r1nx - oix * r1nz);
+            double vciz = s->vel[i][2] + (oix * r1ny - oiy * r1nx);
+            double r2nx = r2 * nx, r2ny = r2 * ny, r2nz = r2 * nz;
+            double vcjx = s->vel[j][0] + (-ojy * r2nz + ojz * r2ny);
+            double vcjy = s->vel[j][1] + (-ojz * r2nx + ojx * r2nz);
+            double vcjz = s->vel[j][2] + (-ojx * r2ny + ojy * r2nx);
+            double vrx = vcjx - vcix, vry = vcjy - vciy, vrz = vcjz - vciz;
+            double v_n = vrx * nx + vry * ny + vrz * nz;
+            double f_diss_n = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+            double f_n_mag = fmax0(k_n * delta - f_diss_n);
+            double fnx = f_n_mag * nx, fny = f_n_mag * ny, fnz = f_n_mag * nz;
+            s->force[i][0] -= fnx;
+            s->force[i][1] -= fny;
+            s->force[i][2] -= fnz;
+            s->force[j][0] += fnx;
+            s->force[j][1] += fny;
+            s->force[j][2] += fnz;
+            double vtx = vrx - v_n * nx, vty = vry - v_n * ny, vtz = vrz - v_n * nz;
+            uint32_t tag_i = s->tag[i], tag_j = s->tag[j];
+            double sign = tag_i < tag_j ? 1.0 : -1.0;
+            double stored[3] = {0.0, 0.0, 0.0};
+            hist_entry* he = NULL;
+            cslot* cs = NULL;
+            if (s->history_mode == ORC_HIST_REFERENCE) {
+                hist_list* hl = &s->hist[i];
+                for (int k = 0; k < hl->n; k++)
+                    if (hl->e[k].tag == tag_j) {
+                        he = &hl->e[k];
+                        break;
+                    }
+                if (he) memcpy(stored, he->s, sizeof(stored));
+            } else {
+                uint32_t a = tag_i < tag_j ? tag_i : tag_j, b = tag_i < tag_j ? tag_j : tag_i;
+                cs = cmap_find(&s->cm, ((uint64_t)a << 32) | b, 1);
+                memcpy(stored, cs->s, sizeof(stored));
+            }
+            double sx = sign * stored[0], sy = sign * stored[1], sz = sign * stored[2];
+            double s_dot_n = sx * nx + sy * ny + sz * nz;
+            sx -= s_dot_n * nx;
+            sy -= s_dot_n * ny;
+            sz -= s_dot_n * nz;
+            sx += vtx * dt;
+            sy += vty * dt;
+            sz += vtz * dt;
+            double s_mag = sqrt(sx * sx + sy * sy + sz * sz);
+            double f_t_spring_mag = k_t * s_mag;
+            double f_t_max = mu * fabs(f_n_mag);
+            if (f_t_spring_mag > f_t_max && f_t_spring_mag > TANGENTIAL_EPSILON) {
+                double sc = f_t_max / f_t_spring_mag;
+                sx *= sc;
+                sy *= sc;
+                sz *= sc;
+            }
+            double gamma_t = 2.0 * SQRT_5_3 * beta * sqrt(k_t * m_r);
+            double ftx = k_t * sx - gamma_t * vtx;
+            double fty = k_t * sy - gamma_t * vty;
+            double ftz = k_t * sz - gamma_t * vtz;
+            double f_t_mag = sqrt(ftx * ftx + fty * fty + ftz * ftz);
+            if (f_t_mag > f_t_max && f_t_mag > TANGENTIAL_EPSILON) {
+                double sc = f_t_max / f_t_mag;
+                ftx *= sc;
+                fty *= sc;
+                ftz *= sc;
+            }
+            double tix = r1ny * ftz - r1nz * fty;
+            double tiy = r1nz * ftx - r1nx * ftz;
+            double tiz = r1nx * fty - r1ny * ftx;
+            double tjx = r2ny * ftz - r2nz * fty;
+            double tjy = r2nz * ftx - r2nx * ftz;
+            double tjz = r2nx * fty - r2ny * ftx;
+            s->force[i][0] += ftx;
+            s->force[i][1] += fty;
+            s->force[i][2] += ftz;
+            s->force[j][0] -= ftx;
+            s->force[j][1] -= fty;
+            s->force[j][2] -= ftz;
+            s->torque[i][0] += tix;
+            s->torque[i][1] += tiy;
+            s->torque[i][2] += tiz;
+            s->torque[j][0] += tjx;
+            s->torque[j][1] += tjy;
+            s->torque[j][2] += tjz;
+            double ns[3] = {sign * sx, sign * sy, sign * sz};
+            if (s->history_mode == ORC_HIST_REFERENCE) {
+                if (he) {
+                    memcpy(he->s, ns, sizeof(ns));
+                    he->active = 1;
+                } else {
+                    hist_push(&s->hist[i], tag_j, ns, 1);
+                }
+            } else {
+                memcpy(cs->s, ns, sizeof(ns));
+                cs->active = 1;
+            }
+        }
+    }
+    if (s->history_mode == ORC_HIST_REFERENCE) {
+        for (int i = 0; i < nlocal; i++) {
+            hist_list* hl = &s->hist[i];
+            int w = 0;
+            for (int k = 0; k < hl->n; k++)
+                if (hl->e[k].active) hl->e[w++] = hl->e[k];
+            hl->n = w;
+        }
+    } else {
+        cmap_prune(&s->cm);
+    }
+    return 0;
+}
+
+void orc_wall_contact_force(orc_sys* s) {
+    /* wall_contact_force, plane walls (dem_wall/src/lib.rs:848-1002);
+     * surface_energy = cohesion = mu_tw = 0, static walls */
+    int nm = s->nmat;
+    for (int w = 0; w < s->nwalls; w++) {
+        const wall_t* W = &s->walls[w];
+        for (int i = 0; i < s->nlocal; i++) {
+            double px = s->pos[i][0], py = s->pos[i][1], pz = s->pos[i][2];
+            if (!(px >= W->blo[0] && px <= W->bhi[0] && py >= W->blo[1] && py <= W->bhi[1] &&
+                  pz >= W->blo[2] && pz <= W->bhi[2]))
+                continue;
+            double dx = px - W->p[0], dy = py - W->p[1], dz = pz - W->p[2];
+            double distance = dx * W->n[0] + dy * W->n[1] + dz * W->n[2];
+            if (distance <= 0.0) continue;
+            double radius = s->radius[i];
+            int mi = (int)s->type[i];
+            double r_eff = radius;
+            double e_eff = s->e_eff_ij[mi * nm + W->mat];
+            double delta = radius - distance;
+            if (0.5 * radius < delta) delta = 0.5 * radius;
+            if (delta <= 0.0) continue;
+            if (delta < -0.0) continue;
+            double sdr = sqrt(delta * r_eff);
+            double s_n = 2.0 * e_eff * sdr;
+            double k_n = 4.0 / 3.0 * e_eff * sdr;
+            double m_r = s->mass[i];
+            double vrx = s->vel[i][0] - W->v[0], vry = s->vel[i][1] - W->v[1],
+                   vrz = s->vel[i][2] - W->v[2];
+            double v_n = vrx * W->n[0] + vry * W->n[1] + vrz * W->n[2];
+            double beta = s->beta_ij[mi * nm + W->mat];
+            double f_diss = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+            double f_net = fmax0(k_n * delta - f_diss);
+            s->force[i][0] += f_net * W->n[0];
+            s->force[i][1] += f_net * W->n[1];
+            s->force[i][2] += f_net * W->n[2];
+        }
+    }
+}
+
+void orc_apply_gravity(orc_sys* s) {
+    /* apply_gravity (mddem_fixes/src/lib.rs:756-761) */
+    if (!s->gravity_on) return;
+    for (int i = 0; i < s->nlocal; i++) {
+        s->force[i][0] += s->mass[i] * s->gravity[0];
+        s->force[i][1] += s->mass[i] * s->gravity[1];
+        s->force[i][2] += s->mass[i] * s->gravity[2];
+    }
+}
+
+void orc_lj_force(orc_sys* s) {
+    /* lj_force (md_lj/src/lib.rs:488-633), single type, virial inactive */
+    double lj1 = s->lj1, lj2 = s->lj2, cut2 = s->lj_cut2;
+    for (int i = 0; i < s->nlocal; i++) {
+        double pix = s->pos[i][0], piy = s->pos[i][1], piz = s->pos[i][2];
+        double fi0 = s->force[i][0], fi1 = s->force[i][1], fi2 = s->force[i][2];
+        for (uint32_t m = s->offsets[i]; m < s->offsets[i + 1]; m++) {
+            int j = (int)s->indices[m];
+            double dx = s->pos[j][0] - pix, dy = s->pos[j][1] - piy, dz = s->pos[j][2] - piz;
+            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (r2 >= cut2) continue;
+            double r2inv = 1.0 / r2;
+            double r6inv = r2inv * r2inv * r2inv;
+            double fpair = r2inv * r6inv * fma(lj1, r6inv, -lj2);
+            fi0 = fma(-fpair, dx, fi0);
+            fi1 = fma(-fpair, dy, fi1);
+            fi2 = fma(-fpair, dz, fi2);
+            s->force[j][0] = fma(fpair, dx, s->force[j][0]);
+            s->force[j][1] = fma(fpair, dy, s->force[j][1]);
+            s->force[j][2] = fma(fpair, dz, s->force[j][2]);
+        }
+        s->force[i][0] = fi0;
+        s->force[i][1] = fi1;
+        s->force[i][2] = fi2;
+    }
+}
+
+/* ---- integrators ------------------------------------------------------------ */
+
+void orc_initial_integration(orc_sys* s) {
+    /* initial_integration (mddem_verlet/src/lib.rs:97-123) */
+    double dt = s->dt;
+    for (int i = 0; i < s->nlocal; i++) {
+        double h = 0.5 * dt * s->inv_mass[i];
+        for (int d = 0; d < 3; d++) s->vel[i][d] += h * s->force[i][d];
+        for (int d = 0; d < 3; d++) s->pos[i][d] += s->vel[i][d] * dt;
+    }
+}
+
+void orc_final_integration(orc_sys* s) {
+    /* final_integration (mddem_verlet/src/lib.rs:136-154) */
+    double dt = s->dt;
+    for (int i = 0; i < s->nlocal; i++) {
+        double h = 0.5 * dt * s->inv_mass[i];
+        for (int d = 0; d < 3; d++) s->vel[i][d] += h * s->force[i][d];
+    }
+}
+
+void orc_initial_rotation(orc_sys* s) {
+    /* initial_rotation (dem_granular/src/rotational.rs:59-84, helpers 26-41) */
+    double dt = s->dt;
+    for (int i = 0; i < s->nlocal; i++) {
+        double ii = s->inv_inertia[i];
+        if (ii == 0.0) continue;
+        s->omega[i][0] += 0.5 * dt * s->torque[i][0] * ii;
+        s->omega[i][1] += 0.5 * dt * s->torque[i][1] * ii;
+        s->omega[i][2] += 0.5 * dt * s->torque[i][2] * ii;
+        double ox = s->omega[i][0], oy = s->omega[i][1], oz = s->omega[i][2];
+        double omega_mag = sqrt(ox * ox + oy * oy + oz * oz);
+        double angle = omega_mag * dt;
+        if (angle > 1e-14) {
+            double inv = 1.0 / omega_mag;
+            double ax = ox * inv, ay = oy * inv, az = oz * inv;
+            double half = angle * 0.5;
+            double sn = sin(half);
+            double a[4] = {cos(half), ax * sn, ay * sn, az * sn};
+            double* b = s->quat[i];
+            double q0 = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+            double q1 = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+            double q2 = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+            double q3 = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+            b[0] = q0;
+            b[1] = q1;
+            b[2] = q2;
+            b[3] = q3;
+        }
+    }
+}
+
+void orc_final_rotation(orc_sys* s) {
+    /* final_rotation (rotational.rs:87-100) */
+    double dt = s->dt;
+    for (int i = 0; i < s->nlocal; i++) {
+        double ii = s->inv_inertia[i];
+        if (ii == 0.0) continue;
+        s->omega[i][0] += 0.5 * dt * s->torque[i][0] * ii;
+        s->omega[i][1] += 0.5 * dt * s->torque[i][1] * ii;
+        s->omega[i][2] += 0.5 * dt * s->torque[i][2] * ii;
+    }
+}
+
+double orc_compute_ke(const orc_sys* s) {
+    /* compute_ke (mddem_core/src/atom.rs:489-502), no group mask */
+    double ke = 0.0;
+    for (int i = 0; i < s->nlocal; i++) {
+        const double* v = s->vel[i];
+        ke += s->mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    return 0.5 * ke;
+}
+
+void orc_setup_nose_hoover(orc_sys* s) {
+    /* setup_nose_hoover (md_thermostat/src/lib.rs:195-243), no group */
+    double n = (double)s->nlocal;
+    double ndof = 3.0 * n - 3.0;
+    s->nh_ndof = ndof;
+    s->nh_Q = ndof * s->nh_T * s->nh_tau * s->nh_tau;
+    s->p_xi = 0.0;
+}
+
+void orc_nh_pre_initial(orc_sys* s) {
+    /* nh_pre_initial (md_thermostat/src/lib.rs:254-301) */
+    double dt = s->dt;
+    double ke = orc_compute_ke(s);
+    s->p_xi += (dt / 2.0) * (2.0 * ke - s->nh_ndof * s->nh_T);
+    double scale = exp(-dt / 2.0 * s->p_xi / s->nh_Q);
+    for (int i = 0; i < s->nlocal; i++) {
+        double* v = s->vel[i];
+        v[0] *= scale;
+        v[1] *= scale;
+        v[2] *= scale;
+        double h = 0.5 * dt * s->inv_mass[i];
+        v[0] += h * s->force[i][0];
+        v[1] += h * s->force[i][1];
+        v[2] += h * s->force[i][2];
+        s->pos[i][0] += v[0] * dt;
+        s->pos[i][1] += v[1] * dt;
+        s->pos[i][2] += v[2] * dt;
+    }
+}
+
+void orc_nh_post_final(orc_sys* s) {
+    /* nh_post_final (md_thermostat/src/lib.rs:313-353) */
+    double dt = s->dt;
+    double scale = exp(-dt / 2.0 * s->p_xi / s->nh_Q);
+    for (int i = 0; i < s->nlocal; i++) {
+        double* v = s->vel[i];
+        double h = 0.5 * dt * s->inv_mass[i];
+        v[0] += h * s->force[i][0];
+        v[1] += h * s->force[i][1];
+        v[2] += h * s->force[i][2];
+        v[0] *= scale;
+        v[1] *= scale;
+        v[2] *= scale;
+    }
+    double ke = orc_compute_ke(s);
+    s->p_xi += (dt / 2.0) * (2.0 * ke - s->nh_ndof * s->nh_T);
+}
+
+/* ---- scheduler: one step in the phase order of schedule.dot ------------------ */
+
+int orc_step(orc_sys* s) {
+    if (!s->setup_done) {
+        orc_neighbor_setup(s);
+        if (s->nh_on) orc_setup_nose_hoover(s);
+    }
+    /* PreInitialIntegration */
+    if (s->nh_on) orc_nh_pre_initial(s);
+    /* InitialIntegration */
+    if (s->vv_on) orc_initial_integration(s);
+    if (s->rot_on) orc_initial_rotation(s);
+    /* PostInitialIntegration */
+    if (s->state == ORC_STATE_COMMUNICATE_ONLY) orc_decide_rebuild(s);
+    orc_zero_all_forces(s);
+    if (s->state == ORC_STATE_FULL_REBUILD) orc_remove_ghost_atoms(s);
+    /* PreExchange */
+    if (s->state == ORC_STATE_FULL_REBUILD) orc_pbc(s);
+    /* PreNeighbor */
+    if (s->state == ORC_STATE_COMMUNICATE_ONLY) orc_forward_comm(s);
+    orc_sort_atoms_by_bin(s);
+    if (s->state == ORC_STATE_FULL_REBUILD) orc_borders(s);
+    /* Neighbor */
+    if (s->state == ORC_STATE_FULL_REBUILD) orc_bin_neighbor_list(s);
+    /* Force */
+    if (s->model == ORC_MODEL_DEM) {
+        if (orc_hertz_mindlin_contact_force(s) != 0) return -2;
+        orc_wall_contact_force(s);
+    } else {
+        orc_lj_force(s);
+    }
+    orc_apply_gravity(s);
+    /* PostForce */
+    if (s->newton) orc_reverse_send_force(s);
+    /* FinalIntegration */
+    if (s->vv_on) orc_final_integration(s);
+    if (s->rot_on) orc_final_rotation(s);
+    /* PostFinalIntegration */
+    if (s->nh_on) orc_nh_post_final(s);
+    s->step++;
+    return 0;
+}
+
+int orc_run(orc_sys* s, int64_t nsteps) {
+    for (int64_t k = 0; k < nsteps; k++) {
+        int rc = orc_step(s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int orc_state(const orc_sys* s) { return s->state; }
+double orc_p_xi(const orc_sys* s) { return s->p_xi; }
+int64_t orc_rebuild_count(const orc_sys* s) { return s->rebuilds; }
+int64_t orc_overlap_warnings(const orc_sys* s) { return s->overlap_total; }
+
+int64_t orc_history_count(const orc_sys* s) {
+    if (s->history_mode == ORC_HIST_CANONICAL) return (int64_t)s->cm.count;
+    int64_t n = 0;
+    for (int i = 0; i < s->nlocal; i++) n += s->hist[i].n;
+    return n;
+}
+
+int64_t orc_get_history(const orc_sys* s, uint32_t* owner_tag, uint32_t* partner_tag,
+                        double* spring) {
+    int64_t k = 0;
+    if (s->history_mode == ORC_HIST_CANONICAL) {
+        for (uint64_t m = 0; m < s->cm.cap; m++)
+            if (s->cm.slots[m].used) {
+                owner_tag[k] = (uint32_t)(s->cm.slots[m].key >> 32);
+                partner_tag[k] = (uint32_t)(s->cm.slots[m].key & 0xffffffffu);
+                memcpy(&spring[3 * k], s->cm.slots[m].s, sizeof(double) * 3);
+                k++;
+            }
+        return k;
+    }
+    for (int i = 0; i < s->nlocal; i++)
+        for (int m = 0; m < s->hist[i].n; m++, k++) {
+            owner_tag[k] = s->tag[i];
+            partner_tag[k] = s->hist[i].e[m].tag;
+            memcpy(&spring[3 * k], s->hist[i].e[m].s, sizeof(double) * 3);
+        }
+    return k;
+}
+
+int orc_set_history(orc_sys* s, int64_t n, const uint32_t* owner_tag, const uint32_t* partner_tag,
+                    const double* spring) {
+    if (s->history_mode == ORC_HIST_CANONICAL) {
+        for (int64_t k = 0; k < n; k++) {
+            uint32_t a = owner_tag[k] < partner_tag[k] ? owner_tag[k] : partner_tag[k];
+            uint32_t b = owner_tag[k] < partner_tag[k] ? partner_tag[k] : owner_tag[k];
+            cslot* c = cmap_find(&s->cm, ((uint64_t)a << 32) | b, 1);
+            memcpy(c->s, &spring[3 * k], sizeof(double) * 3);
+        }
+        return 0;
+    }
+    for (int64_t k = 0; k < n; k++) {
+        int found = -1;
+        for (int i = 0; i < s->nlocal; i++)
+            if (s->tag[i] == owner_tag[k]) {
+                found = i;
+                break;
+            }
+        if (found < 0) {
+            set_err(s, "history owner tag not found");
+            return -1;
+        }
+        hist_push(&s->hist[found], partner_tag[k], &spring[3 * k], 0);
+    }
+    return 0;
+}
