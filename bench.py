@@ -1,0 +1,167 @@
+"""Throughput of the particle hot path on one MI355X.
+
+Default workload: LJ fluid, FCC 40^3 = 256,000 atoms at rho* = 0.85,
+Nose-Hoover at T* = 0.85, dt = 0.005, rc = 2.5, neighbor cutoff 2.8
+(skin 1.12 x 2 x 1.25), mixed precision (FP32 pair forces, FP64 state).
+A step is one full scheduled step (integrate, rebuild when the skin is
+spent, force, integrate + thermostat) of the whole system; particle-steps/s
+= N x steps / wall seconds of the timed loop, with the state resident in HBM.
+
+Prints one JSON line. Extra keys: the all-FP64 GPU run of the same workload,
+the pair-force kernel's roofline (HIP-event timed inside the timed loop) and
+the CPU baseline (the CPU checker on the host, 1 thread, a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s (LJ 256k, DEM 4M) vs FP64 GPU & CPU; ensemble p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(w, precision, steps, warmup, profile_every=10):
+    from mddem_amd import hip
+    s = hip.from_workload(w, precision, box_exit=False)
+    s.run(warmup)
+    s.synchronize()
+    th0 = s.thermo()
+    s.set_profiling(profile_every)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    th1 = s.thermo(energy=(w.model == "lj"))
+    avg_ms, launches = s.force_kernel_stats()
+    res = dict(seconds=t1 - t0, steps=steps, n=w.n, rebuilds=th1["rebuilds"] - th0["rebuilds"],
+               npairs=th1["npairs"], force_ms=avg_ms, force_launches=launches,
+               temperature=th1["temperature"], pe=th1["pe"], ke=th1["ke"])
+    s.close()
+    return res
+
+
+def force_kernel_bytes(n, npairs, force_bytes):
+    """Algorithmic HBM bytes of one pair-force launch: positions read once
+    (24 B FP64), the half list streamed once (4 B per pair) and the force
+    written once (3 x 4 B mixed, 3 x 8 B FP64)."""
+    return n * 24.0 + 4.0 * npairs + n * 3.0 * force_bytes
+
+
+def cpu_baseline(w, seconds_budget=12.0):
+    """The CPU checker (a C port of the reference path) on the host, 1 thread,
+    over a bounded number of steps of the same workload."""
+    from tests.helpers import oracle_from_workload  # the checker; CPU leg only
+    o = oracle_from_workload(w)
+    o.step()  # setup + first rebuild, untimed
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds_budget:
+        o.run(5)
+        steps += 5
+    dt = time.perf_counter() - t0
+    return dict(value=w.n * steps / dt, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{steps} scheduled steps (after 1 untimed setup step) of the same "
+                       f"{w.n}-atom workload on 1 host thread")
+
+
+def traffic_from_profiles(name):
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--ncell", type=int, default=40)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--fp64-steps", type=int, default=5000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU scaling is not "
+              "part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from mddem_amd import hip
+    from mddem_amd import workloads as W
+
+    w = W.lj_fcc(ncell=args.ncell)
+    prec = hip.MIXED if args.precision == "mixed" else hip.FP64
+    log(f"[bench] workload {w.name} n={w.n} precision={args.precision} "
+        f"steps={args.steps} warmup={args.warmup}")
+    r = run_gpu(w, prec, args.steps, args.warmup)
+    value = r["n"] * r["steps"] / r["seconds"]
+    log(f"[bench] {value:.4g} particle-steps/s, {r['rebuilds']} rebuilds, "
+        f"force kernel {r['force_ms'] * 1e3:.1f} us")
+
+    fp64 = None
+    if prec == hip.MIXED and args.fp64_steps > 0:
+        r64 = run_gpu(w, hip.FP64, args.fp64_steps, args.warmup)
+        fp64 = r64["n"] * r64["steps"] / r64["seconds"]
+        log(f"[bench] fp64 {fp64:.4g} particle-steps/s, force kernel "
+            f"{r64['force_ms'] * 1e3:.1f} us")
+
+    fbytes = 4 if prec == hip.MIXED else 8
+    alg = force_kernel_bytes(r["n"], r["npairs"], fbytes)
+    achieved = alg / (r["force_ms"] * 1e-3) / 1e9 if r["force_ms"] > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(w.name),
+                "kernel": "k_lj (pair force)", "bytes_per_launch": alg,
+                "avg_launch_us": round(r["force_ms"] * 1e3, 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(w, args.cpu_seconds)
+        log(f"[bench] cpu {cpu['value']:.4g} particle-steps/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r["seconds"] / r["steps"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-force/fp64-state" if prec == hip.MIXED else "fp64",
+        "data": "synthetic (seeded FCC lattice, Maxwell-Boltzmann velocities)",
+        "config": {"workload": f"LJ fluid FCC {args.ncell}^3 = {w.n} atoms, rho*=0.85, "
+                               "T*=0.85 Nose-Hoover tau=1, dt=0.005, rc=2.5, skin 2.8",
+                   "precision": args.precision, "rebuild": "displacement (lazy wrap)",
+                   "rebuilds_in_timed_loop": r["rebuilds"],
+                   "pairs_per_atom_half": round(r["npairs"] / r["n"], 3)},
+        "fp64_gpu_value": None if fp64 is None else round(fp64, 1),
+        "mixed_over_fp64": None if fp64 is None else round(value / fp64, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": None if cpu is None else round(value / cpu["value"], 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,183 @@
+/*
+ * mddem_hip.h -- C ABI of the MI355X (gfx950) particle hot path.
+ *
+ * One context = one particle system on one GPU, driven by one host thread
+ * (the reference's single-threaded scheduler; distinct contexts may live on
+ * distinct threads). All host buffers are caller-owned; device memory is
+ * owned by the context. Every function returns an int status: 0 = ok,
+ * negative = error (MDD_ERR_*); mdd_last_error() gives the message. Nothing
+ * unwinds or aborts across the ABI.
+ *
+ * Each entry point names the reference system / type it stands in for
+ * (SueHeir/MDDEM v0.1.3, path:line). Array arguments are row-major: "n*3"
+ * means x,y,z per particle.
+ */
+#ifndef MDDEM_HIP_H
+#define MDDEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDD_OK 0
+#define MDD_ERR_INVALID (-1)     /* bad argument or call order */
+#define MDD_ERR_HIP (-2)         /* HIP runtime error */
+#define MDD_ERR_OVERLAP (-3)     /* > 500 excessive overlaps in one step
+                                    (crates/dem_granular/src/contact.rs:197-211) */
+#define MDD_ERR_CAPACITY (-4)    /* allocation failed */
+#define MDD_ERR_UNSUPPORTED (-5) /* configuration outside this path */
+#define MDD_ERR_NO_DEVICE (-6)   /* no usable gfx950 device */
+
+enum { MDD_MODEL_LJ = 0, MDD_MODEL_DEM = 1 };
+enum { MDD_FP64 = 0, MDD_MIXED = 1 };
+
+typedef struct mdd_ctx mdd_ctx;
+
+typedef struct {
+    int device;        /* HIP device ordinal */
+    int model;         /* MDD_MODEL_LJ (LJDefaultPlugins) or MDD_MODEL_DEM
+                          (GranularDefaultPlugins) */
+    int precision;     /* MDD_FP64: all FP64. MDD_MIXED: FP32 pair forces and
+                          accumulation, FP64 positions, velocities, thermostat
+                          and shear displacement */
+    int64_t capacity;  /* max particles; 0 = size on first upload */
+} mdd_params;
+
+typedef struct {
+    int64_t step;          /* completed steps */
+    int64_t rebuilds;      /* neighbor rebuilds so far */
+    int64_t npairs;        /* half-list pairs (reference pair count) */
+    int64_t nghost;        /* periodic images at the last rebuild */
+    int64_t ncontacts;     /* DEM pairs in contact at the last force pass */
+    double ke;             /* 0.5 sum m v^2 (crates/mddem_core/src/atom.rs:489-502) */
+    double temperature;    /* 2 KE / (3N - 3) */
+    double pe;             /* LJ 12-6 energy, unshifted (computed on request) */
+    double p_xi;           /* Nose-Hoover thermostat momentum */
+    double momentum[3];    /* sum m v */
+    double granular_temperature; /* sum m |v - vbar|^2 / (3 M)
+                                    (crates/dem_granular/src/granular_temp.rs:44-134) */
+} mdd_thermo;
+
+/* ---- lifetime (App / resources) ---------------------------------------- */
+int mdd_create(mdd_ctx** ctx, const mdd_params* params);
+int mdd_destroy(mdd_ctx* ctx);
+const char* mdd_last_error(const mdd_ctx* ctx);
+int mdd_version(void);
+
+/* ---- configuration --------------------------------------------------------- */
+/* Domain box and periodicity: [domain] section, Domain::update_derived
+ * (crates/mddem_core/src/domain.rs:105-160) */
+int mdd_set_box(mdd_ctx* ctx, const double lo[3], const double hi[3], const int periodic[3]);
+/* [neighbor] section, NeighborConfig (crates/mddem_core/src/neighbor.rs:45-82);
+ * every must be 0 (displacement rebuilds), newton selects nothing here (the
+ * pair set is the newton half list; forces are gathered) */
+int mdd_set_neighbor(mdd_ctx* ctx, double skin_fraction, double bin_size, int every,
+                     int check, int sort_every, int newton);
+/* 1 (default): rebuild when any particle leaves the periodic box, as
+ * decide_rebuild does (neighbor.rs:651-666). 0: wrap lazily at displacement
+ * rebuilds only (pair images stay valid between rebuilds). */
+int mdd_set_rebuild_on_box_exit(mdd_ctx* ctx, int enabled);
+/* Atom::dt; [run] dt or calculate_delta_time (crates/dem_atom/src/insert.rs:1392-1431) */
+int mdd_set_dt(mdd_ctx* ctx, double dt);
+/* [[dem.materials]] + MaterialTable::build_pair_tables (crates/dem_atom/src/lib.rs:389-515) */
+int mdd_set_materials(mdd_ctx* ctx, int n, const double* youngs, const double* poisson,
+                      const double* restitution, const double* friction);
+/* [lj] section, LJPairCoeffs::from_params (crates/md_lj/src/lib.rs:230-239) */
+int mdd_set_lj(mdd_ctx* ctx, double epsilon, double sigma, double cutoff);
+/* [thermostat] section, NoseHooverPlugin (crates/md_thermostat/src/lib.rs:166-243) */
+int mdd_set_nose_hoover(mdd_ctx* ctx, int enabled, double temperature, double coupling);
+/* VelocityVerletPlugin (crates/mddem_verlet/src/lib.rs:46-85) and
+ * RotationalDynamicsPlugin (crates/dem_granular/src/rotational.rs:49-56) */
+int mdd_set_integrators(mdd_ctx* ctx, int velocity_verlet, int rotation);
+/* [gravity] section, GravityPlugin (crates/mddem_fixes/src/lib.rs:735-761) */
+int mdd_set_gravity(mdd_ctx* ctx, int enabled, const double g[3]);
+/* [[wall]] plane entry, WallPlane (crates/dem_wall/src/lib.rs:302-357); static walls */
+int mdd_add_wall(mdd_ctx* ctx, const double point[3], const double normal[3], int material,
+                 const double bound_lo[3], const double bound_hi[3]);
+int mdd_clear_walls(mdd_ctx* ctx);
+
+/* ---- state ------------------------------------------------------------------ */
+/* Replaces particle insertion into Atom + DemAtom (crates/dem_atom/src/insert.rs:323-355,
+ * crates/md_lattice/src/lib.rs:336-360): per-particle tag, type (material or LJ
+ * type), pos/vel/omega (n*3; omega may be NULL), radius (DEM; NULL for LJ),
+ * cutoff_radius (neighbor skin radius) and mass. Forces start at zero and the
+ * first step integrates with them, as the reference does. */
+int mdd_upload_state(mdd_ctx* ctx, int64_t n, const uint32_t* tag, const uint32_t* type,
+                     const double* pos, const double* vel, const double* omega,
+                     const double* radius, const double* cutoff_radius, const double* mass);
+int64_t mdd_nlocal(const mdd_ctx* ctx);
+/* current particle order (spatially sorted); any pointer may be NULL;
+ * quat is n*4 (w,x,y,z) */
+int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, double* omega,
+                       double* quat, int32_t* image);
+int mdd_download_forces(mdd_ctx* ctx, double* force, double* torque);
+int mdd_upload_forces(mdd_ctx* ctx, const double* force, const double* torque);
+
+/* ---- systems (one call = one reference system) ------------------------------ */
+/* neighbor_setup + compute_bin_grid (crates/mddem_core/src/neighbor.rs:480-541, 290-371) */
+int mdd_neighbor_setup(mdd_ctx* ctx);
+/* One full rebuild now: pbc (domain.rs:413-459), sort_atoms_by_bin
+ * (neighbor.rs:686-794), borders (comm.rs:449-570) and bin_neighbor_list
+ * (neighbor.rs:810-1080). */
+int mdd_bin_neighbor_list(mdd_ctx* ctx);
+/* Inject a half list (CSR over particle indices, no images), the way the
+ * reference unit tests build Neighbor by hand (e.g. contact.rs:1032-1034). */
+int mdd_set_half_list(mdd_ctx* ctx, int64_t nlocal, const uint32_t* offsets,
+                      const uint32_t* indices);
+/* number of half-list pairs (Neighbor::neighbor_indices.len()) */
+int64_t mdd_npairs(mdd_ctx* ctx);
+/* Half-list pairs as (owner tag, partner tag, image code), code =
+ * (sx+1)*9 + (sy+1)*3 + (sz+1) for the partner's periodic image shift;
+ * Neighbor::pairs (neighbor.rs:172-227). Writes at most cap, returns the
+ * total in *n. */
+int mdd_download_pairs(mdd_ctx* ctx, int64_t cap, uint32_t* owner_tag, uint32_t* partner_tag,
+                       uint8_t* code, int64_t* n);
+/* lj_force (crates/md_lj/src/lib.rs:488-633); overwrites forces */
+int mdd_lj_force(mdd_ctx* ctx);
+/* hertz_mindlin_contact_force (crates/dem_granular/src/contact.rs:103-615) with the
+ * shear history store (tangential.rs:54-133) keyed by canonical pair; overwrites
+ * forces and torques */
+int mdd_hertz_mindlin_contact_force(mdd_ctx* ctx);
+/* wall_contact_force, plane walls (crates/dem_wall/src/lib.rs:848-1002); adds */
+int mdd_wall_contact_force(mdd_ctx* ctx);
+/* apply_gravity (crates/mddem_fixes/src/lib.rs:756-761); adds */
+int mdd_apply_gravity(mdd_ctx* ctx);
+/* initial_integration / final_integration (crates/mddem_verlet/src/lib.rs:97-154) */
+int mdd_initial_integration(mdd_ctx* ctx);
+int mdd_final_integration(mdd_ctx* ctx);
+/* initial_rotation / final_rotation (crates/dem_granular/src/rotational.rs:59-100) */
+int mdd_initial_rotation(mdd_ctx* ctx);
+int mdd_final_rotation(mdd_ctx* ctx);
+/* setup_nose_hoover, nh_pre_initial, nh_post_final
+ * (crates/md_thermostat/src/lib.rs:195-353) */
+int mdd_setup_nose_hoover(mdd_ctx* ctx);
+int mdd_nh_pre_initial(mdd_ctx* ctx);
+int mdd_nh_post_final(mdd_ctx* ctx);
+/* nsteps full steps in the ParticleSimScheduleSet phase order
+ * (crates/mddem_core/src/schedule.rs:22-52, schedule.dot); the App run loop */
+int mdd_run(mdd_ctx* ctx, int64_t nsteps);
+
+/* ---- shear history (ContactHistoryStore, tangential.rs:54-133) --------------- */
+/* (min tag, max tag, spring[3]) in the canonical sign of the lower tag;
+ * at most cap entries, total in *n */
+int mdd_download_history(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* tag_hi,
+                         double* spring, int64_t* n);
+
+/* ---- observables (print_thermo, mddem_print/src/lib.rs:595-700) -------------- */
+int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out);
+/* block until queued work is done */
+int mdd_synchronize(mdd_ctx* ctx);
+/* Time every `every`-th scheduled pair-force launch with HIP events on the
+ * context's stream (0 = off); resets the statistics. No reference
+ * counterpart: measurement hook for the roofline report. */
+int mdd_set_profiling(mdd_ctx* ctx, int every);
+/* Average device time (ms) of the timed pair-force launches since the last
+ * mdd_set_profiling, and the number of pair-force launches. */
+int mdd_force_kernel_stats(mdd_ctx* ctx, double* avg_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

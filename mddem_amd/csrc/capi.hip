@@ -1,0 +1,910 @@
+// C ABI of the gfx950 particle path (include/mddem_hip.h) and the host step
+// scheduler that runs the reference phase order (ParticleSimScheduleSet,
+// crates/mddem_core/src/schedule.rs:22-52; schedule.dot) on one GPU.
+//
+// Step scheduling: a chunk of K steps is queued as [pre, force, post] x K
+// with no host synchronisation. The pre kernel of each step takes the
+// rebuild decision on the device; a step that needs a rebuild records itself
+// in Status::abort_step and every later kernel of the chunk returns
+// immediately. The host then reads the status once per chunk, rebuilds the
+// neighbor list and resumes at the force pass of the aborted step, which is
+// exactly where the reference rebuilds (decide_rebuild runs after the
+// initial integration, neighbor.rs:639-674). Periodic spatial sorts
+// (sort_atoms_by_bin every sort_every steps, neighbor.rs:697-701) are
+// scheduled by the host.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "ctx.hpp"
+#include "mddem_hip.h"
+
+using mdd::ApiError;
+using mdd::Ctx;
+
+struct mdd_ctx {
+    Ctx c;
+};
+
+namespace mdd {
+void neighbor_setup(Ctx& c);
+}
+
+namespace {
+
+__global__ void k_status_init(mdd::Status* st) {
+    st->abort_step = INT_MAX;
+    st->error = 0;
+    st->overflow = 0u;
+    st->max_row = 0;
+    st->max_half = 0;
+    st->ghost_total = 0u;
+    st->overlap_step = 0;
+    st->overlap_count = 0;
+    st->ticket = 0u;
+    st->ncontacts = 0u;
+    st->ke = 0.0;
+    st->p_xi = 0.0;
+    st->nh_scale = 1.0;
+    st->p_xi_pre = 0.0;
+}
+
+__global__ void k_set_abort(mdd::Status* st, int v) { st->abort_step = v; }
+__global__ void k_pxi_from_pre(mdd::Status* st) { st->p_xi = st->p_xi_pre; }
+
+constexpr int kEventPool = 2048;
+
+int fail(mdd_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->c.err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(mdd_ctx* ctx, F&& body) {
+    if (!ctx) return MDD_ERR_INVALID;
+    try {
+        MDD_HIP(hipSetDevice(ctx->c.device));
+        body(ctx->c);
+        ctx->c.err.clear();
+        return MDD_OK;
+    } catch (const ApiError& e) {
+        return fail(ctx, e.code, e.what());
+    } catch (const mdd::HipError& e) {
+        return fail(ctx, MDD_ERR_HIP, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, MDD_ERR_CAPACITY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(ctx, MDD_ERR_INVALID, e.what());
+    }
+}
+
+void need_state(Ctx& c) {
+    if (c.n <= 0) throw ApiError(MDD_ERR_INVALID, "no particles uploaded");
+}
+
+void need_list(Ctx& c) {
+    need_state(c);
+    if (!c.list_valid) throw ApiError(MDD_ERR_INVALID, "neighbor list not built");
+}
+
+void sync_status(Ctx& c) {
+    MDD_HIP(hipMemcpyAsync(c.status_host, c.status.p, sizeof(mdd::Status), hipMemcpyDeviceToHost,
+                           c.stream));
+    MDD_HIP(hipStreamSynchronize(c.stream));
+}
+
+void harvest_profile(Ctx& c) {
+    for (int64_t k = 0; k < c.prof_pending; k++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c.ev_pool[2 * k], c.ev_pool[2 * k + 1]) == hipSuccess) {
+            c.prof_ms_sum += ms;
+            c.prof_count++;
+        }
+    }
+    c.prof_pending = 0;
+}
+
+void check_device_error(Ctx& c) {
+    if (c.status_host->error) {
+        const int e = c.status_host->error;
+        if (e == MDD_ERR_OVERLAP)
+            throw ApiError(MDD_ERR_OVERLAP,
+                           "Over 500 excessive overlaps this step -- aborting. Check timestep or "
+                           "initial configuration.");
+        throw ApiError(e, "device reported an error");
+    }
+}
+
+void setup_nose_hoover(Ctx& c) {
+    // setup_nose_hoover (md_thermostat/src/lib.rs:195-243): ndof = 3N - 3,
+    // Q = ndof T tau^2, p_xi = 0, then the first pre half-step's KE
+    const double n = double(c.n);
+    c.nh_ndof = 3.0 * n - 3.0;
+    c.nh_Q = c.nh_ndof * c.nh_T * c.nh_tau * c.nh_tau;
+    k_status_init<<<1, 1, 0, c.stream>>>(c.status.p);
+    mdd::launch_ke(c);
+    c.nh_ready = true;
+}
+
+void run_steps(Ctx& c, int64_t nsteps) {
+    need_state(c);
+    if (c.model == MDD_MODEL_DEM && c.mat.n == 0)
+        throw ApiError(MDD_ERR_INVALID, "materials not set");
+    if (c.every != 0)
+        throw ApiError(MDD_ERR_UNSUPPORTED, "only displacement rebuilds (every = 0) are supported");
+    if (c.step + nsteps > INT_MAX - 2) throw ApiError(MDD_ERR_INVALID, "step count too large");
+    if (!c.setup_done) mdd::neighbor_setup(c);
+    if (c.nh_on && !c.nh_ready) setup_nose_hoover(c);
+    const int64_t end = c.step + nsteps;
+    while (c.step < end) {
+        const int64_t s = c.step + 1;
+        const bool forced = !c.list_valid || c.list_injected ||
+                            (c.sort_every > 0 && s % c.sort_every == 0);
+        if (forced) {
+            // FullRebuild step: integrate, rebuild, force, integrate
+            mdd::launch_pre(c, int(s), false);
+            mdd::rebuild(c);
+            harvest_profile(c);
+            mdd::launch_pair_forces(c, int(s), false);
+            mdd::launch_post(c, int(s));
+            c.step = s;
+            continue;
+        }
+        int64_t K = std::min<int64_t>(c.chunk, end - c.step);
+        if (c.sort_every > 0) {
+            const int64_t next_sort = (c.step / c.sort_every + 1) * c.sort_every;
+            K = std::min<int64_t>(K, next_sort - 1 - c.step);
+        }
+        if (K < 1) K = 1;
+        for (int64_t k = 1; k <= K; k++) {
+            const int st = int(c.step + k);
+            mdd::launch_pre(c, st, true);
+            mdd::launch_pair_forces(c, st, false);
+            mdd::launch_post(c, st);
+        }
+        sync_status(c);
+        harvest_profile(c);
+        check_device_error(c);
+        const int ab = c.status_host->abort_step;
+        if (ab != INT_MAX) {
+            k_set_abort<<<1, 1, 0, c.stream>>>(c.status.p, INT_MAX);
+            mdd::rebuild(c);
+            harvest_profile(c);
+            mdd::launch_pair_forces(c, ab, false);
+            mdd::launch_post(c, ab);
+            const int64_t ran = int64_t(ab) - c.step;
+            c.step = ab;
+            c.chunk = int(std::max<int64_t>(2, ran + ran / 4 + 1));
+        } else {
+            c.step += K;
+            c.chunk = std::min(c.chunk * 2, 1024);
+        }
+    }
+    sync_status(c);
+    harvest_profile(c);
+    check_device_error(c);
+}
+
+void pack_upload(Ctx& c, int64_t n, const uint32_t* tag, const uint32_t* type, const double* pos,
+                 const double* vel, const double* omega, const double* radius,
+                 const double* cutr, const double* mass) {
+    if (n <= 0 || n >= mdd::kMaxParticles)
+        throw ApiError(MDD_ERR_INVALID, "particle count out of range");
+    if (!tag || !pos || !vel || !cutr || !mass)
+        throw ApiError(MDD_ERR_INVALID, "tag, pos, vel, cutoff_radius and mass are required");
+    const bool dem = c.model == MDD_MODEL_DEM;
+    if (dem && !radius) throw ApiError(MDD_ERR_INVALID, "DEM particles need a radius");
+    int64_t cap = std::max<int64_t>(n, c.cap);
+    c.cap = cap;
+    c.n = n;
+    c.npad = (cap + 63) / 64 * 64;
+    const int64_t np = c.npad;
+    c.pos.resize(np);
+    c.vel.resize(np);
+    c.tag.resize(np);
+    c.type.resize(np);
+    c.image.resize(np);
+    c.mass.resize(np);
+    c.cutr.resize(np);
+    c.build_pos.resize(np);
+    c.pos2.resize(np);
+    c.vel2.resize(np);
+    c.tag2.resize(np);
+    c.type2.resize(np);
+    c.image2.resize(np);
+    c.mass2.resize(np);
+    c.cutr2.resize(np);
+    const bool mixed = c.precision == MDD_MIXED;
+    if (mixed) {
+        c.f32.resize(3 * np);
+        c.f32b.resize(3 * np);
+    } else {
+        c.f64.resize(3 * np);
+        c.f64b.resize(3 * np);
+    }
+    if (dem) {
+        c.omega.resize(np);
+        c.quat.resize(np);
+        c.omega2.resize(np);
+        c.quat2.resize(np);
+        if (mixed) {
+            c.t32.resize(3 * np);
+            c.t32b.resize(3 * np);
+        } else {
+            c.t64.resize(3 * np);
+            c.t64b.resize(3 * np);
+        }
+    }
+    std::vector<double4> hp(n), hv(n), hw, hq;
+    std::vector<uint32_t> ht(tag, tag + n), hty(n, 0u);
+    if (type) std::copy(type, type + n, hty.begin());
+    if (dem) {
+        hw.resize(n);
+        hq.resize(n);
+    }
+    for (int64_t i = 0; i < n; i++) {
+        if (!(mass[i] > 0.0)) throw ApiError(MDD_ERR_INVALID, "masses must be positive");
+        if (dem && hty[i] >= uint32_t(c.mat.n > 0 ? c.mat.n : mdd::kMaxMat))
+            throw ApiError(MDD_ERR_INVALID, "particle material out of range");
+        hp[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], dem ? radius[i] : 0.0);
+        hv[i] = make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 1.0 / mass[i]);
+        if (dem) {
+            const double r = radius[i];
+            // insert_single_particle (dem_atom/src/insert.rs:340-350)
+            const double inv_i = 1.0 / (0.4 * mass[i] * r * r);
+            hw[i] = make_double4(omega ? omega[3 * i] : 0.0, omega ? omega[3 * i + 1] : 0.0,
+                                 omega ? omega[3 * i + 2] : 0.0, inv_i);
+            hq[i] = make_double4(1.0, 0.0, 0.0, 0.0);
+        }
+    }
+    MDD_HIP(hipMemcpy(c.pos.p, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    MDD_HIP(hipMemcpy(c.vel.p, hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    MDD_HIP(hipMemcpy(c.tag.p, ht.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    MDD_HIP(hipMemcpy(c.type.p, hty.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    MDD_HIP(hipMemcpy(c.mass.p, mass, sizeof(double) * n, hipMemcpyHostToDevice));
+    MDD_HIP(hipMemcpy(c.cutr.p, cutr, sizeof(double) * n, hipMemcpyHostToDevice));
+    MDD_HIP(hipMemset(c.image.p, 0, sizeof(int4) * np));
+    MDD_HIP(hipMemcpy(c.build_pos.p, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    if (mixed)
+        MDD_HIP(hipMemset(c.f32.p, 0, sizeof(float) * 3 * np));
+    else
+        MDD_HIP(hipMemset(c.f64.p, 0, sizeof(double) * 3 * np));
+    if (dem) {
+        MDD_HIP(hipMemcpy(c.omega.p, hw.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        MDD_HIP(hipMemcpy(c.quat.p, hq.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        if (mixed)
+            MDD_HIP(hipMemset(c.t32.p, 0, sizeof(float) * 3 * np));
+        else
+            MDD_HIP(hipMemset(c.t64.p, 0, sizeof(double) * 3 * np));
+    }
+    c.list_valid = false;
+    c.list_injected = false;
+    c.setup_done = false;
+    c.nh_ready = false;
+    c.step = 0;
+    c.rebuilds = 0;
+    c.nghost = 0;
+    c.chunk = 8;
+    k_status_init<<<1, 1, 0, c.stream>>>(c.status.p);
+    MDD_HIP(hipStreamSynchronize(c.stream));
+}
+
+template <class T>
+std::vector<T> dl(const mdd::DBuf<T>& b, size_t n) {
+    std::vector<T> h(n);
+    if (n) MDD_HIP(hipMemcpy(h.data(), b.p, sizeof(T) * n, hipMemcpyDeviceToHost));
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdd_version(void) { return 1; }
+
+int mdd_create(mdd_ctx** out, const mdd_params* p) {
+    if (!out || !p) return MDD_ERR_INVALID;
+    *out = nullptr;
+    if (p->model != MDD_MODEL_LJ && p->model != MDD_MODEL_DEM) return MDD_ERR_INVALID;
+    if (p->precision != MDD_FP64 && p->precision != MDD_MIXED) return MDD_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MDD_ERR_NO_DEVICE;
+    if (p->device < 0 || p->device >= ndev) return MDD_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) return MDD_ERR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return MDD_ERR_NO_DEVICE;
+    mdd_ctx* ctx = nullptr;
+    try {
+        ctx = new mdd_ctx();
+    } catch (...) {
+        return MDD_ERR_CAPACITY;
+    }
+    Ctx& c = ctx->c;
+    c.device = p->device;
+    c.model = p->model;
+    c.precision = p->precision;
+    c.cap = p->capacity > 0 ? p->capacity : 0;
+    c.vv_on = p->model == MDD_MODEL_DEM;
+    c.rot_on = p->model == MDD_MODEL_DEM;
+    for (int d = 0; d < 3; d++) {
+        c.box.lo[d] = 0.0;
+        c.box.hi[d] = 1.0;
+        c.box.L[d] = 1.0;
+        c.box.periodic[d] = 1;
+    }
+    // LJForcePlugin default table (md_lj/src/lib.rs:313-315)
+    c.lj1 = 48.0;
+    c.lj2 = 24.0;
+    c.lj_cut2 = 6.25;
+    c.lj_e12 = 4.0;
+    c.lj_e6 = 4.0;
+    int rc = guarded(ctx, [&](Ctx& cc) {
+        MDD_HIP(hipStreamCreate(&cc.stream));
+        cc.status.resize(1);
+        MDD_HIP(hipHostMalloc(reinterpret_cast<void**>(&cc.status_host), sizeof(mdd::Status),
+                              hipHostMallocDefault));
+        std::memset(cc.status_host, 0, sizeof(mdd::Status));
+        k_status_init<<<1, 1, 0, cc.stream>>>(cc.status.p);
+        MDD_HIP(hipStreamSynchronize(cc.stream));
+    });
+    if (rc != MDD_OK) {
+        mdd_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return MDD_OK;
+}
+
+int mdd_destroy(mdd_ctx* ctx) {
+    if (!ctx) return MDD_OK;
+    Ctx& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (hipEvent_t e : c.ev_pool) (void)hipEventDestroy(e);
+    if (c.status_host) (void)hipHostFree(c.status_host);
+    c.status_host = nullptr;
+    hipStream_t s = c.stream;
+    delete ctx;  // device buffers free themselves
+    if (s) (void)hipStreamDestroy(s);
+    return MDD_OK;
+}
+
+const char* mdd_last_error(const mdd_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int mdd_set_box(mdd_ctx* ctx, const double lo[3], const double hi[3], const int periodic[3]) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (!lo || !hi || !periodic) throw ApiError(MDD_ERR_INVALID, "null box argument");
+        for (int d = 0; d < 3; d++) {
+            if (!(hi[d] > lo[d])) throw ApiError(MDD_ERR_INVALID, "box hi must exceed lo");
+            c.box.lo[d] = lo[d];
+            c.box.hi[d] = hi[d];
+            c.box.L[d] = hi[d] - lo[d];
+            c.box.periodic[d] = periodic[d] ? 1 : 0;
+        }
+        c.setup_done = false;
+        c.list_valid = false;
+    });
+}
+
+int mdd_set_neighbor(mdd_ctx* ctx, double skin_fraction, double bin_size, int every, int check,
+                     int sort_every, int newton) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (!(skin_fraction >= 1.0)) throw ApiError(MDD_ERR_INVALID, "skin_fraction must be >= 1");
+        if (every != 0)
+            throw ApiError(MDD_ERR_UNSUPPORTED, "only displacement rebuilds (every = 0)");
+        c.skin_fraction = skin_fraction;
+        c.bin_min_size = bin_size;
+        c.every = every;
+        c.check = check;
+        c.sort_every = sort_every < 0 ? 0 : sort_every;
+        c.newton = newton;
+        c.setup_done = false;
+        c.list_valid = false;
+    });
+}
+
+int mdd_set_rebuild_on_box_exit(mdd_ctx* ctx, int enabled) {
+    return guarded(ctx, [&](Ctx& c) { c.box_exit_rebuild = enabled ? 1 : 0; });
+}
+
+int mdd_set_dt(mdd_ctx* ctx, double dt) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (!(dt > 0.0)) throw ApiError(MDD_ERR_INVALID, "dt must be positive");
+        c.dt = dt;
+        if (c.nh_on) c.nh_ready = false;
+    });
+}
+
+int mdd_set_materials(mdd_ctx* ctx, int n, const double* E, const double* nu, const double* e,
+                      const double* mu) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (n <= 0 || n > mdd::kMaxMat || !E || !nu || !e || !mu)
+            throw ApiError(MDD_ERR_INVALID, "material table out of range");
+        // MaterialTable::build_pair_tables (dem_atom/src/lib.rs:389-515)
+        const double PI = 3.141592653589793;
+        c.mat.n = n;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                const double e_ij = std::sqrt(e[i] * e[j]);
+                const double log_e = std::log(e_ij);
+                c.mat.beta[i * n + j] = -log_e / std::sqrt(PI * PI + log_e * log_e);
+                c.mat.mu[i * n + j] = std::sqrt(mu[i] * mu[j]);
+                c.mat.eeff[i * n + j] =
+                    1.0 / ((1.0 - nu[i] * nu[i]) / E[i] + (1.0 - nu[j] * nu[j]) / E[j]);
+                c.mat.geff[i * n + j] = 1.0 / (2.0 * (2.0 - nu[i]) * (1.0 + nu[i]) / E[i] +
+                                               2.0 * (2.0 - nu[j]) * (1.0 + nu[j]) / E[j]);
+            }
+        mdd::upload_materials(c);
+    });
+}
+
+int mdd_set_lj(mdd_ctx* ctx, double epsilon, double sigma, double cutoff) {
+    return guarded(ctx, [&](Ctx& c) {
+        // LJPairCoeffs::from_params (md_lj/src/lib.rs:230-239)
+        const double s2 = sigma * sigma;
+        const double sigma6 = (s2 * s2) * s2;
+        c.lj1 = 48.0 * epsilon * sigma6 * sigma6;
+        c.lj2 = 24.0 * epsilon * sigma6;
+        const double cs = cutoff * sigma;
+        c.lj_cut2 = cs * cs;
+        c.lj_e12 = 4.0 * epsilon * sigma6 * sigma6;
+        c.lj_e6 = 4.0 * epsilon * sigma6;
+    });
+}
+
+int mdd_set_nose_hoover(mdd_ctx* ctx, int enabled, double temperature, double coupling) {
+    return guarded(ctx, [&](Ctx& c) {
+        c.nh_on = enabled ? 1 : 0;
+        c.nh_T = temperature;
+        c.nh_tau = coupling;
+        c.nh_ready = false;
+        if (c.nh_on) c.vv_on = 0;  // NH fuses the Verlet half steps
+    });
+}
+
+int mdd_set_integrators(mdd_ctx* ctx, int velocity_verlet, int rotation) {
+    return guarded(ctx, [&](Ctx& c) {
+        c.vv_on = velocity_verlet ? 1 : 0;
+        c.rot_on = rotation ? 1 : 0;
+    });
+}
+
+int mdd_set_gravity(mdd_ctx* ctx, int enabled, const double g[3]) {
+    return guarded(ctx, [&](Ctx& c) {
+        c.gravity_on = enabled ? 1 : 0;
+        if (g)
+            for (int d = 0; d < 3; d++) c.g[d] = g[d];
+    });
+}
+
+int mdd_add_wall(mdd_ctx* ctx, const double point[3], const double normal[3], int material,
+                 const double blo[3], const double bhi[3]) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (c.walls.count >= mdd::kMaxWalls) throw ApiError(MDD_ERR_INVALID, "too many walls");
+        if (!point || !normal) throw ApiError(MDD_ERR_INVALID, "null wall argument");
+        if (material < 0 || material >= mdd::kMaxMat)
+            throw ApiError(MDD_ERR_INVALID, "wall material out of range");
+        mdd::Wall& w = c.walls.w[c.walls.count++];
+        for (int d = 0; d < 3; d++) {
+            w.p[d] = point[d];
+            w.n[d] = normal[d];
+            w.blo[d] = blo ? blo[d] : -1.0e300;
+            w.bhi[d] = bhi ? bhi[d] : 1.0e300;
+        }
+        w.mat = material;
+    });
+}
+
+int mdd_clear_walls(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) { c.walls.count = 0; });
+}
+
+int mdd_upload_state(mdd_ctx* ctx, int64_t n, const uint32_t* tag, const uint32_t* type,
+                     const double* pos, const double* vel, const double* omega,
+                     const double* radius, const double* cutoff_radius, const double* mass) {
+    return guarded(ctx, [&](Ctx& c) {
+        pack_upload(c, n, tag, type, pos, vel, omega, radius, cutoff_radius, mass);
+    });
+}
+
+int64_t mdd_nlocal(const mdd_ctx* ctx) { return ctx ? ctx->c.n : -1; }
+
+int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, double* omega,
+                       double* quat, int32_t* image) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        const int64_t n = c.n;
+        if (tag) {
+            auto t = dl(c.tag, n);
+            std::copy(t.begin(), t.end(), tag);
+        }
+        if (pos) {
+            auto h = dl(c.pos, n);
+            for (int64_t i = 0; i < n; i++) {
+                pos[3 * i] = h[i].x;
+                pos[3 * i + 1] = h[i].y;
+                pos[3 * i + 2] = h[i].z;
+            }
+        }
+        if (vel) {
+            auto h = dl(c.vel, n);
+            for (int64_t i = 0; i < n; i++) {
+                vel[3 * i] = h[i].x;
+                vel[3 * i + 1] = h[i].y;
+                vel[3 * i + 2] = h[i].z;
+            }
+        }
+        if (omega) {
+            if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "no angular state");
+            auto h = dl(c.omega, n);
+            for (int64_t i = 0; i < n; i++) {
+                omega[3 * i] = h[i].x;
+                omega[3 * i + 1] = h[i].y;
+                omega[3 * i + 2] = h[i].z;
+            }
+        }
+        if (quat) {
+            if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "no orientation state");
+            auto h = dl(c.quat, n);
+            for (int64_t i = 0; i < n; i++) {
+                quat[4 * i] = h[i].x;
+                quat[4 * i + 1] = h[i].y;
+                quat[4 * i + 2] = h[i].z;
+                quat[4 * i + 3] = h[i].w;
+            }
+        }
+        if (image) {
+            auto h = dl(c.image, n);
+            for (int64_t i = 0; i < n; i++) {
+                image[3 * i] = h[i].x;
+                image[3 * i + 1] = h[i].y;
+                image[3 * i + 2] = h[i].z;
+            }
+        }
+    });
+}
+
+int mdd_download_forces(mdd_ctx* ctx, double* force, double* torque) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        const int64_t n = c.n, np = c.npad;
+        const bool mixed = c.precision == MDD_MIXED;
+        auto take = [&](double* out, bool tq) {
+            if (!out) return;
+            if (tq && c.model != MDD_MODEL_DEM)
+                throw ApiError(MDD_ERR_INVALID, "no torque state");
+            if (mixed) {
+                auto h = dl(tq ? c.t32 : c.f32, size_t(3 * np));
+                for (int64_t i = 0; i < n; i++)
+                    for (int d = 0; d < 3; d++) out[3 * i + d] = double(h[d * np + i]);
+            } else {
+                auto h = dl(tq ? c.t64 : c.f64, size_t(3 * np));
+                for (int64_t i = 0; i < n; i++)
+                    for (int d = 0; d < 3; d++) out[3 * i + d] = h[d * np + i];
+            }
+        };
+        take(force, false);
+        take(torque, true);
+    });
+}
+
+int mdd_upload_forces(mdd_ctx* ctx, const double* force, const double* torque) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        const int64_t n = c.n, np = c.npad;
+        const bool mixed = c.precision == MDD_MIXED;
+        auto put = [&](const double* in, bool tq) {
+            if (!in) return;
+            if (tq && c.model != MDD_MODEL_DEM)
+                throw ApiError(MDD_ERR_INVALID, "no torque state");
+            if (mixed) {
+                std::vector<float> h(size_t(3 * np), 0.f);
+                for (int64_t i = 0; i < n; i++)
+                    for (int d = 0; d < 3; d++) h[d * np + i] = float(in[3 * i + d]);
+                MDD_HIP(hipMemcpy((tq ? c.t32 : c.f32).p, h.data(), sizeof(float) * h.size(),
+                                  hipMemcpyHostToDevice));
+            } else {
+                std::vector<double> h(size_t(3 * np), 0.0);
+                for (int64_t i = 0; i < n; i++)
+                    for (int d = 0; d < 3; d++) h[d * np + i] = in[3 * i + d];
+                MDD_HIP(hipMemcpy((tq ? c.t64 : c.f64).p, h.data(), sizeof(double) * h.size(),
+                                  hipMemcpyHostToDevice));
+            }
+        };
+        put(force, false);
+        put(torque, true);
+    });
+}
+
+int mdd_neighbor_setup(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        mdd::neighbor_setup(c);
+    });
+}
+
+int mdd_bin_neighbor_list(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        mdd::rebuild(c);
+    });
+}
+
+int mdd_set_half_list(mdd_ctx* ctx, int64_t nlocal, const uint32_t* offsets,
+                      const uint32_t* indices) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        if (nlocal != c.n || !offsets) throw ApiError(MDD_ERR_INVALID, "row count differs");
+        std::vector<std::vector<uint32_t>> own(c.n), rev(c.n);
+        for (int64_t i = 0; i < c.n; i++)
+            for (uint32_t k = offsets[i]; k < offsets[i + 1]; k++) {
+                const uint32_t j = indices[k];
+                if (int64_t(j) >= c.n || int64_t(j) == i)
+                    throw ApiError(MDD_ERR_INVALID, "half list index out of range");
+                own[i].push_back(mdd::enc_entry(j, mdd::kCodeNone, false));
+                rev[j].push_back(mdd::enc_entry(uint32_t(i), mdd::kCodeNone, true));
+            }
+        int mx = 1;
+        for (int64_t i = 0; i < c.n; i++) mx = std::max<int>(mx, int(own[i].size() + rev[i].size()));
+        if (c.model == MDD_MODEL_DEM && mx > 64)
+            throw ApiError(MDD_ERR_UNSUPPORTED, "DEM rows above 64 entries");
+        c.maxnb = mx;
+        const int64_t np = c.npad;
+        std::vector<uint32_t> hn(size_t(mx) * np, 0u), cnt(np, 0u), half(np, 0u);
+        for (int64_t i = 0; i < c.n; i++) {
+            int k = 0;
+            for (uint32_t e : own[i]) hn[size_t(k++) * np + i] = e;
+            for (uint32_t e : rev[i]) hn[size_t(k++) * np + i] = e;
+            cnt[i] = uint32_t(k);
+            half[i] = uint32_t(own[i].size());
+        }
+        c.nbr.resize(hn.size());
+        c.nnb.resize(np);
+        c.nhalf.resize(np);
+        MDD_HIP(hipMemcpy(c.nbr.p, hn.data(), sizeof(uint32_t) * hn.size(), hipMemcpyHostToDevice));
+        MDD_HIP(hipMemcpy(c.nnb.p, cnt.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice));
+        MDD_HIP(hipMemcpy(c.nhalf.p, half.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice));
+        if (c.model == MDD_MODEL_DEM) {
+            c.hist.resize(size_t(3) * mx * np);
+            c.cmask.resize(np);
+            MDD_HIP(hipMemset(c.cmask.p, 0, sizeof(uint64_t) * np));
+        }
+        c.list_valid = true;
+        c.list_injected = true;
+        c.npairs = offsets[c.n];
+    });
+}
+
+int64_t mdd_npairs(mdd_ctx* ctx) {
+    if (!ctx || !ctx->c.list_valid) return 0;
+    int64_t total = 0;
+    int rc = guarded(ctx, [&](Ctx& c) {
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        auto h = dl(c.nhalf, size_t(c.n));
+        for (uint32_t v : h) total += v;
+    });
+    return rc == MDD_OK ? total : rc;
+}
+
+int mdd_download_pairs(mdd_ctx* ctx, int64_t cap, uint32_t* owner_tag, uint32_t* partner_tag,
+                       uint8_t* code, int64_t* n_out) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_list(c);
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        const int64_t n = c.n, np = c.npad;
+        auto half = dl(c.nhalf, size_t(n));
+        auto nb = dl(c.nbr, size_t(c.maxnb) * np);
+        auto tg = dl(c.tag, size_t(n));
+        int64_t k = 0;
+        for (int64_t i = 0; i < n; i++)
+            for (uint32_t m = 0; m < half[i]; m++, k++) {
+                if (k >= cap) continue;
+                const uint32_t e = nb[size_t(m) * np + i];
+                if (owner_tag) owner_tag[k] = tg[i];
+                if (partner_tag) partner_tag[k] = tg[mdd::entry_idx(e)];
+                if (code) code[k] = uint8_t(mdd::entry_code(e));
+            }
+        if (n_out) *n_out = k;
+    });
+}
+
+int mdd_lj_force(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_list(c);
+        if (c.model != MDD_MODEL_LJ) throw ApiError(MDD_ERR_INVALID, "not an LJ context");
+        mdd::launch_pair_forces(c, -1, false);
+    });
+}
+
+int mdd_hertz_mindlin_contact_force(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_list(c);
+        if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
+        if (c.mat.n == 0) throw ApiError(MDD_ERR_INVALID, "materials not set");
+        mdd::launch_pair_forces(c, -1, false);
+        sync_status(c);
+        check_device_error(c);
+    });
+}
+
+int mdd_wall_contact_force(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
+        mdd::launch_walls_gravity(c, true, false);
+    });
+}
+
+int mdd_apply_gravity(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        mdd::launch_walls_gravity(c, false, true);
+    });
+}
+
+int mdd_initial_integration(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        mdd::launch_initial_integration(c, -1, false);
+    });
+}
+
+int mdd_final_integration(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        mdd::launch_final_integration(c);
+    });
+}
+
+int mdd_initial_rotation(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
+        mdd::launch_initial_rotation(c);
+    });
+}
+
+int mdd_final_rotation(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
+        mdd::launch_final_rotation(c);
+    });
+}
+
+int mdd_setup_nose_hoover(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        setup_nose_hoover(c);
+    });
+}
+
+int mdd_nh_pre_initial(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        if (!c.nh_ready) setup_nose_hoover(c);
+        else mdd::launch_ke(c);  // KE of the current velocities -> p_xi half update
+        k_pxi_from_pre<<<1, 1, 0, c.stream>>>(c.status.p);
+        mdd::launch_nh_pre(c, -1, false);
+    });
+}
+
+int mdd_nh_post_final(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        if (!c.nh_ready) throw ApiError(MDD_ERR_INVALID, "thermostat not set up");
+        mdd::launch_nh_post(c);
+    });
+}
+
+int mdd_run(mdd_ctx* ctx, int64_t nsteps) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (nsteps < 0) throw ApiError(MDD_ERR_INVALID, "negative step count");
+        run_steps(c, nsteps);
+    });
+}
+
+int mdd_download_history(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* tag_hi,
+                         double* spring, int64_t* n_out) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_list(c);
+        if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        const int64_t n = c.n, np = c.npad;
+        const int64_t cs = int64_t(c.maxnb) * np;
+        auto half = dl(c.nhalf, size_t(n));
+        auto nb = dl(c.nbr, size_t(cs));
+        auto tg = dl(c.tag, size_t(n));
+        auto mk = dl(c.cmask, size_t(n));
+        auto hs = dl(c.hist, size_t(3 * cs));
+        int64_t k = 0;
+        for (int64_t i = 0; i < n; i++)
+            for (uint32_t m = 0; m < half[i]; m++) {
+                if (!((mk[i] >> m) & 1ull)) continue;
+                if (k < cap) {
+                    const uint32_t e = nb[size_t(m) * np + i];
+                    const uint32_t a = tg[i], b = tg[mdd::entry_idx(e)];
+                    if (tag_lo) tag_lo[k] = std::min(a, b);
+                    if (tag_hi) tag_hi[k] = std::max(a, b);
+                    if (spring)
+                        for (int d = 0; d < 3; d++) spring[3 * k + d] = hs[d * cs + size_t(m) * np + i];
+                }
+                k++;
+            }
+        if (n_out) *n_out = k;
+    });
+}
+
+int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        if (!out) throw ApiError(MDD_ERR_INVALID, "null output");
+        std::memset(out, 0, sizeof(*out));
+        double s[8];
+        mdd::thermo_reduce(c, s);
+        sync_status(c);
+        out->step = c.step;
+        out->rebuilds = c.rebuilds;
+        out->nghost = c.nghost;
+        out->ke = 0.5 * s[0];
+        const double ndof = 3.0 * double(c.n) - 3.0;
+        out->temperature = ndof > 0 ? 2.0 * out->ke / ndof : 0.0;
+        out->momentum[0] = s[1];
+        out->momentum[1] = s[2];
+        out->momentum[2] = s[3];
+        out->granular_temperature = s[4] > 0 ? s[5] / (3.0 * s[4]) : 0.0;
+        out->p_xi = c.status_host->p_xi;
+        if (c.list_valid) {
+            auto h = dl(c.nhalf, size_t(c.n));
+            int64_t t = 0;
+            for (uint32_t v : h) t += v;
+            out->npairs = t;
+            if (c.model == MDD_MODEL_DEM) {
+                auto mk = dl(c.cmask, size_t(c.n));
+                int64_t nc = 0;
+                for (int64_t i = 0; i < c.n; i++) nc += __builtin_popcountll(mk[i]);
+                out->ncontacts = nc / 2;
+            }
+        }
+        if (with_energy && c.model == MDD_MODEL_LJ && c.list_valid) {
+            mdd::launch_pair_forces(c, -1, true);
+            MDD_HIP(hipStreamSynchronize(c.stream));
+            auto h = dl(c.pe, size_t(c.n));
+            double pe = 0.0;
+            for (double v : h) pe += v;
+            out->pe = pe;
+        }
+    });
+}
+
+int mdd_synchronize(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) { MDD_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+int mdd_set_profiling(mdd_ctx* ctx, int every) {
+    return guarded(ctx, [&](Ctx& c) {
+        c.prof_every = every > 0 ? every : 0;
+        if (c.prof_every && c.ev_pool.empty()) {
+            c.ev_pool.resize(2 * kEventPool);
+            for (auto& e : c.ev_pool) MDD_HIP(hipEventCreate(&e));
+        }
+        c.prof_ms_sum = 0;
+        c.prof_count = 0;
+        c.prof_pending = 0;
+        c.force_launches = 0;
+        c.prof_ctr = 0;
+    });
+}
+
+int mdd_force_kernel_stats(mdd_ctx* ctx, double* avg_ms, int64_t* launches) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (avg_ms) *avg_ms = c.prof_count ? c.prof_ms_sum / double(c.prof_count) : 0.0;
+        if (launches) *launches = c.force_launches;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,273 @@
+// Internal types of the gfx950 particle path: device buffers, the context
+// behind the C ABI, and the encoding of neighbor-list entries.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mdd {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const char* expr, const char* file, int line)
+        : std::runtime_error(std::string(hipGetErrorString(c)) + " in " + expr + " (" + file +
+                             ":" + std::to_string(line) + ")"),
+          code(c) {}
+};
+
+struct ApiError : std::runtime_error {
+    int code;
+    ApiError(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+#define MDD_HIP(expr)                                                   \
+    do {                                                                \
+        hipError_t e_ = (expr);                                         \
+        if (e_ != hipSuccess) throw ::mdd::HipError(e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+// ---- neighbor entry encoding -------------------------------------------------
+// One 32-bit word per (particle, partner) entry of the full list:
+//   bits  0..25  partner particle index (root particle for periodic images)
+//   bits 26..30  image code = (sx+1)*9 + (sy+1)*3 + (sz+1); 13 = no shift.
+//                The shift applies to the non-owner of the pair.
+//   bit  31      set when this particle is NOT the owner of the pair, i.e.
+//                the pair sits in the partner's half-list row.
+constexpr uint32_t kIdxBits = 26;
+constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
+constexpr uint32_t kRevBit = 1u << 31;
+constexpr int kCodeNone = 13;
+constexpr int64_t kMaxParticles = int64_t(1) << kIdxBits;
+
+__host__ __device__ inline uint32_t enc_entry(uint32_t idx, int code, bool rev) {
+    return idx | (uint32_t(code) << kIdxBits) | (rev ? kRevBit : 0u);
+}
+__host__ __device__ inline uint32_t entry_idx(uint32_t e) { return e & kIdxMask; }
+__host__ __device__ inline int entry_code(uint32_t e) { return int((e >> kIdxBits) & 31u); }
+__host__ __device__ inline bool entry_rev(uint32_t e) { return (e & kRevBit) != 0u; }
+__host__ __device__ inline int code_of(int sx, int sy, int sz) {
+    return (sx + 1) * 9 + (sy + 1) * 3 + (sz + 1);
+}
+// periodic displacement of one axis: exactly +L, -L or 0.0 (comm.rs:280 packs
+// pos + periodic_offset * size with periodic_offset = +-1.0)
+__host__ __device__ inline double image_shift(int s, double L) {
+    return s == 0 ? 0.0 : (s > 0 ? L : -L);
+}
+__host__ __device__ inline int code_sx(int code) { return code / 9 - 1; }
+__host__ __device__ inline int code_sy(int code) { return (code / 3) % 3 - 1; }
+__host__ __device__ inline int code_sz(int code) { return code % 3 - 1; }
+
+// Periodic image displacement per image code: exactly +L, -L or 0.0 per axis,
+// so x + shift reproduces the reference ghost coordinate fl(x +- L).
+struct Shift27 {
+    double v[27][3];
+};
+
+struct BoxP {
+    double lo[3], hi[3], L[3];
+    int periodic[3];
+};
+
+struct GridP {
+    double origin[3], inv_bs[3];
+    int n[3];
+    int nfwd;
+    int64_t ncells;
+};
+
+struct Wall {
+    double p[3], n[3], blo[3], bhi[3];
+    int mat;
+};
+constexpr int kMaxWalls = 8;
+constexpr int kMaxMat = 16;
+
+struct WallsP {
+    int count;
+    Wall w[kMaxWalls];
+};
+
+// Material pair tables (beta, friction, E*, G*) in row-major nmat x nmat.
+struct MatP {
+    int n;
+    double beta[kMaxMat * kMaxMat];
+    double mu[kMaxMat * kMaxMat];
+    double eeff[kMaxMat * kMaxMat];
+    double geff[kMaxMat * kMaxMat];
+};
+
+// Device-resident run status, read by the host at chunk boundaries.
+struct Status {
+    int abort_step;          // first step whose drift needs a rebuild; INT_MAX none
+    int error;               // 0 or an MDD_ERR_* code raised on the device
+    unsigned overflow;       // bit 0: image capacity, bit 1: neighbor row capacity
+    int max_row;             // longest full-list row at the last build
+    int max_half;            // longest half-list row at the last build
+    unsigned ghost_total;    // periodic images at the last build
+    int overlap_step;        // step of overlap_count
+    int overlap_count;       // large-overlap pairs in that step
+    unsigned ticket;         // last-block reduction counter
+    unsigned ncontacts;      // DEM pairs in contact (owner side)
+    int pad[2];
+    double ke;               // kinetic energy after the last post kernel
+    double p_xi;             // Nose-Hoover momentum after the last post kernel
+    double nh_scale;         // velocity scale for the next pre kernel
+    double p_xi_pre;         // p_xi after the next pre half-update
+};
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void resize(size_t count) {
+        if (count <= n && p) return;
+        release();
+        size_t c = count ? count : 1;
+        MDD_HIP(hipMalloc(&p, c * sizeof(T)));
+        n = c;
+    }
+    void swap(DBuf& o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
+};
+
+struct Ctx {
+    int device = 0, model = 0, precision = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // ---- configuration
+    BoxP box{};
+    double dt = 1.0;
+    double skin_fraction = 1.0, bin_min_size = 1.0;
+    int every = 0, check = 1, sort_every = 1000, newton = 1;
+    int box_exit_rebuild = 1;
+    MatP mat{};
+    double lj1 = 0, lj2 = 0, lj_cut2 = 0, lj_e12 = 0, lj_e6 = 0;
+    int nh_on = 0;
+    double nh_T = 0, nh_tau = 1, nh_ndof = 0, nh_Q = 1;
+    bool nh_ready = false;
+    int vv_on = 0, rot_on = 0;
+    int gravity_on = 0;
+    double g[3] = {0, 0, 0};
+    WallsP walls{};
+
+    // ---- neighbor grid (neighbor_setup)
+    bool setup_done = false;
+    double max_cutoff = 0, ghost_cutoff = 0, bin_size[3] = {1, 1, 1};
+    GridP grid{};
+    std::vector<int> fwd;
+    bool uniform = true;
+    double cutoff_sq = 0, sf2 = 1, min_skin = 0;
+    Shift27 shifts{};
+
+    // ---- sizes
+    int64_t n = 0, cap = 0, npad = 0;
+    int64_t ghost_cap = 0, nghost = 0;
+    int maxnb = 0;          // row capacity of the full list
+    int64_t npairs = 0;
+    bool list_valid = false;
+    bool list_injected = false;
+
+    // ---- particle arrays (current order)
+    DBuf<double4> pos, vel, omega, quat;  // pos.w radius, vel.w 1/m, omega.w 1/I
+    DBuf<uint32_t> tag, type;
+    DBuf<int4> image;
+    DBuf<double> mass, cutr;
+    DBuf<double> f64, t64;  // 3*npad SoA force / torque (FP64 mode)
+    DBuf<float> f32, t32;   // 3*npad SoA force / torque (mixed mode)
+    DBuf<double4> build_pos;
+    // alternates for the rebuild permutation
+    DBuf<double4> pos2, vel2, omega2, quat2;
+    DBuf<uint32_t> tag2, type2;
+    DBuf<int4> image2;
+    DBuf<double> mass2, cutr2;
+    DBuf<double> f64b, t64b;
+    DBuf<float> f32b, t32b;
+
+    // ---- rebuild scratch
+    DBuf<uint32_t> key, key2, order, slot_of, cnt, start, scan_tmp, gcnt, goff;
+    DBuf<uint32_t> ghost_root;
+    DBuf<uint8_t> ghost_code;
+    DBuf<double4> ghost_pos;
+    DBuf<uint32_t> all_key, bin_start, bin_order, bin_slot, bin_entry;
+    DBuf<double4> bin_pos;
+    DBuf<int> stencil;
+
+    // ---- full neighbor list (ELL, column-major: entry k of particle i at k*npad+i)
+    DBuf<uint32_t> nbr, nbr_old;
+    DBuf<uint32_t> nnb, nnb_old, nhalf;
+    int64_t npad_old = 0;
+    int maxnb_old = 0;
+    // DEM shear history per list slot, canonical sign (lower tag's frame)
+    DBuf<double> hist, hist_old;  // 3 * maxnb * npad, SoA by component
+    // bit k of cmask[i]: slot k of particle i was in contact at the last force pass
+    DBuf<uint64_t> cmask, cmask_old;
+    DBuf<double> mat_dev;         // beta | mu | E* | G*, each nmat x nmat
+    DBuf<double> pe;              // per-particle LJ energy (thermo only)
+
+    // ---- run state
+    DBuf<Status> status;
+    Status* status_host = nullptr;  // pinned mirror
+    DBuf<double> partial;           // per-block partial sums
+    int64_t step = 0, rebuilds = 0;
+    int64_t sort_counter = 0;
+    int chunk = 8;
+
+    // ---- profiling of the pair-force kernel
+    int prof_every = 0;
+    std::vector<hipEvent_t> ev_pool;
+    double prof_ms_sum = 0;
+    int64_t prof_count = 0;
+    int64_t force_launches = 0;
+    int64_t prof_ctr = 0;
+    int64_t prof_pending = 0;
+};
+
+// ---- launchers implemented in the .hip files
+// scan_sort.hip
+void exclusive_scan(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n);
+void counting_sort(Ctx& c, const uint32_t* key, int64_t n, int64_t ncells, uint32_t* cnt,
+                   uint32_t* start, uint32_t* order, uint32_t* slot_of);
+void fill_u32(Ctx& c, uint32_t* p, uint32_t v, int64_t n);
+void fill_f64(Ctx& c, double* p, double v, int64_t n);
+void fill_f32(Ctx& c, float* p, float v, int64_t n);
+// neighbor.hip
+void rebuild(Ctx& c);
+// forces.hip
+void launch_pair_forces(Ctx& c, int step, bool energy);
+void launch_walls_gravity(Ctx& c, bool walls, bool gravity);
+void upload_materials(Ctx& c);
+// integrate.hip
+void launch_pre(Ctx& c, int step, bool check_rebuild);
+void launch_post(Ctx& c, int step);
+void launch_initial_integration(Ctx& c, int step, bool check_rebuild);
+void launch_final_integration(Ctx& c);
+void launch_initial_rotation(Ctx& c);
+void launch_final_rotation(Ctx& c);
+void launch_nh_pre(Ctx& c, int step, bool check_rebuild);
+void launch_nh_post(Ctx& c);
+void launch_ke(Ctx& c);
+void thermo_reduce(Ctx& c, double out[8]);
+
+inline int blocks_for(int64_t n, int threads) {
+    int64_t b = (n + threads - 1) / threads;
+    return int(b > 0 ? b : 1);
+}
+
+}  // namespace mdd

@@ -1,0 +1,520 @@
+// Pair-force kernels of the gfx950 particle path: Lennard-Jones 12-6 and
+// fused Hertz-Mindlin contact with tangential shear history, plus plane walls
+// and gravity.
+//
+// One thread owns one particle and gathers over its full neighbor row, so
+// forces accumulate in registers and are written once: no atomics, no
+// reverse (ghost -> owner) pass, and bitwise run-to-run determinism. Every
+// pair is evaluated in its owner's frame (the frame the reference half list
+// evaluates it in), so the two particles of a pair see bitwise-equal,
+// opposite forces.
+//
+// Precision: R = double reproduces the reference arithmetic (operation order
+// and fused multiply-adds as written there). R = float ("mixed") keeps the
+// FP64 pair geometry (image shift, separation, cutoff test, overlap, normal,
+// relative velocity) and the FP64 shear spring, and does the force algebra
+// and per-particle accumulation in FP32.
+#include "ctx.hpp"
+
+namespace mdd {
+
+namespace {
+
+// dem_atom/src/lib.rs:63
+constexpr double kSqrt53 = 0.9128709291752768;
+// dem_granular/src/lib.rs:93-108
+constexpr double kTangEps = 1e-30;
+constexpr double kLargeOverlap = 0.90;
+constexpr int kMaxOverlapWarnings = 500;
+
+struct ForceArgs {
+    int64_t n, npad;
+    int maxnb;
+    const double4* pos;
+    const double4* vel;
+    const double4* omega;
+    const uint32_t* tag;
+    const uint32_t* type;
+    const double* mass;
+    const uint32_t* nbr;
+    const uint32_t* nnb;
+    double L[3];
+    double lj1, lj2, cut2, e12, e6;
+    const double* mat;
+    int nm;
+    double dt;
+    double* hist;
+    uint64_t* cmask;
+    int64_t hist_cs;
+    WallsP walls;
+    int use_walls, use_gravity, use_pairs;
+    double g[3];
+    double* f64;
+    float* f32;
+    double* t64;
+    float* t32;
+    double* pe;
+    Status* st;
+    int step;
+};
+
+__device__ inline bool gated(const ForceArgs& a) {
+    return a.step >= 0 && a.st->abort_step <= a.step;
+}
+
+template <typename R>
+__device__ inline R rcp(R x);
+template <>
+__device__ inline double rcp<double>(double x) {
+    return 1.0 / x;
+}
+template <>
+__device__ inline float rcp<float>(float x) {
+    return __builtin_amdgcn_rcpf(x);
+}
+
+template <typename R>
+__device__ inline R sqrt_r(R x);
+template <>
+__device__ inline double sqrt_r<double>(double x) {
+    return sqrt(x);
+}
+template <>
+__device__ inline float sqrt_r<float>(float x) {
+    return __builtin_sqrtf(x);
+}
+
+// lj_force (md_lj/src/lib.rs:488-633): for each neighbor in cutoff,
+// fpair = r^-2 r^-6 (lj1 r^-6 - lj2), owner gets fma(-fpair, d, f),
+// partner fma(fpair, d, f), d = partner image - owner.
+template <typename R, bool ENERGY>
+__global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= a.n) return;
+    if (gated(a)) return;
+    const double4 xi = a.pos[i];
+    int cnt = int(a.nnb[i]);
+    if (cnt > a.maxnb) cnt = a.maxnb;
+    R fx = R(0), fy = R(0), fz = R(0);
+    double pe = 0.0;
+    const R lj1 = R(a.lj1), lj2 = R(a.lj2);
+    const uint32_t* row = a.nbr + i;
+    for (int k = 0; k < cnt; k++) {
+        const uint32_t e = row[int64_t(k) * a.npad];
+        const uint32_t j = entry_idx(e);
+        const int code = entry_code(e);
+        const double4 xj = a.pos[j];
+        const double shx = image_shift(code_sx(code), a.L[0]);
+        const double shy = image_shift(code_sy(code), a.L[1]);
+        const double shz = image_shift(code_sz(code), a.L[2]);
+        double dx, dy, dz;
+        const bool rev = entry_rev(e);
+        if (rev) {
+            dx = (xi.x + shx) - xj.x;
+            dy = (xi.y + shy) - xj.y;
+            dz = (xi.z + shz) - xj.z;
+        } else {
+            dx = (xj.x + shx) - xi.x;
+            dy = (xj.y + shy) - xi.y;
+            dz = (xj.z + shz) - xi.z;
+        }
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        if (r2 >= a.cut2) continue;
+        const R r2inv = rcp<R>(R(r2));
+        const R r6inv = r2inv * r2inv * r2inv;
+        const R fpair = r2inv * r6inv * fma(lj1, r6inv, -lj2);
+        const R s = rev ? fpair : -fpair;
+        fx = fma(s, R(dx), fx);
+        fy = fma(s, R(dy), fy);
+        fz = fma(s, R(dz), fz);
+        if (ENERGY) {
+            const double r6 = double(r6inv);
+            pe += 0.5 * (r6 * (a.e12 * r6 - a.e6));
+        }
+    }
+    if constexpr (sizeof(R) == 8) {
+        a.f64[i] = fx;
+        a.f64[a.npad + i] = fy;
+        a.f64[2 * a.npad + i] = fz;
+    } else {
+        a.f32[i] = fx;
+        a.f32[a.npad + i] = fy;
+        a.f32[2 * a.npad + i] = fz;
+    }
+    if (ENERGY) a.pe[i] = pe;
+}
+
+template <typename R>
+__device__ inline void store_force(const ForceArgs& a, int64_t i, R fx, R fy, R fz, R tx, R ty,
+                                   R tz) {
+    if constexpr (sizeof(R) == 8) {
+        a.f64[i] = fx;
+        a.f64[a.npad + i] = fy;
+        a.f64[2 * a.npad + i] = fz;
+        a.t64[i] = tx;
+        a.t64[a.npad + i] = ty;
+        a.t64[2 * a.npad + i] = tz;
+    } else {
+        a.f32[i] = fx;
+        a.f32[a.npad + i] = fy;
+        a.f32[2 * a.npad + i] = fz;
+        a.t32[i] = tx;
+        a.t32[a.npad + i] = ty;
+        a.t32[2 * a.npad + i] = tz;
+    }
+}
+
+// plane walls (dem_wall/src/lib.rs:862-1002): normal Hertz + damping, no
+// wall friction, static wall, mu_tw = 0
+template <typename R>
+__device__ inline void wall_forces(const ForceArgs& a, const double4& p, const double4& v,
+                                   double mass, uint32_t ti, R& fx, R& fy, R& fz) {
+    const int nm = a.nm;
+    for (int w = 0; w < a.walls.count; w++) {
+        const Wall& W = a.walls.w[w];
+        if (!(p.x >= W.blo[0] && p.x <= W.bhi[0] && p.y >= W.blo[1] && p.y <= W.bhi[1] &&
+              p.z >= W.blo[2] && p.z <= W.bhi[2]))
+            continue;
+        const double dx = p.x - W.p[0], dy = p.y - W.p[1], dz = p.z - W.p[2];
+        const double distance = dx * W.n[0] + dy * W.n[1] + dz * W.n[2];
+        if (distance <= 0.0) continue;
+        const double radius = p.w;
+        double delta = radius - distance;
+        if (0.5 * radius < delta) delta = 0.5 * radius;
+        if (delta <= 0.0) continue;
+        const R e_eff = R(a.mat[2 * nm * nm + ti * nm + W.mat]);
+        const R beta = R(a.mat[ti * nm + W.mat]);
+        const R sdr = sqrt_r<R>(R(delta) * R(radius));
+        const R s_n = R(2.0) * e_eff * sdr;
+        const R k_n = R(4.0 / 3.0) * e_eff * sdr;
+        const R m_r = R(mass);
+        const double v_n = v.x * W.n[0] + v.y * W.n[1] + v.z * W.n[2];
+        const R f_diss = R(2.0) * beta * R(kSqrt53) * sqrt_r<R>(s_n * m_r) * R(v_n);
+        R f_net = k_n * R(delta) - f_diss;
+        if (!(f_net > R(0))) f_net = R(0);
+        fx += f_net * R(W.n[0]);
+        fy += f_net * R(W.n[1]);
+        fz += f_net * R(W.n[2]);
+    }
+}
+
+// hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615) with
+// surface energy, cohesion and rolling / twisting friction all zero
+template <typename R>
+__global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= a.n) return;
+    if (gated(a)) return;
+    const double4 pi = a.pos[i];
+    const double4 vi = a.vel[i];
+    R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
+    const uint32_t ti = a.type[i];
+    if (a.use_pairs) {
+        const double4 wi = a.omega[i];
+        const uint32_t tagi = a.tag[i];
+        int cnt = int(a.nnb[i]);
+        if (cnt > a.maxnb) cnt = a.maxnb;
+        const uint64_t mask_in = a.cmask[i];
+        uint64_t mask_out = 0ull;
+        const int nm = a.nm;
+        const double* beta_t = a.mat;
+        const double* mu_t = a.mat + nm * nm;
+        const double* eeff_t = a.mat + 2 * nm * nm;
+        const double* geff_t = a.mat + 3 * nm * nm;
+        const double dt = a.dt;
+        for (int k = 0; k < cnt; k++) {
+            const int64_t slot = int64_t(k) * a.npad + i;
+            const uint32_t e = a.nbr[slot];
+            const uint32_t j = entry_idx(e);
+            const int code = entry_code(e);
+            const bool rev = entry_rev(e);
+            const double4 pj = a.pos[j];
+            const double shx = image_shift(code_sx(code), a.L[0]);
+            const double shy = image_shift(code_sy(code), a.L[1]);
+            const double shz = image_shift(code_sz(code), a.L[2]);
+            // owner frame: d = partner image - owner; r1 owner radius
+            double dx, dy, dz, r1, r2;
+            if (rev) {
+                dx = (pi.x + shx) - pj.x;
+                dy = (pi.y + shy) - pj.y;
+                dz = (pi.z + shz) - pj.z;
+                r1 = pj.w;
+                r2 = pi.w;
+            } else {
+                dx = (pj.x + shx) - pi.x;
+                dy = (pj.y + shy) - pi.y;
+                dz = (pj.z + shz) - pi.z;
+                r1 = pi.w;
+                r2 = pj.w;
+            }
+            const double dist_sq = dx * dx + dy * dy + dz * dz;
+            const double sum_r = r1 + r2;
+            if (dist_sq >= sum_r * sum_r) continue;
+            const double distance = sqrt(dist_sq);
+            if (distance == 0.0) continue;
+            const double r_min = r1 < r2 ? r1 : r2;
+            double delta = sum_r - distance;
+            if (0.5 * r_min < delta) delta = 0.5 * r_min;
+            if (delta > 0.0 && !rev && distance / sum_r < kLargeOverlap) {
+                int c = atomicAdd(&a.st->overlap_count, 1) + 1;
+                if (c > kMaxOverlapWarnings) {
+                    atomicCAS(&a.st->error, 0, -3);
+                    if (a.step >= 0) atomicMin(&a.st->abort_step, a.step);
+                }
+            }
+            if (delta <= 0.0) continue;
+            const double4 vj = a.vel[j];
+            const double4 wj = a.omega[j];
+            const uint32_t tagj = a.tag[j];
+            const uint32_t tj = a.type[j];
+            // owner (O) / partner (P) quantities
+            const double4& vO = rev ? vj : vi;
+            const double4& vP = rev ? vi : vj;
+            const double4& wO = rev ? wj : wi;
+            const double4& wP = rev ? wi : wj;
+            const uint32_t tagO = rev ? tagj : tagi;
+            const uint32_t tagP = rev ? tagi : tagj;
+            const int mO = int(rev ? tj : ti), mP = int(rev ? ti : tj);
+            const int mij = mO * nm + mP;
+            const R r_eff = R(r1) * R(r2) / R(sum_r);
+            const R e_eff = R(eeff_t[mij]);
+            const R g_eff = R(geff_t[mij]);
+            const R beta = R(beta_t[mij]);
+            const R mu = R(mu_t[mij]);
+            const double inv_dist = 1.0 / distance;
+            const double nx = dx * inv_dist, ny = dy * inv_dist, nz = dz * inv_dist;
+            const R m_r = R(1.0) / (R(vO.w) + R(vP.w));
+            const R sdr = sqrt_r<R>(R(delta) * r_eff);
+            const R s_n = R(2.0) * e_eff * sdr;
+            const R k_n = R(4.0 / 3.0) * e_eff * sdr;
+            const R k_t = R(8.0) * g_eff * sdr;
+            // contact-point velocities v + w x (+-r n), FP64
+            const double r1nx = r1 * nx, r1ny = r1 * ny, r1nz = r1 * nz;
+            const double vcix = vO.x + (wO.y * r1nz - wO.z * r1ny);
+            const double vciy = vO.y + (wO.z * r1nx - wO.x * r1nz);
+            const double vciz = vO.z + (wO.x * r1ny - wO.y * r1nx);
+            const double r2nx = r2 * nx, r2ny = r2 * ny, r2nz = r2 * nz;
+            const double vcjx = vP.x + (-wP.y * r2nz + wP.z * r2ny);
+            const double vcjy = vP.y + (-wP.z * r2nx + wP.x * r2nz);
+            const double vcjz = vP.z + (-wP.x * r2ny + wP.y * r2nx);
+            const double vrx = vcjx - vcix, vry = vcjy - vciy, vrz = vcjz - vciz;
+            const double v_n = vrx * nx + vry * ny + vrz * nz;
+            const R f_diss_n = R(2.0) * beta * R(kSqrt53) * sqrt_r<R>(s_n * m_r) * R(v_n);
+            R f_n_mag = k_n * R(delta) - f_diss_n;
+            if (!(f_n_mag > R(0))) f_n_mag = R(0);
+            const R fnx = f_n_mag * R(nx), fny = f_n_mag * R(ny), fnz = f_n_mag * R(nz);
+            const double vtx = vrx - v_n * nx, vty = vry - v_n * ny, vtz = vrz - v_n * nz;
+            // shear spring in the lower tag's frame (tangential.rs:23-26)
+            const double sign = tagO < tagP ? 1.0 : -1.0;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            if ((mask_in >> k) & 1ull) {
+                sx = sign * a.hist[slot];
+                sy = sign * a.hist[a.hist_cs + slot];
+                sz = sign * a.hist[2 * a.hist_cs + slot];
+            }
+            const double s_dot_n = sx * nx + sy * ny + sz * nz;
+            sx -= s_dot_n * nx;
+            sy -= s_dot_n * ny;
+            sz -= s_dot_n * nz;
+            sx += vtx * dt;
+            sy += vty * dt;
+            sz += vtz * dt;
+            const R s_mag = R(sqrt(sx * sx + sy * sy + sz * sz));
+            const R f_t_spring_mag = k_t * s_mag;
+            const R f_t_max = mu * fabs(f_n_mag);
+            if (f_t_spring_mag > f_t_max && f_t_spring_mag > R(kTangEps)) {
+                const R scale = f_t_max / f_t_spring_mag;
+                sx *= double(scale);
+                sy *= double(scale);
+                sz *= double(scale);
+            }
+            const R gamma_t = R(2.0) * R(kSqrt53) * beta * sqrt_r<R>(k_t * m_r);
+            R ftx = k_t * R(sx) - gamma_t * R(vtx);
+            R fty = k_t * R(sy) - gamma_t * R(vty);
+            R ftz = k_t * R(sz) - gamma_t * R(vtz);
+            const R f_t_mag = sqrt_r<R>(ftx * ftx + fty * fty + ftz * ftz);
+            if (f_t_mag > f_t_max && f_t_mag > R(kTangEps)) {
+                const R scale = f_t_max / f_t_mag;
+                ftx *= scale;
+                fty *= scale;
+                ftz *= scale;
+            }
+            if (rev) {
+                // partner: +F_n - F_t, torque (r2 n) x F_t
+                fx += fnx;
+                fy += fny;
+                fz += fnz;
+                fx -= ftx;
+                fy -= fty;
+                fz -= ftz;
+                tx += R(r2ny) * ftz - R(r2nz) * fty;
+                ty += R(r2nz) * ftx - R(r2nx) * ftz;
+                tz += R(r2nx) * fty - R(r2ny) * ftx;
+            } else {
+                // owner: -F_n + F_t, torque (r1 n) x F_t
+                fx -= fnx;
+                fy -= fny;
+                fz -= fnz;
+                fx += ftx;
+                fy += fty;
+                fz += ftz;
+                tx += R(r1ny) * ftz - R(r1nz) * fty;
+                ty += R(r1nz) * ftx - R(r1nx) * ftz;
+                tz += R(r1nx) * fty - R(r1ny) * ftx;
+            }
+            a.hist[slot] = sign * sx;
+            a.hist[a.hist_cs + slot] = sign * sy;
+            a.hist[2 * a.hist_cs + slot] = sign * sz;
+            mask_out |= 1ull << k;
+        }
+        a.cmask[i] = mask_out;
+    }
+    if (a.use_walls) wall_forces<R>(a, pi, vi, a.mass[i], ti, fx, fy, fz);
+    if (a.use_gravity) {
+        const double m = a.mass[i];
+        fx += R(m * a.g[0]);
+        fy += R(m * a.g[1]);
+        fz += R(m * a.g[2]);
+    }
+    store_force<R>(a, i, fx, fy, fz, tx, ty, tz);
+}
+
+// standalone wall / gravity system: adds to the current forces
+template <typename R>
+__global__ void k_walls_gravity(ForceArgs a) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= a.n) return;
+    R* f = nullptr;
+    if constexpr (sizeof(R) == 8)
+        f = a.f64;
+    else
+        f = a.f32;
+    R fx = f[i], fy = f[a.npad + i], fz = f[2 * a.npad + i];
+    if (a.use_walls) wall_forces<R>(a, a.pos[i], a.vel[i], a.mass[i], a.type[i], fx, fy, fz);
+    if (a.use_gravity) {
+        const double m = a.mass[i];
+        fx += R(m * a.g[0]);
+        fy += R(m * a.g[1]);
+        fz += R(m * a.g[2]);
+    }
+    f[i] = fx;
+    f[a.npad + i] = fy;
+    f[2 * a.npad + i] = fz;
+}
+
+__global__ void k_reset_overlap(Status* st) { st->overlap_count = 0; }
+
+ForceArgs make_args(Ctx& c, int step) {
+    ForceArgs a{};
+    a.n = c.n;
+    a.npad = c.npad;
+    a.maxnb = c.maxnb;
+    a.pos = c.pos.p;
+    a.vel = c.vel.p;
+    a.omega = c.omega.p;
+    a.tag = c.tag.p;
+    a.type = c.type.p;
+    a.mass = c.mass.p;
+    a.nbr = c.nbr.p;
+    a.nnb = c.nnb.p;
+    for (int d = 0; d < 3; d++) a.L[d] = c.box.L[d];
+    a.lj1 = c.lj1;
+    a.lj2 = c.lj2;
+    a.cut2 = c.lj_cut2;
+    a.e12 = c.lj_e12;
+    a.e6 = c.lj_e6;
+    a.mat = c.mat_dev.p;
+    a.nm = c.mat.n;
+    a.dt = c.dt;
+    a.hist = c.hist.p;
+    a.cmask = c.cmask.p;
+    a.hist_cs = int64_t(c.maxnb) * c.npad;
+    a.walls = c.walls;
+    a.g[0] = c.g[0];
+    a.g[1] = c.g[1];
+    a.g[2] = c.g[2];
+    a.f64 = c.f64.p;
+    a.f32 = c.f32.p;
+    a.t64 = c.t64.p;
+    a.t32 = c.t32.p;
+    a.pe = c.pe.p;
+    a.st = c.status.p;
+    a.step = step;
+    return a;
+}
+
+}  // namespace
+
+void upload_materials(Ctx& c) {
+    const int nm = c.mat.n;
+    std::vector<double> h(size_t(4) * nm * nm);
+    for (int k = 0; k < nm * nm; k++) {
+        h[k] = c.mat.beta[k];
+        h[nm * nm + k] = c.mat.mu[k];
+        h[2 * nm * nm + k] = c.mat.eeff[k];
+        h[3 * nm * nm + k] = c.mat.geff[k];
+    }
+    c.mat_dev.resize(h.size());
+    MDD_HIP(hipMemcpy(c.mat_dev.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+}
+
+// one pair-force pass (Force phase). step < 0: ungated standalone call.
+void launch_pair_forces(Ctx& c, int step, bool energy) {
+    ForceArgs a = make_args(c, step);
+    const int nb = blocks_for(c.n, 256);
+    const bool mixed = c.precision == 1;
+    if (step < 0) {
+        k_reset_overlap<<<1, 1, 0, c.stream>>>(c.status.p);
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool prof = c.prof_every > 0 && step >= 0 && (c.prof_ctr++ % c.prof_every) == 0 &&
+                      c.prof_pending < int64_t(c.ev_pool.size() / 2);
+    if (prof) {
+        e0 = c.ev_pool[2 * c.prof_pending];
+        e1 = c.ev_pool[2 * c.prof_pending + 1];
+        MDD_HIP(hipEventRecord(e0, c.stream));
+    }
+    if (c.model == 0) {
+        if (energy) {
+            c.pe.resize(c.npad);
+            a.pe = c.pe.p;
+            if (mixed)
+                k_lj<float, true><<<nb, 256, 0, c.stream>>>(a);
+            else
+                k_lj<double, true><<<nb, 256, 0, c.stream>>>(a);
+        } else {
+            if (mixed)
+                k_lj<float, false><<<nb, 256, 0, c.stream>>>(a);
+            else
+                k_lj<double, false><<<nb, 256, 0, c.stream>>>(a);
+        }
+    } else {
+        a.use_pairs = 1;
+        a.use_walls = step >= 0 && c.walls.count > 0;
+        a.use_gravity = step >= 0 && c.gravity_on;
+        if (mixed)
+            k_hertz<float><<<nb, 256, 0, c.stream>>>(a);
+        else
+            k_hertz<double><<<nb, 256, 0, c.stream>>>(a);
+    }
+    MDD_HIP(hipGetLastError());
+    c.force_launches++;
+    if (prof) {
+        MDD_HIP(hipEventRecord(e1, c.stream));
+        c.prof_pending++;
+    }
+}
+
+void launch_walls_gravity(Ctx& c, bool walls, bool gravity) {
+    ForceArgs a = make_args(c, -1);
+    a.use_walls = walls && c.walls.count > 0;
+    a.use_gravity = gravity && c.gravity_on;
+    const int nb = blocks_for(c.n, 256);
+    if (c.precision == 1)
+        k_walls_gravity<float><<<nb, 256, 0, c.stream>>>(a);
+    else
+        k_walls_gravity<double><<<nb, 256, 0, c.stream>>>(a);
+    MDD_HIP(hipGetLastError());
+}
+
+}  // namespace mdd

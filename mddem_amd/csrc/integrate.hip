@@ -1,0 +1,457 @@
+// Time integration kernels of the gfx950 particle path: Velocity Verlet
+// (crates/mddem_verlet/src/lib.rs:97-154), quaternion rotation
+// (crates/dem_granular/src/rotational.rs:59-100) and the Nose-Hoover fused
+// half steps (crates/md_thermostat/src/lib.rs:254-353).
+//
+// The pre kernel of a step ends with the reference's rebuild decision
+// (decide_rebuild, crates/mddem_core/src/neighbor.rs:639-674): a particle
+// that moved more than the skin budget since the last build, or (when
+// enabled) left the periodic box, records the step in Status::abort_step.
+// Every later kernel of the step sequence then returns at once, the host
+// rebuilds the neighbor list and resumes with the force pass of that step.
+//
+// The Nose-Hoover kinetic energy is reduced deterministically: fixed-order
+// tree sums per block, then the last block to finish sums the block partials
+// in index order and advances the thermostat.
+#include "ctx.hpp"
+
+namespace mdd {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+struct IntArgs {
+    int64_t n, npad;
+    double dt;
+    double4* pos;
+    double4* vel;
+    double4* omega;
+    double4* quat;
+    const double* mass;
+    const double4* build_pos;
+    const double* f64;
+    const float* f32;
+    const double* t64;
+    const float* t32;
+    int rot;         // apply rotation half-steps
+    int vv;          // apply translational half-steps
+    int check;       // evaluate the rebuild decision
+    double thr2;     // ((skin_fraction - 1) * min_skin)^2
+    BoxP box;
+    int box_exit;
+    Status* st;
+    int step;
+    double* partial;
+    double ndof, temp, q_mass;
+};
+
+template <typename F>
+__device__ inline void load_f(const IntArgs& a, int64_t i, double& fx, double& fy, double& fz) {
+    if constexpr (sizeof(F) == 8) {
+        fx = a.f64[i];
+        fy = a.f64[a.npad + i];
+        fz = a.f64[2 * a.npad + i];
+    } else {
+        fx = double(a.f32[i]);
+        fy = double(a.f32[a.npad + i]);
+        fz = double(a.f32[2 * a.npad + i]);
+    }
+}
+
+template <typename F>
+__device__ inline void load_t(const IntArgs& a, int64_t i, double& tx, double& ty, double& tz) {
+    if constexpr (sizeof(F) == 8) {
+        tx = a.t64[i];
+        ty = a.t64[a.npad + i];
+        tz = a.t64[2 * a.npad + i];
+    } else {
+        tx = double(a.t32[i]);
+        ty = double(a.t32[a.npad + i]);
+        tz = double(a.t32[2 * a.npad + i]);
+    }
+}
+
+// displacement_exceeded (neighbor.rs:564-601) + periodic box exit (651-666)
+__device__ inline void rebuild_check(const IntArgs& a, int64_t i, const double4& p) {
+    const double4 b = a.build_pos[i];
+    double d[3] = {p.x - b.x, p.y - b.y, p.z - b.z};
+    const double x[3] = {p.x, p.y, p.z};
+    bool out = false;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (a.box.periodic[k]) {
+            const double L = a.box.L[k], h = L * 0.5;
+            if (d[k] > h)
+                d[k] -= L;
+            else if (d[k] < -h)
+                d[k] += L;
+            if (a.box_exit && (x[k] < a.box.lo[k] || x[k] >= a.box.hi[k])) out = true;
+        }
+    }
+    if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > a.thr2 || out)
+        atomicMin(&a.st->abort_step, a.step);
+}
+
+// initial_rotation body (rotational.rs:59-84)
+__device__ inline void rotate_initial(const IntArgs& a, int64_t i, double tx, double ty,
+                                      double tz) {
+    double4 w = a.omega[i];
+    const double ii = w.w;
+    if (ii == 0.0) return;
+    const double dt = a.dt;
+    w.x += 0.5 * dt * tx * ii;
+    w.y += 0.5 * dt * ty * ii;
+    w.z += 0.5 * dt * tz * ii;
+    a.omega[i] = w;
+    const double omega_mag = sqrt(w.x * w.x + w.y * w.y + w.z * w.z);
+    const double angle = omega_mag * dt;
+    if (angle > 1e-14) {
+        const double inv = 1.0 / omega_mag;
+        const double ax = w.x * inv, ay = w.y * inv, az = w.z * inv;
+        const double half = angle * 0.5;
+        const double s = sin(half);
+        const double a0 = cos(half), a1 = ax * s, a2 = ay * s, a3 = az * s;
+        const double4 b = a.quat[i];
+        double4 q;
+        q.x = a0 * b.x - a1 * b.y - a2 * b.z - a3 * b.w;
+        q.y = a0 * b.y + a1 * b.x + a2 * b.w - a3 * b.z;
+        q.z = a0 * b.z - a1 * b.w + a2 * b.x + a3 * b.y;
+        q.w = a0 * b.w + a1 * b.z - a2 * b.y + a3 * b.x;
+        a.quat[i] = q;
+    }
+}
+
+// final_rotation body (rotational.rs:87-100)
+__device__ inline void rotate_final(const IntArgs& a, int64_t i, double tx, double ty,
+                                    double tz) {
+    double4 w = a.omega[i];
+    const double ii = w.w;
+    if (ii == 0.0) return;
+    const double dt = a.dt;
+    w.x += 0.5 * dt * tx * ii;
+    w.y += 0.5 * dt * ty * ii;
+    w.z += 0.5 * dt * tz * ii;
+    a.omega[i] = w;
+}
+
+// PreInitialIntegration + InitialIntegration of one step:
+// NH rescale (if nh), velocity half kick, drift, rotation, rebuild decision
+template <typename F, bool NH>
+__global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (a.step >= 0 && a.st->abort_step < a.step) return;
+    if (i == 0) a.st->overlap_count = 0;
+    if (i >= a.n) return;
+    double4 v = a.vel[i];
+    double4 p = a.pos[i];
+    const double dt = a.dt;
+    if (NH) {
+        const double scale = a.st->nh_scale;
+        v.x *= scale;
+        v.y *= scale;
+        v.z *= scale;
+    }
+    if (NH || a.vv) {
+        double fx, fy, fz;
+        load_f<F>(a, i, fx, fy, fz);
+        const double h = 0.5 * dt * v.w;
+        v.x += h * fx;
+        v.y += h * fy;
+        v.z += h * fz;
+        p.x += v.x * dt;
+        p.y += v.y * dt;
+        p.z += v.z * dt;
+        a.vel[i] = v;
+        a.pos[i] = p;
+    }
+    if (a.rot) {
+        double tx, ty, tz;
+        load_t<F>(a, i, tx, ty, tz);
+        rotate_initial(a, i, tx, ty, tz);
+    }
+    if (a.check) rebuild_check(a, i, p);
+}
+
+// deterministic block sum of one double (256 threads)
+__device__ inline double block_sum(double v, double* lds) {
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+// FinalIntegration + PostFinalIntegration of one step: half kick, rotation,
+// NH rescale with this step's factor, kinetic energy and the thermostat
+// updates (post half-step now, next step's pre half-step ahead of time: the
+// reference recomputes the same KE there, md_thermostat/src/lib.rs:262-265)
+template <typename F, bool NH>
+__global__ void __launch_bounds__(kThreads) k_post(IntArgs a) {
+    __shared__ double lds[kThreads];
+    __shared__ bool last;
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (a.step >= 0 && a.st->abort_step <= a.step) return;
+    double ke = 0.0;
+    if (i < a.n) {
+        double4 v = a.vel[i];
+        if (NH || a.vv) {
+            double fx, fy, fz;
+            load_f<F>(a, i, fx, fy, fz);
+            const double h = 0.5 * a.dt * v.w;
+            v.x += h * fx;
+            v.y += h * fy;
+            v.z += h * fz;
+        }
+        if (NH) {
+            const double scale = a.st->nh_scale;
+            v.x *= scale;
+            v.y *= scale;
+            v.z *= scale;
+        }
+        if (NH || a.vv) a.vel[i] = v;
+        if (a.rot) {
+            double tx, ty, tz;
+            load_t<F>(a, i, tx, ty, tz);
+            rotate_final(a, i, tx, ty, tz);
+        }
+        if (NH) ke = a.mass[i] * (v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    if (!NH) return;
+    const double bs = block_sum(ke, lds);
+    if (threadIdx.x == 0) {
+        a.partial[blockIdx.x] = bs;
+        __threadfence();
+        const unsigned t = atomicAdd(&a.st->ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double s = 0.0;
+    for (int64_t b = threadIdx.x; b < gridDim.x; b += kThreads) s += a.partial[b];
+    const double tot = block_sum(s, lds);
+    if (threadIdx.x == 0) {
+        const double KE = 0.5 * tot;
+        const double dt = a.dt;
+        const double p_post = a.st->p_xi_pre + (dt / 2.0) * (2.0 * KE - a.ndof * a.temp);
+        const double p_next = p_post + (dt / 2.0) * (2.0 * KE - a.ndof * a.temp);
+        a.st->ke = KE;
+        a.st->p_xi = p_post;
+        a.st->p_xi_pre = p_next;
+        a.st->nh_scale = exp(-dt / 2.0 * p_next / a.q_mass);
+        a.st->ticket = 0u;
+    }
+}
+
+// kinetic energy of the current velocities -> Status (setup_nose_hoover and
+// the first nh_pre_initial, md_thermostat/src/lib.rs:195-265)
+__global__ void __launch_bounds__(kThreads) k_nh_init(IntArgs a) {
+    __shared__ double lds[kThreads];
+    __shared__ bool last;
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    double ke = 0.0;
+    if (i < a.n) {
+        const double4 v = a.vel[i];
+        ke = a.mass[i] * (v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    const double bs = block_sum(ke, lds);
+    if (threadIdx.x == 0) {
+        a.partial[blockIdx.x] = bs;
+        __threadfence();
+        const unsigned t = atomicAdd(&a.st->ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double s = 0.0;
+    for (int64_t b = threadIdx.x; b < gridDim.x; b += kThreads) s += a.partial[b];
+    const double tot = block_sum(s, lds);
+    if (threadIdx.x == 0) {
+        const double KE = 0.5 * tot;
+        const double dt = a.dt;
+        const double p = a.st->p_xi;
+        const double p_next = p + (dt / 2.0) * (2.0 * KE - a.ndof * a.temp);
+        a.st->ke = KE;
+        a.st->p_xi_pre = p_next;
+        a.st->nh_scale = exp(-dt / 2.0 * p_next / a.q_mass);
+        a.st->ticket = 0u;
+    }
+}
+
+// per-block partial sums of the thermo quantities: sum m v^2, sum m v, sum m,
+// and (second pass, given vbar) sum m |v - vbar|^2
+__global__ void __launch_bounds__(kThreads) k_thermo(IntArgs a, double vbx, double vby,
+                                                     double vbz, double* out) {
+    __shared__ double lds[kThreads];
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    if (i < a.n) {
+        const double4 v = a.vel[i];
+        const double m = a.mass[i];
+        q[0] = m * (v.x * v.x + v.y * v.y + v.z * v.z);
+        q[1] = m * v.x;
+        q[2] = m * v.y;
+        q[3] = m * v.z;
+        q[4] = m;
+        const double ux = v.x - vbx, uy = v.y - vby, uz = v.z - vbz;
+        q[5] = m * (ux * ux + uy * uy + uz * uz);
+    }
+    for (int k = 0; k < 6; k++) {
+        const double s = block_sum(q[k], lds);
+        if (threadIdx.x == 0) out[blockIdx.x * 6 + k] = s;
+    }
+}
+
+IntArgs make_args(Ctx& c, int step, bool check) {
+    IntArgs a{};
+    a.n = c.n;
+    a.npad = c.npad;
+    a.dt = c.dt;
+    a.pos = c.pos.p;
+    a.vel = c.vel.p;
+    a.omega = c.omega.p;
+    a.quat = c.quat.p;
+    a.mass = c.mass.p;
+    a.build_pos = c.build_pos.p;
+    a.f64 = c.f64.p;
+    a.f32 = c.f32.p;
+    a.t64 = c.t64.p;
+    a.t32 = c.t32.p;
+    a.rot = 0;
+    a.vv = 0;
+    a.check = check && c.list_valid && !c.list_injected ? 1 : 0;
+    const double skin = (c.skin_fraction - 1.0) * c.min_skin;
+    a.thr2 = skin * skin;
+    a.box = c.box;
+    a.box_exit = c.box_exit_rebuild;
+    a.st = c.status.p;
+    a.step = step;
+    c.partial.resize(size_t(blocks_for(c.n, kThreads)) * 6 + 64);
+    a.partial = c.partial.p;
+    a.ndof = c.nh_ndof;
+    a.temp = c.nh_T;
+    a.q_mass = c.nh_Q;
+    return a;
+}
+
+template <bool NH>
+void pre_impl(Ctx& c, IntArgs& a) {
+    const int nb = blocks_for(c.n, kThreads);
+    if (c.precision == 1)
+        k_pre<float, NH><<<nb, kThreads, 0, c.stream>>>(a);
+    else
+        k_pre<double, NH><<<nb, kThreads, 0, c.stream>>>(a);
+    MDD_HIP(hipGetLastError());
+}
+
+template <bool NH>
+void post_impl(Ctx& c, IntArgs& a) {
+    const int nb = blocks_for(c.n, kThreads);
+    if (c.precision == 1)
+        k_post<float, NH><<<nb, kThreads, 0, c.stream>>>(a);
+    else
+        k_post<double, NH><<<nb, kThreads, 0, c.stream>>>(a);
+    MDD_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+// scheduled pre kernel of a step (all enabled integrators)
+void launch_pre(Ctx& c, int step, bool check_rebuild) {
+    IntArgs a = make_args(c, step, check_rebuild);
+    a.rot = c.rot_on && c.model == 1;
+    a.vv = c.vv_on;
+    if (c.nh_on)
+        pre_impl<true>(c, a);
+    else
+        pre_impl<false>(c, a);
+}
+
+void launch_post(Ctx& c, int step) {
+    IntArgs a = make_args(c, step, false);
+    a.rot = c.rot_on && c.model == 1;
+    a.vv = c.vv_on;
+    if (c.nh_on)
+        post_impl<true>(c, a);
+    else
+        post_impl<false>(c, a);
+}
+
+void launch_initial_integration(Ctx& c, int step, bool check_rebuild) {
+    IntArgs a = make_args(c, step, check_rebuild);
+    a.vv = 1;
+    pre_impl<false>(c, a);
+}
+
+void launch_final_integration(Ctx& c) {
+    IntArgs a = make_args(c, -1, false);
+    a.vv = 1;
+    post_impl<false>(c, a);
+}
+
+void launch_initial_rotation(Ctx& c) {
+    IntArgs a = make_args(c, -1, false);
+    a.rot = 1;
+    pre_impl<false>(c, a);
+}
+
+void launch_final_rotation(Ctx& c) {
+    IntArgs a = make_args(c, -1, false);
+    a.rot = 1;
+    post_impl<false>(c, a);
+}
+
+void launch_nh_pre(Ctx& c, int step, bool check_rebuild) {
+    IntArgs a = make_args(c, step, check_rebuild);
+    pre_impl<true>(c, a);
+}
+
+void launch_nh_post(Ctx& c) {
+    IntArgs a = make_args(c, -1, false);
+    post_impl<true>(c, a);
+}
+
+// kinetic energy of the current velocities; seeds the thermostat's next
+// pre half-step from Status::p_xi
+void launch_ke(Ctx& c) {
+    IntArgs a = make_args(c, -1, false);
+    k_nh_init<<<blocks_for(c.n, kThreads), kThreads, 0, c.stream>>>(a);
+    MDD_HIP(hipGetLastError());
+}
+
+// out: [sum m v^2, sum m vx, sum m vy, sum m vz, sum m, sum m |v - vbar|^2]
+void thermo_reduce(Ctx& c, double out[8]) {
+    IntArgs a = make_args(c, -1, false);
+    const int nb = blocks_for(c.n, kThreads);
+    std::vector<double> h(size_t(nb) * 6);
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int pass = 0; pass < 2; pass++) {
+        double vb[3] = {0, 0, 0};
+        if (pass == 1 && s[4] > 0) {
+            vb[0] = s[1] / s[4];
+            vb[1] = s[2] / s[4];
+            vb[2] = s[3] / s[4];
+        }
+        k_thermo<<<nb, kThreads, 0, c.stream>>>(a, vb[0], vb[1], vb[2], c.partial.p);
+        MDD_HIP(hipGetLastError());
+        MDD_HIP(hipMemcpyAsync(h.data(), c.partial.p, sizeof(double) * h.size(),
+                               hipMemcpyDeviceToHost, c.stream));
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        double t[6] = {0, 0, 0, 0, 0, 0};
+        for (int b = 0; b < nb; b++)
+            for (int k = 0; k < 6; k++) t[k] += h[size_t(b) * 6 + k];
+        if (pass == 0)
+            for (int k = 0; k < 6; k++) s[k] = t[k];
+        else
+            s[5] = t[5];
+    }
+    for (int k = 0; k < 6; k++) out[k] = s[k];
+}
+
+}  // namespace mdd

@@ -1,0 +1,299 @@
+"""Parity of the HIP path (through the C ABI) with the CPU checker.
+
+Pair sets and history remaps are compared bit for bit; forces to 1e-12 (FP64,
+only summation order differs) or 1e-5 (mixed: FP32 force math) relative
+error in the global 2-norm. Every case prints its name before touching the
+GPU so a failure names it."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from mddem_amd import hip
+from mddem_amd import workloads as W
+from oracle.oracle import HIST_CANONICAL, MODEL_DEM, MODEL_LJ, Oracle, dem_mass
+from tests.helpers import canonical_pairs, kats, oracle_from_workload, rel_err
+
+pytestmark = pytest.mark.gpu
+K = kats()
+FP64_TOL = 1e-12
+MIXED_TOL = 1e-5
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu case] {request.node.name}", flush=True)
+    sys.stdout.flush()
+    yield
+
+
+def by_tag(tags, arr):
+    order = np.argsort(tags)
+    return np.asarray(arr)[order]
+
+
+def two_sphere_gpu(precision, sep, vel1=(0, 0, 0), vel0=(0, 0, 0), om0=(0, 0, 0),
+                   om1=(0, 0, 0), e=0.95, mu=0.4, dt=1e-7, r=1e-3):
+    s = hip.System(hip.MODEL_DEM, precision)
+    s.set_box((-1, -1, -1), (1, 1, 1), (False, False, False))
+    s.set_materials([8.7e9], [0.3], [e], [mu])
+    s.set_dt(dt)
+    m = dem_mass(2500.0, r)
+    s.upload([[0, 0, 0], [sep, 0, 0]], [vel0, vel1], [m, m], [r, r], radius=[r, r],
+             omega=[om0, om1])
+    s.set_half_list([0, 1, 1], [1])
+    return s
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+def test_hertz_force_matches_analytical_value(prec):
+    r, d = K["hertz_radius"], K["hertz_delta"]
+    s = two_sphere_gpu(prec, 2 * r - d)
+    s.hertz_mindlin_contact_force()
+    f, t = s.forces()
+    tol = 1e-10 if prec == hip.FP64 else MIXED_TOL
+    assert abs(abs(f[0, 0]) - K["hertz_force"]) / K["hertz_force"] < tol
+    assert f[0, 0] < 0 < f[1, 0]
+    assert np.array_equal(f[0], -f[1])
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+@pytest.mark.parametrize("r", [0.9, 1.0, 1.12, 1.5, 2.4, 3.0])
+def test_lj_two_body_force(prec, r):
+    s = hip.System(hip.MODEL_LJ, prec)
+    s.set_box((-10, -10, -10), (10, 10, 10), (False, False, False))
+    s.set_lj(1.0, 1.0, 2.5)
+    s.upload([[0, 0, 0], [r, 0, 0]], np.zeros((2, 3)), [1, 1], [0.5, 0.5])
+    s.set_half_list([0, 1, 1], [1])
+    s.lj_force()
+    f, _ = s.forces()
+    want = K["lj_force_on_plus_x"][repr(r)]
+    if want == 0.0:
+        assert f[1, 0] == 0.0
+    else:
+        tol = 1e-13 if prec == hip.FP64 else 2e-6
+        assert abs(f[1, 0] - want) / abs(want) < tol
+    assert f[0, 0] == -f[1, 0]
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+def test_sliding_contact_history_matches_oracle(prec):
+    # spring_history_accumulates_tangential_force (tangential.rs:353-408),
+    # five force passes with history carried between them
+    kw = dict(vel0=(0.1, 0.2, -0.1), vel1=(-0.3, 0.1, 0.05), om0=(10, 20, -5), om1=(-15, 5, 10))
+    s = two_sphere_gpu(prec, 0.0019, **kw)
+    o = Oracle(MODEL_DEM)
+    o.set_box((-1, -1, -1), (1, 1, 1), (False, False, False))
+    o.set_materials([8.7e9], [0.3], [0.95], [0.4])
+    o.set_dt(1e-7)
+    o.set_history_mode(HIST_CANONICAL)
+    m = dem_mass(2500.0, 1e-3)
+    o.set_atoms([[0, 0, 0], [0.0019, 0, 0]], [kw["vel0"], kw["vel1"]], [m, m], [1e-3, 1e-3],
+                radius=[1e-3, 1e-3], omega=[kw["om0"], kw["om1"]])
+    o.set_half_list([0, 1, 1], [1])
+    for _ in range(5):
+        o.zero_all_forces()
+        o.hertz_mindlin_contact_force()
+        s.hertz_mindlin_contact_force()
+    f, t = s.forces()
+    ref = o.atoms()
+    tol = FP64_TOL if prec == hip.FP64 else MIXED_TOL
+    assert rel_err(f, ref["force"]) < tol
+    assert rel_err(t, ref["torque"]) < tol
+    a, b, spring = s.history()
+    oa, ob, ospring = o.history()
+    assert list(zip(a, b)) == list(zip(oa, ob))
+    if prec == hip.FP64:
+        assert rel_err(spring, ospring) < FP64_TOL
+    else:
+        assert rel_err(spring, ospring) < 1e-6
+    assert np.array_equal(f[0], -f[1])
+
+
+def _oracle_rebuild(o):
+    o.neighbor_setup()
+    o.pbc()
+    o.sort_atoms_by_bin()
+    o.borders()
+    o.bin_neighbor_list()
+
+
+def _random_periodic(seed, n=300, poly=False, zperiodic=True):
+    st = W.Stream(seed)
+    L = np.array([6.0, 5.0, 7.0])
+    pos = st.uniform(3 * n).reshape(n, 3) * L
+    cut_r = 0.55 + 0.3 * st.uniform(n) if poly else np.full(n, 0.7)
+    return pos, L, cut_r, (True, True, zperiodic)
+
+
+def _pair_cases():
+    return ["trig30", "rand_uniform", "rand_slab", "rand_poly", "dem_gas_500", "lj_fcc_6",
+            "dem_settling_slab"]
+
+
+def _build_case(name):
+    """(oracle, gpu system) configured identically, lists not built yet."""
+    if name == "trig30":
+        pos = np.array(K["trig30_pos"])
+        o = Oracle(MODEL_LJ)
+        o.set_box((0, 0, 0), (4, 4, 4), (False, False, False))
+        o.set_neighbor(1.0, 0.5, 0, True, 1000, True)
+        o.set_atoms(pos, np.zeros_like(pos), np.ones(30), np.full(30, 0.3))
+        s = hip.System(hip.MODEL_LJ)
+        s.set_box((0, 0, 0), (4, 4, 4), (False, False, False))
+        s.set_neighbor(1.0, 0.5, 0, True, 1000, True)
+        s.upload(pos, np.zeros_like(pos), np.ones(30), np.full(30, 0.3))
+        return o, s
+    if name.startswith("rand"):
+        seed = {"rand_uniform": 1, "rand_slab": 2, "rand_poly": 3}[name]
+        pos, L, cut_r, per = _random_periodic(seed, poly=name == "rand_poly",
+                                              zperiodic=name != "rand_slab")
+        o = Oracle(MODEL_LJ)
+        o.set_box((0, 0, 0), L, per)
+        o.set_neighbor(1.1, 0.4, 0, True, 1000, True)
+        o.set_atoms(pos, np.zeros_like(pos), np.ones(len(pos)), cut_r)
+        s = hip.System(hip.MODEL_LJ)
+        s.set_box((0, 0, 0), L, per)
+        s.set_neighbor(1.1, 0.4, 0, True, 1000, True)
+        s.upload(pos, np.zeros_like(pos), np.ones(len(pos)), cut_r)
+        return o, s
+    if name == "dem_gas_500":
+        w = W.dem_gas(n=500)
+    elif name == "lj_fcc_6":
+        w = W.lj_fcc(ncell=6)
+    elif name == "dem_settling_slab":
+        w = W.dem_settling(n=3000, box=(0.04, 0.04, 0.08))
+    return oracle_from_workload(w), hip.from_workload(w)
+
+
+@pytest.mark.parametrize("name", _pair_cases())
+def test_pair_set_bit_exact(name):
+    o, s = _build_case(name)
+    _oracle_rebuild(o)
+    s.bin_neighbor_list()
+    go, gp, gc = s.pairs()
+    oo, op, oc = o.pairs()
+    # identical half list: same owners, partners and image shifts
+    key = lambda a, b, c: np.sort((a.astype(np.int64) << 37) | (b.astype(np.int64) << 5) | c)
+    assert len(go) == len(oo)
+    assert np.array_equal(key(go, gp, gc), key(oo, op, oc))
+    assert np.array_equal(canonical_pairs(go, gp, gc), canonical_pairs(oo, op, oc))
+    assert s.npairs() == len(oo)
+
+
+def _perturbed_lj(n=6, jitter=0.05, seed=11):
+    w = W.lj_fcc(ncell=n)
+    w.pos = w.pos + (W.Stream(seed).uniform(3 * w.n).reshape(-1, 3) - 0.5) * jitter
+    w.pos = np.mod(w.pos, w.hi - w.lo)
+    return w
+
+
+def _dense_dem(n=2000, seed=5):
+    # a gas compressed so that many spheres overlap slightly
+    w = W.dem_gas(n=n, box=0.025 * (n / 500) ** (1 / 3), seed=seed)
+    w.pos = w.pos * 0.83
+    w.hi = w.hi * 0.83
+    w.vel = (W.Stream(seed + 1).uniform(3 * n).reshape(-1, 3) - 0.5) * 0.4
+    w.omega = (W.Stream(seed + 2).uniform(3 * n).reshape(-1, 3) - 0.5) * 400.0
+    return w
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+@pytest.mark.parametrize("case", ["lj", "dem"])
+def test_forces_match_oracle(case, prec):
+    w = _perturbed_lj() if case == "lj" else _dense_dem()
+    o = oracle_from_workload(w)
+    s = hip.from_workload(w, prec)
+    _oracle_rebuild(o)
+    o.zero_all_forces()
+    if case == "lj":
+        o.lj_force()
+    else:
+        o.hertz_mindlin_contact_force()
+    o.reverse_send_force()
+    s.bin_neighbor_list()
+    if case == "lj":
+        s.lj_force()
+    else:
+        s.hertz_mindlin_contact_force()
+    ref = o.atoms()
+    st = s.state()
+    f, t = s.forces()
+    tol = FP64_TOL if prec == hip.FP64 else MIXED_TOL
+    assert rel_err(by_tag(st["tag"], f), by_tag(ref["tag"], ref["force"])) < tol
+    if case == "dem":
+        assert rel_err(by_tag(st["tag"], t), by_tag(ref["tag"], ref["torque"])) < tol
+        assert np.abs(f.sum(axis=0)).max() <= 1e-9 * np.abs(f).max()
+
+
+def _trajectory(w, steps, prec=hip.FP64):
+    o = oracle_from_workload(w)
+    s = hip.from_workload(w, prec, box_exit=True)
+    o.run(steps)
+    s.run(steps)
+    return o, s
+
+
+def test_dem_gas_trajectory_matches_oracle():
+    w = W.dem_gas(n=500)
+    o, s = _trajectory(w, 400)
+    ref = o.atoms()
+    st = s.state()
+    L = 0.025
+    dp = by_tag(st["tag"], st["pos"]) - by_tag(ref["tag"], ref["pos"])
+    dp -= L * np.round(dp / L)
+    assert np.abs(dp).max() < 1e-12 * L * 1e3
+    assert rel_err(by_tag(st["tag"], st["vel"]), by_tag(ref["tag"], ref["vel"])) < 1e-9
+    assert rel_err(by_tag(st["tag"], st["omega"]), by_tag(ref["tag"], ref["omega"])) < 1e-6
+    assert s.thermo()["rebuilds"] == o.rebuilds
+
+
+def test_lj_nvt_trajectory_matches_oracle():
+    w = W.lj_fcc(ncell=4)
+    o, s = _trajectory(w, 300)
+    ref = o.atoms()
+    st = s.state()
+    assert rel_err(by_tag(st["tag"], st["vel"]), by_tag(ref["tag"], ref["vel"])) < 1e-9
+    th = s.thermo()
+    assert abs(th["p_xi"] - o.p_xi) < 1e-9 * max(1.0, abs(o.p_xi))
+    assert th["rebuilds"] == o.rebuilds
+
+
+def test_history_remap_is_bit_exact():
+    w = _dense_dem(n=1500, seed=9)
+    s = hip.from_workload(w, hip.FP64, box_exit=False)
+    s.run(20)
+    a0, b0, s0 = s.history()
+    assert len(a0) > 100
+    s.bin_neighbor_list()  # rebuild at the same positions: wrap, sort, new rows
+    a1, b1, s1 = s.history()
+    k0 = np.lexsort((b0, a0))
+    k1 = np.lexsort((b1, a1))
+    assert np.array_equal(a0[k0], a1[k1]) and np.array_equal(b0[k0], b1[k1])
+    assert np.array_equal(s0[k0], s1[k1])
+
+
+@pytest.mark.parametrize("model", ["lj", "dem"])
+def test_runs_are_bitwise_deterministic(model):
+    w = W.lj_fcc(ncell=5) if model == "lj" else _dense_dem(n=1200, seed=3)
+    outs = []
+    for _ in range(2):
+        s = hip.from_workload(w, hip.MIXED)
+        s.run(150)
+        st = s.state()
+        outs.append((st["pos"], st["vel"], s.forces()[0]))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_lj_256k_fcc_full_size_properties():
+    w = W.lj_fcc(ncell=40)
+    s = hip.from_workload(w, hip.MIXED, box_exit=False)
+    s.bin_neighbor_list()
+    assert s.npairs() == w.n * K["fcc_neighbors_within_2p8"] // 2
+    s.run(200)
+    th = s.thermo(energy=True)
+    assert np.all(np.abs(th["momentum"]) < 1e-9 * w.n)
+    assert 0.3 < th["temperature"] < 1.5
+    assert math.isfinite(th["pe"]) and th["pe"] < 0
