@@ -40,18 +40,17 @@ __global__ void k_status_init(mdd::Status* st) {
     st->max_row = 0;
     st->max_half = 0;
     st->ghost_total = 0u;
-    st->overlap_step = 0;
     st->overlap_count = 0;
-    st->ticket = 0u;
-    st->ncontacts = 0u;
+    st->nh_kind[0] = st->nh_kind[1] = 1;
+    st->pad = 0;
     st->ke = 0.0;
     st->p_xi = 0.0;
-    st->nh_scale = 1.0;
-    st->p_xi_pre = 0.0;
+    st->nh_p[0] = st->nh_p[1] = 0.0;
+    st->nh_scale[0] = st->nh_scale[1] = 1.0;
 }
 
 __global__ void k_set_abort(mdd::Status* st, int v) { st->abort_step = v; }
-__global__ void k_pxi_from_pre(mdd::Status* st) { st->p_xi = st->p_xi_pre; }
+__global__ void k_set_pxi(mdd::Status* st, double v) { st->p_xi = v; }
 
 constexpr int kEventPool = 2048;
 
@@ -118,12 +117,12 @@ void check_device_error(Ctx& c) {
 
 void setup_nose_hoover(Ctx& c) {
     // setup_nose_hoover (md_thermostat/src/lib.rs:195-243): ndof = 3N - 3,
-    // Q = ndof T tau^2, p_xi = 0, then the first pre half-step's KE
+    // Q = ndof T tau^2, p_xi = 0
     const double n = double(c.n);
     c.nh_ndof = 3.0 * n - 3.0;
     c.nh_Q = c.nh_ndof * c.nh_T * c.nh_tau * c.nh_tau;
-    k_status_init<<<1, 1, 0, c.stream>>>(c.status.p);
-    mdd::launch_ke(c);
+    k_set_pxi<<<1, 1, 0, c.stream>>>(c.status.p, 0.0);
+    MDD_HIP(hipGetLastError());
     c.nh_ready = true;
 }
 
@@ -136,6 +135,13 @@ void run_steps(Ctx& c, int64_t nsteps) {
     if (c.step + nsteps > INT_MAX - 2) throw ApiError(MDD_ERR_INVALID, "step count too large");
     if (!c.setup_done) mdd::neighbor_setup(c);
     if (c.nh_on && !c.nh_ready) setup_nose_hoover(c);
+    if (nsteps == 0) return;
+    if (c.nh_on) {
+        // the first pre kernel reads the KE of the current velocities and
+        // the current p_xi from the slot of the previous step's parity
+        mdd::launch_ke_partials(c);
+        mdd::launch_nh_seed(c, int(c.step & 1));
+    }
     const int64_t end = c.step + nsteps;
     while (c.step < end) {
         const int64_t s = c.step + 1;
@@ -181,6 +187,8 @@ void run_steps(Ctx& c, int64_t nsteps) {
             c.chunk = std::min(c.chunk * 2, 1024);
         }
     }
+    // post half of the last step's thermostat update -> Status::p_xi, ke
+    if (c.nh_on) mdd::launch_nh_finish(c, int(c.step & 1));
     sync_status(c);
     harvest_profile(c);
     check_device_error(c);
@@ -639,8 +647,10 @@ int mdd_set_half_list(mdd_ctx* ctx, int64_t nlocal, const uint32_t* offsets,
     return guarded(ctx, [&](Ctx& c) {
         need_state(c);
         if (nlocal != c.n || !offsets) throw ApiError(MDD_ERR_INVALID, "row count differs");
+        if (offsets[c.n] > 0 && !indices) throw ApiError(MDD_ERR_INVALID, "null index array");
         std::vector<std::vector<uint32_t>> own(c.n), rev(c.n);
-        for (int64_t i = 0; i < c.n; i++)
+        for (int64_t i = 0; i < c.n; i++) {
+            if (offsets[i + 1] < offsets[i]) throw ApiError(MDD_ERR_INVALID, "offsets decrease");
             for (uint32_t k = offsets[i]; k < offsets[i + 1]; k++) {
                 const uint32_t j = indices[k];
                 if (int64_t(j) >= c.n || int64_t(j) == i)
@@ -648,17 +658,20 @@ int mdd_set_half_list(mdd_ctx* ctx, int64_t nlocal, const uint32_t* offsets,
                 own[i].push_back(mdd::enc_entry(j, mdd::kCodeNone, false));
                 rev[j].push_back(mdd::enc_entry(uint32_t(i), mdd::kCodeNone, true));
             }
+        }
         int mx = 1;
         for (int64_t i = 0; i < c.n; i++) mx = std::max<int>(mx, int(own[i].size() + rev[i].size()));
         if (c.model == MDD_MODEL_DEM && mx > 64)
             throw ApiError(MDD_ERR_UNSUPPORTED, "DEM rows above 64 entries");
+        const int G = 1 << c.lg;
+        mx = (mx + G - 1) / G * G;
         c.maxnb = mx;
         const int64_t np = c.npad;
         std::vector<uint32_t> hn(size_t(mx) * np, 0u), cnt(np, 0u), half(np, 0u);
         for (int64_t i = 0; i < c.n; i++) {
             int k = 0;
-            for (uint32_t e : own[i]) hn[size_t(k++) * np + i] = e;
-            for (uint32_t e : rev[i]) hn[size_t(k++) * np + i] = e;
+            for (uint32_t e : own[i]) hn[mdd::row_addr(i, k++, np, c.lg)] = e;
+            for (uint32_t e : rev[i]) hn[mdd::row_addr(i, k++, np, c.lg)] = e;
             cnt[i] = uint32_t(k);
             half[i] = uint32_t(own[i].size());
         }
@@ -696,18 +709,23 @@ int mdd_download_pairs(mdd_ctx* ctx, int64_t cap, uint32_t* owner_tag, uint32_t*
         need_list(c);
         MDD_HIP(hipStreamSynchronize(c.stream));
         const int64_t n = c.n, np = c.npad;
-        auto half = dl(c.nhalf, size_t(n));
+        auto nn = dl(c.nnb, size_t(n));
         auto nb = dl(c.nbr, size_t(c.maxnb) * np);
         auto tg = dl(c.tag, size_t(n));
         int64_t k = 0;
-        for (int64_t i = 0; i < n; i++)
-            for (uint32_t m = 0; m < half[i]; m++, k++) {
-                if (k >= cap) continue;
-                const uint32_t e = nb[size_t(m) * np + i];
-                if (owner_tag) owner_tag[k] = tg[i];
-                if (partner_tag) partner_tag[k] = tg[mdd::entry_idx(e)];
-                if (code) code[k] = uint8_t(mdd::entry_code(e));
+        for (int64_t i = 0; i < n; i++) {
+            const int cnt = std::min<int>(int(nn[i]), c.maxnb);
+            for (int m = 0; m < cnt; m++) {
+                const uint32_t e = nb[mdd::row_addr(i, m, np, c.lg)];
+                if (mdd::entry_rev(e)) continue;
+                if (k < cap) {
+                    if (owner_tag) owner_tag[k] = tg[i];
+                    if (partner_tag) partner_tag[k] = tg[mdd::entry_idx(e)];
+                    if (code) code[k] = uint8_t(mdd::entry_code(e));
+                }
+                k++;
             }
+        }
         if (n_out) *n_out = k;
     });
 }
@@ -787,9 +805,7 @@ int mdd_nh_pre_initial(mdd_ctx* ctx) {
     return guarded(ctx, [&](Ctx& c) {
         need_state(c);
         if (!c.nh_ready) setup_nose_hoover(c);
-        else mdd::launch_ke(c);  // KE of the current velocities -> p_xi half update
-        k_pxi_from_pre<<<1, 1, 0, c.stream>>>(c.status.p);
-        mdd::launch_nh_pre(c, -1, false);
+        mdd::launch_nh_pre_standalone(c);
     });
 }
 
@@ -797,7 +813,7 @@ int mdd_nh_post_final(mdd_ctx* ctx) {
     return guarded(ctx, [&](Ctx& c) {
         need_state(c);
         if (!c.nh_ready) throw ApiError(MDD_ERR_INVALID, "thermostat not set up");
-        mdd::launch_nh_post(c);
+        mdd::launch_nh_post_standalone(c);
     });
 }
 
@@ -816,25 +832,29 @@ int mdd_download_history(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* 
         MDD_HIP(hipStreamSynchronize(c.stream));
         const int64_t n = c.n, np = c.npad;
         const int64_t cs = int64_t(c.maxnb) * np;
-        auto half = dl(c.nhalf, size_t(n));
+        auto nn = dl(c.nnb, size_t(n));
         auto nb = dl(c.nbr, size_t(cs));
         auto tg = dl(c.tag, size_t(n));
         auto mk = dl(c.cmask, size_t(n));
         auto hs = dl(c.hist, size_t(3 * cs));
         int64_t k = 0;
-        for (int64_t i = 0; i < n; i++)
-            for (uint32_t m = 0; m < half[i]; m++) {
+        for (int64_t i = 0; i < n; i++) {
+            const int cnt = std::min<int>(int(nn[i]), c.maxnb);
+            for (int m = 0; m < cnt; m++) {
                 if (!((mk[i] >> m) & 1ull)) continue;
+                const int64_t slot = mdd::row_addr(i, m, np, c.lg);
+                const uint32_t e = nb[slot];
+                if (mdd::entry_rev(e)) continue;
                 if (k < cap) {
-                    const uint32_t e = nb[size_t(m) * np + i];
                     const uint32_t a = tg[i], b = tg[mdd::entry_idx(e)];
                     if (tag_lo) tag_lo[k] = std::min(a, b);
                     if (tag_hi) tag_hi[k] = std::max(a, b);
                     if (spring)
-                        for (int d = 0; d < 3; d++) spring[3 * k + d] = hs[d * cs + size_t(m) * np + i];
+                        for (int d = 0; d < 3; d++) spring[3 * k + d] = hs[d * cs + slot];
                 }
                 k++;
             }
+        }
         if (n_out) *n_out = k;
     });
 }

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 #include <stdexcept>
@@ -58,6 +59,12 @@ __host__ __device__ inline int code_of(int sx, int sy, int sz) {
 __host__ __device__ inline double image_shift(int s, double L) {
     return s == 0 ? 0.0 : (s > 0 ? L : -L);
 }
+// Neighbor-row layout: entry k of particle p. G consecutive entries of a
+// row are contiguous (G lanes of one particle read them together), and
+// entry group k / G of consecutive particles is contiguous across particles.
+__host__ __device__ inline int64_t row_addr(int64_t p, int k, int64_t npad, int lg) {
+    return ((int64_t(k >> lg) * npad + p) << lg) + (k & ((1 << lg) - 1));
+}
 __host__ __device__ inline int code_sx(int code) { return code / 9 - 1; }
 __host__ __device__ inline int code_sy(int code) { return (code / 3) % 3 - 1; }
 __host__ __device__ inline int code_sz(int code) { return code % 3 - 1; }
@@ -105,19 +112,19 @@ struct MatP {
 struct Status {
     int abort_step;          // first step whose drift needs a rebuild; INT_MAX none
     int error;               // 0 or an MDD_ERR_* code raised on the device
-    unsigned overflow;       // bit 0: image capacity, bit 1: neighbor row capacity
+    unsigned overflow;       // bit 0: image capacity, bit 1: neighbor row capacity,
+                             // bit 2: particle outside a non-periodic face
     int max_row;             // longest full-list row at the last build
     int max_half;            // longest half-list row at the last build
     unsigned ghost_total;    // periodic images at the last build
-    int overlap_step;        // step of overlap_count
-    int overlap_count;       // large-overlap pairs in that step
-    unsigned ticket;         // last-block reduction counter
-    unsigned ncontacts;      // DEM pairs in contact (owner side)
-    int pad[2];
-    double ke;               // kinetic energy after the last post kernel
-    double p_xi;             // Nose-Hoover momentum after the last post kernel
-    double nh_scale;         // velocity scale for the next pre kernel
-    double p_xi_pre;         // p_xi after the next pre half-update
+    int overlap_count;       // large-overlap pairs in the current step
+    int nh_kind[2];          // per step parity: 1 = nh_p is p_xi after a post
+                             // half-update (or the initial p_xi), 0 = after a pre
+    int pad;
+    double ke;               // kinetic energy at the last thermostat finish
+    double p_xi;             // Nose-Hoover momentum at the last thermostat finish
+    double nh_p[2];          // p_xi by step parity
+    double nh_scale[2];      // velocity scale by step parity
 };
 
 template <class T>
@@ -179,7 +186,10 @@ struct Ctx {
     // ---- sizes
     int64_t n = 0, cap = 0, npad = 0;
     int64_t ghost_cap = 0, nghost = 0;
-    int maxnb = 0;          // row capacity of the full list
+    int maxnb = 0;          // row capacity of the full list (multiple of 1 << lg)
+    int lg = 2;             // log2 of lanes per particle in the pair-force kernels
+    int lg_old = 2;
+    int build_lg = 3;       // log2 of lanes per particle in the list build
     int64_t npairs = 0;
     bool list_valid = false;
     bool list_injected = false;
@@ -205,11 +215,13 @@ struct Ctx {
     DBuf<uint32_t> ghost_root;
     DBuf<uint8_t> ghost_code;
     DBuf<double4> ghost_pos;
-    DBuf<uint32_t> all_key, bin_start, bin_order, bin_slot, bin_entry;
+    DBuf<uint32_t> bin_start, bin_entry, gstart, placed, ghost_key, ghost_order;
     DBuf<double4> bin_pos;
-    DBuf<int> stencil;
+    DBuf<int> stencil;       // forward offsets
+    DBuf<int> stencil_full;  // all offsets whose cells can hold a pair, fixed order
+    std::vector<int> full;
 
-    // ---- full neighbor list (ELL, column-major: entry k of particle i at k*npad+i)
+    // ---- full neighbor list: entry k of particle i at row_addr(i, k, npad, lg)
     DBuf<uint32_t> nbr, nbr_old;
     DBuf<uint32_t> nnb, nnb_old, nhalf;
     int64_t npad_old = 0;
@@ -224,7 +236,8 @@ struct Ctx {
     // ---- run state
     DBuf<Status> status;
     Status* status_host = nullptr;  // pinned mirror
-    DBuf<double> partial;           // per-block partial sums
+    DBuf<double> partial;           // per-block kinetic-energy partial sums
+    DBuf<double> partial2;          // per-block thermo partial sums
     int64_t step = 0, rebuilds = 0;
     int64_t sort_counter = 0;
     int chunk = 8;
@@ -242,8 +255,9 @@ struct Ctx {
 // ---- launchers implemented in the .hip files
 // scan_sort.hip
 void exclusive_scan(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n);
-void counting_sort(Ctx& c, const uint32_t* key, int64_t n, int64_t ncells, uint32_t* cnt,
-                   uint32_t* start, uint32_t* order, uint32_t* slot_of);
+void counting_sort(Ctx& c, const uint32_t* key, int64_t n, const uint32_t* n_dev, int64_t ncells,
+                   uint32_t* cnt, uint32_t* start, uint32_t* placed, uint32_t* order,
+                   uint32_t* slot_of);
 void fill_u32(Ctx& c, uint32_t* p, uint32_t v, int64_t n);
 void fill_f64(Ctx& c, double* p, double v, int64_t n);
 void fill_f32(Ctx& c, float* p, float v, int64_t n);
@@ -260,9 +274,11 @@ void launch_initial_integration(Ctx& c, int step, bool check_rebuild);
 void launch_final_integration(Ctx& c);
 void launch_initial_rotation(Ctx& c);
 void launch_final_rotation(Ctx& c);
-void launch_nh_pre(Ctx& c, int step, bool check_rebuild);
-void launch_nh_post(Ctx& c);
-void launch_ke(Ctx& c);
+void launch_ke_partials(Ctx& c);
+void launch_nh_finish(Ctx& c, int parity);
+void launch_nh_seed(Ctx& c, int parity);
+void launch_nh_pre_standalone(Ctx& c);
+void launch_nh_post_standalone(Ctx& c);
 void thermo_reduce(Ctx& c, double out[8]);
 
 inline int blocks_for(int64_t n, int threads) {

@@ -2,12 +2,13 @@
 // fused Hertz-Mindlin contact with tangential shear history, plus plane walls
 // and gravity.
 //
-// One thread owns one particle and gathers over its full neighbor row, so
-// forces accumulate in registers and are written once: no atomics, no
-// reverse (ghost -> owner) pass, and bitwise run-to-run determinism. Every
-// pair is evaluated in its owner's frame (the frame the reference half list
-// evaluates it in), so the two particles of a pair see bitwise-equal,
-// opposite forces.
+// A group of G = 2^LG lanes owns one particle and gathers over its full
+// neighbor row (lane t takes entries t, t+G, ...), so forces accumulate in
+// registers, are combined by a fixed shuffle tree and written once: no
+// atomics, no reverse (image -> particle) pass, and bitwise run-to-run
+// determinism. Every pair is evaluated in its owner's frame (the frame the
+// reference half list evaluates it in), so the two particles of a pair see
+// bitwise-equal, opposite pair forces.
 //
 // Precision: R = double reproduces the reference arithmetic (operation order
 // and fused multiply-adds as written there). R = float ("mixed") keeps the
@@ -84,12 +85,34 @@ __device__ inline float sqrt_r<float>(float x) {
     return __builtin_sqrtf(x);
 }
 
-// lj_force (md_lj/src/lib.rs:488-633): for each neighbor in cutoff,
+template <int G, typename T>
+__device__ inline T group_sum(T v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int G>
+__device__ inline uint64_t group_or(uint64_t v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor(uint32_t(v), o, 64);
+        const uint32_t hi = __shfl_xor(uint32_t(v >> 32), o, 64);
+        v |= (uint64_t(hi) << 32) | lo;
+    }
+    return v;
+}
+
+// lj_force (md_lj/src/lib.rs:488-633): for each neighbor inside the cutoff,
 // fpair = r^-2 r^-6 (lj1 r^-6 - lj2), owner gets fma(-fpair, d, f),
-// partner fma(fpair, d, f), d = partner image - owner.
-template <typename R, bool ENERGY>
+// partner fma(fpair, d, f), d = partner image - owner. Pairs outside the
+// cutoff contribute fma(0, d, f) = f, which leaves the sum unchanged.
+template <typename R, bool ENERGY, int LG>
 __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
-    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    constexpr int G = 1 << LG;
+    const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const int64_t i = gt >> LG;
+    const int t = int(gt & (G - 1));
     if (i >= a.n) return;
     if (gated(a)) return;
     const double4 xi = a.pos[i];
@@ -98,40 +121,42 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     R fx = R(0), fy = R(0), fz = R(0);
     double pe = 0.0;
     const R lj1 = R(a.lj1), lj2 = R(a.lj2);
-    const uint32_t* row = a.nbr + i;
-    for (int k = 0; k < cnt; k++) {
-        const uint32_t e = row[int64_t(k) * a.npad];
+    const uint32_t* row = a.nbr + (i << LG) + t;
+    const int64_t stride = a.npad << LG;
+#pragma unroll 4
+    for (int k = t; k < cnt; k += G) {
+        const uint32_t e = row[int64_t(k >> LG) * stride];
         const uint32_t j = entry_idx(e);
         const int code = entry_code(e);
         const double4 xj = a.pos[j];
         const double shx = image_shift(code_sx(code), a.L[0]);
         const double shy = image_shift(code_sy(code), a.L[1]);
         const double shz = image_shift(code_sz(code), a.L[2]);
-        double dx, dy, dz;
         const bool rev = entry_rev(e);
-        if (rev) {
-            dx = (xi.x + shx) - xj.x;
-            dy = (xi.y + shy) - xj.y;
-            dz = (xi.z + shz) - xj.z;
-        } else {
-            dx = (xj.x + shx) - xi.x;
-            dy = (xj.y + shy) - xi.y;
-            dz = (xj.z + shz) - xi.z;
-        }
+        const double ox = rev ? xj.x : xi.x, oy = rev ? xj.y : xi.y, oz = rev ? xj.z : xi.z;
+        const double px = rev ? xi.x : xj.x, py = rev ? xi.y : xj.y, pz = rev ? xi.z : xj.z;
+        const double dx = (px + shx) - ox;
+        const double dy = (py + shy) - oy;
+        const double dz = (pz + shz) - oz;
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        if (r2 >= a.cut2) continue;
+        const bool in = r2 < a.cut2;
         const R r2inv = rcp<R>(R(r2));
         const R r6inv = r2inv * r2inv * r2inv;
         const R fpair = r2inv * r6inv * fma(lj1, r6inv, -lj2);
-        const R s = rev ? fpair : -fpair;
+        const R s = in ? (rev ? fpair : -fpair) : R(0);
         fx = fma(s, R(dx), fx);
         fy = fma(s, R(dy), fy);
         fz = fma(s, R(dz), fz);
-        if (ENERGY) {
+        if (ENERGY && in) {
             const double r6 = double(r6inv);
             pe += 0.5 * (r6 * (a.e12 * r6 - a.e6));
         }
     }
+    fx = group_sum<G>(fx);
+    fy = group_sum<G>(fy);
+    fz = group_sum<G>(fz);
+    if (ENERGY) pe = group_sum<G>(pe);
+    if (t != 0) return;
     if constexpr (sizeof(R) == 8) {
         a.f64[i] = fx;
         a.f64[a.npad + i] = fy;
@@ -200,9 +225,12 @@ __device__ inline void wall_forces(const ForceArgs& a, const double4& p, const d
 
 // hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615) with
 // surface energy, cohesion and rolling / twisting friction all zero
-template <typename R>
+template <typename R, int LG>
 __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
-    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    constexpr int G = 1 << LG;
+    const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const int64_t i = gt >> LG;
+    const int t = int(gt & (G - 1));
     if (i >= a.n) return;
     if (gated(a)) return;
     const double4 pi = a.pos[i];
@@ -222,8 +250,8 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
         const double* eeff_t = a.mat + 2 * nm * nm;
         const double* geff_t = a.mat + 3 * nm * nm;
         const double dt = a.dt;
-        for (int k = 0; k < cnt; k++) {
-            const int64_t slot = int64_t(k) * a.npad + i;
+        for (int k = t; k < cnt; k += G) {
+            const int64_t slot = row_addr(i, k, a.npad, LG);
             const uint32_t e = a.nbr[slot];
             const uint32_t j = entry_idx(e);
             const int code = entry_code(e);
@@ -367,8 +395,16 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
             a.hist[2 * a.hist_cs + slot] = sign * sz;
             mask_out |= 1ull << k;
         }
-        a.cmask[i] = mask_out;
+        fx = group_sum<G>(fx);
+        fy = group_sum<G>(fy);
+        fz = group_sum<G>(fz);
+        tx = group_sum<G>(tx);
+        ty = group_sum<G>(ty);
+        tz = group_sum<G>(tz);
+        mask_out = group_or<G>(mask_out);
+        if (t == 0) a.cmask[i] = mask_out;
     }
+    if (t != 0) return;
     if (a.use_walls) wall_forces<R>(a, pi, vi, a.mass[i], ti, fx, fy, fz);
     if (a.use_gravity) {
         const double m = a.mass[i];
@@ -443,6 +479,30 @@ ForceArgs make_args(Ctx& c, int step) {
     return a;
 }
 
+template <typename R, bool E>
+void launch_lj(Ctx& c, const ForceArgs& a) {
+    const int64_t threads = c.n << c.lg;
+    const int nb = blocks_for(threads, 256);
+    switch (c.lg) {
+        case 0: k_lj<R, E, 0><<<nb, 256, 0, c.stream>>>(a); break;
+        case 1: k_lj<R, E, 1><<<nb, 256, 0, c.stream>>>(a); break;
+        case 3: k_lj<R, E, 3><<<nb, 256, 0, c.stream>>>(a); break;
+        default: k_lj<R, E, 2><<<nb, 256, 0, c.stream>>>(a); break;
+    }
+}
+
+template <typename R>
+void launch_hertz(Ctx& c, const ForceArgs& a) {
+    const int64_t threads = c.n << c.lg;
+    const int nb = blocks_for(threads, 256);
+    switch (c.lg) {
+        case 0: k_hertz<R, 0><<<nb, 256, 0, c.stream>>>(a); break;
+        case 1: k_hertz<R, 1><<<nb, 256, 0, c.stream>>>(a); break;
+        case 3: k_hertz<R, 3><<<nb, 256, 0, c.stream>>>(a); break;
+        default: k_hertz<R, 2><<<nb, 256, 0, c.stream>>>(a); break;
+    }
+}
+
 }  // namespace
 
 void upload_materials(Ctx& c) {
@@ -461,7 +521,6 @@ void upload_materials(Ctx& c) {
 // one pair-force pass (Force phase). step < 0: ungated standalone call.
 void launch_pair_forces(Ctx& c, int step, bool energy) {
     ForceArgs a = make_args(c, step);
-    const int nb = blocks_for(c.n, 256);
     const bool mixed = c.precision == 1;
     if (step < 0) {
         k_reset_overlap<<<1, 1, 0, c.stream>>>(c.status.p);
@@ -479,23 +538,23 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
             c.pe.resize(c.npad);
             a.pe = c.pe.p;
             if (mixed)
-                k_lj<float, true><<<nb, 256, 0, c.stream>>>(a);
+                launch_lj<float, true>(c, a);
             else
-                k_lj<double, true><<<nb, 256, 0, c.stream>>>(a);
+                launch_lj<double, true>(c, a);
         } else {
             if (mixed)
-                k_lj<float, false><<<nb, 256, 0, c.stream>>>(a);
+                launch_lj<float, false>(c, a);
             else
-                k_lj<double, false><<<nb, 256, 0, c.stream>>>(a);
+                launch_lj<double, false>(c, a);
         }
     } else {
         a.use_pairs = 1;
         a.use_walls = step >= 0 && c.walls.count > 0;
         a.use_gravity = step >= 0 && c.gravity_on;
         if (mixed)
-            k_hertz<float><<<nb, 256, 0, c.stream>>>(a);
+            launch_hertz<float>(c, a);
         else
-            k_hertz<double><<<nb, 256, 0, c.stream>>>(a);
+            launch_hertz<double>(c, a);
     }
     MDD_HIP(hipGetLastError());
     c.force_launches++;

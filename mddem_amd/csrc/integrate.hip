@@ -10,9 +10,16 @@
 // Every later kernel of the step sequence then returns at once, the host
 // rebuilds the neighbor list and resumes with the force pass of that step.
 //
-// The Nose-Hoover kinetic energy is reduced deterministically: fixed-order
-// tree sums per block, then the last block to finish sums the block partials
-// in index order and advances the thermostat.
+// Nose-Hoover without a grid-wide barrier: the post kernel of step s leaves
+// per-block kinetic-energy partials (a fixed number of blocks, grid-stride),
+// and every block of the pre kernel of step s+1 sums all of them in the same
+// fixed order, so each block derives the bitwise-identical thermostat update
+// by itself. The reference computes that same KE twice (end of step s and
+// start of step s+1, md_thermostat/src/lib.rs:262-265 and 345-352) and
+// advances p_xi by two equal terms; both are applied here. The thermostat
+// momentum lives in two slots indexed by step parity: the pre kernel of step
+// s reads slot (s-1)&1 and its block 0 writes slot s&1, so no block reads a
+// value another block of the same launch is writing.
 #include "ctx.hpp"
 
 namespace mdd {
@@ -20,6 +27,7 @@ namespace mdd {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kMaxPartials = 1024;  // blocks of a thermostatted post kernel
 
 struct IntArgs {
     int64_t n, npad;
@@ -35,7 +43,7 @@ struct IntArgs {
     const double* t64;
     const float* t32;
     int rot;         // apply rotation half-steps
-    int vv;          // apply translational half-steps
+    int vv;          // apply Verlet translational half-steps
     int check;       // evaluate the rebuild decision
     double thr2;     // ((skin_fraction - 1) * min_skin)^2
     BoxP box;
@@ -43,6 +51,8 @@ struct IntArgs {
     Status* st;
     int step;
     double* partial;
+    int nparts;      // KE partials left by the last post kernel
+    int q_in, q_out; // thermostat slots read / written
     double ndof, temp, q_mass;
 };
 
@@ -70,6 +80,31 @@ __device__ inline void load_t(const IntArgs& a, int64_t i, double& tx, double& t
         ty = double(a.t32[a.npad + i]);
         tz = double(a.t32[2 * a.npad + i]);
     }
+}
+
+// Sum over the 256 threads of a block in a fixed order; every thread gets
+// the same bits (xor butterfly inside each wavefront: both partners add the
+// same two operands; then the four wave sums in fixed order).
+__device__ inline double block_sum(double v, double* lds4) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    return (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+}
+
+// total of the KE partials (sum m v^2), identical in every block
+__device__ inline double sum_partials(const IntArgs& a, double* lds4) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < a.nparts; b += kThreads) s += a.partial[b];
+    return block_sum(s, lds4);
+}
+
+// p_xi half update (md_thermostat/src/lib.rs:266 and 352)
+__device__ inline double nh_term(const IntArgs& a, double ke) {
+    return (a.dt / 2.0) * (2.0 * ke - a.ndof * a.temp);
 }
 
 // displacement_exceeded (neighbor.rs:564-601) + periodic box exit (651-666)
@@ -135,33 +170,55 @@ __device__ inline void rotate_final(const IntArgs& a, int64_t i, double tx, doub
     a.omega[i] = w;
 }
 
-// PreInitialIntegration + InitialIntegration of one step:
-// NH rescale (if nh), velocity half kick, drift, rotation, rebuild decision
+__device__ inline void kick_drift(const IntArgs& a, double4& v, double4& p, double fx, double fy,
+                                  double fz) {
+    const double dt = a.dt;
+    const double h = 0.5 * dt * v.w;
+    v.x += h * fx;
+    v.y += h * fy;
+    v.z += h * fz;
+    p.x += v.x * dt;
+    p.y += v.y * dt;
+    p.z += v.z * dt;
+}
+
+// PreInitialIntegration + InitialIntegration of one step: thermostat
+// (nh_pre_initial: p_xi update, rescale, kick, drift), Verlet kick + drift,
+// rotation, then the rebuild decision.
 template <typename F, bool NH>
 __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
+    __shared__ double lds4[4];
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    // uniform over the launch: only earlier kernels lower abort_step below step
     if (a.step >= 0 && a.st->abort_step < a.step) return;
     if (i == 0) a.st->overlap_count = 0;
+    double scale = 1.0;
+    if (NH) {
+        const double ke = 0.5 * sum_partials(a, lds4);
+        const double term = nh_term(a, ke);
+        double p = a.st->nh_p[a.q_in];
+        if (a.st->nh_kind[a.q_in] == 0) p += term;  // post half of the previous step
+        p += term;                                  // this step's pre half
+        scale = exp(-a.dt / 2.0 * p / a.q_mass);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.st->nh_p[a.q_out] = p;
+            a.st->nh_scale[a.q_out] = scale;
+            a.st->nh_kind[a.q_out] = 0;
+        }
+    }
     if (i >= a.n) return;
     double4 v = a.vel[i];
     double4 p = a.pos[i];
-    const double dt = a.dt;
-    if (NH) {
-        const double scale = a.st->nh_scale;
-        v.x *= scale;
-        v.y *= scale;
-        v.z *= scale;
-    }
     if (NH || a.vv) {
         double fx, fy, fz;
         load_f<F>(a, i, fx, fy, fz);
-        const double h = 0.5 * dt * v.w;
-        v.x += h * fx;
-        v.y += h * fy;
-        v.z += h * fz;
-        p.x += v.x * dt;
-        p.y += v.y * dt;
-        p.z += v.z * dt;
+        if (NH) {
+            v.x *= scale;
+            v.y *= scale;
+            v.z *= scale;
+            kick_drift(a, v, p, fx, fy, fz);
+        }
+        if (a.vv) kick_drift(a, v, p, fx, fy, fz);
         a.vel[i] = v;
         a.pos[i] = p;
     }
@@ -173,122 +230,96 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
     if (a.check) rebuild_check(a, i, p);
 }
 
-// deterministic block sum of one double (256 threads)
-__device__ inline double block_sum(double v, double* lds) {
-    lds[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
-        __syncthreads();
-    }
-    double r = lds[0];
-    __syncthreads();
-    return r;
-}
-
-// FinalIntegration + PostFinalIntegration of one step: half kick, rotation,
-// NH rescale with this step's factor, kinetic energy and the thermostat
-// updates (post half-step now, next step's pre half-step ahead of time: the
-// reference recomputes the same KE there, md_thermostat/src/lib.rs:262-265)
+// FinalIntegration + PostFinalIntegration of one step: Verlet kick,
+// rotation, thermostat kick + rescale with this step's factor and the
+// per-block kinetic-energy partials (grid-stride over a fixed block count).
 template <typename F, bool NH>
 __global__ void __launch_bounds__(kThreads) k_post(IntArgs a) {
-    __shared__ double lds[kThreads];
-    __shared__ bool last;
-    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    __shared__ double lds4[4];
     if (a.step >= 0 && a.st->abort_step <= a.step) return;
+    const double scale = NH ? a.st->nh_scale[a.q_in] : 1.0;
     double ke = 0.0;
-    if (i < a.n) {
+    const int64_t stride = int64_t(gridDim.x) * kThreads;
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < a.n; i += stride) {
         double4 v = a.vel[i];
         if (NH || a.vv) {
             double fx, fy, fz;
             load_f<F>(a, i, fx, fy, fz);
             const double h = 0.5 * a.dt * v.w;
-            v.x += h * fx;
-            v.y += h * fy;
-            v.z += h * fz;
+            if (a.vv) {
+                v.x += h * fx;
+                v.y += h * fy;
+                v.z += h * fz;
+            }
+            if (NH) {
+                v.x += h * fx;
+                v.y += h * fy;
+                v.z += h * fz;
+                v.x *= scale;
+                v.y *= scale;
+                v.z *= scale;
+            }
+            a.vel[i] = v;
         }
-        if (NH) {
-            const double scale = a.st->nh_scale;
-            v.x *= scale;
-            v.y *= scale;
-            v.z *= scale;
-        }
-        if (NH || a.vv) a.vel[i] = v;
         if (a.rot) {
             double tx, ty, tz;
             load_t<F>(a, i, tx, ty, tz);
             rotate_final(a, i, tx, ty, tz);
         }
-        if (NH) ke = a.mass[i] * (v.x * v.x + v.y * v.y + v.z * v.z);
+        if (NH) ke += a.mass[i] * (v.x * v.x + v.y * v.y + v.z * v.z);
     }
     if (!NH) return;
-    const double bs = block_sum(ke, lds);
-    if (threadIdx.x == 0) {
-        a.partial[blockIdx.x] = bs;
-        __threadfence();
-        const unsigned t = atomicAdd(&a.st->ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double s = 0.0;
-    for (int64_t b = threadIdx.x; b < gridDim.x; b += kThreads) s += a.partial[b];
-    const double tot = block_sum(s, lds);
-    if (threadIdx.x == 0) {
-        const double KE = 0.5 * tot;
-        const double dt = a.dt;
-        const double p_post = a.st->p_xi_pre + (dt / 2.0) * (2.0 * KE - a.ndof * a.temp);
-        const double p_next = p_post + (dt / 2.0) * (2.0 * KE - a.ndof * a.temp);
-        a.st->ke = KE;
-        a.st->p_xi = p_post;
-        a.st->p_xi_pre = p_next;
-        a.st->nh_scale = exp(-dt / 2.0 * p_next / a.q_mass);
-        a.st->ticket = 0u;
-    }
+    const double bs = block_sum(ke, lds4);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = bs;
 }
 
-// kinetic energy of the current velocities -> Status (setup_nose_hoover and
-// the first nh_pre_initial, md_thermostat/src/lib.rs:195-265)
-__global__ void __launch_bounds__(kThreads) k_nh_init(IntArgs a) {
-    __shared__ double lds[kThreads];
-    __shared__ bool last;
-    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+// KE partials of the current velocities, laid out like the post kernel's
+__global__ void __launch_bounds__(kThreads) k_ke_partials(IntArgs a) {
+    __shared__ double lds4[4];
     double ke = 0.0;
-    if (i < a.n) {
+    const int64_t stride = int64_t(gridDim.x) * kThreads;
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < a.n; i += stride) {
         const double4 v = a.vel[i];
-        ke = a.mass[i] * (v.x * v.x + v.y * v.y + v.z * v.z);
+        ke += a.mass[i] * (v.x * v.x + v.y * v.y + v.z * v.z);
     }
-    const double bs = block_sum(ke, lds);
-    if (threadIdx.x == 0) {
-        a.partial[blockIdx.x] = bs;
-        __threadfence();
-        const unsigned t = atomicAdd(&a.st->ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double s = 0.0;
-    for (int64_t b = threadIdx.x; b < gridDim.x; b += kThreads) s += a.partial[b];
-    const double tot = block_sum(s, lds);
-    if (threadIdx.x == 0) {
-        const double KE = 0.5 * tot;
-        const double dt = a.dt;
-        const double p = a.st->p_xi;
-        const double p_next = p + (dt / 2.0) * (2.0 * KE - a.ndof * a.temp);
-        a.st->ke = KE;
-        a.st->p_xi_pre = p_next;
-        a.st->nh_scale = exp(-dt / 2.0 * p_next / a.q_mass);
-        a.st->ticket = 0u;
-    }
+    const double bs = block_sum(ke, lds4);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = bs;
+}
+
+// slot q <- Status::p_xi with the matching velocity scale. kind 1: a
+// post-half value (the reference's p_xi between steps; a following pre
+// kernel adds one KE term); kind 0: a pre-half value (a following finish
+// adds the post-half term)
+__global__ void k_nh_seed(IntArgs a, int q, int kind) {
+    const double p = a.st->p_xi;
+    a.st->nh_p[q] = p;
+    a.st->nh_kind[q] = kind;
+    a.st->nh_scale[q] = exp(-a.dt / 2.0 * p / a.q_mass);
+}
+
+// Status::p_xi <- slot q (after a standalone pre half-step)
+__global__ void k_nh_pxi_from_slot(Status* st, int q) { st->p_xi = st->nh_p[q]; }
+
+// finish the post half-update of slot q from the KE partials: Status::ke and
+// Status::p_xi hold the reference's values between steps afterwards, and the
+// slot becomes a post-half value (a following pre kernel adds one term)
+__global__ void __launch_bounds__(kThreads) k_nh_finish(IntArgs a, int q) {
+    __shared__ double lds4[4];
+    const double ke = 0.5 * sum_partials(a, lds4);
+    if (threadIdx.x != 0) return;
+    double p = a.st->nh_p[q];
+    if (a.st->nh_kind[q] == 0) p += nh_term(a, ke);
+    a.st->nh_p[q] = p;
+    a.st->nh_kind[q] = 1;
+    a.st->ke = ke;
+    a.st->p_xi = p;
 }
 
 // per-block partial sums of the thermo quantities: sum m v^2, sum m v, sum m,
 // and (second pass, given vbar) sum m |v - vbar|^2
 __global__ void __launch_bounds__(kThreads) k_thermo(IntArgs a, double vbx, double vby,
                                                      double vbz, double* out) {
-    __shared__ double lds[kThreads];
+    __shared__ double lds4[4];
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     double q[6] = {0, 0, 0, 0, 0, 0};
     if (i < a.n) {
@@ -303,9 +334,14 @@ __global__ void __launch_bounds__(kThreads) k_thermo(IntArgs a, double vbx, doub
         q[5] = m * (ux * ux + uy * uy + uz * uz);
     }
     for (int k = 0; k < 6; k++) {
-        const double s = block_sum(q[k], lds);
+        const double s = block_sum(q[k], lds4);
         if (threadIdx.x == 0) out[blockIdx.x * 6 + k] = s;
     }
+}
+
+int post_blocks(const Ctx& c, bool nh) {
+    const int nb = blocks_for(c.n, kThreads);
+    return nh ? std::min(nb, kMaxPartials) : nb;
 }
 
 IntArgs make_args(Ctx& c, int step, bool check) {
@@ -332,8 +368,11 @@ IntArgs make_args(Ctx& c, int step, bool check) {
     a.box_exit = c.box_exit_rebuild;
     a.st = c.status.p;
     a.step = step;
-    c.partial.resize(size_t(blocks_for(c.n, kThreads)) * 6 + 64);
+    c.partial.resize(kMaxPartials);
     a.partial = c.partial.p;
+    a.nparts = post_blocks(c, true);
+    a.q_in = step >= 0 ? ((step - 1) & 1) : 0;
+    a.q_out = step >= 0 ? (step & 1) : 1;
     a.ndof = c.nh_ndof;
     a.temp = c.nh_T;
     a.q_mass = c.nh_Q;
@@ -352,7 +391,7 @@ void pre_impl(Ctx& c, IntArgs& a) {
 
 template <bool NH>
 void post_impl(Ctx& c, IntArgs& a) {
-    const int nb = blocks_for(c.n, kThreads);
+    const int nb = post_blocks(c, NH);
     if (c.precision == 1)
         k_post<float, NH><<<nb, kThreads, 0, c.stream>>>(a);
     else
@@ -373,10 +412,12 @@ void launch_pre(Ctx& c, int step, bool check_rebuild) {
         pre_impl<false>(c, a);
 }
 
+// scheduled post kernel of a step; its thermostat half uses slot step & 1
 void launch_post(Ctx& c, int step) {
     IntArgs a = make_args(c, step, false);
     a.rot = c.rot_on && c.model == 1;
     a.vv = c.vv_on;
+    a.q_in = step & 1;
     if (c.nh_on)
         post_impl<true>(c, a);
     else
@@ -407,28 +448,52 @@ void launch_final_rotation(Ctx& c) {
     post_impl<false>(c, a);
 }
 
-void launch_nh_pre(Ctx& c, int step, bool check_rebuild) {
-    IntArgs a = make_args(c, step, check_rebuild);
-    pre_impl<true>(c, a);
-}
-
-void launch_nh_post(Ctx& c) {
+// KE partials of the current velocities (thermostat start)
+void launch_ke_partials(Ctx& c) {
     IntArgs a = make_args(c, -1, false);
-    post_impl<true>(c, a);
-}
-
-// kinetic energy of the current velocities; seeds the thermostat's next
-// pre half-step from Status::p_xi
-void launch_ke(Ctx& c) {
-    IntArgs a = make_args(c, -1, false);
-    k_nh_init<<<blocks_for(c.n, kThreads), kThreads, 0, c.stream>>>(a);
+    k_ke_partials<<<post_blocks(c, true), kThreads, 0, c.stream>>>(a);
     MDD_HIP(hipGetLastError());
+}
+
+void launch_nh_seed(Ctx& c, int parity) {
+    IntArgs a = make_args(c, -1, false);
+    k_nh_seed<<<1, 1, 0, c.stream>>>(a, parity & 1, 1);
+    MDD_HIP(hipGetLastError());
+}
+
+void launch_nh_finish(Ctx& c, int parity) {
+    IntArgs a = make_args(c, -1, false);
+    k_nh_finish<<<1, kThreads, 0, c.stream>>>(a, parity & 1);
+    MDD_HIP(hipGetLastError());
+}
+
+// standalone nh_pre_initial: p_xi from Status, KE of the current velocities
+void launch_nh_pre_standalone(Ctx& c) {
+    launch_ke_partials(c);
+    launch_nh_seed(c, 0);
+    IntArgs a = make_args(c, -1, false);
+    a.q_in = 0;
+    a.q_out = 1;
+    pre_impl<true>(c, a);
+    k_nh_pxi_from_slot<<<1, 1, 0, c.stream>>>(c.status.p, 1);
+    MDD_HIP(hipGetLastError());
+}
+
+// standalone nh_post_final: scale from Status::p_xi, then the KE update
+void launch_nh_post_standalone(Ctx& c) {
+    IntArgs a = make_args(c, -1, false);
+    k_nh_seed<<<1, 1, 0, c.stream>>>(a, 0, 0);
+    MDD_HIP(hipGetLastError());
+    a.q_in = 0;
+    post_impl<true>(c, a);
+    launch_nh_finish(c, 0);
 }
 
 // out: [sum m v^2, sum m vx, sum m vy, sum m vz, sum m, sum m |v - vbar|^2]
 void thermo_reduce(Ctx& c, double out[8]) {
     IntArgs a = make_args(c, -1, false);
     const int nb = blocks_for(c.n, kThreads);
+    c.partial2.resize(size_t(nb) * 6);
     std::vector<double> h(size_t(nb) * 6);
     double s[6] = {0, 0, 0, 0, 0, 0};
     for (int pass = 0; pass < 2; pass++) {
@@ -438,9 +503,9 @@ void thermo_reduce(Ctx& c, double out[8]) {
             vb[1] = s[2] / s[4];
             vb[2] = s[3] / s[4];
         }
-        k_thermo<<<nb, kThreads, 0, c.stream>>>(a, vb[0], vb[1], vb[2], c.partial.p);
+        k_thermo<<<nb, kThreads, 0, c.stream>>>(a, vb[0], vb[1], vb[2], c.partial2.p);
         MDD_HIP(hipGetLastError());
-        MDD_HIP(hipMemcpyAsync(h.data(), c.partial.p, sizeof(double) * h.size(),
+        MDD_HIP(hipMemcpyAsync(h.data(), c.partial2.p, sizeof(double) * h.size(),
                                hipMemcpyDeviceToHost, c.stream));
         MDD_HIP(hipStreamSynchronize(c.stream));
         double t[6] = {0, 0, 0, 0, 0, 0};

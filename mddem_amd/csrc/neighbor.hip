@@ -6,20 +6,28 @@
 //   sort_atoms_by_bin (crates/mddem_core/src/neighbor.rs:686-794), stable
 //   borders self-sends (crates/mddem_core/src/comm.rs:449-570) as periodic images
 //   bin_neighbor_list (crates/mddem_core/src/neighbor.rs:810-1080)
-// and then derives, for every particle, a full row: the pairs it owns in the
-// reference half list (same bin grid, same candidate order, same FP64 fma
-// distance test) followed by the pairs that other particles own with it. A
-// particle finds the latter by scanning its backward stencil cells and the
-// cells around each of its periodic images, applying the owner's distance
-// test bit for bit, so no atomics or transposition are needed and every row
-// comes out in a deterministic order.
+// on the reference's bin grid, so the reference half list (same owners,
+// same FP64 fma distance test) comes out bit for bit. Instead of storing
+// only the half list, every particle gets a full row: the pairs it owns
+// plus the pairs other particles own with it, each entry flagged, so the
+// force kernels gather without atomics. A row is built by a group of lanes
+// that scans the particle's stencil cells once: a candidate in a forward
+// cell (or later in the same cell) is owned by the particle; a particle (not
+// a periodic image) in a backward cell (or earlier in the same cell) owns
+// the pair instead, and for two particles the distance test is symmetric
+// bit for bit. Pairs owned through the particle's own periodic images come
+// from a second scan around each image. Passing candidates are compacted
+// with a wavefront ballot, in a fixed order, so rows are deterministic.
+#include <algorithm>
+#include <cmath>
+
 #include "ctx.hpp"
 
 namespace mdd {
 
-void counting_sort_dev(Ctx& c, const uint32_t* key, const uint32_t* n_dev, int64_t n_max,
-                       int64_t ncells, uint32_t* cnt, uint32_t* start, uint32_t* order,
-                       uint32_t* slot_of);
+void counting_sort(Ctx& c, const uint32_t* key, int64_t n, const uint32_t* n_dev, int64_t ncells,
+                   uint32_t* cnt, uint32_t* start, uint32_t* placed, uint32_t* order,
+                   uint32_t* slot_of);
 
 namespace {
 
@@ -145,89 +153,92 @@ __global__ void k_ghost_count(int64_t n, const double4* __restrict__ pos, BoxP b
     gcnt[i] = uint32_t((1 + lf[0] + hf[0]) * (1 + lf[1] + hf[1]) * (1 + lf[2] + hf[2]) - 1);
 }
 
+// periodic images of every particle (x shifted by exactly +-L per axis);
+// when they do not all fit, none is written and the live count is 0
 __global__ void k_ghost_emit(int64_t n, const double4* __restrict__ pos, BoxP box, double gc,
-                             const uint32_t* __restrict__ goff, int64_t ghost_cap,
+                             GridP grid, const uint32_t* __restrict__ goff, int64_t ghost_cap,
                              uint32_t* __restrict__ groot, uint8_t* __restrict__ gcode,
-                             double4* __restrict__ gpos, Status* st) {
+                             double4* __restrict__ gpos, uint32_t* __restrict__ gkey,
+                             uint32_t* ng_live, Status* st) {
     int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    uint32_t total = goff[n];
-    if (i == 0) st->ghost_total = total;
-    if (int64_t(total) > ghost_cap) {
-        if (i == 0) atomicOr(&st->overflow, 1u);
-        return;
+    const uint32_t total = goff[n];
+    const bool fits = int64_t(total) <= ghost_cap;
+    if (i == 0) {
+        st->ghost_total = total;
+        *ng_live = fits ? total : 0u;
+        if (!fits) atomicOr(&st->overflow, 1u);
     }
-    if (i >= n) return;
+    if (!fits || i >= n) return;
     double4 p = pos[i];
     double x[3] = {p.x, p.y, p.z};
     int lf[3], hf[3];
     image_flags(box, gc, x, lf, hf);
     uint32_t g = goff[i];
-    int sxs[3] = {0, 1, -1};
+    const int sv[3] = {0, 1, -1};
     for (int a = 0; a < 3; a++) {
-        int sx = sxs[a];
+        const int sx = sv[a];
         if ((sx == 1 && !lf[0]) || (sx == -1 && !hf[0])) continue;
         for (int b = 0; b < 3; b++) {
-            int sy = sxs[b];
+            const int sy = sv[b];
             if ((sy == 1 && !lf[1]) || (sy == -1 && !hf[1])) continue;
-            for (int cidx = 0; cidx < 3; cidx++) {
-                int sz = sxs[cidx];
+            for (int cz = 0; cz < 3; cz++) {
+                const int sz = sv[cz];
                 if ((sz == 1 && !lf[2]) || (sz == -1 && !hf[2])) continue;
                 if (sx == 0 && sy == 0 && sz == 0) continue;
-                int code = code_of(sx, sy, sz);
+                const double gx = x[0] + image_shift(sx, box.L[0]);
+                const double gy = x[1] + image_shift(sy, box.L[1]);
+                const double gz = x[2] + image_shift(sz, box.L[2]);
                 groot[g] = uint32_t(i);
-                gcode[g] = uint8_t(code);
-                gpos[g] = make_double4(x[0] + image_shift(sx, box.L[0]),
-                                       x[1] + image_shift(sy, box.L[1]),
-                                       x[2] + image_shift(sz, box.L[2]), 0.0);
+                gcode[g] = uint8_t(code_of(sx, sy, sz));
+                gpos[g] = make_double4(gx, gy, gz, 0.0);
+                gkey[g] = cell_of(grid, gx, gy, gz);
                 g++;
             }
         }
     }
 }
 
-// bin keys of particles (already sorted) followed by their periodic images
-__global__ void k_all_keys(int64_t n, int64_t ghost_cap, const uint32_t* __restrict__ key_local,
-                           const double4* __restrict__ gpos, const uint32_t* __restrict__ goff,
-                           GridP g, uint32_t* __restrict__ all_key, uint32_t* n_all) {
-    int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    int64_t ng = goff[n];
-    if (ng > ghost_cap) ng = ghost_cap;
-    if (t == 0) *n_all = uint32_t(n + ng);
-    if (t < n) {
-        all_key[t] = key_local[t];
-    } else if (t < n + ng) {
-        double4 q = gpos[t - n];
-        all_key[t] = cell_of(g, q.x, q.y, q.z);
-    }
+// combined bin starts: particles of a cell first (already sorted), then images
+__global__ void k_bin_start(int64_t ncells, const uint32_t* __restrict__ lstart,
+                            const uint32_t* __restrict__ gstart, uint32_t* __restrict__ bstart) {
+    int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (c <= ncells) bstart[c] = lstart[c] + gstart[c];
 }
 
-// binned copies of position / neighbor skin radius and the pre-encoded entry
-__global__ void k_bin_fill(int64_t n, const uint32_t* __restrict__ n_all,
-                           const uint32_t* __restrict__ order, const double4* __restrict__ pos,
-                           const double* __restrict__ cutr, const double4* __restrict__ gpos,
-                           const uint32_t* __restrict__ groot,
+// binned copies of position / neighbor skin radius and the encoded entry
+__global__ void k_bin_fill(int64_t n, int64_t ghost_cap, const uint32_t* __restrict__ ng_live,
+                           const uint32_t* __restrict__ key, const uint32_t* __restrict__ lstart,
+                           const uint32_t* __restrict__ gstart,
+                           const uint32_t* __restrict__ bstart,
+                           const uint32_t* __restrict__ gorder, const uint32_t* __restrict__ gkey,
+                           const double4* __restrict__ pos, const double* __restrict__ cutr,
+                           const double4* __restrict__ gpos, const uint32_t* __restrict__ groot,
                            const uint8_t* __restrict__ gcode, double4* __restrict__ bin_pos,
                            uint32_t* __restrict__ bin_entry, uint32_t* __restrict__ slot_local) {
-    int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (s >= int64_t(*n_all)) return;
-    uint32_t a = order[s];
-    if (a < n) {
-        double4 p = pos[a];
-        bin_pos[s] = make_double4(p.x, p.y, p.z, cutr[a]);
-        bin_entry[s] = enc_entry(a, kCodeNone, false);
-        slot_local[a] = uint32_t(s);
-    } else {
-        uint32_t gi = a - uint32_t(n);
-        double4 q = gpos[gi];
-        uint32_t r = groot[gi];
-        bin_pos[s] = make_double4(q.x, q.y, q.z, cutr[r]);
-        bin_entry[s] = enc_entry(r, gcode[gi], false);
+    const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (t < n) {
+        const uint32_t c = key[t];
+        const uint32_t s = bstart[c] + uint32_t(t - lstart[c]);
+        const double4 p = pos[t];
+        bin_pos[s] = make_double4(p.x, p.y, p.z, cutr[t]);
+        bin_entry[s] = enc_entry(uint32_t(t), kCodeNone, false);
+        slot_local[t] = s;
+        return;
     }
+    const int64_t gs = t - n;
+    if (gs >= int64_t(*ng_live) || gs >= ghost_cap) return;
+    const uint32_t g = gorder[gs];
+    const uint32_t c = gkey[g];
+    const uint32_t s = bstart[c] + (lstart[c + 1] - lstart[c]) + (uint32_t(gs) - gstart[c]);
+    const double4 q = gpos[g];
+    const uint32_t r = groot[g];
+    bin_pos[s] = make_double4(q.x, q.y, q.z, cutr[r]);
+    bin_entry[s] = enc_entry(r, gcode[g], false);
 }
 
 struct ListArgs {
     int64_t n, npad;
-    int maxnb;
+    int maxnb, lg;
     const double4* bin_pos;
     const uint32_t* bin_entry;
     const uint32_t* bin_start;
@@ -235,6 +246,8 @@ struct ListArgs {
     const uint32_t* key;
     const int* fwd;
     int nfwd;
+    const int* full;
+    int nfull;
     BoxP box;
     GridP grid;
     double gc;
@@ -257,102 +270,110 @@ __device__ inline bool pair_in(const ListArgs& a, double dx, double dy, double d
     return r2 < sum * sum * a.sf2;
 }
 
-__global__ void __launch_bounds__(256) k_build_list(ListArgs a) {
-    int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (p >= a.n) return;
+template <int LGB>
+__global__ void __launch_bounds__(256) k_build_rows(ListArgs a) {
+    constexpr int GB = 1 << LGB;
+    const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const int64_t p = gt >> LGB;
+    const uint32_t t = uint32_t(gt & (GB - 1));
+    const int lane = threadIdx.x & 63;
+    const uint64_t gmask =
+        GB == 64 ? ~0ull : (((1ull << GB) - 1ull) << (lane & ~(GB - 1)));
+    const uint64_t below = gmask & ((1ull << lane) - 1ull);
+    if (p >= a.n) return;  // whole group: p is the same on its lanes
     const uint32_t slot = a.slot_local[p];
     const uint32_t c = a.key[p];
     const double4 xp = a.bin_pos[slot];
     const double sp = xp.w;
-    int cnt = 0;
-    auto push = [&](uint32_t e) {
-        if (cnt < a.maxnb) a.nbr[int64_t(cnt) * a.npad + p] = e;
-        cnt++;
-    };
-    // (1) pairs this particle owns: rest of its own cell, then forward cells
-    {
-        const uint32_t end = a.bin_start[c + 1];
-        for (uint32_t m = slot + 1; m < end; m++) {
-            double4 q = a.bin_pos[m];
-            if (pair_in(a, q.x - xp.x, q.y - xp.y, q.z - xp.z, sp, q.w)) push(a.bin_entry[m]);
-        }
-        for (int k = 0; k < a.nfwd; k++) {
-            const int64_t cc = int64_t(c) + a.fwd[k];
-            if (cc < 0 || cc >= a.grid.ncells) continue;  // only off-grid (clamped) cells
-            const uint32_t b = a.bin_start[cc], e = a.bin_start[cc + 1];
-            for (uint32_t m = b; m < e; m++) {
-                double4 q = a.bin_pos[m];
-                if (pair_in(a, q.x - xp.x, q.y - xp.y, q.z - xp.z, sp, q.w))
-                    push(a.bin_entry[m]);
+    int cnt = 0, nh = 0;
+    // the particle's own stencil cells: owned pairs and pairs owned by
+    // other particles (not images) with it
+    for (int s = 0; s < a.nfull; s++) {
+        const int off = a.full[s];
+        const int64_t cc = int64_t(c) + off;
+        if (cc < 0 || cc >= a.grid.ncells) continue;
+        const uint32_t b = a.bin_start[cc], e = a.bin_start[cc + 1];
+        for (uint32_t m0 = b; m0 < e; m0 += GB) {
+            const uint32_t m = m0 + t;
+            bool pass = false, own = false;
+            uint32_t ent = 0u;
+            if (m < e && m != slot) {
+                ent = a.bin_entry[m];
+                own = off > 0 || (off == 0 && m > slot);
+                if (own || entry_code(ent) == kCodeNone) {
+                    const double4 q = a.bin_pos[m];
+                    pass = pair_in(a, q.x - xp.x, q.y - xp.y, q.z - xp.z, sp, q.w);
+                }
             }
+            const uint64_t bal = __ballot(pass) & gmask;
+            const uint64_t obal = __ballot(pass && own) & gmask;
+            if (pass) {
+                const int k = cnt + __popcll(bal & below);
+                if (k < a.maxnb) a.nbr[row_addr(p, k, a.npad, a.lg)] = own ? ent : (ent | kRevBit);
+            }
+            cnt += __popcll(bal);
+            nh += __popcll(obal);
         }
     }
-    const int nh = cnt;
-    // (2) pairs owned by other particles with this particle itself: earlier
-    //     slots of its cell and particles (not images) of backward cells
-    {
-        for (uint32_t m = a.bin_start[c]; m < slot; m++) {
-            double4 q = a.bin_pos[m];
-            if (pair_in(a, xp.x - q.x, xp.y - q.y, xp.z - q.z, q.w, sp))
-                push(a.bin_entry[m] | kRevBit);
-        }
-        for (int k = 0; k < a.nfwd; k++) {
-            const int64_t cc = int64_t(c) - a.fwd[k];
-            if (cc < 0 || cc >= a.grid.ncells) continue;
-            const uint32_t b = a.bin_start[cc], e = a.bin_start[cc + 1];
-            for (uint32_t m = b; m < e; m++) {
-                uint32_t ent = a.bin_entry[m];
-                if (entry_code(ent) != kCodeNone) continue;
-                double4 q = a.bin_pos[m];
-                if (pair_in(a, xp.x - q.x, xp.y - q.y, xp.z - q.z, q.w, sp))
-                    push(ent | kRevBit);
-            }
-        }
-    }
-    // (3) pairs owned by particles near one of this particle's periodic images
-    {
-        double x[3] = {xp.x, xp.y, xp.z};
-        int lf[3], hf[3];
-        image_flags(a.box, a.gc, x, lf, hf);
-        if (lf[0] | hf[0] | lf[1] | hf[1] | lf[2] | hf[2]) {
-            const int sv[3] = {0, 1, -1};
-            for (int ia = 0; ia < 3; ia++) {
-                int sx = sv[ia];
-                if ((sx == 1 && !lf[0]) || (sx == -1 && !hf[0])) continue;
-                for (int ib = 0; ib < 3; ib++) {
-                    int sy = sv[ib];
-                    if ((sy == 1 && !lf[1]) || (sy == -1 && !hf[1])) continue;
-                    for (int ic = 0; ic < 3; ic++) {
-                        int sz = sv[ic];
-                        if ((sz == 1 && !lf[2]) || (sz == -1 && !hf[2])) continue;
-                        if (sx == 0 && sy == 0 && sz == 0) continue;
-                        const int code = code_of(sx, sy, sz);
-                        const double gx = x[0] + image_shift(sx, a.box.L[0]);
-                        const double gy = x[1] + image_shift(sy, a.box.L[1]);
-                        const double gz = x[2] + image_shift(sz, a.box.L[2]);
-                        const uint32_t cg = cell_of(a.grid, gx, gy, gz);
-                        for (int k = -1; k < a.nfwd; k++) {
-                            const int64_t cc = k < 0 ? int64_t(cg) : int64_t(cg) - a.fwd[k];
-                            if (cc < 0 || cc >= a.grid.ncells) continue;
-                            const uint32_t b = a.bin_start[cc], e = a.bin_start[cc + 1];
-                            for (uint32_t m = b; m < e; m++) {
-                                uint32_t ent = a.bin_entry[m];
-                                if (entry_code(ent) != kCodeNone) continue;
-                                double4 q = a.bin_pos[m];
-                                if (pair_in(a, gx - q.x, gy - q.y, gz - q.z, q.w, sp))
-                                    push(enc_entry(entry_idx(ent), code, true));
+    // pairs owned by particles near one of this particle's periodic images
+    double x[3] = {xp.x, xp.y, xp.z};
+    int lf[3], hf[3];
+    image_flags(a.box, a.gc, x, lf, hf);
+    if (lf[0] | hf[0] | lf[1] | hf[1] | lf[2] | hf[2]) {
+        const int sv[3] = {0, 1, -1};
+        for (int ia = 0; ia < 3; ia++) {
+            const int sx = sv[ia];
+            if ((sx == 1 && !lf[0]) || (sx == -1 && !hf[0])) continue;
+            for (int ib = 0; ib < 3; ib++) {
+                const int sy = sv[ib];
+                if ((sy == 1 && !lf[1]) || (sy == -1 && !hf[1])) continue;
+                for (int ic = 0; ic < 3; ic++) {
+                    const int sz = sv[ic];
+                    if ((sz == 1 && !lf[2]) || (sz == -1 && !hf[2])) continue;
+                    if (sx == 0 && sy == 0 && sz == 0) continue;
+                    const int code = code_of(sx, sy, sz);
+                    const double gx = x[0] + image_shift(sx, a.box.L[0]);
+                    const double gy = x[1] + image_shift(sy, a.box.L[1]);
+                    const double gz = x[2] + image_shift(sz, a.box.L[2]);
+                    const uint32_t cg = cell_of(a.grid, gx, gy, gz);
+                    // owners: every particle in the image's cell (images sit
+                    // after particles there) and particles of its backward cells
+                    for (int k = -1; k < a.nfwd; k++) {
+                        const int64_t cc = k < 0 ? int64_t(cg) : int64_t(cg) - a.fwd[k];
+                        if (cc < 0 || cc >= a.grid.ncells) continue;
+                        const uint32_t b = a.bin_start[cc], e = a.bin_start[cc + 1];
+                        for (uint32_t m0 = b; m0 < e; m0 += GB) {
+                            const uint32_t m = m0 + t;
+                            bool pass = false;
+                            uint32_t ent = 0u;
+                            if (m < e) {
+                                ent = a.bin_entry[m];
+                                if (entry_code(ent) == kCodeNone) {
+                                    const double4 q = a.bin_pos[m];
+                                    pass = pair_in(a, gx - q.x, gy - q.y, gz - q.z, q.w, sp);
+                                }
                             }
+                            const uint64_t bal = __ballot(pass) & gmask;
+                            if (pass) {
+                                const int kk = cnt + __popcll(bal & below);
+                                if (kk < a.maxnb)
+                                    a.nbr[row_addr(p, kk, a.npad, a.lg)] =
+                                        enc_entry(entry_idx(ent), code, true);
+                            }
+                            cnt += __popcll(bal);
                         }
                     }
                 }
             }
         }
     }
-    a.nnb[p] = uint32_t(cnt);
-    a.nhalf[p] = uint32_t(nh);
-    atomicMax(&a.st->max_row, cnt);
-    atomicMax(&a.st->max_half, nh);
-    if (cnt > a.maxnb) atomicOr(&a.st->overflow, 2u);
+    if (t == 0) {
+        a.nnb[p] = uint32_t(cnt);
+        a.nhalf[p] = uint32_t(nh);
+        atomicMax(&a.st->max_row, cnt);
+        atomicMax(&a.st->max_half, nh);
+        if (cnt > a.maxnb) atomicOr(&a.st->overflow, 2u);
+    }
 }
 
 // carry shear history across a rebuild: a new slot takes the old slot of the
@@ -361,12 +382,12 @@ __global__ void __launch_bounds__(256) k_build_list(ListArgs a) {
 // lower tag's frame, so it transfers unchanged). Only slots that were in
 // contact at the last force pass carry history (the reference prunes the
 // rest, contact.rs:586-589); their contact bits move with them.
-__global__ void k_remap_history(int64_t n, int64_t npad, int maxnb,
+__global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
                                 const uint32_t* __restrict__ nbr,
                                 const uint32_t* __restrict__ nnb,
                                 const uint32_t* __restrict__ tag,
                                 const uint32_t* __restrict__ order, int64_t npad_old,
-                                int maxnb_old, const uint32_t* __restrict__ nbr_old,
+                                int maxnb_old, int lg_old, const uint32_t* __restrict__ nbr_old,
                                 const uint32_t* __restrict__ nnb_old,
                                 const uint32_t* __restrict__ tag_old,
                                 const double* __restrict__ hist_old,
@@ -386,13 +407,12 @@ __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb,
     uint64_t nm = 0ull;
     if (om) {
         for (int k = 0; k < cnt; k++) {
-            const uint32_t tp = tag[entry_idx(nbr[int64_t(k) * npad + i])];
+            const int64_t s = row_addr(i, k, npad, lg);
+            const uint32_t tp = tag[entry_idx(nbr[s])];
             for (int ko = 0; ko < ocnt; ko++) {
                 if (!((om >> ko) & 1ull)) continue;
-                const uint32_t eo = nbr_old[int64_t(ko) * npad_old + oi];
-                if (tag_old[entry_idx(eo)] == tp) {
-                    const int64_t so = int64_t(ko) * npad_old + oi;
-                    const int64_t s = int64_t(k) * npad + i;
+                const int64_t so = row_addr(oi, ko, npad_old, lg_old);
+                if (tag_old[entry_idx(nbr_old[so])] == tp) {
                     hist[s] = hist_old[so];
                     hist[cs + s] = hist_old[cso + so];
                     hist[2 * cs + s] = hist_old[2 * cso + so];
@@ -405,57 +425,7 @@ __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb,
     cmask[i] = nm;
 }
 
-__global__ void k_count_dev(const uint32_t* __restrict__ key, const uint32_t* __restrict__ n_dev,
-                            uint32_t* cnt) {
-    int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (i < int64_t(*n_dev)) atomicAdd(&cnt[key[i]], 1u);
-}
-
-__global__ void k_scatter_dev(const uint32_t* __restrict__ key,
-                              const uint32_t* __restrict__ n_dev,
-                              const uint32_t* __restrict__ start, uint32_t* cursor,
-                              uint32_t* __restrict__ order) {
-    int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (i < int64_t(*n_dev)) {
-        uint32_t c = key[i];
-        uint32_t p = atomicAdd(&cursor[c], 1u);
-        order[start[c] + p] = uint32_t(i);
-    }
-}
-
-__global__ void k_segsort2(const uint32_t* __restrict__ start, int64_t ncells,
-                           uint32_t* __restrict__ order) {
-    int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (c >= ncells) return;
-    uint32_t b = start[c], e = start[c + 1];
-    for (uint32_t i = b + 1; i < e; i++) {
-        uint32_t v = order[i];
-        uint32_t j = i;
-        while (j > b && order[j - 1] > v) {
-            order[j] = order[j - 1];
-            j--;
-        }
-        order[j] = v;
-    }
-}
-
 }  // namespace
-
-void counting_sort_dev(Ctx& c, const uint32_t* key, const uint32_t* n_dev, int64_t n_max,
-                       int64_t ncells, uint32_t* cnt, uint32_t* start, uint32_t* order,
-                       uint32_t* slot_of) {
-    (void)slot_of;
-    fill_u32(c, cnt, 0u, ncells);
-    if (n_max > 0) k_count_dev<<<blocks_for(n_max, 256), 256, 0, c.stream>>>(key, n_dev, cnt);
-    exclusive_scan(c, cnt, start, ncells);
-    fill_u32(c, cnt, 0u, ncells);
-    if (n_max > 0) {
-        k_scatter_dev<<<blocks_for(n_max, 256), 256, 0, c.stream>>>(key, n_dev, start, cnt,
-                                                                   order);
-        k_segsort2<<<blocks_for(ncells, 256), 256, 0, c.stream>>>(start, ncells, order);
-    }
-    MDD_HIP(hipGetLastError());
-}
 
 // neighbor_setup + compute_bin_grid (neighbor.rs:480-541, 290-371) on the host
 void neighbor_setup(Ctx& c);
@@ -472,15 +442,21 @@ void rebuild(Ctx& c) {
     c.cnt.resize(ncells);
     c.start.resize(ncells + 1);
     c.bin_start.resize(ncells + 1);
+    c.gstart.resize(ncells + 1);
     c.gcnt.resize(n);
     c.goff.resize(n + 1);
+    c.placed.resize(std::max<int64_t>(n, c.ghost_cap) + 1);
+    // scan scratch sized once for every scan of this rebuild, so no buffer is
+    // reallocated while kernels that use it are queued
+    c.scan_tmp.resize(size_t((std::max<int64_t>(ncells, n) + 2047) / 2048 + 2));
 
     k_status_reset_build<<<1, 1, 0, c.stream>>>(c.status.p);
     // pbc + keys
     k_wrap_key<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, c.pos.p, c.image.p, c.box, c.grid,
                                                           c.key.p, c.status.p);
     // sort_atoms_by_bin: stable counting sort of particles by bin
-    counting_sort(c, c.key.p, n, ncells, c.cnt.p, c.start.p, c.order.p, nullptr);
+    counting_sort(c, c.key.p, n, nullptr, ncells, c.cnt.p, c.start.p, c.placed.p, c.order.p,
+                  nullptr);
     // permute every particle array into the alternates, then swap
     {
         PermArgs a{};
@@ -550,19 +526,23 @@ void rebuild(Ctx& c) {
     c.cmask.swap(c.cmask_old);
     c.npad_old = c.npad;
     c.maxnb_old = c.maxnb;
+    c.lg_old = c.lg;
 
     for (int attempt = 0; attempt < 6; attempt++) {
         if (c.ghost_cap < 1024) c.ghost_cap = 1024;
         c.ghost_root.resize(c.ghost_cap);
         c.ghost_code.resize(c.ghost_cap);
         c.ghost_pos.resize(c.ghost_cap);
+        c.ghost_key.resize(c.ghost_cap + 1);  // last word: live image count
+        c.ghost_order.resize(c.ghost_cap);
+        c.placed.resize(std::max<int64_t>(n, c.ghost_cap) + 1);
         const int64_t nall_max = n + c.ghost_cap;
-        c.all_key.resize(nall_max + 1);  // last word holds the live count
-        c.bin_order.resize(nall_max);
         c.bin_pos.resize(nall_max);
         c.bin_entry.resize(nall_max);
         c.slot_of.resize(n);
-        if (c.maxnb < 8) c.maxnb = 8;
+        const int G = 1 << c.lg;
+        if (c.maxnb < G) c.maxnb = G;
+        c.maxnb = (c.maxnb + G - 1) / G * G;
         c.nbr.resize(size_t(c.maxnb) * c.npad);
         c.nnb.resize(c.npad);
         c.nhalf.resize(c.npad);
@@ -572,29 +552,33 @@ void rebuild(Ctx& c) {
             c.hist.resize(size_t(3) * c.maxnb * c.npad);
             c.cmask.resize(c.npad);
         }
-        uint32_t* n_all = c.all_key.p + nall_max;
+        uint32_t* ng_live = c.ghost_key.p + c.ghost_cap;
 
         k_status_reset_build<<<1, 1, 0, c.stream>>>(c.status.p);
-        // borders: periodic images
+        // borders: periodic images and their bins
         k_ghost_count<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, c.pos.p, c.box, c.ghost_cutoff,
                                                                  c.gcnt.p);
         exclusive_scan(c, c.gcnt.p, c.goff.p, n);
         k_ghost_emit<<<blocks_for(n, 256), 256, 0, c.stream>>>(
-            n, c.pos.p, c.box, c.ghost_cutoff, c.goff.p, c.ghost_cap, c.ghost_root.p,
-            c.ghost_code.p, c.ghost_pos.p, c.status.p);
-        // bin particles and images (bin_neighbor_list steps 1-2b)
-        k_all_keys<<<blocks_for(nall_max, 256), 256, 0, c.stream>>>(
-            n, c.ghost_cap, c.key.p, c.ghost_pos.p, c.goff.p, c.grid, c.all_key.p, n_all);
-        counting_sort_dev(c, c.all_key.p, n_all, nall_max, ncells, c.cnt.p, c.bin_start.p,
-                          c.bin_order.p, nullptr);
+            n, c.pos.p, c.box, c.ghost_cutoff, c.grid, c.goff.p, c.ghost_cap, c.ghost_root.p,
+            c.ghost_code.p, c.ghost_pos.p, c.ghost_key.p, ng_live, c.status.p);
+        // images binned after the particles of each cell (bin_neighbor_list
+        // steps 1-2b: counting sort in index order, images have the larger indices)
+        counting_sort(c, c.ghost_key.p, c.ghost_cap, ng_live, ncells, c.cnt.p, c.gstart.p,
+                      c.placed.p, c.ghost_order.p, nullptr);
+        k_bin_start<<<blocks_for(ncells + 1, 256), 256, 0, c.stream>>>(ncells, c.start.p,
+                                                                         c.gstart.p,
+                                                                         c.bin_start.p);
         k_bin_fill<<<blocks_for(nall_max, 256), 256, 0, c.stream>>>(
-            n, n_all, c.bin_order.p, c.pos.p, c.cutr.p, c.ghost_pos.p, c.ghost_root.p,
+            n, c.ghost_cap, ng_live, c.key.p, c.start.p, c.gstart.p, c.bin_start.p,
+            c.ghost_order.p, c.ghost_key.p, c.pos.p, c.cutr.p, c.ghost_pos.p, c.ghost_root.p,
             c.ghost_code.p, c.bin_pos.p, c.bin_entry.p, c.slot_of.p);
-        // full rows (bin_neighbor_list step 3 plus the mirrored entries)
+        // full rows
         ListArgs la{};
         la.n = n;
         la.npad = c.npad;
         la.maxnb = c.maxnb;
+        la.lg = c.lg;
         la.bin_pos = c.bin_pos.p;
         la.bin_entry = c.bin_entry.p;
         la.bin_start = c.bin_start.p;
@@ -602,6 +586,8 @@ void rebuild(Ctx& c) {
         la.key = c.key.p;
         la.fwd = c.stencil.p;
         la.nfwd = int(c.fwd.size());
+        la.full = c.stencil_full.p;
+        la.nfull = int(c.full.size());
         la.box = c.box;
         la.grid = c.grid;
         la.gc = c.ghost_cutoff;
@@ -612,13 +598,24 @@ void rebuild(Ctx& c) {
         la.nnb = c.nnb.p;
         la.nhalf = c.nhalf.p;
         la.st = c.status.p;
-        k_build_list<<<blocks_for(n, 256), 256, 0, c.stream>>>(la);
+        const int64_t threads = n << c.build_lg;
+        switch (c.build_lg) {
+            case 2:
+                k_build_rows<2><<<blocks_for(threads, 256), 256, 0, c.stream>>>(la);
+                break;
+            case 4:
+                k_build_rows<4><<<blocks_for(threads, 256), 256, 0, c.stream>>>(la);
+                break;
+            default:
+                k_build_rows<3><<<blocks_for(threads, 256), 256, 0, c.stream>>>(la);
+                break;
+        }
         MDD_HIP(hipGetLastError());
         if (dem) {
             k_remap_history<<<blocks_for(n, 256), 256, 0, c.stream>>>(
-                n, c.npad, c.maxnb, c.nbr.p, c.nnb.p, c.tag.p, c.order.p, c.npad_old,
-                c.maxnb_old, c.nbr_old.p, c.nnb_old.p, c.tag2.p, c.hist_old.p, c.cmask_old.p,
-                c.hist.p, c.cmask.p, have_old ? 1 : 0);
+                n, c.npad, c.maxnb, c.lg, c.nbr.p, c.nnb.p, c.tag.p, c.order.p, c.npad_old,
+                c.maxnb_old, c.lg_old, c.nbr_old.p, c.nnb_old.p, c.tag2.p, c.hist_old.p,
+                c.cmask_old.p, c.hist.p, c.cmask.p, have_old ? 1 : 0);
             MDD_HIP(hipGetLastError());
         }
         MDD_HIP(hipMemcpyAsync(c.status_host, c.status.p, sizeof(Status), hipMemcpyDeviceToHost,
@@ -685,6 +682,7 @@ void neighbor_setup(Ctx& c) {
         throw ApiError(-5, "bin grid too large; raise bin_size");
     double cutoff_sq = max_cutoff * max_cutoff;
     c.fwd.clear();
+    c.full.clear();
     const int ny = c.grid.n[1], nz = c.grid.n[2];
     for (int dx = -sl[0]; dx <= sl[0]; dx++)
         for (int dy = -sl[1]; dy <= sl[1]; dy++)
@@ -695,12 +693,16 @@ void neighbor_setup(Ctx& c) {
                 double mdz = double(az > 0 ? az : 0) * c.bin_size[2];
                 if (mdx * mdx + mdy * mdy + mdz * mdz < cutoff_sq) {
                     int off = dx * ny * nz + dy * nz + dz;
+                    c.full.push_back(off);
                     if (off > 0) c.fwd.push_back(off);
                 }
             }
     c.grid.nfwd = int(c.fwd.size());
     c.stencil.resize(c.fwd.size());
     MDD_HIP(hipMemcpy(c.stencil.p, c.fwd.data(), sizeof(int) * c.fwd.size(),
+                      hipMemcpyHostToDevice));
+    c.stencil_full.resize(c.full.size());
+    MDD_HIP(hipMemcpy(c.stencil_full.p, c.full.data(), sizeof(int) * c.full.size(),
                       hipMemcpyHostToDevice));
     // save_build_positions cutoff cache (neighbor.rs:545-561)
     c.min_skin = mn;
@@ -712,12 +714,13 @@ void neighbor_setup(Ctx& c) {
         c.uniform = false;
     }
     c.sf2 = c.skin_fraction * c.skin_fraction;
-    for (int code = 0; code < 27; code++) {
-        int s[3] = {code / 9 - 1, (code / 3) % 3 - 1, code % 3 - 1};
+    if (c.ghost_cap == 0) {
+        // expected images per particle: prod over periodic axes of (1 + 2 gc / L) - 1
+        double per = 1.0;
         for (int d = 0; d < 3; d++)
-            c.shifts.v[code][d] = s[d] == 0 ? 0.0 : (s[d] > 0 ? c.box.L[d] : -c.box.L[d]);
+            if (c.box.periodic[d]) per *= 1.0 + std::min(1.0, 2.0 * c.ghost_cutoff / c.box.L[d]);
+        c.ghost_cap = int64_t(double(c.n) * (per - 1.0) * 1.25) + 1024;
     }
-    if (c.ghost_cap == 0) c.ghost_cap = c.n / 2 + 1024;
     c.setup_done = true;
 }
 

@@ -2,8 +2,10 @@
 //
 // The counting sort reproduces the order of the reference's binning
 // (crates/mddem_core/src/neighbor.rs:857-889): within a cell, elements stay
-// in index order. Placement uses atomics for speed, and a per-cell insertion
-// sort then restores index order, so the result is deterministic.
+// in index order. Elements are first placed with atomics (aggregated over
+// runs of equal keys inside a wavefront, since the input is mostly sorted
+// already), then every element computes its index rank inside its cell in
+// parallel and is written to its final slot, so the result is deterministic.
 #include "ctx.hpp"
 
 namespace mdd {
@@ -112,43 +114,78 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const uint32_t* __r
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sums[nb];
 }
 
-__global__ void k_count(const uint32_t* __restrict__ key, int64_t n, uint32_t* cnt) {
-    int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[key[i]], 1u);
+// Runs of equal keys inside a wavefront: the lane of the run head (h) and
+// of the run tail (t). Invalid lanes form runs of their own.
+__device__ inline void wave_runs(uint32_t key, bool valid, int& h, int& t) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t kp = __shfl_up(key, 1, 64);
+    const int vp = __shfl_up(int(valid), 1, 64);
+    const uint32_t kn = __shfl_down(key, 1, 64);
+    const int vn = __shfl_down(int(valid), 1, 64);
+    const bool head = lane == 0 || !valid || !vp || kp != key;
+    const bool tail = lane == 63 || !valid || !vn || kn != key;
+    int hv = head ? lane : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int x = __shfl_up(hv, d, 64);
+        if (lane >= d && x > hv) hv = x;
+    }
+    int tv = tail ? lane : 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int x = __shfl_down(tv, d, 64);
+        if (lane + d < 64 && x < tv) tv = x;
+    }
+    h = hv;
+    t = tv;
+}
+
+// count per key; n taken from n_dev when given (device-side element count)
+__global__ void k_count(const uint32_t* __restrict__ key, int64_t n,
+                        const uint32_t* __restrict__ n_dev, uint32_t* cnt) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const int64_t nn = n_dev ? int64_t(*n_dev) : n;
+    const bool valid = i < nn;
+    const uint32_t k = valid ? key[i] : 0u;
+    int h, t;
+    wave_runs(k, valid, h, t);
+    const int lane = threadIdx.x & 63;
+    if (valid && lane == t) atomicAdd(&cnt[k], uint32_t(t - h + 1));
 }
 
 __global__ void k_scatter(const uint32_t* __restrict__ key, int64_t n,
+                          const uint32_t* __restrict__ n_dev,
                           const uint32_t* __restrict__ start, uint32_t* cursor,
                           uint32_t* __restrict__ order) {
-    int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (i < n) {
-        uint32_t c = key[i];
-        uint32_t p = atomicAdd(&cursor[c], 1u);
-        order[start[c] + p] = uint32_t(i);
-    }
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const int64_t nn = n_dev ? int64_t(*n_dev) : n;
+    const bool valid = i < nn;
+    const uint32_t k = valid ? key[i] : 0u;
+    int h, t;
+    wave_runs(k, valid, h, t);
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (valid && lane == t) base = start[k] + atomicAdd(&cursor[k], uint32_t(t - h + 1));
+    base = __shfl(base, t, 64);
+    if (valid) order[base + uint32_t(lane - h)] = uint32_t(i);
 }
 
-// restore index order inside each cell (insertion sort; cells hold few items)
-__global__ void k_segsort(const uint32_t* __restrict__ start, int64_t ncells,
-                          uint32_t* __restrict__ order) {
-    int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (c >= ncells) return;
-    uint32_t b = start[c], e = start[c + 1];
-    for (uint32_t i = b + 1; i < e; i++) {
-        uint32_t v = order[i];
-        uint32_t j = i;
-        while (j > b && order[j - 1] > v) {
-            order[j] = order[j - 1];
-            j--;
-        }
-        order[j] = v;
-    }
-}
-
-__global__ void k_invert(const uint32_t* __restrict__ order, int64_t n,
-                         uint32_t* __restrict__ slot_of) {
-    int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (s < n) slot_of[order[s]] = uint32_t(s);
+// final slot of every element: its cell's start plus the number of elements
+// of the same cell with a smaller index (index order inside each cell)
+__global__ void k_rank(const uint32_t* __restrict__ key, int64_t n,
+                       const uint32_t* __restrict__ n_dev, const uint32_t* __restrict__ start,
+                       const uint32_t* __restrict__ placed, uint32_t* __restrict__ order,
+                       uint32_t* __restrict__ slot_of) {
+    const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const int64_t nn = n_dev ? int64_t(*n_dev) : n;
+    if (s >= nn) return;
+    const uint32_t v = placed[s];
+    const uint32_t c = key[v];
+    const uint32_t b = start[c], e = start[c + 1];
+    uint32_t r = 0;
+    for (uint32_t t = b; t < e; t++) r += placed[t] < v ? 1u : 0u;
+    order[b + r] = v;
+    if (slot_of) slot_of[v] = b + r;
 }
 
 int fill_blocks(int64_t n) {
@@ -179,26 +216,28 @@ void fill_f32(Ctx& c, float* p, float v, int64_t n) {
 void exclusive_scan(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n) {
     int64_t nb = (n + kScanTile - 1) / kScanTile;
     if (nb < 1) nb = 1;
-    c.scan_tmp.resize(size_t(nb + 1));
+    c.scan_tmp.resize(size_t(nb + 2));
     k_scan_reduce<<<int(nb), kScanThreads, 0, c.stream>>>(in, n, c.scan_tmp.p);
     k_scan_sums<<<1, kScanThreads, 0, c.stream>>>(c.scan_tmp.p, nb);
     k_scan_apply<<<int(nb), kScanThreads, 0, c.stream>>>(in, out, n, c.scan_tmp.p, nb);
     MDD_HIP(hipGetLastError());
 }
 
-// Stable counting sort of indices 0..n-1 by key in [0, ncells).
-// start: ncells+1, order: n (slot -> index), slot_of: n (index -> slot) or null.
-// cnt: ncells scratch.
-void counting_sort(Ctx& c, const uint32_t* key, int64_t n, int64_t ncells, uint32_t* cnt,
-                   uint32_t* start, uint32_t* order, uint32_t* slot_of) {
+// Stable counting sort of indices 0..n-1 by key in [0, ncells). The element
+// count is n, or *n_dev (<= n) when n_dev is given. start: ncells+1;
+// order: n (slot -> index); slot_of: n (index -> slot) or null; cnt: ncells
+// scratch; placed: n scratch.
+void counting_sort(Ctx& c, const uint32_t* key, int64_t n, const uint32_t* n_dev, int64_t ncells,
+                   uint32_t* cnt, uint32_t* start, uint32_t* placed, uint32_t* order,
+                   uint32_t* slot_of) {
     fill_u32(c, cnt, 0u, ncells);
-    if (n > 0) k_count<<<blocks_for(n, 256), 256, 0, c.stream>>>(key, n, cnt);
+    if (n > 0) k_count<<<blocks_for(n, 256), 256, 0, c.stream>>>(key, n, n_dev, cnt);
     exclusive_scan(c, cnt, start, ncells);
     fill_u32(c, cnt, 0u, ncells);
     if (n > 0) {
-        k_scatter<<<blocks_for(n, 256), 256, 0, c.stream>>>(key, n, start, cnt, order);
-        k_segsort<<<blocks_for(ncells, 256), 256, 0, c.stream>>>(start, ncells, order);
-        if (slot_of) k_invert<<<blocks_for(n, 256), 256, 0, c.stream>>>(order, n, slot_of);
+        k_scatter<<<blocks_for(n, 256), 256, 0, c.stream>>>(key, n, n_dev, start, cnt, placed);
+        k_rank<<<blocks_for(n, 256), 256, 0, c.stream>>>(key, n, n_dev, start, placed, order,
+                                                          slot_of);
     }
     MDD_HIP(hipGetLastError());
 }

@@ -71,9 +71,12 @@ def test_lj_two_body_force(prec, r):
     want = K["lj_force_on_plus_x"][repr(r)]
     if want == 0.0:
         assert f[1, 0] == 0.0
+    elif prec == hip.FP64:
+        assert abs(f[1, 0] - want) / abs(want) < 1e-13
     else:
-        tol = 1e-13 if prec == hip.FP64 else 2e-6
-        assert abs(f[1, 0] - want) / abs(want) < tol
+        # FP32 error is relative to the two terms, which cancel near 2^(1/6)
+        scale = 48.0 * r ** -13 + 24.0 * r ** -7
+        assert abs(f[1, 0] - want) < 4e-6 * scale
     assert f[0, 0] == -f[1, 0]
 
 
@@ -129,7 +132,7 @@ def _random_periodic(seed, n=300, poly=False, zperiodic=True):
 
 def _pair_cases():
     return ["trig30", "rand_uniform", "rand_slab", "rand_poly", "dem_gas_500", "lj_fcc_6",
-            "dem_settling_slab"]
+            "dem_settling_slab", "dem_dense"]
 
 
 def _build_case(name):
@@ -159,11 +162,16 @@ def _build_case(name):
         s.upload(pos, np.zeros_like(pos), np.ones(len(pos)), cut_r)
         return o, s
     if name == "dem_gas_500":
+        # spheres start > 2.2 mm apart: widen the skin so the list is not empty
         w = W.dem_gas(n=500)
+        w.skin_fraction = 1.6
     elif name == "lj_fcc_6":
         w = W.lj_fcc(ncell=6)
     elif name == "dem_settling_slab":
         w = W.dem_settling(n=3000, box=(0.04, 0.04, 0.08))
+        w.skin_fraction = 1.6
+    elif name == "dem_dense":
+        w = _dense_dem(n=2000, seed=13)
     return oracle_from_workload(w), hip.from_workload(w)
 
 
@@ -176,6 +184,7 @@ def test_pair_set_bit_exact(name):
     oo, op, oc = o.pairs()
     # identical half list: same owners, partners and image shifts
     key = lambda a, b, c: np.sort((a.astype(np.int64) << 37) | (b.astype(np.int64) << 5) | c)
+    assert len(oo) > 0
     assert len(go) == len(oo)
     assert np.array_equal(key(go, gp, gc), key(oo, op, oc))
     assert np.array_equal(canonical_pairs(go, gp, gc), canonical_pairs(oo, op, oc))
@@ -224,7 +233,13 @@ def test_forces_match_oracle(case, prec):
     assert rel_err(by_tag(st["tag"], f), by_tag(ref["tag"], ref["force"])) < tol
     if case == "dem":
         assert rel_err(by_tag(st["tag"], t), by_tag(ref["tag"], ref["torque"])) < tol
-        assert np.abs(f.sum(axis=0)).max() <= 1e-9 * np.abs(f).max()
+        # pair forces are antisymmetric bit for bit; what remains is the
+        # rounding of each particle's own accumulation (FP64 or FP32)
+        net = np.abs(f.sum(axis=0)).max()
+        if prec == hip.FP64:
+            assert net <= 1e-9 * np.abs(f).max()
+        else:
+            assert net <= 1e-6 * np.abs(f).sum()
 
 
 def _trajectory(w, steps, prec=hip.FP64):
