@@ -189,7 +189,6 @@ struct Ctx {
     int maxnb = 0;          // row capacity of the full list (multiple of 1 << lg)
     int lg = 2;             // log2 of lanes per particle in the pair-force kernels
     int lg_old = 2;
-    int build_lg = 3;       // log2 of lanes per particle in the list build
     int64_t npairs = 0;
     bool list_valid = false;
     bool list_injected = false;
@@ -218,8 +217,11 @@ struct Ctx {
     DBuf<uint32_t> bin_start, bin_entry, gstart, placed, ghost_key, ghost_order;
     DBuf<double4> bin_pos;
     DBuf<int> stencil;       // forward offsets
-    DBuf<int> stencil_full;  // all offsets whose cells can hold a pair, fixed order
-    std::vector<int> full;
+    std::vector<int> full;   // all offsets whose cells can hold a pair, ascending
+    DBuf<int> run_first, run_last;  // the full stencil as runs of consecutive offsets
+    int nruns = 0, run0 = 0;        // run0: the run holding offset 0
+    DBuf<uint32_t> bin_gidx;        // bin slot of a periodic image -> image index
+    DBuf<uint32_t> gnbr, gcnt_img;  // per image: particles owning a pair with it
 
     // ---- full neighbor list: entry k of particle i at row_addr(i, k, npad, lg)
     DBuf<uint32_t> nbr, nbr_old;

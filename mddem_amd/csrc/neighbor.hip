@@ -214,7 +214,8 @@ __global__ void k_bin_fill(int64_t n, int64_t ghost_cap, const uint32_t* __restr
                            const double4* __restrict__ pos, const double* __restrict__ cutr,
                            const double4* __restrict__ gpos, const uint32_t* __restrict__ groot,
                            const uint8_t* __restrict__ gcode, double4* __restrict__ bin_pos,
-                           uint32_t* __restrict__ bin_entry, uint32_t* __restrict__ slot_local) {
+                           uint32_t* __restrict__ bin_entry, uint32_t* __restrict__ bin_gidx,
+                           uint32_t* __restrict__ slot_local) {
     const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (t < n) {
         const uint32_t c = key[t];
@@ -234,145 +235,280 @@ __global__ void k_bin_fill(int64_t n, int64_t ghost_cap, const uint32_t* __restr
     const uint32_t r = groot[g];
     bin_pos[s] = make_double4(q.x, q.y, q.z, cutr[r]);
     bin_entry[s] = enc_entry(r, gcode[g], false);
+    bin_gidx[s] = g;
 }
 
-struct ListArgs {
-    int64_t n, npad;
+constexpr int kCellWaves = 4;  // cells (one per wavefront) per 256-thread block
+constexpr int kNch = 10;       // register-resident 64-candidate chunks per window
+constexpr int kMaxRuns = 64;   // contiguous bin ranges of one cell's stencil
+
+struct CellArgs {
+    int64_t ncells, npad;
     int maxnb, lg;
-    const double4* bin_pos;
-    const uint32_t* bin_entry;
-    const uint32_t* bin_start;
-    const uint32_t* slot_local;
-    const uint32_t* key;
-    const int* fwd;
-    int nfwd;
-    const int* full;
-    int nfull;
-    BoxP box;
-    GridP grid;
-    double gc;
-    int uniform;
+    const double4* __restrict__ bin_pos;
+    const uint32_t* __restrict__ bin_entry;
+    const uint32_t* __restrict__ bin_gidx;
+    const uint32_t* __restrict__ bin_start;
+    const int* __restrict__ run_first;
+    const int* __restrict__ run_last;
+    int nruns, run0;
     double cutoff_sq, sf2;
-    uint32_t* nbr;
-    uint32_t* nnb;
-    uint32_t* nhalf;
-    Status* st;
+    uint32_t* __restrict__ nbr;
+    uint32_t* __restrict__ nnb;
+    uint32_t* __restrict__ nhalf;
+    uint32_t* __restrict__ gnbr;
+    uint32_t* __restrict__ gcnt;
 };
 
-// reference pair test, evaluated from the owner's side: r2 of
-// (partner image - owner) with the fma chain of neighbor.rs:955-958 and
-// either the uniform cutoff or (s_i + s_j)^2 sf^2 (neighbor.rs:1003-1004)
-__device__ inline bool pair_in(const ListArgs& a, double dx, double dy, double dz,
-                               double s_owner, double s_partner) {
-    double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    if (a.uniform) return r2 < a.cutoff_sq;
-    double sum = s_owner + s_partner;
-    return r2 < sum * sum * a.sf2;
+// Blocks are dispatched round-robin over the 8 XCDs; map them so that each
+// XCD works on one contiguous range of cells, whose neighbourhoods share
+// that XCD's L2.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t x = b & 7u, i = b >> 3, q = nb >> 3, r = nb & 7u;
+    return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + i;
 }
 
-template <int LGB>
-__global__ void __launch_bounds__(256) k_build_rows(ListArgs a) {
-    constexpr int GB = 1 << LGB;
-    const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    const int64_t p = gt >> LGB;
-    const uint32_t t = uint32_t(gt & (GB - 1));
-    const int lane = threadIdx.x & 63;
-    const uint64_t gmask =
-        GB == 64 ? ~0ull : (((1ull << GB) - 1ull) << (lane & ~(GB - 1)));
-    const uint64_t below = gmask & ((1ull << lane) - 1ull);
-    if (p >= a.n) return;  // whole group: p is the same on its lanes
-    const uint32_t slot = a.slot_local[p];
-    const uint32_t c = a.key[p];
-    const double4 xp = a.bin_pos[slot];
-    const double sp = xp.w;
-    int cnt = 0, nh = 0;
-    // the particle's own stencil cells: owned pairs and pairs owned by
-    // other particles (not images) with it
-    for (int s = 0; s < a.nfull; s++) {
-        const int off = a.full[s];
-        const int64_t cc = int64_t(c) + off;
-        if (cc < 0 || cc >= a.grid.ncells) continue;
-        const uint32_t b = a.bin_start[cc], e = a.bin_start[cc + 1];
-        for (uint32_t m0 = b; m0 < e; m0 += GB) {
-            const uint32_t m = m0 + t;
-            bool pass = false, own = false;
-            uint32_t ent = 0u;
-            if (m < e && m != slot) {
-                ent = a.bin_entry[m];
-                own = off > 0 || (off == 0 && m > slot);
-                if (own || entry_code(ent) == kCodeNone) {
-                    const double4 q = a.bin_pos[m];
-                    pass = pair_in(a, q.x - xp.x, q.y - xp.y, q.z - xp.z, sp, q.w);
-                }
+// lanes above / below lane `rel` of a 64-lane chunk
+__device__ inline uint64_t lanes_above(int rel) {
+    if (rel < 0) return ~0ull;
+    if (rel >= 63) return 0ull;
+    return ~0ull << (rel + 1);
+}
+__device__ inline uint64_t lanes_below(int rel) {
+    if (rel <= 0) return 0ull;
+    if (rel >= 64) return ~0ull;
+    return (1ull << rel) - 1ull;
+}
+
+__device__ inline double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(b), l));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(b >> 32), l));
+    return __longlong_as_double(static_cast<long long>((uint64_t(hi) << 32) | lo));
+}
+
+__device__ inline uint32_t lanes_before(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+// Candidate chunks of one window, held in registers: lane l of chunk c is
+// flat candidate win + 64 c + l of the cell's stencil ranges.
+template <bool UNIFORM>
+struct Window {
+    double x[kNch], y[kNch], z[kNch], w[kNch];
+    uint32_t e[kNch];
+    uint64_t local[kNch];
+    int avail;  // candidates from the window start on
+};
+
+// Pair tests of one member of the cell against the window. A particle
+// member writes its own row: candidates after it in bin order are pairs it
+// owns (particles and periodic images), particles before it own the pair
+// with it. An image member collects the particles before it in bin order
+// (they own the pair with the image) into the image's scratch row.
+template <bool UNIFORM, bool IMAGE>
+__device__ inline void test_member(const CellArgs& a, const Window<UNIFORM>& W, int nch, int fs,
+                                   const double4& xi, uint32_t code_i, uint32_t* __restrict__ out,
+                                   uint32_t q, uint32_t& cnt, uint32_t& own) {
+    const int lane = int(threadIdx.x & 63);
+    const uint32_t G = 1u << a.lg;
+#pragma unroll
+    for (int c = 0; c < kNch; c++) {
+        const int rel = fs - c * 64;
+        const uint64_t above = lanes_above(rel), below = lanes_below(rel);
+        const uint64_t valid = lanes_below(W.avail - c * 64);
+        const uint64_t cm = c >= nch ? 0ull
+                            : IMAGE ? (W.local[c] & below)
+                                    : (valid & (above | (below & W.local[c])));
+        if (cm != 0ull) {
+            const double dx = W.x[c] - xi.x, dy = W.y[c] - xi.y, dz = W.z[c] - xi.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            bool in;
+            if (UNIFORM) {
+                in = r2 < a.cutoff_sq;
+            } else {
+                const double s = xi.w + W.w[c];
+                in = r2 < s * s * a.sf2;
             }
-            const uint64_t bal = __ballot(pass) & gmask;
-            const uint64_t obal = __ballot(pass && own) & gmask;
-            if (pass) {
-                const int k = cnt + __popcll(bal & below);
-                if (k < a.maxnb) a.nbr[row_addr(p, k, a.npad, a.lg)] = own ? ent : (ent | kRevBit);
-            }
-            cnt += __popcll(bal);
-            nh += __popcll(obal);
-        }
-    }
-    // pairs owned by particles near one of this particle's periodic images
-    double x[3] = {xp.x, xp.y, xp.z};
-    int lf[3], hf[3];
-    image_flags(a.box, a.gc, x, lf, hf);
-    if (lf[0] | hf[0] | lf[1] | hf[1] | lf[2] | hf[2]) {
-        const int sv[3] = {0, 1, -1};
-        for (int ia = 0; ia < 3; ia++) {
-            const int sx = sv[ia];
-            if ((sx == 1 && !lf[0]) || (sx == -1 && !hf[0])) continue;
-            for (int ib = 0; ib < 3; ib++) {
-                const int sy = sv[ib];
-                if ((sy == 1 && !lf[1]) || (sy == -1 && !hf[1])) continue;
-                for (int ic = 0; ic < 3; ic++) {
-                    const int sz = sv[ic];
-                    if ((sz == 1 && !lf[2]) || (sz == -1 && !hf[2])) continue;
-                    if (sx == 0 && sy == 0 && sz == 0) continue;
-                    const int code = code_of(sx, sy, sz);
-                    const double gx = x[0] + image_shift(sx, a.box.L[0]);
-                    const double gy = x[1] + image_shift(sy, a.box.L[1]);
-                    const double gz = x[2] + image_shift(sz, a.box.L[2]);
-                    const uint32_t cg = cell_of(a.grid, gx, gy, gz);
-                    // owners: every particle in the image's cell (images sit
-                    // after particles there) and particles of its backward cells
-                    for (int k = -1; k < a.nfwd; k++) {
-                        const int64_t cc = k < 0 ? int64_t(cg) : int64_t(cg) - a.fwd[k];
-                        if (cc < 0 || cc >= a.grid.ncells) continue;
-                        const uint32_t b = a.bin_start[cc], e = a.bin_start[cc + 1];
-                        for (uint32_t m0 = b; m0 < e; m0 += GB) {
-                            const uint32_t m = m0 + t;
-                            bool pass = false;
-                            uint32_t ent = 0u;
-                            if (m < e) {
-                                ent = a.bin_entry[m];
-                                if (entry_code(ent) == kCodeNone) {
-                                    const double4 q = a.bin_pos[m];
-                                    pass = pair_in(a, gx - q.x, gy - q.y, gz - q.z, q.w, sp);
-                                }
-                            }
-                            const uint64_t bal = __ballot(pass) & gmask;
-                            if (pass) {
-                                const int kk = cnt + __popcll(bal & below);
-                                if (kk < a.maxnb)
-                                    a.nbr[row_addr(p, kk, a.npad, a.lg)] =
-                                        enc_entry(entry_idx(ent), code, true);
-                            }
-                            cnt += __popcll(bal);
-                        }
+            const uint64_t pass = __ballot(in) & cm;
+            if (pass != 0ull) {
+                const uint32_t k = cnt + lanes_before(pass);
+                if (((pass >> lane) & 1ull) && k < uint32_t(a.maxnb)) {
+                    if (IMAGE) {
+                        out[k] = enc_entry(entry_idx(W.e[c]), int(code_i), true);
+                    } else {
+                        const bool mine = (above >> lane) & 1ull;
+                        const uint32_t at =
+                            (((k >> a.lg) * uint32_t(a.npad) + q) << a.lg) + (k & (G - 1u));
+                        out[at] = mine ? W.e[c] : (W.e[c] | kRevBit);
                     }
                 }
+                cnt += uint32_t(__popcll(pass));
+                if (!IMAGE) own += uint32_t(__popcll(pass & above));
             }
         }
     }
-    if (t == 0) {
-        a.nnb[p] = uint32_t(cnt);
-        a.nhalf[p] = uint32_t(nh);
-        atomicMax(&a.st->max_row, cnt);
-        atomicMax(&a.st->max_half, nh);
-        if (cnt > a.maxnb) atomicOr(&a.st->overflow, 2u);
+}
+
+// Neighbor rows, one wavefront per bin cell. The cell's stencil cells form
+// a few runs of consecutive cells, i.e. contiguous bin ranges; concatenated
+// they are the cell's candidate list, whose order is bin order. Candidates
+// are loaded 64 per chunk into registers once, and every member of the cell
+// (particles, then periodic images) is tested against them with uniform
+// control flow; passing lanes are compacted with a ballot, so each row lists
+// its entries in bin order.
+template <bool UNIFORM>
+__global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int lane = int(threadIdx.x & 63);
+    const int64_t C = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kCellWaves + w;
+    if (C >= a.ncells) return;
+    const uint32_t i0 = a.bin_start[C], i1 = a.bin_start[C + 1];
+    if (i0 == i1) return;
+    // stencil runs in lanes: bin range [rb, rb + rcnt), flat offset excl
+    uint32_t rcnt = 0, rb = 0;
+    if (lane < a.nruns) {
+        int64_t lo = C + a.run_first[lane], hi = C + a.run_last[lane];
+        if (lo < 0) lo = 0;
+        if (hi > a.ncells - 1) hi = a.ncells - 1;
+        if (lo <= hi) {
+            rb = a.bin_start[lo];
+            rcnt = a.bin_start[hi + 1] - rb;
+        }
+    }
+    uint32_t incl = rcnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    const uint32_t excl = incl - rcnt;
+    const uint32_t rbase = rb - excl;  // bin index = flat index + rbase (mod 2^32)
+    const uint32_t total = __builtin_amdgcn_readlane(incl, a.nruns - 1);
+    const uint32_t base0 = __builtin_amdgcn_readlane(rbase, a.run0);
+    const uint32_t ni = i1 - i0;
+    for (uint32_t ib = 0; ib < ni; ib += 64) {
+        const uint32_t nib = ni - ib < 64u ? ni - ib : 64u;
+        uint32_t cntv = 0, ownv = 0;  // lane l: member ib + l
+        // the members of this batch, lane l holding member ib + l
+        double4 mx = make_double4(0.0, 0.0, 0.0, 0.0);
+        uint32_t me = 0u, mg = 0u;
+        if (uint32_t(lane) < nib) {
+            const uint32_t slot = i0 + ib + uint32_t(lane);
+            mx = a.bin_pos[slot];
+            me = a.bin_entry[slot];
+            if (entry_code(me) != kCodeNone) mg = a.bin_gidx[slot];
+        }
+        for (uint32_t win = 0; win < total; win += kNch * 64) {
+            Window<UNIFORM> W;
+            W.avail = int(total - win);
+            uint32_t base[kNch];
+#pragma unroll
+            for (int c = 0; c < kNch; c++) base[c] = 0;
+            for (int r = 0; r < a.nruns; r++) {
+                const uint32_t pr = __builtin_amdgcn_readlane(excl, r);
+                const uint32_t br = __builtin_amdgcn_readlane(rbase, r);
+#pragma unroll
+                for (int c = 0; c < kNch; c++)
+                    if (win + uint32_t(c * 64 + lane) >= pr) base[c] = br;
+            }
+#pragma unroll
+            for (int c = 0; c < kNch; c++) {
+                const uint32_t f = win + uint32_t(c * 64 + lane);
+                const bool valid = f < total;
+                double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
+                uint32_t e = 0u;
+                if (valid) {
+                    q = a.bin_pos[f + base[c]];
+                    e = a.bin_entry[f + base[c]];
+                }
+                W.x[c] = q.x;
+                W.y[c] = q.y;
+                W.z[c] = q.z;
+                W.w[c] = q.w;
+                W.e[c] = e;
+                W.local[c] = __ballot(valid && entry_code(e) == kCodeNone);
+            }
+            const uint32_t left = (total - win + 63u) / 64u;
+            const int nch = left < uint32_t(kNch) ? int(left) : kNch;
+            for (uint32_t ii = 0; ii < nib; ii++) {
+                const uint32_t slot = i0 + ib + ii;
+                const uint32_t ei = uint32_t(__builtin_amdgcn_readlane(int(me), int(ii)));
+                double4 xi;
+                xi.x = readlane_f64(mx.x, int(ii));
+                xi.y = readlane_f64(mx.y, int(ii));
+                xi.z = readlane_f64(mx.z, int(ii));
+                xi.w = UNIFORM ? 0.0 : readlane_f64(mx.w, int(ii));
+                const int fs = int(slot - base0 - win);
+                uint32_t cnt = __builtin_amdgcn_readlane(cntv, int(ii));
+                uint32_t own = __builtin_amdgcn_readlane(ownv, int(ii));
+                if (entry_code(ei) == kCodeNone) {
+                    test_member<UNIFORM, false>(a, W, nch, fs, xi, 0u, a.nbr, entry_idx(ei), cnt,
+                                                own);
+                } else {
+                    const uint32_t g = uint32_t(__builtin_amdgcn_readlane(int(mg), int(ii)));
+                    test_member<UNIFORM, true>(a, W, nch, fs, xi, uint32_t(entry_code(ei)),
+                                               a.gnbr + size_t(g) * uint32_t(a.maxnb), 0u, cnt,
+                                               own);
+                }
+                if (uint32_t(lane) == ii) {
+                    cntv = cnt;
+                    ownv = own;
+                }
+            }
+        }
+        if (uint32_t(lane) < nib) {
+            if (entry_code(me) == kCodeNone) {
+                a.nnb[entry_idx(me)] = cntv;
+                a.nhalf[entry_idx(me)] = ownv;
+            } else {
+                a.gcnt[mg] = cntv;
+            }
+        }
+    }
+}
+
+// Append to each particle's row the pairs owned through its periodic images
+// (in image order), then report the row maxima and capacity overflow.
+__global__ void __launch_bounds__(256) k_merge_rows(int64_t n, int64_t npad, int maxnb, int lg,
+                                                    const uint32_t* __restrict__ goff,
+                                                    const uint32_t* __restrict__ ng_live,
+                                                    const uint32_t* __restrict__ gcnt,
+                                                    const uint32_t* __restrict__ gnbr,
+                                                    uint32_t* __restrict__ nbr,
+                                                    uint32_t* __restrict__ nnb,
+                                                    const uint32_t* __restrict__ nhalf, Status* st) {
+    const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    uint32_t cnt = 0, nh = 0;
+    if (q < n) {
+        cnt = nnb[q];
+        nh = nhalf[q];
+        const uint32_t g0 = goff[q], g1 = goff[q + 1];
+        if (*ng_live != 0u && g1 > g0) {
+            for (uint32_t g = g0; g < g1; g++) {
+                const uint32_t c = gcnt[g];
+                const uint32_t cc = c < uint32_t(maxnb) ? c : uint32_t(maxnb);
+                for (uint32_t k = 0; k < cc; k++) {
+                    const uint32_t kk = cnt + k;
+                    if (kk < uint32_t(maxnb))
+                        nbr[row_addr(q, int(kk), npad, lg)] = gnbr[size_t(g) * uint32_t(maxnb) + k];
+                }
+                cnt += c;
+            }
+            nnb[q] = cnt;
+        }
+    }
+    uint32_t mc = cnt, mh = nh;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t a = __shfl_xor(mc, o, 64), b = __shfl_xor(mh, o, 64);
+        mc = a > mc ? a : mc;
+        mh = b > mh ? b : mh;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&st->max_row, int(mc));
+        atomicMax(&st->max_half, int(mh));
+        if (mc > uint32_t(maxnb)) atomicOr(&st->overflow, 2u);
     }
 }
 
@@ -539,11 +675,16 @@ void rebuild(Ctx& c) {
         const int64_t nall_max = n + c.ghost_cap;
         c.bin_pos.resize(nall_max);
         c.bin_entry.resize(nall_max);
+        c.bin_gidx.resize(nall_max);
         c.slot_of.resize(n);
         const int G = 1 << c.lg;
         if (c.maxnb < G) c.maxnb = G;
         c.maxnb = (c.maxnb + G - 1) / G * G;
+        if (int64_t(c.maxnb) * c.npad >= (int64_t(1) << 31) || c.npad >= (int64_t(1) << 24))
+            throw ApiError(-5, "neighbor list too large for 32-bit row indexing");
         c.nbr.resize(size_t(c.maxnb) * c.npad);
+        c.gnbr.resize(size_t(c.ghost_cap) * c.maxnb);
+        c.gcnt_img.resize(c.ghost_cap);
         c.nnb.resize(c.npad);
         c.nhalf.resize(c.npad);
         if (dem) {
@@ -572,44 +713,36 @@ void rebuild(Ctx& c) {
         k_bin_fill<<<blocks_for(nall_max, 256), 256, 0, c.stream>>>(
             n, c.ghost_cap, ng_live, c.key.p, c.start.p, c.gstart.p, c.bin_start.p,
             c.ghost_order.p, c.ghost_key.p, c.pos.p, c.cutr.p, c.ghost_pos.p, c.ghost_root.p,
-            c.ghost_code.p, c.bin_pos.p, c.bin_entry.p, c.slot_of.p);
-        // full rows
-        ListArgs la{};
-        la.n = n;
-        la.npad = c.npad;
-        la.maxnb = c.maxnb;
-        la.lg = c.lg;
-        la.bin_pos = c.bin_pos.p;
-        la.bin_entry = c.bin_entry.p;
-        la.bin_start = c.bin_start.p;
-        la.slot_local = c.slot_of.p;
-        la.key = c.key.p;
-        la.fwd = c.stencil.p;
-        la.nfwd = int(c.fwd.size());
-        la.full = c.stencil_full.p;
-        la.nfull = int(c.full.size());
-        la.box = c.box;
-        la.grid = c.grid;
-        la.gc = c.ghost_cutoff;
-        la.uniform = c.uniform ? 1 : 0;
-        la.cutoff_sq = c.cutoff_sq;
-        la.sf2 = c.sf2;
-        la.nbr = c.nbr.p;
-        la.nnb = c.nnb.p;
-        la.nhalf = c.nhalf.p;
-        la.st = c.status.p;
-        const int64_t threads = n << c.build_lg;
-        switch (c.build_lg) {
-            case 2:
-                k_build_rows<2><<<blocks_for(threads, 256), 256, 0, c.stream>>>(la);
-                break;
-            case 4:
-                k_build_rows<4><<<blocks_for(threads, 256), 256, 0, c.stream>>>(la);
-                break;
-            default:
-                k_build_rows<3><<<blocks_for(threads, 256), 256, 0, c.stream>>>(la);
-                break;
-        }
+            c.ghost_code.p, c.bin_pos.p, c.bin_entry.p, c.bin_gidx.p, c.slot_of.p);
+        // rows: one wavefront per cell, then image-owned pairs appended
+        CellArgs ca{};
+        ca.ncells = ncells;
+        ca.npad = c.npad;
+        ca.maxnb = c.maxnb;
+        ca.lg = c.lg;
+        ca.bin_pos = c.bin_pos.p;
+        ca.bin_entry = c.bin_entry.p;
+        ca.bin_gidx = c.bin_gidx.p;
+        ca.bin_start = c.bin_start.p;
+        ca.run_first = c.run_first.p;
+        ca.run_last = c.run_last.p;
+        ca.nruns = c.nruns;
+        ca.run0 = c.run0;
+        ca.cutoff_sq = c.cutoff_sq;
+        ca.sf2 = c.sf2;
+        ca.nbr = c.nbr.p;
+        ca.nnb = c.nnb.p;
+        ca.nhalf = c.nhalf.p;
+        ca.gnbr = c.gnbr.p;
+        ca.gcnt = c.gcnt_img.p;
+        const int cblocks = blocks_for(ncells, kCellWaves);
+        if (c.uniform)
+            k_build_cells<true><<<cblocks, 256, 0, c.stream>>>(ca);
+        else
+            k_build_cells<false><<<cblocks, 256, 0, c.stream>>>(ca);
+        k_merge_rows<<<blocks_for(n, 256), 256, 0, c.stream>>>(
+            n, c.npad, c.maxnb, c.lg, c.goff.p, ng_live, c.gcnt_img.p, c.gnbr.p, c.nbr.p, c.nnb.p,
+            c.nhalf.p, c.status.p);
         MDD_HIP(hipGetLastError());
         if (dem) {
             k_remap_history<<<blocks_for(n, 256), 256, 0, c.stream>>>(
@@ -701,9 +834,26 @@ void neighbor_setup(Ctx& c) {
     c.stencil.resize(c.fwd.size());
     MDD_HIP(hipMemcpy(c.stencil.p, c.fwd.data(), sizeof(int) * c.fwd.size(),
                       hipMemcpyHostToDevice));
-    c.stencil_full.resize(c.full.size());
-    MDD_HIP(hipMemcpy(c.stencil_full.p, c.full.data(), sizeof(int) * c.full.size(),
-                      hipMemcpyHostToDevice));
+    // the full stencil as runs of consecutive cell offsets (contiguous bins)
+    std::sort(c.full.begin(), c.full.end());
+    std::vector<int> rf, rl;
+    c.run0 = -1;
+    for (size_t k = 0; k < c.full.size(); k++) {
+        if (k == 0 || c.full[k] != c.full[k - 1] + 1) {
+            rf.push_back(c.full[k]);
+            rl.push_back(c.full[k]);
+        } else {
+            rl.back() = c.full[k];
+        }
+        if (c.full[k] == 0) c.run0 = int(rf.size()) - 1;
+    }
+    if (rf.size() > size_t(kMaxRuns) || c.run0 < 0)
+        throw ApiError(-5, "bin stencil has too many cell runs; raise bin_size");
+    c.nruns = int(rf.size());
+    c.run_first.resize(rf.size());
+    c.run_last.resize(rl.size());
+    MDD_HIP(hipMemcpy(c.run_first.p, rf.data(), sizeof(int) * rf.size(), hipMemcpyHostToDevice));
+    MDD_HIP(hipMemcpy(c.run_last.p, rl.data(), sizeof(int) * rl.size(), hipMemcpyHostToDevice));
     // save_build_positions cutoff cache (neighbor.rs:545-561)
     c.min_skin = mn;
     if (std::fabs(mx - mn) < 1e-15) {
