@@ -32,10 +32,32 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(w, precision, steps, warmup, profile_every=10):
+def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True):
+    """Warm up (the first three quarters of the warmup steps time the
+    pair-force launch shapes, 2, 4 and 8 lanes per particle, and keep the
+    fastest), then time exactly `steps` steps."""
     from mddem_amd import hip
     s = hip.from_workload(w, precision, box_exit=False)
-    s.run(warmup)
+    lanes_log2 = 2
+    trial = warmup // 4 if tune else 0
+    if trial >= 100:
+        s.run(warmup - 3 * trial - 1)  # equilibrate first
+        best = None
+        for lg in (1, 2, 3):
+            s.set_force_lanes(lg)
+            s.run(trial // 4)  # relayout rebuild
+            s.set_profiling(1)
+            s.run(trial - trial // 4)
+            ms, _ = s.force_kernel_stats()
+            log(f"[bench]   tune: {1 << lg} lanes/particle, pair-force kernel {ms * 1e3:.1f} us")
+            if best is None or ms < best[0]:
+                best = (ms, lg)
+        lanes_log2 = best[1]
+        s.set_force_lanes(lanes_log2)
+        s.set_profiling(0)
+        s.run(1)  # the last warmup step lays the list out for the chosen shape
+    else:
+        s.run(warmup)
     s.synchronize()
     th0 = s.thermo()
     s.set_profiling(profile_every)
@@ -48,7 +70,8 @@ def run_gpu(w, precision, steps, warmup, profile_every=10):
     avg_ms, launches = s.force_kernel_stats()
     res = dict(seconds=t1 - t0, steps=steps, n=w.n, rebuilds=th1["rebuilds"] - th0["rebuilds"],
                npairs=th1["npairs"], force_ms=avg_ms, force_launches=launches,
-               temperature=th1["temperature"], pe=th1["pe"], ke=th1["ke"])
+               temperature=th1["temperature"], pe=th1["pe"], ke=th1["ke"],
+               lanes=1 << lanes_log2)
     s.close()
     return res
 
@@ -90,7 +113,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--ncell", type=int, default=40)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--fp64-steps", type=int, default=5000)
@@ -153,6 +176,7 @@ def main():
                                "T*=0.85 Nose-Hoover tau=1, dt=0.005, rc=2.5, skin 2.8",
                    "precision": args.precision, "rebuild": "displacement (lazy wrap)",
                    "rebuilds_in_timed_loop": r["rebuilds"],
+                   "force_lanes_per_particle": r["lanes"],
                    "pairs_per_atom_half": round(r["npairs"] / r["n"], 3)},
         "fp64_gpu_value": None if fp64 is None else round(fp64, 1),
         "mixed_over_fp64": None if fp64 is None else round(value / fp64, 3),

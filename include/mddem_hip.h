@@ -176,6 +176,10 @@ int mdd_set_profiling(mdd_ctx* ctx, int every);
 /* Average device time (ms) of the timed pair-force launches since the last
  * mdd_set_profiling, and the number of pair-force launches. */
 int mdd_force_kernel_stats(mdd_ctx* ctx, double* avg_ms, int64_t* launches);
+/* Lanes per particle in the pair-force kernels, as log2 (0..3; default 2).
+ * Changes the neighbor-row layout, so the next step rebuilds the list. No
+ * reference counterpart: launch-shape tuning. */
+int mdd_set_force_lanes(mdd_ctx* ctx, int log2_lanes);
 
 #ifdef __cplusplus
 }

@@ -143,50 +143,74 @@ void run_steps(Ctx& c, int64_t nsteps) {
         mdd::launch_nh_seed(c, int(c.step & 1));
     }
     const int64_t end = c.step + nsteps;
-    while (c.step < end) {
-        const int64_t s = c.step + 1;
-        const bool forced = !c.list_valid || c.list_injected ||
-                            (c.sort_every > 0 && s % c.sort_every == 0);
-        if (forced) {
-            // FullRebuild step: integrate, rebuild, force, integrate
-            mdd::launch_pre(c, int(s), false);
-            mdd::rebuild(c);
-            harvest_profile(c);
-            mdd::launch_pair_forces(c, int(s), false);
-            mdd::launch_post(c, int(s));
-            c.step = s;
-            continue;
+    // resume at the force pass of step ab after the device stopped there:
+    // a rebuild request from the drift, or a rebuild that did not fit
+    auto resume = [&](int ab) {
+        const bool retry = (c.status_host->overflow & 3u) != 0u;
+        k_set_abort<<<1, 1, 0, c.stream>>>(c.status.p, INT_MAX);
+        if (retry) {
+            mdd::rebuild_grow(c, *c.status_host);
+            if (++c.retries > 6) throw ApiError(MDD_ERR_CAPACITY, "neighbor rebuild did not fit");
+        } else {
+            c.retries = 0;
         }
-        int64_t K = std::min<int64_t>(c.chunk, end - c.step);
-        if (c.sort_every > 0) {
-            const int64_t next_sort = (c.step / c.sort_every + 1) * c.sort_every;
-            K = std::min<int64_t>(K, next_sort - 1 - c.step);
+        mdd::rebuild_async(c, ab, retry);
+        mdd::launch_pair_forces(c, ab, false);
+        mdd::launch_post(c, ab);
+        if (!retry) {
+            const int64_t ran = int64_t(ab) - c.step;
+            if (ran > 0) c.chunk = int(std::max<int64_t>(2, ran + ran / 4 + 1));
         }
-        if (K < 1) K = 1;
-        for (int64_t k = 1; k <= K; k++) {
-            const int st = int(c.step + k);
-            mdd::launch_pre(c, st, true);
-            mdd::launch_pair_forces(c, st, false);
-            mdd::launch_post(c, st);
-        }
+        c.step = ab;
+    };
+    auto check_status = [&]() {
         sync_status(c);
         harvest_profile(c);
         check_device_error(c);
-        const int ab = c.status_host->abort_step;
-        if (ab != INT_MAX) {
-            k_set_abort<<<1, 1, 0, c.stream>>>(c.status.p, INT_MAX);
-            mdd::rebuild(c);
-            harvest_profile(c);
-            mdd::launch_pair_forces(c, ab, false);
-            mdd::launch_post(c, ab);
-            const int64_t ran = int64_t(ab) - c.step;
-            c.step = ab;
-            c.chunk = int(std::max<int64_t>(2, ran + ran / 4 + 1));
-        } else {
-            c.step += K;
-            c.chunk = std::min(c.chunk * 2, 1024);
+        if (c.status_host->overflow & 4u) mdd::rebuild_grow(c, *c.status_host);  // throws
+        return c.status_host->abort_step;
+    };
+    for (;;) {
+        while (c.step < end) {
+            const int64_t s = c.step + 1;
+            const bool forced = !c.list_valid || c.list_injected || c.list_lg != c.lg ||
+                                (c.sort_every > 0 && s % c.sort_every == 0);
+            if (forced) {
+                // FullRebuild step: integrate, rebuild, force, integrate
+                mdd::launch_pre(c, int(s), false);
+                mdd::rebuild_async(c, int(s), false);
+                mdd::launch_pair_forces(c, int(s), false);
+                mdd::launch_post(c, int(s));
+                c.step = s;
+                continue;
+            }
+            int64_t K = std::min<int64_t>(c.chunk, end - c.step);
+            if (c.sort_every > 0) {
+                const int64_t next_sort = (c.step / c.sort_every + 1) * c.sort_every;
+                K = std::min<int64_t>(K, next_sort - 1 - c.step);
+            }
+            if (K < 1) K = 1;
+            for (int64_t k = 1; k <= K; k++) {
+                const int st = int(c.step + k);
+                mdd::launch_pre(c, st, true);
+                mdd::launch_pair_forces(c, st, false);
+                mdd::launch_post(c, st);
+            }
+            const int ab = check_status();
+            if (ab != INT_MAX) {
+                resume(ab);
+            } else {
+                c.step += K;
+                c.retries = 0;
+                c.nghost = c.status_host->ghost_total;
+                c.chunk = std::min(c.chunk * 2, 1024);
+            }
         }
+        const int ab = check_status();
+        if (ab == INT_MAX) break;
+        resume(ab);
     }
+    c.nghost = c.status_host->ghost_total;
     // post half of the last step's thermostat update -> Status::p_xi, ke
     if (c.nh_on) mdd::launch_nh_finish(c, int(c.step & 1));
     sync_status(c);
@@ -688,6 +712,7 @@ int mdd_set_half_list(mdd_ctx* ctx, int64_t nlocal, const uint32_t* offsets,
         }
         c.list_valid = true;
         c.list_injected = true;
+        c.list_lg = c.lg;
         c.npairs = offsets[c.n];
     });
 }
@@ -716,7 +741,7 @@ int mdd_download_pairs(mdd_ctx* ctx, int64_t cap, uint32_t* owner_tag, uint32_t*
         for (int64_t i = 0; i < n; i++) {
             const int cnt = std::min<int>(int(nn[i]), c.maxnb);
             for (int m = 0; m < cnt; m++) {
-                const uint32_t e = nb[mdd::row_addr(i, m, np, c.lg)];
+                const uint32_t e = nb[mdd::row_addr(i, m, np, c.list_lg)];
                 if (mdd::entry_rev(e)) continue;
                 if (k < cap) {
                     if (owner_tag) owner_tag[k] = tg[i];
@@ -842,7 +867,7 @@ int mdd_download_history(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* 
             const int cnt = std::min<int>(int(nn[i]), c.maxnb);
             for (int m = 0; m < cnt; m++) {
                 if (!((mk[i] >> m) & 1ull)) continue;
-                const int64_t slot = mdd::row_addr(i, m, np, c.lg);
+                const int64_t slot = mdd::row_addr(i, m, np, c.list_lg);
                 const uint32_t e = nb[slot];
                 if (mdd::entry_rev(e)) continue;
                 if (k < cap) {
@@ -924,6 +949,14 @@ int mdd_force_kernel_stats(mdd_ctx* ctx, double* avg_ms, int64_t* launches) {
     return guarded(ctx, [&](Ctx& c) {
         if (avg_ms) *avg_ms = c.prof_count ? c.prof_ms_sum / double(c.prof_count) : 0.0;
         if (launches) *launches = c.force_launches;
+    });
+}
+
+int mdd_set_force_lanes(mdd_ctx* ctx, int log2_lanes) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (log2_lanes < 0 || log2_lanes > 3)
+            throw ApiError(MDD_ERR_INVALID, "log2 lanes per particle must be 0..3");
+        c.lg = log2_lanes;  // the next step rebuilds the list in the new layout
     });
 }
 

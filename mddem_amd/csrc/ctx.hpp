@@ -188,7 +188,8 @@ struct Ctx {
     int64_t ghost_cap = 0, nghost = 0;
     int maxnb = 0;          // row capacity of the full list (multiple of 1 << lg)
     int lg = 2;             // log2 of lanes per particle in the pair-force kernels
-    int lg_old = 2;
+    int list_lg = 2;        // the row layout the current list was built with
+    int lg_old = 2;         // the row layout of the previous list
     int64_t npairs = 0;
     bool list_valid = false;
     bool list_injected = false;
@@ -222,11 +223,14 @@ struct Ctx {
     int nruns = 0, run0 = 0;        // run0: the run holding offset 0
     DBuf<uint32_t> bin_gidx;        // bin slot of a periodic image -> image index
     DBuf<uint32_t> gnbr, gcnt_img;  // per image: particles owning a pair with it
+    DBuf<uint32_t> nmain;           // row length from the particle's own cell pass
+    DBuf<uint32_t> row_blk;         // per-block row maxima of the last build
 
     // ---- full neighbor list: entry k of particle i at row_addr(i, k, npad, lg)
     DBuf<uint32_t> nbr, nbr_old;
     DBuf<uint32_t> nnb, nnb_old, nhalf;
     int64_t npad_old = 0;
+    bool remap_have_old = false;
     int maxnb_old = 0;
     // DEM shear history per list slot, canonical sign (lower tag's frame)
     DBuf<double> hist, hist_old;  // 3 * maxnb * npad, SoA by component
@@ -243,6 +247,7 @@ struct Ctx {
     int64_t step = 0, rebuilds = 0;
     int64_t sort_counter = 0;
     int chunk = 8;
+    int retries = 0;  // consecutive builds that did not fit
 
     // ---- profiling of the pair-force kernel
     int prof_every = 0;
@@ -265,6 +270,8 @@ void fill_f64(Ctx& c, double* p, double v, int64_t n);
 void fill_f32(Ctx& c, float* p, float v, int64_t n);
 // neighbor.hip
 void rebuild(Ctx& c);
+void rebuild_async(Ctx& c, int step, bool retry);
+bool rebuild_grow(Ctx& c, const Status& s);
 // forces.hip
 void launch_pair_forces(Ctx& c, int step, bool energy);
 void launch_walls_gravity(Ctx& c, bool walls, bool gravity);

@@ -39,6 +39,7 @@ struct ForceArgs {
     const double* mass;
     const uint32_t* nbr;
     const uint32_t* nnb;
+    const uint32_t* gcnt;  // periodic images per particle at the last build, or null
     double L[3];
     double lj1, lj2, cut2, e12, e6;
     const double* mat;
@@ -105,8 +106,15 @@ __device__ inline uint64_t group_or(uint64_t v) {
 
 // lj_force (md_lj/src/lib.rs:488-633): for each neighbor inside the cutoff,
 // fpair = r^-2 r^-6 (lj1 r^-6 - lj2), owner gets fma(-fpair, d, f),
-// partner fma(fpair, d, f), d = partner image - owner. Pairs outside the
-// cutoff contribute fma(0, d, f) = f, which leaves the sum unchanged.
+// partner fma(fpair, d, f), d = partner image - owner.
+//
+// Seen from particle i, with the periodic shift S applied to the non-owner:
+//   i owns the pair:  d  = (x_j + S) - x_i,  F_i = -fpair d
+//   j owns the pair:  d' = (x_i + S) - x_j,  F_i = +fpair d'
+// and since fl(a - b) = -fl(b - a), d'' = x_j - (x_i + S) = -d' exactly, so
+// F_i = -fpair d'' in both cases: d = (x_j + S_j) - (x_i + S_i) with S on
+// the non-owner. Without an image both shifts vanish and d = x_j - x_i.
+// Pairs outside the cutoff contribute fma(0, d, f) = f.
 template <typename R, bool ENERGY, int LG>
 __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     constexpr int G = 1 << LG;
@@ -123,33 +131,44 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     const R lj1 = R(a.lj1), lj2 = R(a.lj2);
     const uint32_t* row = a.nbr + (i << LG) + t;
     const int64_t stride = a.npad << LG;
-#pragma unroll 4
-    for (int k = t; k < cnt; k += G) {
-        const uint32_t e = row[int64_t(k >> LG) * stride];
-        const uint32_t j = entry_idx(e);
-        const int code = entry_code(e);
-        const double4 xj = a.pos[j];
-        const double shx = image_shift(code_sx(code), a.L[0]);
-        const double shy = image_shift(code_sy(code), a.L[1]);
-        const double shz = image_shift(code_sz(code), a.L[2]);
-        const bool rev = entry_rev(e);
-        const double ox = rev ? xj.x : xi.x, oy = rev ? xj.y : xi.y, oz = rev ? xj.z : xi.z;
-        const double px = rev ? xi.x : xj.x, py = rev ? xi.y : xj.y, pz = rev ? xi.z : xj.z;
-        const double dx = (px + shx) - ox;
-        const double dy = (py + shy) - oy;
-        const double dz = (pz + shz) - oz;
+    // a row holds periodic-image entries only if its particle had images at
+    // the last build; wavefronts without any take the shift-free loop
+    const bool img = a.gcnt == nullptr || a.gcnt[i] != 0u;
+    auto pair = [&](uint32_t e, double dx, double dy, double dz) {
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
         const bool in = r2 < a.cut2;
         const R r2inv = rcp<R>(R(r2));
         const R r6inv = r2inv * r2inv * r2inv;
         const R fpair = r2inv * r6inv * fma(lj1, r6inv, -lj2);
-        const R s = in ? (rev ? fpair : -fpair) : R(0);
+        const R s = in ? -fpair : R(0);
         fx = fma(s, R(dx), fx);
         fy = fma(s, R(dy), fy);
         fz = fma(s, R(dz), fz);
         if (ENERGY && in) {
             const double r6 = double(r6inv);
             pe += 0.5 * (r6 * (a.e12 * r6 - a.e6));
+        }
+    };
+    if (__ballot(img) == 0ull) {
+#pragma unroll 4
+        for (int k = t; k < cnt; k += G) {
+            const uint32_t e = row[int64_t(k >> LG) * stride];
+            const double4 xj = a.pos[entry_idx(e)];
+            pair(e, xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
+        }
+    } else {
+#pragma unroll 4
+        for (int k = t; k < cnt; k += G) {
+            const uint32_t e = row[int64_t(k >> LG) * stride];
+            const double4 xj = a.pos[entry_idx(e)];
+            const int code = entry_code(e);
+            const bool rev = entry_rev(e);
+            const double shx = image_shift(code_sx(code), a.L[0]);
+            const double shy = image_shift(code_sy(code), a.L[1]);
+            const double shz = image_shift(code_sz(code), a.L[2]);
+            pair(e, (xj.x + (rev ? 0.0 : shx)) - (xi.x + (rev ? shx : 0.0)),
+                 (xj.y + (rev ? 0.0 : shy)) - (xi.y + (rev ? shy : 0.0)),
+                 (xj.z + (rev ? 0.0 : shz)) - (xi.z + (rev ? shz : 0.0)));
         }
     }
     fx = group_sum<G>(fx);
@@ -453,6 +472,7 @@ ForceArgs make_args(Ctx& c, int step) {
     a.mass = c.mass.p;
     a.nbr = c.nbr.p;
     a.nnb = c.nnb.p;
+    a.gcnt = c.list_injected ? nullptr : c.gcnt.p;
     for (int d = 0; d < 3; d++) a.L[d] = c.box.L[d];
     a.lj1 = c.lj1;
     a.lj2 = c.lj2;
@@ -481,9 +501,9 @@ ForceArgs make_args(Ctx& c, int step) {
 
 template <typename R, bool E>
 void launch_lj(Ctx& c, const ForceArgs& a) {
-    const int64_t threads = c.n << c.lg;
+    const int64_t threads = c.n << c.list_lg;
     const int nb = blocks_for(threads, 256);
-    switch (c.lg) {
+    switch (c.list_lg) {
         case 0: k_lj<R, E, 0><<<nb, 256, 0, c.stream>>>(a); break;
         case 1: k_lj<R, E, 1><<<nb, 256, 0, c.stream>>>(a); break;
         case 3: k_lj<R, E, 3><<<nb, 256, 0, c.stream>>>(a); break;
@@ -493,9 +513,9 @@ void launch_lj(Ctx& c, const ForceArgs& a) {
 
 template <typename R>
 void launch_hertz(Ctx& c, const ForceArgs& a) {
-    const int64_t threads = c.n << c.lg;
+    const int64_t threads = c.n << c.list_lg;
     const int nb = blocks_for(threads, 256);
-    switch (c.lg) {
+    switch (c.list_lg) {
         case 0: k_hertz<R, 0><<<nb, 256, 0, c.stream>>>(a); break;
         case 1: k_hertz<R, 1><<<nb, 256, 0, c.stream>>>(a); break;
         case 3: k_hertz<R, 3><<<nb, 256, 0, c.stream>>>(a); break;

@@ -468,47 +468,106 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
     }
 }
 
-// Append to each particle's row the pairs owned through its periodic images
-// (in image order), then report the row maxima and capacity overflow.
-__global__ void __launch_bounds__(256) k_merge_rows(int64_t n, int64_t npad, int maxnb, int lg,
-                                                    const uint32_t* __restrict__ goff,
+// Row length of every particle: its own cell pass plus the pairs owned
+// through each of its periodic images. Row maxima go to per-block slots
+// (blk[3 b .. 3 b + 2]); k_row_finish reduces them, so no two blocks contend
+// for one address.
+__global__ void __launch_bounds__(256) k_row_totals(int64_t n, const uint32_t* __restrict__ goff,
                                                     const uint32_t* __restrict__ ng_live,
                                                     const uint32_t* __restrict__ gcnt,
-                                                    const uint32_t* __restrict__ gnbr,
-                                                    uint32_t* __restrict__ nbr,
+                                                    const uint32_t* __restrict__ nmain,
+                                                    const uint32_t* __restrict__ nhalf,
                                                     uint32_t* __restrict__ nnb,
-                                                    const uint32_t* __restrict__ nhalf, Status* st) {
+                                                    uint32_t* __restrict__ blk) {
+    __shared__ uint32_t lds[2][4];
     const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     uint32_t cnt = 0, nh = 0;
     if (q < n) {
-        cnt = nnb[q];
+        cnt = nmain[q];
         nh = nhalf[q];
-        const uint32_t g0 = goff[q], g1 = goff[q + 1];
-        if (*ng_live != 0u && g1 > g0) {
-            for (uint32_t g = g0; g < g1; g++) {
-                const uint32_t c = gcnt[g];
-                const uint32_t cc = c < uint32_t(maxnb) ? c : uint32_t(maxnb);
-                for (uint32_t k = 0; k < cc; k++) {
-                    const uint32_t kk = cnt + k;
-                    if (kk < uint32_t(maxnb))
-                        nbr[row_addr(q, int(kk), npad, lg)] = gnbr[size_t(g) * uint32_t(maxnb) + k];
-                }
-                cnt += c;
-            }
-            nnb[q] = cnt;
-        }
+        if (*ng_live != 0u)
+            for (uint32_t g = goff[q]; g < goff[q + 1]; g++) cnt += gcnt[g];
+        nnb[q] = cnt;
     }
-    uint32_t mc = cnt, mh = nh;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t a = __shfl_xor(mc, o, 64), b = __shfl_xor(mh, o, 64);
-        mc = a > mc ? a : mc;
-        mh = b > mh ? b : mh;
+        const uint32_t x = __shfl_xor(cnt, o, 64), y = __shfl_xor(nh, o, 64);
+        cnt = x > cnt ? x : cnt;
+        nh = y > nh ? y : nh;
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&st->max_row, int(mc));
-        atomicMax(&st->max_half, int(mh));
+        lds[0][w] = cnt;
+        lds[1][w] = nh;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t mc = 0, mh = 0;
+        for (int k = 0; k < int(blockDim.x >> 6); k++) {
+            mc = lds[0][k] > mc ? lds[0][k] : mc;
+            mh = lds[1][k] > mh ? lds[1][k] : mh;
+        }
+        blk[2 * blockIdx.x] = mc;
+        blk[2 * blockIdx.x + 1] = mh;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_row_finish(int nblk, const uint32_t* __restrict__ blk,
+                                                    int maxnb, Status* st) {
+    __shared__ uint32_t lds[2][4];
+    uint32_t mc = 0, mh = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        mc = blk[2 * b] > mc ? blk[2 * b] : mc;
+        mh = blk[2 * b + 1] > mh ? blk[2 * b + 1] : mh;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t x = __shfl_xor(mc, o, 64), y = __shfl_xor(mh, o, 64);
+        mc = x > mc ? x : mc;
+        mh = y > mh ? y : mh;
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][w] = mc;
+        lds[1][w] = mh;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < int(blockDim.x >> 6); k++) {
+            mc = lds[0][k] > mc ? lds[0][k] : mc;
+            mh = lds[1][k] > mh ? lds[1][k] : mh;
+        }
+        st->max_row = int(mc);
+        st->max_half = int(mh);
         if (mc > uint32_t(maxnb)) atomicOr(&st->overflow, 2u);
+    }
+}
+
+// Copy each image's scratch row to the end of its particle's row: after the
+// particle's own pass and the rows of the particle's earlier images.
+constexpr int kCopyLanes = 16;
+__global__ void __launch_bounds__(256) k_copy_image_rows(int64_t ghost_cap, int64_t npad,
+                                                         int maxnb, int lg,
+                                                         const uint32_t* __restrict__ ng_live,
+                                                         const uint32_t* __restrict__ groot,
+                                                         const uint32_t* __restrict__ goff,
+                                                         const uint32_t* __restrict__ gcnt,
+                                                         const uint32_t* __restrict__ nmain,
+                                                         const uint32_t* __restrict__ gnbr,
+                                                         uint32_t* __restrict__ nbr) {
+    const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const int64_t g = gt / kCopyLanes;
+    const uint32_t t = uint32_t(gt % kCopyLanes);
+    if (g >= ghost_cap || g >= int64_t(*ng_live)) return;
+    const uint32_t p = groot[g];
+    uint32_t off = nmain[p];
+    for (uint32_t h = goff[p]; h < uint32_t(g); h++) off += gcnt[h];
+    uint32_t c = gcnt[g];
+    if (c > uint32_t(maxnb)) c = uint32_t(maxnb);
+    for (uint32_t k = t; k < c; k += kCopyLanes) {
+        const uint32_t kk = off + k;
+        if (kk < uint32_t(maxnb))
+            nbr[row_addr(p, int(kk), npad, lg)] = gnbr[size_t(g) * uint32_t(maxnb) + k];
     }
 }
 
@@ -566,7 +625,23 @@ __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
 // neighbor_setup + compute_bin_grid (neighbor.rs:480-541, 290-371) on the host
 void neighbor_setup(Ctx& c);
 
-void rebuild(Ctx& c) {
+namespace {
+
+__global__ void k_status_reset_attempt(Status* st) {
+    st->overflow &= 4u;  // keep the fixed-boundary exit seen by the wrap
+    st->max_row = 0;
+    st->max_half = 0;
+    st->ghost_total = 0u;
+}
+
+// a build that did not fit stops the step sequence at this step
+__global__ void k_overflow_gate(Status* st, int step) {
+    if (st->overflow != 0u) atomicMin(&st->abort_step, step);
+}
+
+// pbc, spatial sort of the particles, and the previous list kept as the
+// source of the shear-history remap
+void rebuild_prepare(Ctx& c) {
     if (!c.setup_done) neighbor_setup(c);
     const int64_t n = c.n;
     const int64_t ncells = c.grid.ncells;
@@ -594,186 +669,227 @@ void rebuild(Ctx& c) {
     counting_sort(c, c.key.p, n, nullptr, ncells, c.cnt.p, c.start.p, c.placed.p, c.order.p,
                   nullptr);
     // permute every particle array into the alternates, then swap
-    {
-        PermArgs a{};
-        a.n = n;
-        a.npad = c.npad;
-        a.order = c.order.p;
-        a.pos = c.pos.p;
-        a.vel = c.vel.p;
-        a.omega = c.omega.p;
-        a.quat = c.quat.p;
-        a.pos2 = c.pos2.p;
-        a.vel2 = c.vel2.p;
-        a.omega2 = c.omega2.p;
-        a.quat2 = c.quat2.p;
-        a.build_pos = c.build_pos.p;
-        a.tag = c.tag.p;
-        a.type = c.type.p;
-        a.key = c.key.p;
-        a.tag2 = c.tag2.p;
-        a.type2 = c.type2.p;
-        a.key2 = c.key2.p;
-        a.image = c.image.p;
-        a.image2 = c.image2.p;
-        a.mass = c.mass.p;
-        a.cutr = c.cutr.p;
-        a.mass2 = c.mass2.p;
-        a.cutr2 = c.cutr2.p;
-        if (mixed) {
-            a.f32 = c.f32.p;
-            a.f32b = c.f32b.p;
-            a.t32 = dem ? c.t32.p : nullptr;
-            a.t32b = dem ? c.t32b.p : nullptr;
-        } else {
-            a.f64 = c.f64.p;
-            a.f64b = c.f64b.p;
-            a.t64 = dem ? c.t64.p : nullptr;
-            a.t64b = dem ? c.t64b.p : nullptr;
-        }
-        a.dem = dem;
-        k_permute<<<blocks_for(n, 256), 256, 0, c.stream>>>(a);
-        MDD_HIP(hipGetLastError());
-        c.pos.swap(c.pos2);
-        c.vel.swap(c.vel2);
-        c.key.swap(c.key2);
-        c.tag.swap(c.tag2);  // tag2 keeps the pre-sort tags for the history remap
-        c.type.swap(c.type2);
-        c.image.swap(c.image2);
-        c.mass.swap(c.mass2);
-        c.cutr.swap(c.cutr2);
-        if (mixed) {
-            c.f32.swap(c.f32b);
-            if (dem) c.t32.swap(c.t32b);
-        } else {
-            c.f64.swap(c.f64b);
-            if (dem) c.t64.swap(c.t64b);
-        }
-        if (dem) {
-            c.omega.swap(c.omega2);
-            c.quat.swap(c.quat2);
-        }
+    PermArgs a{};
+    a.n = n;
+    a.npad = c.npad;
+    a.order = c.order.p;
+    a.pos = c.pos.p;
+    a.vel = c.vel.p;
+    a.omega = c.omega.p;
+    a.quat = c.quat.p;
+    a.pos2 = c.pos2.p;
+    a.vel2 = c.vel2.p;
+    a.omega2 = c.omega2.p;
+    a.quat2 = c.quat2.p;
+    a.build_pos = c.build_pos.p;
+    a.tag = c.tag.p;
+    a.type = c.type.p;
+    a.key = c.key.p;
+    a.tag2 = c.tag2.p;
+    a.type2 = c.type2.p;
+    a.key2 = c.key2.p;
+    a.image = c.image.p;
+    a.image2 = c.image2.p;
+    a.mass = c.mass.p;
+    a.cutr = c.cutr.p;
+    a.mass2 = c.mass2.p;
+    a.cutr2 = c.cutr2.p;
+    if (mixed) {
+        a.f32 = c.f32.p;
+        a.f32b = c.f32b.p;
+        a.t32 = dem ? c.t32.p : nullptr;
+        a.t32b = dem ? c.t32b.p : nullptr;
+    } else {
+        a.f64 = c.f64.p;
+        a.f64b = c.f64b.p;
+        a.t64 = dem ? c.t64.p : nullptr;
+        a.t64b = dem ? c.t64b.p : nullptr;
+    }
+    a.dem = dem;
+    k_permute<<<blocks_for(n, 256), 256, 0, c.stream>>>(a);
+    MDD_HIP(hipGetLastError());
+    c.pos.swap(c.pos2);
+    c.vel.swap(c.vel2);
+    c.key.swap(c.key2);
+    c.tag.swap(c.tag2);  // tag2 keeps the pre-sort tags for the history remap
+    c.type.swap(c.type2);
+    c.image.swap(c.image2);
+    c.mass.swap(c.mass2);
+    c.cutr.swap(c.cutr2);
+    if (mixed) {
+        c.f32.swap(c.f32b);
+        if (dem) c.t32.swap(c.t32b);
+    } else {
+        c.f64.swap(c.f64b);
+        if (dem) c.t64.swap(c.t64b);
+    }
+    if (dem) {
+        c.omega.swap(c.omega2);
+        c.quat.swap(c.quat2);
     }
     // the previous list becomes the remap source
-    const bool have_old = c.list_valid && dem;
+    c.remap_have_old = c.list_valid && dem;
     c.nbr.swap(c.nbr_old);
     c.nnb.swap(c.nnb_old);
     c.hist.swap(c.hist_old);
     c.cmask.swap(c.cmask_old);
     c.npad_old = c.npad;
     c.maxnb_old = c.maxnb;
-    c.lg_old = c.lg;
+    c.lg_old = c.list_lg;
+}
 
-    for (int attempt = 0; attempt < 6; attempt++) {
-        if (c.ghost_cap < 1024) c.ghost_cap = 1024;
-        c.ghost_root.resize(c.ghost_cap);
-        c.ghost_code.resize(c.ghost_cap);
-        c.ghost_pos.resize(c.ghost_cap);
-        c.ghost_key.resize(c.ghost_cap + 1);  // last word: live image count
-        c.ghost_order.resize(c.ghost_cap);
-        c.placed.resize(std::max<int64_t>(n, c.ghost_cap) + 1);
-        const int64_t nall_max = n + c.ghost_cap;
-        c.bin_pos.resize(nall_max);
-        c.bin_entry.resize(nall_max);
-        c.bin_gidx.resize(nall_max);
-        c.slot_of.resize(n);
-        const int G = 1 << c.lg;
-        if (c.maxnb < G) c.maxnb = G;
-        c.maxnb = (c.maxnb + G - 1) / G * G;
-        if (int64_t(c.maxnb) * c.npad >= (int64_t(1) << 31) || c.npad >= (int64_t(1) << 24))
-            throw ApiError(-5, "neighbor list too large for 32-bit row indexing");
-        c.nbr.resize(size_t(c.maxnb) * c.npad);
-        c.gnbr.resize(size_t(c.ghost_cap) * c.maxnb);
-        c.gcnt_img.resize(c.ghost_cap);
-        c.nnb.resize(c.npad);
-        c.nhalf.resize(c.npad);
-        if (dem) {
-            if (c.maxnb > 64)
-                throw ApiError(-5, "DEM neighbor rows above 64 entries are not supported");
-            c.hist.resize(size_t(3) * c.maxnb * c.npad);
-            c.cmask.resize(c.npad);
-        }
-        uint32_t* ng_live = c.ghost_key.p + c.ghost_cap;
+// one build of images, bins and rows with the current capacities; the
+// status says afterwards whether everything fitted
+void rebuild_attempt(Ctx& c) {
+    const int64_t n = c.n;
+    const int64_t ncells = c.grid.ncells;
+    const bool dem = c.model == 1;
+    if (c.ghost_cap < 1024) c.ghost_cap = 1024;
+    c.ghost_root.resize(c.ghost_cap);
+    c.ghost_code.resize(c.ghost_cap);
+    c.ghost_pos.resize(c.ghost_cap);
+    c.ghost_key.resize(c.ghost_cap + 1);  // last word: live image count
+    c.ghost_order.resize(c.ghost_cap);
+    c.placed.resize(std::max<int64_t>(n, c.ghost_cap) + 1);
+    const int64_t nall_max = n + c.ghost_cap;
+    c.bin_pos.resize(nall_max);
+    c.bin_entry.resize(nall_max);
+    c.bin_gidx.resize(nall_max);
+    c.slot_of.resize(n);
+    const int G = 1 << c.lg;
+    if (c.maxnb < G) c.maxnb = G;
+    c.maxnb = (c.maxnb + G - 1) / G * G;
+    if (int64_t(c.maxnb) * c.npad >= (int64_t(1) << 31) || c.npad >= (int64_t(1) << 24))
+        throw ApiError(-5, "neighbor list too large for 32-bit row indexing");
+    c.nbr.resize(size_t(c.maxnb) * c.npad);
+    c.gnbr.resize(size_t(c.ghost_cap) * c.maxnb);
+    c.gcnt_img.resize(c.ghost_cap);
+    c.nnb.resize(c.npad);
+    c.nmain.resize(c.npad);
+    c.nhalf.resize(c.npad);
+    if (dem) {
+        if (c.maxnb > 64)
+            throw ApiError(-5, "DEM neighbor rows above 64 entries are not supported");
+        c.hist.resize(size_t(3) * c.maxnb * c.npad);
+        c.cmask.resize(c.npad);
+    }
+    uint32_t* ng_live = c.ghost_key.p + c.ghost_cap;
 
-        k_status_reset_build<<<1, 1, 0, c.stream>>>(c.status.p);
-        // borders: periodic images and their bins
-        k_ghost_count<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, c.pos.p, c.box, c.ghost_cutoff,
-                                                                 c.gcnt.p);
-        exclusive_scan(c, c.gcnt.p, c.goff.p, n);
-        k_ghost_emit<<<blocks_for(n, 256), 256, 0, c.stream>>>(
-            n, c.pos.p, c.box, c.ghost_cutoff, c.grid, c.goff.p, c.ghost_cap, c.ghost_root.p,
-            c.ghost_code.p, c.ghost_pos.p, c.ghost_key.p, ng_live, c.status.p);
-        // images binned after the particles of each cell (bin_neighbor_list
-        // steps 1-2b: counting sort in index order, images have the larger indices)
-        counting_sort(c, c.ghost_key.p, c.ghost_cap, ng_live, ncells, c.cnt.p, c.gstart.p,
-                      c.placed.p, c.ghost_order.p, nullptr);
-        k_bin_start<<<blocks_for(ncells + 1, 256), 256, 0, c.stream>>>(ncells, c.start.p,
-                                                                         c.gstart.p,
-                                                                         c.bin_start.p);
-        k_bin_fill<<<blocks_for(nall_max, 256), 256, 0, c.stream>>>(
-            n, c.ghost_cap, ng_live, c.key.p, c.start.p, c.gstart.p, c.bin_start.p,
-            c.ghost_order.p, c.ghost_key.p, c.pos.p, c.cutr.p, c.ghost_pos.p, c.ghost_root.p,
-            c.ghost_code.p, c.bin_pos.p, c.bin_entry.p, c.bin_gidx.p, c.slot_of.p);
-        // rows: one wavefront per cell, then image-owned pairs appended
-        CellArgs ca{};
-        ca.ncells = ncells;
-        ca.npad = c.npad;
-        ca.maxnb = c.maxnb;
-        ca.lg = c.lg;
-        ca.bin_pos = c.bin_pos.p;
-        ca.bin_entry = c.bin_entry.p;
-        ca.bin_gidx = c.bin_gidx.p;
-        ca.bin_start = c.bin_start.p;
-        ca.run_first = c.run_first.p;
-        ca.run_last = c.run_last.p;
-        ca.nruns = c.nruns;
-        ca.run0 = c.run0;
-        ca.cutoff_sq = c.cutoff_sq;
-        ca.sf2 = c.sf2;
-        ca.nbr = c.nbr.p;
-        ca.nnb = c.nnb.p;
-        ca.nhalf = c.nhalf.p;
-        ca.gnbr = c.gnbr.p;
-        ca.gcnt = c.gcnt_img.p;
-        const int cblocks = blocks_for(ncells, kCellWaves);
-        if (c.uniform)
-            k_build_cells<true><<<cblocks, 256, 0, c.stream>>>(ca);
-        else
-            k_build_cells<false><<<cblocks, 256, 0, c.stream>>>(ca);
-        k_merge_rows<<<blocks_for(n, 256), 256, 0, c.stream>>>(
-            n, c.npad, c.maxnb, c.lg, c.goff.p, ng_live, c.gcnt_img.p, c.gnbr.p, c.nbr.p, c.nnb.p,
-            c.nhalf.p, c.status.p);
+    k_status_reset_attempt<<<1, 1, 0, c.stream>>>(c.status.p);
+    // borders: periodic images and their bins
+    k_ghost_count<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, c.pos.p, c.box, c.ghost_cutoff,
+                                                             c.gcnt.p);
+    exclusive_scan(c, c.gcnt.p, c.goff.p, n);
+    k_ghost_emit<<<blocks_for(n, 256), 256, 0, c.stream>>>(
+        n, c.pos.p, c.box, c.ghost_cutoff, c.grid, c.goff.p, c.ghost_cap, c.ghost_root.p,
+        c.ghost_code.p, c.ghost_pos.p, c.ghost_key.p, ng_live, c.status.p);
+    // images binned after the particles of each cell (bin_neighbor_list
+    // steps 1-2b: counting sort in index order, images have the larger indices)
+    counting_sort(c, c.ghost_key.p, c.ghost_cap, ng_live, ncells, c.cnt.p, c.gstart.p, c.placed.p,
+                  c.ghost_order.p, nullptr);
+    k_bin_start<<<blocks_for(ncells + 1, 256), 256, 0, c.stream>>>(ncells, c.start.p, c.gstart.p,
+                                                                     c.bin_start.p);
+    k_bin_fill<<<blocks_for(nall_max, 256), 256, 0, c.stream>>>(
+        n, c.ghost_cap, ng_live, c.key.p, c.start.p, c.gstart.p, c.bin_start.p, c.ghost_order.p,
+        c.ghost_key.p, c.pos.p, c.cutr.p, c.ghost_pos.p, c.ghost_root.p, c.ghost_code.p,
+        c.bin_pos.p, c.bin_entry.p, c.bin_gidx.p, c.slot_of.p);
+    // rows: one wavefront per cell, then image-owned pairs appended
+    CellArgs ca{};
+    ca.ncells = ncells;
+    ca.npad = c.npad;
+    ca.maxnb = c.maxnb;
+    ca.lg = c.lg;
+    ca.bin_pos = c.bin_pos.p;
+    ca.bin_entry = c.bin_entry.p;
+    ca.bin_gidx = c.bin_gidx.p;
+    ca.bin_start = c.bin_start.p;
+    ca.run_first = c.run_first.p;
+    ca.run_last = c.run_last.p;
+    ca.nruns = c.nruns;
+    ca.run0 = c.run0;
+    ca.cutoff_sq = c.cutoff_sq;
+    ca.sf2 = c.sf2;
+    ca.nbr = c.nbr.p;
+    ca.nnb = c.nmain.p;
+    ca.nhalf = c.nhalf.p;
+    ca.gnbr = c.gnbr.p;
+    ca.gcnt = c.gcnt_img.p;
+    const int cblocks = blocks_for(ncells, kCellWaves);
+    if (c.uniform)
+        k_build_cells<true><<<cblocks, 256, 0, c.stream>>>(ca);
+    else
+        k_build_cells<false><<<cblocks, 256, 0, c.stream>>>(ca);
+    const int tblocks = blocks_for(n, 256);
+    c.row_blk.resize(size_t(2) * tblocks);
+    k_row_totals<<<tblocks, 256, 0, c.stream>>>(n, c.goff.p, ng_live, c.gcnt_img.p, c.nmain.p,
+                                                c.nhalf.p, c.nnb.p, c.row_blk.p);
+    k_row_finish<<<1, 256, 0, c.stream>>>(tblocks, c.row_blk.p, c.maxnb, c.status.p);
+    k_copy_image_rows<<<blocks_for(c.ghost_cap * kCopyLanes, 256), 256, 0, c.stream>>>(
+        c.ghost_cap, c.npad, c.maxnb, c.lg, ng_live, c.ghost_root.p, c.goff.p, c.gcnt_img.p,
+        c.nmain.p, c.gnbr.p, c.nbr.p);
+    MDD_HIP(hipGetLastError());
+    if (dem) {
+        k_remap_history<<<blocks_for(n, 256), 256, 0, c.stream>>>(
+            n, c.npad, c.maxnb, c.lg, c.nbr.p, c.nnb.p, c.tag.p, c.order.p, c.npad_old,
+            c.maxnb_old, c.lg_old, c.nbr_old.p, c.nnb_old.p, c.tag2.p, c.hist_old.p,
+            c.cmask_old.p, c.hist.p, c.cmask.p, c.remap_have_old ? 1 : 0);
         MDD_HIP(hipGetLastError());
-        if (dem) {
-            k_remap_history<<<blocks_for(n, 256), 256, 0, c.stream>>>(
-                n, c.npad, c.maxnb, c.lg, c.nbr.p, c.nnb.p, c.tag.p, c.order.p, c.npad_old,
-                c.maxnb_old, c.lg_old, c.nbr_old.p, c.nnb_old.p, c.tag2.p, c.hist_old.p,
-                c.cmask_old.p, c.hist.p, c.cmask.p, have_old ? 1 : 0);
-            MDD_HIP(hipGetLastError());
-        }
+    }
+    c.list_valid = true;
+    c.list_injected = false;
+    c.list_lg = c.lg;
+    c.npairs = -1;  // computed on request
+}
+
+}  // namespace
+
+// Capacities for the next attempt after a build that did not fit; false when
+// the status reports a complete build.
+bool rebuild_grow(Ctx& c, const Status& s) {
+    if (s.overflow & 4u)
+        throw ApiError(-5, "a particle left a non-periodic boundary (removal is not supported)");
+    bool again = false;
+    if (s.overflow & 1u) {
+        c.ghost_cap = int64_t(s.ghost_total) + int64_t(s.ghost_total) / 4 + 1024;
+        again = true;
+    }
+    if (s.overflow & 2u) {
+        c.maxnb = s.max_row + s.max_row / 8 + 8;
+        if (c.model == 1 && c.maxnb > 64) c.maxnb = s.max_row <= 64 ? 64 : s.max_row;
+        again = true;
+    }
+    return again;
+}
+
+// full rebuild, checked on the host before it returns
+void rebuild(Ctx& c) {
+    rebuild_prepare(c);
+    bool fitted = false;
+    for (int attempt = 0; attempt < 6 && !fitted; attempt++) {
+        rebuild_attempt(c);
         MDD_HIP(hipMemcpyAsync(c.status_host, c.status.p, sizeof(Status), hipMemcpyDeviceToHost,
                                c.stream));
         MDD_HIP(hipStreamSynchronize(c.stream));
-        const Status& s = *c.status_host;
-        if (s.overflow & 4u)
-            throw ApiError(-5, "a particle left a non-periodic boundary (removal is not supported)");
-        if (s.overflow & 1u) {
-            c.ghost_cap = int64_t(s.ghost_total) + int64_t(s.ghost_total) / 4 + 1024;
-            continue;
-        }
-        if (s.overflow & 2u) {
-            c.maxnb = s.max_row + s.max_row / 8 + 8;
-            if (dem && c.maxnb > 64) c.maxnb = s.max_row <= 64 ? 64 : s.max_row;
-            continue;
-        }
-        c.nghost = s.ghost_total;
-        break;
+        fitted = !rebuild_grow(c, *c.status_host);
     }
-    if (c.status_host->overflow) throw ApiError(-4, "neighbor rebuild did not fit after retries");
-    c.list_valid = true;
-    c.list_injected = false;
-    c.npairs = -1;  // computed on request
+    if (!fitted) throw ApiError(-4, "neighbor rebuild did not fit after retries");
+    c.nghost = c.status_host->ghost_total;
     c.rebuilds++;
+}
+
+// Rebuild inside a step sequence without waiting for the device: if the
+// build does not fit, the step's later kernels are gated off (as after a
+// rebuild request) and the host repeats the build with larger capacities,
+// starting from the same sorted state (retry = true), when it next reads
+// the status.
+void rebuild_async(Ctx& c, int step, bool retry) {
+    if (!retry) {
+        rebuild_prepare(c);
+        c.rebuilds++;
+    }
+    rebuild_attempt(c);
+    k_overflow_gate<<<1, 1, 0, c.stream>>>(c.status.p, step);
+    MDD_HIP(hipGetLastError());
 }
 
 void neighbor_setup(Ctx& c) {

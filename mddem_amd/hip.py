@@ -98,6 +98,7 @@ SIGNATURES = {
     "mdd_synchronize": (C.c_int, [_vp]),
     "mdd_set_profiling": (C.c_int, [_vp, C.c_int]),
     "mdd_force_kernel_stats": (C.c_int, [_vp, _dp, _i64p]),
+    "mdd_set_force_lanes": (C.c_int, [_vp, C.c_int]),
 }
 
 
@@ -325,6 +326,9 @@ class System:
         n = C.c_int64(0)
         self._c(self.L.mdd_force_kernel_stats(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def set_force_lanes(self, log2_lanes: int):
+        self._c(self.L.mdd_set_force_lanes(self.h, int(log2_lanes)))
 
 
 def from_workload(w, precision=FP64, box_exit=True, sort_every=1000, device=0) -> System:
