@@ -292,34 +292,71 @@ __device__ inline uint32_t lanes_before(uint64_t m) {
 }
 
 // Candidate chunks of one window, held in registers: lane l of chunk c is
-// flat candidate win + 64 c + l of the cell's stencil ranges.
+// flat candidate win + 64 c + l of the cell's stencil ranges (bin order).
 template <bool UNIFORM>
 struct Window {
     double x[kNch], y[kNch], z[kNch], w[kNch];
     uint32_t e[kNch];
-    uint64_t local[kNch];
-    int avail;  // candidates from the window start on
+    // lanes a particle of the cell may pair with: every particle, and every
+    // periodic image that is not in a backward cell (those pairs belong to
+    // the image's particle, found through its own images)
+    uint64_t okp[kNch];
 };
 
-// Pair tests of one member of the cell against the window. A particle
-// member writes its own row: candidates after it in bin order are pairs it
-// owns (particles and periodic images), particles before it own the pair
-// with it. An image member collects the particles before it in bin order
-// (they own the pair with the image) into the image's scratch row.
-template <bool UNIFORM, bool IMAGE>
-__device__ inline void test_member(const CellArgs& a, const Window<UNIFORM>& W, int nch, int fs,
-                                   const double4& xi, uint32_t code_i, uint32_t* __restrict__ out,
-                                   uint32_t q, uint32_t& cnt, uint32_t& own) {
+// Pair tests of one particle of the cell (flat index fs relative to the
+// window) against the window. Candidates before it in bin order own the
+// pair with it (entry flagged), candidates after it are pairs it owns.
+template <bool UNIFORM>
+__device__ inline void test_particle(const CellArgs& a, const Window<UNIFORM>& W, int nch,
+                                     int fs, const double4& xi, uint32_t q, uint32_t& cnt,
+                                     uint32_t& nrev) {
     const int lane = int(threadIdx.x & 63);
     const uint32_t G = 1u << a.lg;
 #pragma unroll
     for (int c = 0; c < kNch; c++) {
-        const int rel = fs - c * 64;
-        const uint64_t above = lanes_above(rel), below = lanes_below(rel);
-        const uint64_t valid = lanes_below(W.avail - c * 64);
-        const uint64_t cm = c >= nch ? 0ull
-                            : IMAGE ? (W.local[c] & below)
-                                    : (valid & (above | (below & W.local[c])));
+        if (c < nch) {
+            const int f = c * 64 + lane;
+            const double dx = W.x[c] - xi.x, dy = W.y[c] - xi.y, dz = W.z[c] - xi.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            bool in;
+            if (UNIFORM) {
+                in = r2 < a.cutoff_sq;
+            } else {
+                const double s = xi.w + W.w[c];
+                in = r2 < s * s * a.sf2;
+            }
+            const uint64_t pass = __ballot(in) & __ballot(f != fs) & W.okp[c];
+            if (pass != 0ull) {
+                const uint64_t before = __ballot(f < fs);
+                const uint32_t k = cnt + lanes_before(pass);
+                if (__builtin_amdgcn_inverse_ballot_w64(pass) && k < uint32_t(a.maxnb)) {
+                    // row_addr with 24-bit multiplies (npad < 2^24 is checked by the host)
+                    const uint32_t at =
+                        ((__umul24(k >> a.lg, uint32_t(a.npad)) + q) << a.lg) + (k & (G - 1u));
+                    a.nbr[at] = __builtin_amdgcn_inverse_ballot_w64(before) ? (W.e[c] | kRevBit)
+                                                                            : W.e[c];
+                }
+                cnt += uint32_t(__popcll(pass));
+                nrev += uint32_t(__popcll(pass & before));
+            }
+        }
+    }
+}
+
+// Pair tests of one periodic image of the cell: the particles before it in
+// bin order (backward cells and this cell) own the pair with it; they are
+// collected, flagged, into the image's scratch row.
+template <bool UNIFORM>
+__device__ inline void test_image(const CellArgs& a, const Window<UNIFORM>& W, int nch, int f_end,
+                                  const double4& xi, uint32_t code_i, uint32_t* __restrict__ out,
+                                  uint32_t& cnt) {
+    const int lane = int(threadIdx.x & 63);
+    const uint32_t tagbits = (code_i << kIdxBits) | kRevBit;
+#pragma unroll
+    for (int c = 0; c < kNch; c++) {
+        const uint64_t cm =
+            c < nch ? (__ballot(entry_code(W.e[c]) == kCodeNone) & lanes_below(f_end - c * 64))
+                    : 0ull;
         if (cm != 0ull) {
             const double dx = W.x[c] - xi.x, dy = W.y[c] - xi.y, dz = W.z[c] - xi.z;
             const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -333,18 +370,9 @@ __device__ inline void test_member(const CellArgs& a, const Window<UNIFORM>& W, 
             const uint64_t pass = __ballot(in) & cm;
             if (pass != 0ull) {
                 const uint32_t k = cnt + lanes_before(pass);
-                if (((pass >> lane) & 1ull) && k < uint32_t(a.maxnb)) {
-                    if (IMAGE) {
-                        out[k] = enc_entry(entry_idx(W.e[c]), int(code_i), true);
-                    } else {
-                        const bool mine = (above >> lane) & 1ull;
-                        const uint32_t at =
-                            (((k >> a.lg) * uint32_t(a.npad) + q) << a.lg) + (k & (G - 1u));
-                        out[at] = mine ? W.e[c] : (W.e[c] | kRevBit);
-                    }
-                }
+                if (__builtin_amdgcn_inverse_ballot_w64(pass) && k < uint32_t(a.maxnb))
+                    out[k] = entry_idx(W.e[c]) | tagbits;
                 cnt += uint32_t(__popcll(pass));
-                if (!IMAGE) own += uint32_t(__popcll(pass & above));
             }
         }
     }
@@ -386,10 +414,12 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
     const uint32_t rbase = rb - excl;  // bin index = flat index + rbase (mod 2^32)
     const uint32_t total = __builtin_amdgcn_readlane(incl, a.nruns - 1);
     const uint32_t base0 = __builtin_amdgcn_readlane(rbase, a.run0);
+    // flat indices of this cell's first slot and of the slot after it
+    const int f_i0 = int(i0 - base0), f_i1 = int(i1 - base0);
     const uint32_t ni = i1 - i0;
     for (uint32_t ib = 0; ib < ni; ib += 64) {
         const uint32_t nib = ni - ib < 64u ? ni - ib : 64u;
-        uint32_t cntv = 0, ownv = 0;  // lane l: member ib + l
+        uint32_t cntv = 0, revv = 0;  // lane l: member ib + l
         // the members of this batch, lane l holding member ib + l
         double4 mx = make_double4(0.0, 0.0, 0.0, 0.0);
         uint32_t me = 0u, mg = 0u;
@@ -399,9 +429,11 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
             me = a.bin_entry[slot];
             if (entry_code(me) != kCodeNone) mg = a.bin_gidx[slot];
         }
+        // members ib .. ib + nloc - 1 are particles (they precede the images)
+        const uint32_t nloc =
+            uint32_t(__popcll(__ballot(uint32_t(lane) < nib && entry_code(me) == kCodeNone)));
         for (uint32_t win = 0; win < total; win += kNch * 64) {
             Window<UNIFORM> W;
-            W.avail = int(total - win);
             uint32_t base[kNch];
 #pragma unroll
             for (int c = 0; c < kNch; c++) base[c] = 0;
@@ -427,40 +459,47 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
                 W.z[c] = q.z;
                 W.w[c] = q.w;
                 W.e[c] = e;
-                W.local[c] = __ballot(valid && entry_code(e) == kCodeNone);
+                // invalid lanes hold entry 0, whose image code is not kCodeNone
+                const bool particle = valid && entry_code(e) == kCodeNone;
+                W.okp[c] = __ballot(particle || (valid && int(f) >= f_i0));
             }
             const uint32_t left = (total - win + 63u) / 64u;
             const int nch = left < uint32_t(kNch) ? int(left) : kNch;
-            for (uint32_t ii = 0; ii < nib; ii++) {
-                const uint32_t slot = i0 + ib + ii;
+            const int shift = int(win);
+            for (uint32_t ii = 0; ii < nloc; ii++) {
                 const uint32_t ei = uint32_t(__builtin_amdgcn_readlane(int(me), int(ii)));
                 double4 xi;
                 xi.x = readlane_f64(mx.x, int(ii));
                 xi.y = readlane_f64(mx.y, int(ii));
                 xi.z = readlane_f64(mx.z, int(ii));
                 xi.w = UNIFORM ? 0.0 : readlane_f64(mx.w, int(ii));
-                const int fs = int(slot - base0 - win);
                 uint32_t cnt = __builtin_amdgcn_readlane(cntv, int(ii));
-                uint32_t own = __builtin_amdgcn_readlane(ownv, int(ii));
-                if (entry_code(ei) == kCodeNone) {
-                    test_member<UNIFORM, false>(a, W, nch, fs, xi, 0u, a.nbr, entry_idx(ei), cnt,
-                                                own);
-                } else {
-                    const uint32_t g = uint32_t(__builtin_amdgcn_readlane(int(mg), int(ii)));
-                    test_member<UNIFORM, true>(a, W, nch, fs, xi, uint32_t(entry_code(ei)),
-                                               a.gnbr + size_t(g) * uint32_t(a.maxnb), 0u, cnt,
-                                               own);
-                }
+                uint32_t nrev = __builtin_amdgcn_readlane(revv, int(ii));
+                const int fs = f_i0 + int(ib + ii) - shift;
+                test_particle<UNIFORM>(a, W, nch, fs, xi, entry_idx(ei), cnt, nrev);
                 if (uint32_t(lane) == ii) {
                     cntv = cnt;
-                    ownv = own;
+                    revv = nrev;
                 }
+            }
+            for (uint32_t ii = nloc; ii < nib; ii++) {
+                const uint32_t ei = uint32_t(__builtin_amdgcn_readlane(int(me), int(ii)));
+                double4 xi;
+                xi.x = readlane_f64(mx.x, int(ii));
+                xi.y = readlane_f64(mx.y, int(ii));
+                xi.z = readlane_f64(mx.z, int(ii));
+                xi.w = UNIFORM ? 0.0 : readlane_f64(mx.w, int(ii));
+                uint32_t cnt = __builtin_amdgcn_readlane(cntv, int(ii));
+                const uint32_t g = uint32_t(__builtin_amdgcn_readlane(int(mg), int(ii)));
+                test_image<UNIFORM>(a, W, nch, f_i1 - shift, xi, uint32_t(entry_code(ei)),
+                                    a.gnbr + size_t(g) * uint32_t(a.maxnb), cnt);
+                if (uint32_t(lane) == ii) cntv = cnt;
             }
         }
         if (uint32_t(lane) < nib) {
             if (entry_code(me) == kCodeNone) {
                 a.nnb[entry_idx(me)] = cntv;
-                a.nhalf[entry_idx(me)] = ownv;
+                a.nhalf[entry_idx(me)] = cntv - revv;
             } else {
                 a.gcnt[mg] = cntv;
             }
