@@ -76,6 +76,48 @@ def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True):
     return res
 
 
+def run_dem(steps, warmup, n=4_000_000):
+    """DEM settling line: polydisperse glass spheres between plane walls under
+    gravity, mixed and all-FP64, the same steps timed for each."""
+    from mddem_amd import hip
+    from mddem_amd import workloads as W
+    w = W.dem_settling(n=n)
+    out = {}
+    for name, prec in (("mixed", hip.MIXED), ("fp64", hip.FP64)):
+        s = hip.from_workload(w, prec, box_exit=False)
+        s.set_force_lanes(0)  # one lane per sphere: rows are short here
+        s.run(warmup)
+        s.synchronize()
+        th0 = s.thermo()
+        s.set_profiling(5)
+        t0 = time.perf_counter()
+        s.run(steps)
+        s.synchronize()
+        t1 = time.perf_counter()
+        th1 = s.thermo()
+        ms, _ = s.force_kernel_stats()
+        rate = w.n * steps / (t1 - t0)
+        # whole-step algorithmic bytes: 216 B (mixed) / 264 B (FP64) of state,
+        # 4 B per half-list pair, 56 B per contact with shear history
+        state = 216.0 if prec == hip.MIXED else 264.0
+        bps = state + (4.0 * th1["npairs"] + 56.0 * th1["ncontacts"]) / w.n
+        out[name] = dict(value=round(rate, 1), step_roofline_frac=round(rate * bps / 8e12, 4),
+                         ms_per_step=round(1e3 * (t1 - t0) / steps, 4),
+                         force_kernel_us=round(ms * 1e3, 2), npairs=th1["npairs"],
+                         ncontacts=th1["ncontacts"],
+                         rebuilds=th1["rebuilds"] - th0["rebuilds"])
+        s.close()
+        log(f"[bench] dem {name}: {out[name]['value']:.4g} particle-steps/s, "
+            f"{out[name]['ms_per_step']} ms/step, pairs {th1['npairs']}, "
+            f"contacts {th1['ncontacts']}")
+    out["mixed_over_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+    out["workload"] = (f"DEM settling, {w.n} glass spheres r~U[0.8,1.2] mm, box 0.4x0.4x1.2 m "
+                       "periodic x/y, plane walls z=0 and z=1.2 m, gravity, dt=0.05 t_Rayleigh")
+    out["steps"] = steps
+    out["warmup"] = warmup
+    return out
+
+
 def force_kernel_bytes(n, npairs, force_bytes):
     """Algorithmic HBM bytes of one pair-force launch: positions read once
     (24 B FP64), the half list streamed once (4 B per pair) and the force
@@ -119,6 +161,10 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--precision-report", type=int, default=2000,
+                    help="NVE steps of the mixed-vs-FP64 precision report (0 = skip)")
+    ap.add_argument("--dem-steps", type=int, default=500,
+                    help="timed steps of the DEM 4M line per precision (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU scaling is not "
@@ -153,6 +199,17 @@ def main():
                 "kernel": "k_lj (pair force)", "bytes_per_launch": alg,
                 "avg_launch_us": round(r["force_ms"] * 1e3, 2)}
 
+    precision = None
+    if args.precision_report > 0:
+        from mddem_amd import precision as P
+        precision = dict(force_error=P.force_errors(w),
+                         nve=P.nve_drift(w, args.precision_report))
+        log(f"[bench] precision: {json.dumps(precision)}")
+
+    dem = None
+    if args.dem_steps > 0:
+        dem = run_dem(args.dem_steps, 50)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -183,6 +240,13 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_over_cpu": None if cpu is None else round(value / cpu["value"], 1),
+        # whole-step algorithmic bytes per particle-step: 120 B (mixed) or 144 B
+        # (FP64) of state + 4 B per half-list pair, against the 8 TB/s HBM roofline
+        "step_roofline_frac": round(
+            value * ((120.0 if prec == hip.MIXED else 144.0) + 4.0 * r["npairs"] / r["n"]) / 8e12,
+            4),
+        "precision": precision,
+        "dem_4m": dem,
     }
     print(json.dumps(out), flush=True)
 
