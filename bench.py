@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precision-report", type=int, default=2000,
                     help="NVE steps of the mixed-vs-FP64 precision report (0 = skip)")
+    ap.add_argument("--serve-jobs", type=int, default=24,
+                    help="jobs in the serving-stream measurement (0 = skip)")
     ap.add_argument("--dem-steps", type=int, default=500,
                     help="timed steps of the DEM 4M line per precision (0 = skip)")
     args = ap.parse_args()
@@ -210,6 +212,17 @@ def main():
     if args.dem_steps > 0:
         dem = run_dem(args.dem_steps, 50)
 
+    serving = None
+    if args.serve_jobs > 0:
+        from mddem_amd import serve
+        jobs = serve.make_stream(args.serve_jobs, rate=30.0, steps=(1000, 5000))
+        for j in jobs:
+            j.workload = serve.build_workload(j)
+        serving = serve.Server(serve.default_budget(), slots=8).run(jobs)
+        serving["stream"] = (f"{args.serve_jobs} jobs, Poisson 30/s, 70% granular gas 1k-200k, "
+                             "30% LJ FCC 4k-500k, steps log-uniform [1000, 5000], mixed")
+        log(f"[bench] serving: {json.dumps(serving)}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -247,6 +260,7 @@ def main():
             4),
         "precision": precision,
         "dem_4m": dem,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -176,6 +176,9 @@ int mdd_set_profiling(mdd_ctx* ctx, int every);
 /* Average device time (ms) of the timed pair-force launches since the last
  * mdd_set_profiling, and the number of pair-force launches. */
 int mdd_force_kernel_stats(mdd_ctx* ctx, double* avg_ms, int64_t* launches);
+/* Free and total device memory of a device (hipMemGetInfo), for admission by
+ * footprint. No reference counterpart: serving-loop support. */
+int mdd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
 /* Lanes per particle in the pair-force kernels, as log2 (0..3; default 2).
  * Changes the neighbor-row layout, so the next step rebuilds the list. No
  * reference counterpart: launch-shape tuning. */

@@ -952,6 +952,18 @@ int mdd_force_kernel_stats(mdd_ctx* ctx, double* avg_ms, int64_t* launches) {
     });
 }
 
+int mdd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return MDD_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return MDD_ERR_HIP;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return MDD_ERR_HIP;
+    if (free_bytes) *free_bytes = int64_t(f);
+    if (total_bytes) *total_bytes = int64_t(t);
+    return MDD_OK;
+}
+
 int mdd_set_force_lanes(mdd_ctx* ctx, int log2_lanes) {
     return guarded(ctx, [&](Ctx& c) {
         if (log2_lanes < 0 || log2_lanes > 3)

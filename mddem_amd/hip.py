@@ -99,6 +99,7 @@ SIGNATURES = {
     "mdd_set_profiling": (C.c_int, [_vp, C.c_int]),
     "mdd_force_kernel_stats": (C.c_int, [_vp, _dp, _i64p]),
     "mdd_set_force_lanes": (C.c_int, [_vp, C.c_int]),
+    "mdd_device_memory": (C.c_int, [C.c_int, _i64p, _i64p]),
 }
 
 
@@ -329,6 +330,15 @@ class System:
 
     def set_force_lanes(self, log2_lanes: int):
         self._c(self.L.mdd_set_force_lanes(self.h, int(log2_lanes)))
+
+
+def device_memory(device: int = 0):
+    """(free, total) device memory in bytes."""
+    f, t = C.c_int64(0), C.c_int64(0)
+    rc = lib().mdd_device_memory(int(device), C.byref(f), C.byref(t))
+    if rc != 0:
+        raise MddError(rc, f"mdd_device_memory failed ({rc})")
+    return f.value, t.value
 
 
 def from_workload(w, precision=FP64, box_exit=True, sort_every=1000, device=0) -> System:
