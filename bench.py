@@ -163,6 +163,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precision-report", type=int, default=2000,
                     help="NVE steps of the mixed-vs-FP64 precision report (0 = skip)")
+    ap.add_argument("--ensemble-systems", type=int, default=1024,
+                    help="systems in the batched-ensemble measurement (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs in the serving-stream measurement (0 = skip)")
     ap.add_argument("--dem-steps", type=int, default=500,
@@ -212,6 +215,12 @@ def main():
     if args.dem_steps > 0:
         dem = run_dem(args.dem_steps, 50)
 
+    ens = None
+    if args.ensemble_systems > 0:
+        from mddem_amd import ensemble
+        ens = ensemble.run(args.ensemble_systems, args.ensemble_steps, log=log)
+        log(f"[bench] ensemble: {json.dumps(ens)}")
+
     serving = None
     if args.serve_jobs > 0:
         from mddem_amd import serve
@@ -260,6 +269,7 @@ def main():
             4),
         "precision": precision,
         "dem_4m": dem,
+        "ensemble": ens,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -108,6 +108,26 @@ int mdd_upload_state(mdd_ctx* ctx, int64_t n, const uint32_t* tag, const uint32_
                      const double* pos, const double* vel, const double* omega,
                      const double* radius, const double* cutoff_radius, const double* mass);
 int64_t mdd_nlocal(const mdd_ctx* ctx);
+
+/* ---- batched ensemble ------------------------------------------------------ */
+/* Append one independent, fully periodic DEM system (its own box [lo, hi),
+ * single material, particles) to the context. All systems then step together:
+ * one launch per phase covers every system, particles and bins are
+ * offset-indexed by system, and a neighbor rebuild of every system happens
+ * when any system's skin is spent. Each system follows the same per-system
+ * physics as a context holding it alone (setup: neighbor_setup per system,
+ * neighbor.rs:480-541; materials: MaterialTable::build_pair_tables,
+ * crates/dem_atom/src/lib.rs:389-515). Tags only need to be unique within a
+ * system. Replaces running many reference App instances, one per system. */
+int mdd_add_system(mdd_ctx* ctx, const double lo[3], const double hi[3], double youngs,
+                   double poisson, double restitution, double friction, int64_t n,
+                   const uint32_t* tag, const double* pos, const double* vel,
+                   const double* omega, const double* radius, const double* cutoff_radius,
+                   const double* mass);
+/* number of systems (0: a single system uploaded with mdd_upload_state) */
+int mdd_nsystems(mdd_ctx* ctx);
+/* system sys occupies particles [first, first + count) of every download */
+int mdd_system_range(mdd_ctx* ctx, int sys, int64_t* first, int64_t* count);
 /* current particle order (spatially sorted); any pointer may be NULL;
  * quat is n*4 (w,x,y,z) */
 int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, double* omega,

@@ -78,7 +78,10 @@ int guarded(mdd_ctx* ctx, F&& body) {
     }
 }
 
+void commit_systems(Ctx& c);
+
 void need_state(Ctx& c) {
+    if (c.systems_dirty) commit_systems(c);
     if (c.n <= 0) throw ApiError(MDD_ERR_INVALID, "no particles uploaded");
 }
 
@@ -128,8 +131,10 @@ void setup_nose_hoover(Ctx& c) {
 
 void run_steps(Ctx& c, int64_t nsteps) {
     need_state(c);
-    if (c.model == MDD_MODEL_DEM && c.mat.n == 0)
+    if (c.model == MDD_MODEL_DEM && c.mat.n == 0 && c.nsys == 0)
         throw ApiError(MDD_ERR_INVALID, "materials not set");
+    if (c.nsys > 0 && c.walls.count > 0)
+        throw ApiError(MDD_ERR_UNSUPPORTED, "walls in a batched ensemble");
     if (c.every != 0)
         throw ApiError(MDD_ERR_UNSUPPORTED, "only displacement rebuilds (every = 0) are supported");
     if (c.step + nsteps > INT_MAX - 2) throw ApiError(MDD_ERR_INVALID, "step count too large");
@@ -322,6 +327,58 @@ void pack_upload(Ctx& c, int64_t n, const uint32_t* tag, const uint32_t* type, c
     MDD_HIP(hipStreamSynchronize(c.stream));
 }
 
+// the staged systems of a batched ensemble as one particle set
+void commit_systems(Ctx& c) {
+    c.systems_dirty = false;
+    const int ns = int(c.staged.size());
+    int64_t n = 0;
+    for (const auto& S : c.staged) n += int64_t(S.tag.size());
+    std::vector<uint32_t> tag;
+    std::vector<uint32_t> type(static_cast<size_t>(n), 0u);
+    std::vector<uint32_t> psys(static_cast<size_t>(n), 0u);
+    std::vector<double> pos, vel, omega, radius, cutr, mass;
+    tag.reserve(n);
+    c.sys_first.assign(size_t(ns) + 1, 0);
+    c.sys_h.assign(size_t(ns), mdd::SysP{});
+    c.sys_rmin.assign(size_t(ns), 0.0);
+    c.sys_rmax.assign(size_t(ns), 0.0);
+    for (int s = 0; s < ns; s++) {
+        const auto& S = c.staged[s];
+        const int64_t k = int64_t(S.tag.size());
+        std::fill(psys.begin() + c.sys_first[s], psys.begin() + c.sys_first[s] + k, uint32_t(s));
+        c.sys_first[s + 1] = c.sys_first[s] + k;
+        tag.insert(tag.end(), S.tag.begin(), S.tag.end());
+        pos.insert(pos.end(), S.pos.begin(), S.pos.end());
+        vel.insert(vel.end(), S.vel.begin(), S.vel.end());
+        omega.insert(omega.end(), S.omega.begin(), S.omega.end());
+        radius.insert(radius.end(), S.radius.begin(), S.radius.end());
+        cutr.insert(cutr.end(), S.cutr.begin(), S.cutr.end());
+        mass.insert(mass.end(), S.mass.begin(), S.mass.end());
+        mdd::SysP& P = c.sys_h[s];
+        for (int d = 0; d < 3; d++) {
+            P.box.lo[d] = S.lo[d];
+            P.box.hi[d] = S.hi[d];
+            P.box.L[d] = S.hi[d] - S.lo[d];
+            P.box.periodic[d] = 1;
+        }
+        double k4[4];
+        mdd::pair_constants(S.youngs, S.poisson, S.restitution, S.friction, S.youngs, S.poisson,
+                            S.restitution, S.friction, k4);
+        P.beta = k4[0];
+        P.mu = k4[1];
+        P.eeff = k4[2];
+        P.geff = k4[3];
+        c.sys_rmin[s] = *std::min_element(S.cutr.begin(), S.cutr.end());
+        c.sys_rmax[s] = *std::max_element(S.cutr.begin(), S.cutr.end());
+    }
+    pack_upload(c, n, tag.data(), type.data(), pos.data(), vel.data(), omega.data(),
+                radius.data(), cutr.data(), mass.data());
+    c.psys.resize(size_t(c.npad));
+    MDD_HIP(hipMemcpy(c.psys.p, psys.data(), sizeof(uint32_t) * size_t(n), hipMemcpyHostToDevice));
+    c.nsys = ns;
+    c.setup_done = false;
+}
+
 template <class T>
 std::vector<T> dl(const mdd::DBuf<T>& b, size_t n) {
     std::vector<T> h(n);
@@ -453,19 +510,15 @@ int mdd_set_materials(mdd_ctx* ctx, int n, const double* E, const double* nu, co
     return guarded(ctx, [&](Ctx& c) {
         if (n <= 0 || n > mdd::kMaxMat || !E || !nu || !e || !mu)
             throw ApiError(MDD_ERR_INVALID, "material table out of range");
-        // MaterialTable::build_pair_tables (dem_atom/src/lib.rs:389-515)
-        const double PI = 3.141592653589793;
         c.mat.n = n;
         for (int i = 0; i < n; i++)
             for (int j = 0; j < n; j++) {
-                const double e_ij = std::sqrt(e[i] * e[j]);
-                const double log_e = std::log(e_ij);
-                c.mat.beta[i * n + j] = -log_e / std::sqrt(PI * PI + log_e * log_e);
-                c.mat.mu[i * n + j] = std::sqrt(mu[i] * mu[j]);
-                c.mat.eeff[i * n + j] =
-                    1.0 / ((1.0 - nu[i] * nu[i]) / E[i] + (1.0 - nu[j] * nu[j]) / E[j]);
-                c.mat.geff[i * n + j] = 1.0 / (2.0 * (2.0 - nu[i]) * (1.0 + nu[i]) / E[i] +
-                                               2.0 * (2.0 - nu[j]) * (1.0 + nu[j]) / E[j]);
+                double k[4];
+                mdd::pair_constants(E[i], nu[i], e[i], mu[i], E[j], nu[j], e[j], mu[j], k);
+                c.mat.beta[i * n + j] = k[0];
+                c.mat.mu[i * n + j] = k[1];
+                c.mat.eeff[i * n + j] = k[2];
+                c.mat.geff[i * n + j] = k[3];
             }
         mdd::upload_materials(c);
     });
@@ -536,7 +589,64 @@ int mdd_upload_state(mdd_ctx* ctx, int64_t n, const uint32_t* tag, const uint32_
                      const double* pos, const double* vel, const double* omega,
                      const double* radius, const double* cutoff_radius, const double* mass) {
     return guarded(ctx, [&](Ctx& c) {
+        c.staged.clear();
+        c.systems_dirty = false;
+        c.nsys = 0;
         pack_upload(c, n, tag, type, pos, vel, omega, radius, cutoff_radius, mass);
+    });
+}
+
+int mdd_add_system(mdd_ctx* ctx, const double lo[3], const double hi[3], double youngs,
+                   double poisson, double restitution, double friction, int64_t n,
+                   const uint32_t* tag, const double* pos, const double* vel,
+                   const double* omega, const double* radius, const double* cutoff_radius,
+                   const double* mass) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (c.model != MDD_MODEL_DEM)
+            throw ApiError(MDD_ERR_UNSUPPORTED, "ensembles hold DEM systems");
+        if (n <= 0 || !lo || !hi || !tag || !pos || !vel || !radius || !cutoff_radius || !mass)
+            throw ApiError(MDD_ERR_INVALID, "bad system arguments");
+        if (c.nsys == 0 && c.n > 0 && c.staged.empty())
+            throw ApiError(MDD_ERR_INVALID, "context already holds a single system");
+        Ctx::Staged S;
+        for (int d = 0; d < 3; d++) {
+            if (!(hi[d] > lo[d])) throw ApiError(MDD_ERR_INVALID, "box hi must exceed lo");
+            S.lo[d] = lo[d];
+            S.hi[d] = hi[d];
+        }
+        S.youngs = youngs;
+        S.poisson = poisson;
+        S.restitution = restitution;
+        S.friction = friction;
+        S.tag.assign(tag, tag + n);
+        S.pos.assign(pos, pos + 3 * n);
+        S.vel.assign(vel, vel + 3 * n);
+        S.omega = omega ? std::vector<double>(omega, omega + 3 * n) : std::vector<double>(3 * n, 0.0);
+        S.radius.assign(radius, radius + n);
+        S.cutr.assign(cutoff_radius, cutoff_radius + n);
+        S.mass.assign(mass, mass + n);
+        c.staged.push_back(std::move(S));
+        c.systems_dirty = true;
+        c.list_valid = false;
+    });
+}
+
+int mdd_nsystems(mdd_ctx* ctx) {
+    if (!ctx) return MDD_ERR_INVALID;
+    int ns = 0;
+    const int rc = guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        ns = c.nsys;
+    });
+    return rc == MDD_OK ? ns : rc;
+}
+
+int mdd_system_range(mdd_ctx* ctx, int sys, int64_t* first, int64_t* count) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        if (sys < 0 || sys >= c.nsys) throw ApiError(MDD_ERR_INVALID, "no such system");
+        if (first) *first = c.sys_first[sys];
+        if (count) *count = c.sys_first[sys + 1] - c.sys_first[sys];
     });
 }
 
@@ -767,7 +877,7 @@ int mdd_hertz_mindlin_contact_force(mdd_ctx* ctx) {
     return guarded(ctx, [&](Ctx& c) {
         need_list(c);
         if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
-        if (c.mat.n == 0) throw ApiError(MDD_ERR_INVALID, "materials not set");
+        if (c.mat.n == 0 && c.nsys == 0) throw ApiError(MDD_ERR_INVALID, "materials not set");
         mdd::launch_pair_forces(c, -1, false);
         sync_status(c);
         check_device_error(c);

@@ -87,6 +87,24 @@ struct GridP {
     int64_t ncells;
 };
 
+// One system of a batched ensemble: its box, bin grid (cells
+// cell_off .. cell_off + grid.ncells - 1 of the shared cell space), stencil
+// runs (run_off .. run_off + nruns - 1 of the shared run tables), pair
+// cutoff, rebuild threshold and material constants. Particles of system s
+// occupy one contiguous index range.
+struct SysP {
+    BoxP box;
+    GridP grid;
+    int64_t cell_off;
+    double gc;            // ghost (image) cutoff
+    double cutoff_sq;     // uniform pair cutoff squared
+    double sf2;           // skin_fraction^2 (per-pair cutoff)
+    double thr2;          // rebuild threshold: ((skin_fraction - 1) * min radius)^2
+    int uniform;
+    int run_off, nruns, run0;
+    double beta, mu, eeff, geff;  // single-material pair constants
+};
+
 struct Wall {
     double p[3], n[3], blo[3], bhi[3];
     int mat;
@@ -126,6 +144,19 @@ struct Status {
     double nh_p[2];          // p_xi by step parity
     double nh_scale[2];      // velocity scale by step parity
 };
+
+// MaterialTable::build_pair_tables (crates/dem_atom/src/lib.rs:389-515) for
+// one pair of materials: beta, friction, E*, G*
+inline void pair_constants(double Ei, double nui, double ei, double mui, double Ej, double nuj,
+                           double ej, double muj, double out[4]) {
+    const double PI = 3.141592653589793;
+    const double e_ij = __builtin_sqrt(ei * ej);
+    const double log_e = __builtin_log(e_ij);
+    out[0] = -log_e / __builtin_sqrt(PI * PI + log_e * log_e);
+    out[1] = __builtin_sqrt(mui * muj);
+    out[2] = 1.0 / ((1.0 - nui * nui) / Ei + (1.0 - nuj * nuj) / Ej);
+    out[3] = 1.0 / (2.0 * (2.0 - nui) * (1.0 + nui) / Ei + 2.0 * (2.0 - nuj) * (1.0 + nuj) / Ej);
+}
 
 template <class T>
 struct DBuf {
@@ -174,11 +205,29 @@ struct Ctx {
     double g[3] = {0, 0, 0};
     WallsP walls{};
 
+    // ---- batched ensemble (nsys > 0): particles of system s are
+    // [sys_first[s], sys_first[s + 1]); per-particle system index psys
+    int nsys = 0;
+    std::vector<SysP> sys_h;
+    std::vector<int64_t> sys_first;
+    std::vector<double> sys_rmin, sys_rmax;  // neighbor radius range per system
+    DBuf<SysP> sysp;
+    DBuf<uint32_t> psys;
+    DBuf<int64_t> sys_cell_off;  // nsys + 1 cell offsets (last = total cells)
+    struct Staged {
+        double lo[3], hi[3];
+        double youngs, poisson, restitution, friction;
+        std::vector<uint32_t> tag;
+        std::vector<double> pos, vel, omega, radius, cutr, mass;
+    };
+    std::vector<Staged> staged;
+    bool systems_dirty = false;
+    const SysP* sysp_or_null() const { return nsys > 0 ? sysp.p : nullptr; }
+
     // ---- neighbor grid (neighbor_setup)
     bool setup_done = false;
     double max_cutoff = 0, ghost_cutoff = 0, bin_size[3] = {1, 1, 1};
     GridP grid{};
-    std::vector<int> fwd;
     bool uniform = true;
     double cutoff_sq = 0, sf2 = 1, min_skin = 0;
     Shift27 shifts{};
@@ -217,7 +266,6 @@ struct Ctx {
     DBuf<double4> ghost_pos;
     DBuf<uint32_t> bin_start, bin_entry, gstart, placed, ghost_key, ghost_order;
     DBuf<double4> bin_pos;
-    DBuf<int> stencil;       // forward offsets
     std::vector<int> full;   // all offsets whose cells can hold a pair, ascending
     DBuf<int> run_first, run_last;  // the full stencil as runs of consecutive offsets
     int nruns = 0, run0 = 0;        // run0: the run holding offset 0

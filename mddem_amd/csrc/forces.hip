@@ -58,6 +58,8 @@ struct ForceArgs {
     double* pe;
     Status* st;
     int step;
+    const SysP* sysp;     // batched ensemble: per-system box and materials, or null
+    const uint32_t* psys;
 };
 
 __device__ inline bool gated(const ForceArgs& a) {
@@ -256,6 +258,14 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
     const double4 vi = a.vel[i];
     R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
     const uint32_t ti = a.type[i];
+    // box lengths and (batched ensemble) the system's single-material constants
+    double Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
+    const SysP* S = a.sysp ? &a.sysp[a.psys[i]] : nullptr;
+    if (S) {
+        Lx = S->box.L[0];
+        Ly = S->box.L[1];
+        Lz = S->box.L[2];
+    }
     if (a.use_pairs) {
         const double4 wi = a.omega[i];
         const uint32_t tagi = a.tag[i];
@@ -276,9 +286,9 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
             const int code = entry_code(e);
             const bool rev = entry_rev(e);
             const double4 pj = a.pos[j];
-            const double shx = image_shift(code_sx(code), a.L[0]);
-            const double shy = image_shift(code_sy(code), a.L[1]);
-            const double shz = image_shift(code_sz(code), a.L[2]);
+            const double shx = image_shift(code_sx(code), Lx);
+            const double shy = image_shift(code_sy(code), Ly);
+            const double shz = image_shift(code_sz(code), Lz);
             // owner frame: d = partner image - owner; r1 owner radius
             double dx, dy, dz, r1, r2;
             if (rev) {
@@ -324,10 +334,10 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
             const int mO = int(rev ? tj : ti), mP = int(rev ? ti : tj);
             const int mij = mO * nm + mP;
             const R r_eff = R(r1) * R(r2) / R(sum_r);
-            const R e_eff = R(eeff_t[mij]);
-            const R g_eff = R(geff_t[mij]);
-            const R beta = R(beta_t[mij]);
-            const R mu = R(mu_t[mij]);
+            const R e_eff = R(S ? S->eeff : eeff_t[mij]);
+            const R g_eff = R(S ? S->geff : geff_t[mij]);
+            const R beta = R(S ? S->beta : beta_t[mij]);
+            const R mu = R(S ? S->mu : mu_t[mij]);
             const double inv_dist = 1.0 / distance;
             const double nx = dx * inv_dist, ny = dy * inv_dist, nz = dz * inv_dist;
             const R m_r = R(1.0) / (R(vO.w) + R(vP.w));
@@ -496,6 +506,8 @@ ForceArgs make_args(Ctx& c, int step) {
     a.pe = c.pe.p;
     a.st = c.status.p;
     a.step = step;
+    a.sysp = c.sysp_or_null();
+    a.psys = c.psys.p;
     return a;
 }
 

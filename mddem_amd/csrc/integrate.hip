@@ -54,6 +54,8 @@ struct IntArgs {
     int nparts;      // KE partials left by the last post kernel
     int q_in, q_out; // thermostat slots read / written
     double ndof, temp, q_mass;
+    const SysP* sysp;       // batched ensemble: per-system box and threshold, or null
+    const uint32_t* psys;
 };
 
 template <typename F>
@@ -109,22 +111,29 @@ __device__ inline double nh_term(const IntArgs& a, double ke) {
 
 // displacement_exceeded (neighbor.rs:564-601) + periodic box exit (651-666)
 __device__ inline void rebuild_check(const IntArgs& a, int64_t i, const double4& p) {
+    const BoxP* box = &a.box;
+    double thr2 = a.thr2;
+    if (a.sysp) {
+        const SysP& S = a.sysp[a.psys[i]];
+        box = &S.box;
+        thr2 = S.thr2;
+    }
     const double4 b = a.build_pos[i];
     double d[3] = {p.x - b.x, p.y - b.y, p.z - b.z};
     const double x[3] = {p.x, p.y, p.z};
     bool out = false;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        if (a.box.periodic[k]) {
-            const double L = a.box.L[k], h = L * 0.5;
+        if (box->periodic[k]) {
+            const double L = box->L[k], h = L * 0.5;
             if (d[k] > h)
                 d[k] -= L;
             else if (d[k] < -h)
                 d[k] += L;
-            if (a.box_exit && (x[k] < a.box.lo[k] || x[k] >= a.box.hi[k])) out = true;
+            if (a.box_exit && (x[k] < box->lo[k] || x[k] >= box->hi[k])) out = true;
         }
     }
-    if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > a.thr2 || out)
+    if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > thr2 || out)
         atomicMin(&a.st->abort_step, a.step);
 }
 
@@ -376,6 +385,8 @@ IntArgs make_args(Ctx& c, int step, bool check) {
     a.ndof = c.nh_ndof;
     a.temp = c.nh_T;
     a.q_mass = c.nh_Q;
+    a.sysp = c.sysp_or_null();
+    a.psys = c.psys.p;
     return a;
 }
 

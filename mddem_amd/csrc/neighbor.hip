@@ -10,14 +10,17 @@
 // same FP64 fma distance test) comes out bit for bit. Instead of storing
 // only the half list, every particle gets a full row: the pairs it owns
 // plus the pairs other particles own with it, each entry flagged, so the
-// force kernels gather without atomics. A row is built by a group of lanes
-// that scans the particle's stencil cells once: a candidate in a forward
-// cell (or later in the same cell) is owned by the particle; a particle (not
-// a periodic image) in a backward cell (or earlier in the same cell) owns
-// the pair instead, and for two particles the distance test is symmetric
-// bit for bit. Pairs owned through the particle's own periodic images come
-// from a second scan around each image. Passing candidates are compacted
-// with a wavefront ballot, in a fixed order, so rows are deterministic.
+// force kernels gather without atomics. Rows are built one wavefront per bin
+// cell (k_build_cells): a candidate after the particle in bin order (a
+// forward cell, or later in the same cell) is a pair the particle owns; a
+// particle before it owns the pair instead, and for two particles the
+// distance test is symmetric bit for bit. Pairs owned through a particle's
+// periodic images are collected by the images themselves and appended to
+// the particle's row. Passing candidates are compacted with a wavefront
+// ballot, in a fixed order, so rows are deterministic.
+//
+// A batched ensemble shares one cell space: system s owns cells
+// [cell_off, cell_off + ncells) and its own stencil runs and cutoffs.
 #include <algorithm>
 #include <cmath>
 
@@ -49,10 +52,20 @@ __global__ void k_status_reset_build(Status* st) {
 }
 
 // pbc wrap (wrap_periodic, domain.rs:396-407) + bin key of every particle
+// Batched ensembles pass the per-system table: each particle then uses its
+// own system's box and grid, and its key is offset into the shared cell space.
 __global__ void k_wrap_key(int64_t n, double4* __restrict__ pos, int4* __restrict__ image,
-                           BoxP box, GridP g, uint32_t* __restrict__ key, Status* st) {
+                           BoxP box, GridP g, uint32_t* __restrict__ key, Status* st,
+                           const SysP* __restrict__ sysp, const uint32_t* __restrict__ psys) {
     int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (i >= n) return;
+    int64_t coff = 0;
+    if (sysp) {
+        const SysP& S = sysp[psys[i]];
+        box = S.box;
+        g = S.grid;
+        coff = S.cell_off;
+    }
     double4 p = pos[i];
     int4 im = image[i];
     double x[3] = {p.x, p.y, p.z};
@@ -77,7 +90,7 @@ __global__ void k_wrap_key(int64_t n, double4* __restrict__ pos, int4* __restric
     p.z = x[2];
     pos[i] = p;
     image[i] = make_int4(m[0], m[1], m[2], 0);
-    key[i] = cell_of(g, x[0], x[1], x[2]);
+    key[i] = uint32_t(coff + cell_of(g, x[0], x[1], x[2]));
 }
 
 struct PermArgs {
@@ -143,9 +156,15 @@ __device__ inline void image_flags(const BoxP& box, double gc, const double x[3]
 }
 
 __global__ void k_ghost_count(int64_t n, const double4* __restrict__ pos, BoxP box, double gc,
-                              uint32_t* __restrict__ gcnt) {
+                              uint32_t* __restrict__ gcnt, const SysP* __restrict__ sysp,
+                              const uint32_t* __restrict__ psys) {
     int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (i >= n) return;
+    if (sysp) {
+        const SysP& S = sysp[psys[i]];
+        box = S.box;
+        gc = S.gc;
+    }
     double4 p = pos[i];
     double x[3] = {p.x, p.y, p.z};
     int lf[3], hf[3];
@@ -159,7 +178,8 @@ __global__ void k_ghost_emit(int64_t n, const double4* __restrict__ pos, BoxP bo
                              GridP grid, const uint32_t* __restrict__ goff, int64_t ghost_cap,
                              uint32_t* __restrict__ groot, uint8_t* __restrict__ gcode,
                              double4* __restrict__ gpos, uint32_t* __restrict__ gkey,
-                             uint32_t* ng_live, Status* st) {
+                             uint32_t* ng_live, Status* st, const SysP* __restrict__ sysp,
+                             const uint32_t* __restrict__ psys) {
     int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     const uint32_t total = goff[n];
     const bool fits = int64_t(total) <= ghost_cap;
@@ -169,6 +189,14 @@ __global__ void k_ghost_emit(int64_t n, const double4* __restrict__ pos, BoxP bo
         if (!fits) atomicOr(&st->overflow, 1u);
     }
     if (!fits || i >= n) return;
+    int64_t coff = 0;
+    if (sysp) {
+        const SysP& S = sysp[psys[i]];
+        box = S.box;
+        gc = S.gc;
+        grid = S.grid;
+        coff = S.cell_off;
+    }
     double4 p = pos[i];
     double x[3] = {p.x, p.y, p.z};
     int lf[3], hf[3];
@@ -191,7 +219,7 @@ __global__ void k_ghost_emit(int64_t n, const double4* __restrict__ pos, BoxP bo
                 groot[g] = uint32_t(i);
                 gcode[g] = uint8_t(code_of(sx, sy, sz));
                 gpos[g] = make_double4(gx, gy, gz, 0.0);
-                gkey[g] = cell_of(grid, gx, gy, gz);
+                gkey[g] = uint32_t(coff + cell_of(grid, gx, gy, gz));
                 g++;
             }
         }
@@ -258,6 +286,9 @@ struct CellArgs {
     uint32_t* __restrict__ nhalf;
     uint32_t* __restrict__ gnbr;
     uint32_t* __restrict__ gcnt;
+    const SysP* __restrict__ sysp;
+    const int64_t* __restrict__ sys_cell_off;
+    int nsys;
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs; map them so that each
@@ -291,6 +322,21 @@ __device__ inline uint32_t lanes_before(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 
+// Pair cutoff of the cell's system: uniform (r2 < cut2) or per pair
+// (r2 < (s_i + s_j)^2 sf2), as neighbor.rs:938 and 1003-1004 choose it.
+struct CellCut {
+    double cut2, sf2;
+    int uni;
+};
+
+template <bool UNIFORM>
+__device__ inline bool within(const CellCut& cc, double r2, double si, double sj) {
+    if (UNIFORM) return r2 < cc.cut2;
+    if (cc.uni) return r2 < cc.cut2;
+    const double s = si + sj;
+    return r2 < s * s * cc.sf2;
+}
+
 // Candidate chunks of one window, held in registers: lane l of chunk c is
 // flat candidate win + 64 c + l of the cell's stencil ranges (bin order).
 template <bool UNIFORM>
@@ -307,9 +353,9 @@ struct Window {
 // window) against the window. Candidates before it in bin order own the
 // pair with it (entry flagged), candidates after it are pairs it owns.
 template <bool UNIFORM>
-__device__ inline void test_particle(const CellArgs& a, const Window<UNIFORM>& W, int nch,
-                                     int fs, const double4& xi, uint32_t q, uint32_t& cnt,
-                                     uint32_t& nrev) {
+__device__ inline void test_particle(const CellArgs& a, const CellCut& cc,
+                                     const Window<UNIFORM>& W, int nch, int fs, const double4& xi,
+                                     uint32_t q, uint32_t& cnt, uint32_t& nrev) {
     const int lane = int(threadIdx.x & 63);
     const uint32_t G = 1u << a.lg;
 #pragma unroll
@@ -318,13 +364,7 @@ __device__ inline void test_particle(const CellArgs& a, const Window<UNIFORM>& W
             const int f = c * 64 + lane;
             const double dx = W.x[c] - xi.x, dy = W.y[c] - xi.y, dz = W.z[c] - xi.z;
             const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            bool in;
-            if (UNIFORM) {
-                in = r2 < a.cutoff_sq;
-            } else {
-                const double s = xi.w + W.w[c];
-                in = r2 < s * s * a.sf2;
-            }
+            const bool in = within<UNIFORM>(cc, r2, xi.w, W.w[c]);
             const uint64_t pass = __ballot(in) & __ballot(f != fs) & W.okp[c];
             if (pass != 0ull) {
                 const uint64_t before = __ballot(f < fs);
@@ -347,7 +387,8 @@ __device__ inline void test_particle(const CellArgs& a, const Window<UNIFORM>& W
 // bin order (backward cells and this cell) own the pair with it; they are
 // collected, flagged, into the image's scratch row.
 template <bool UNIFORM>
-__device__ inline void test_image(const CellArgs& a, const Window<UNIFORM>& W, int nch, int f_end,
+__device__ inline void test_image(const CellArgs& a, const CellCut& cc,
+                                  const Window<UNIFORM>& W, int nch, int f_end,
                                   const double4& xi, uint32_t code_i, uint32_t* __restrict__ out,
                                   uint32_t& cnt) {
     const int lane = int(threadIdx.x & 63);
@@ -360,13 +401,7 @@ __device__ inline void test_image(const CellArgs& a, const Window<UNIFORM>& W, i
         if (cm != 0ull) {
             const double dx = W.x[c] - xi.x, dy = W.y[c] - xi.y, dz = W.z[c] - xi.z;
             const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            bool in;
-            if (UNIFORM) {
-                in = r2 < a.cutoff_sq;
-            } else {
-                const double s = xi.w + W.w[c];
-                in = r2 < s * s * a.sf2;
-            }
+            const bool in = within<UNIFORM>(cc, r2, xi.w, W.w[c]);
             const uint64_t pass = __ballot(in) & cm;
             if (pass != 0ull) {
                 const uint32_t k = cnt + lanes_before(pass);
@@ -385,7 +420,10 @@ __device__ inline void test_image(const CellArgs& a, const Window<UNIFORM>& W, i
 // (particles, then periodic images) is tested against them with uniform
 // control flow; passing lanes are compacted with a ballot, so each row lists
 // its entries in bin order.
-template <bool UNIFORM>
+//
+// In a batched ensemble (BATCH) the cell space is shared: the wave finds its
+// cell's system and uses that system's cell range, stencil runs and cutoff.
+template <bool UNIFORM, bool BATCH>
 __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
     const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const int lane = int(threadIdx.x & 63);
@@ -393,12 +431,35 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
     if (C >= a.ncells) return;
     const uint32_t i0 = a.bin_start[C], i1 = a.bin_start[C + 1];
     if (i0 == i1) return;
+    int64_t cmin = 0, cmax = a.ncells - 1;
+    int run_off = 0, nruns = a.nruns, run0 = a.run0;
+    CellCut cc{a.cutoff_sq, a.sf2, UNIFORM ? 1 : 0};
+    if (BATCH) {
+        // the system whose cell range holds C
+        int lo = 0, hi = a.nsys - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.sys_cell_off[mid] <= C)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const SysP& S = a.sysp[lo];
+        cmin = S.cell_off;
+        cmax = S.cell_off + S.grid.ncells - 1;
+        run_off = S.run_off;
+        nruns = S.nruns;
+        run0 = S.run0;
+        cc.cut2 = S.cutoff_sq;
+        cc.sf2 = S.sf2;
+        cc.uni = S.uniform;
+    }
     // stencil runs in lanes: bin range [rb, rb + rcnt), flat offset excl
     uint32_t rcnt = 0, rb = 0;
-    if (lane < a.nruns) {
-        int64_t lo = C + a.run_first[lane], hi = C + a.run_last[lane];
-        if (lo < 0) lo = 0;
-        if (hi > a.ncells - 1) hi = a.ncells - 1;
+    if (lane < nruns) {
+        int64_t lo = C + a.run_first[run_off + lane], hi = C + a.run_last[run_off + lane];
+        if (lo < cmin) lo = cmin;
+        if (hi > cmax) hi = cmax;
         if (lo <= hi) {
             rb = a.bin_start[lo];
             rcnt = a.bin_start[hi + 1] - rb;
@@ -412,8 +473,8 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
     }
     const uint32_t excl = incl - rcnt;
     const uint32_t rbase = rb - excl;  // bin index = flat index + rbase (mod 2^32)
-    const uint32_t total = __builtin_amdgcn_readlane(incl, a.nruns - 1);
-    const uint32_t base0 = __builtin_amdgcn_readlane(rbase, a.run0);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, nruns - 1);
+    const uint32_t base0 = __builtin_amdgcn_readlane(rbase, run0);
     // flat indices of this cell's first slot and of the slot after it
     const int f_i0 = int(i0 - base0), f_i1 = int(i1 - base0);
     const uint32_t ni = i1 - i0;
@@ -437,7 +498,7 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
             uint32_t base[kNch];
 #pragma unroll
             for (int c = 0; c < kNch; c++) base[c] = 0;
-            for (int r = 0; r < a.nruns; r++) {
+            for (int r = 0; r < nruns; r++) {
                 const uint32_t pr = __builtin_amdgcn_readlane(excl, r);
                 const uint32_t br = __builtin_amdgcn_readlane(rbase, r);
 #pragma unroll
@@ -476,7 +537,7 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
                 uint32_t cnt = __builtin_amdgcn_readlane(cntv, int(ii));
                 uint32_t nrev = __builtin_amdgcn_readlane(revv, int(ii));
                 const int fs = f_i0 + int(ib + ii) - shift;
-                test_particle<UNIFORM>(a, W, nch, fs, xi, entry_idx(ei), cnt, nrev);
+                test_particle<UNIFORM>(a, cc, W, nch, fs, xi, entry_idx(ei), cnt, nrev);
                 if (uint32_t(lane) == ii) {
                     cntv = cnt;
                     revv = nrev;
@@ -491,7 +552,7 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
                 xi.w = UNIFORM ? 0.0 : readlane_f64(mx.w, int(ii));
                 uint32_t cnt = __builtin_amdgcn_readlane(cntv, int(ii));
                 const uint32_t g = uint32_t(__builtin_amdgcn_readlane(int(mg), int(ii)));
-                test_image<UNIFORM>(a, W, nch, f_i1 - shift, xi, uint32_t(entry_code(ei)),
+                test_image<UNIFORM>(a, cc, W, nch, f_i1 - shift, xi, uint32_t(entry_code(ei)),
                                     a.gnbr + size_t(g) * uint32_t(a.maxnb), cnt);
                 if (uint32_t(lane) == ii) cntv = cnt;
             }
@@ -703,7 +764,8 @@ void rebuild_prepare(Ctx& c) {
     k_status_reset_build<<<1, 1, 0, c.stream>>>(c.status.p);
     // pbc + keys
     k_wrap_key<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, c.pos.p, c.image.p, c.box, c.grid,
-                                                          c.key.p, c.status.p);
+                                                          c.key.p, c.status.p, c.sysp_or_null(),
+                                                          c.psys.p);
     // sort_atoms_by_bin: stable counting sort of particles by bin
     counting_sort(c, c.key.p, n, nullptr, ncells, c.cnt.p, c.start.p, c.placed.p, c.order.p,
                   nullptr);
@@ -817,11 +879,12 @@ void rebuild_attempt(Ctx& c) {
     k_status_reset_attempt<<<1, 1, 0, c.stream>>>(c.status.p);
     // borders: periodic images and their bins
     k_ghost_count<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, c.pos.p, c.box, c.ghost_cutoff,
-                                                             c.gcnt.p);
+                                                             c.gcnt.p, c.sysp_or_null(), c.psys.p);
     exclusive_scan(c, c.gcnt.p, c.goff.p, n);
     k_ghost_emit<<<blocks_for(n, 256), 256, 0, c.stream>>>(
         n, c.pos.p, c.box, c.ghost_cutoff, c.grid, c.goff.p, c.ghost_cap, c.ghost_root.p,
-        c.ghost_code.p, c.ghost_pos.p, c.ghost_key.p, ng_live, c.status.p);
+        c.ghost_code.p, c.ghost_pos.p, c.ghost_key.p, ng_live, c.status.p, c.sysp_or_null(),
+        c.psys.p);
     // images binned after the particles of each cell (bin_neighbor_list
     // steps 1-2b: counting sort in index order, images have the larger indices)
     counting_sort(c, c.ghost_key.p, c.ghost_cap, ng_live, ncells, c.cnt.p, c.gstart.p, c.placed.p,
@@ -853,11 +916,16 @@ void rebuild_attempt(Ctx& c) {
     ca.nhalf = c.nhalf.p;
     ca.gnbr = c.gnbr.p;
     ca.gcnt = c.gcnt_img.p;
+    ca.sysp = c.sysp_or_null();
+    ca.sys_cell_off = c.sys_cell_off.p;
+    ca.nsys = c.nsys;
     const int cblocks = blocks_for(ncells, kCellWaves);
-    if (c.uniform)
-        k_build_cells<true><<<cblocks, 256, 0, c.stream>>>(ca);
+    if (c.nsys > 0)
+        k_build_cells<false, true><<<cblocks, 256, 0, c.stream>>>(ca);
+    else if (c.uniform)
+        k_build_cells<true, false><<<cblocks, 256, 0, c.stream>>>(ca);
     else
-        k_build_cells<false><<<cblocks, 256, 0, c.stream>>>(ca);
+        k_build_cells<false, false><<<cblocks, 256, 0, c.stream>>>(ca);
     const int tblocks = blocks_for(n, 256);
     c.row_blk.resize(size_t(2) * tblocks);
     k_row_totals<<<tblocks, 256, 0, c.stream>>>(n, c.goff.p, ng_live, c.gcnt_img.p, c.nmain.p,
@@ -931,9 +999,152 @@ void rebuild_async(Ctx& c, int step, bool retry) {
     MDD_HIP(hipGetLastError());
 }
 
-void neighbor_setup(Ctx& c) {
+namespace {
+
+// neighbor_setup + compute_bin_grid for one box and radius range
+struct Geometry {
+    GridP grid;
+    double max_cutoff = 0, ghost_cutoff = 0, bin_size[3] = {1, 1, 1};
+    std::vector<int> full, run_first, run_last;
+    int run0 = -1, nfwd = 0;
+    bool uniform = true;
+    double cutoff_sq = 0, sf2 = 1, min_skin = 0;
+};
+
+Geometry make_geometry(const BoxP& box, double mn, double mx, double skin_fraction,
+                       double bin_size) {
     // neighbor_setup (neighbor.rs:480-541), single process
+    Geometry g;
+    const double max_cutoff = mx > 0.0 ? 2.0 * mx * skin_fraction : bin_size;
+    const double buffer = mn < 1.7976931348623157e308 * 0.5 ? (skin_fraction - 1.0) * mn : 0.0;
+    double ghost_cut = max_cutoff + 2.0 * buffer;
+    if (ghost_cut < 0.0) ghost_cut = 0.0;
+    g.ghost_cutoff = ghost_cut;
+    g.max_cutoff = max_cutoff;
+    const double bin_min = bin_size < max_cutoff ? max_cutoff : bin_size;
+    // compute_bin_grid (neighbor.rs:290-371), comm_size = 1
+    int nint[3], sl[3];
+    for (int d = 0; d < 3; d++) {
+        double q = std::floor(box.L[d] / bin_min);
+        if (q < 1.0) q = 1.0;
+        nint[d] = int(q);
+        g.bin_size[d] = box.L[d] / double(nint[d]);
+    }
+    const double gcut = g.ghost_cutoff > max_cutoff ? g.ghost_cutoff : max_cutoff;
+    for (int d = 0; d < 3; d++) {
+        const int range = int(std::ceil(max_cutoff / g.bin_size[d]));
+        sl[d] = int(std::ceil(gcut / g.bin_size[d])) + range;
+        g.grid.n[d] = nint[d] + 2 * sl[d];
+        g.grid.origin[d] = box.lo[d] - g.bin_size[d] * double(sl[d]);
+        g.grid.inv_bs[d] = 1.0 / g.bin_size[d];
+    }
+    g.grid.ncells = int64_t(g.grid.n[0]) * g.grid.n[1] * g.grid.n[2];
+    if (g.grid.ncells > (int64_t(1) << 31))
+        throw ApiError(-5, "bin grid too large; raise bin_size");
+    const double cutoff_sq = max_cutoff * max_cutoff;
+    const int ny = g.grid.n[1], nz = g.grid.n[2];
+    for (int dx = -sl[0]; dx <= sl[0]; dx++)
+        for (int dy = -sl[1]; dy <= sl[1]; dy++)
+            for (int dz = -sl[2]; dz <= sl[2]; dz++) {
+                const int ax = std::abs(dx) - 1, ay = std::abs(dy) - 1, az = std::abs(dz) - 1;
+                const double mdx = double(ax > 0 ? ax : 0) * g.bin_size[0];
+                const double mdy = double(ay > 0 ? ay : 0) * g.bin_size[1];
+                const double mdz = double(az > 0 ? az : 0) * g.bin_size[2];
+                if (mdx * mdx + mdy * mdy + mdz * mdz < cutoff_sq) {
+                    const int off = dx * ny * nz + dy * nz + dz;
+                    g.full.push_back(off);
+                    if (off > 0) g.nfwd++;
+                }
+            }
+    g.grid.nfwd = g.nfwd;
+    // the full stencil as runs of consecutive cell offsets (contiguous bins)
+    std::sort(g.full.begin(), g.full.end());
+    for (size_t k = 0; k < g.full.size(); k++) {
+        if (k == 0 || g.full[k] != g.full[k - 1] + 1) {
+            g.run_first.push_back(g.full[k]);
+            g.run_last.push_back(g.full[k]);
+        } else {
+            g.run_last.back() = g.full[k];
+        }
+        if (g.full[k] == 0) g.run0 = int(g.run_first.size()) - 1;
+    }
+    if (g.run_first.size() > size_t(kMaxRuns) || g.run0 < 0)
+        throw ApiError(-5, "bin stencil has too many cell runs; raise bin_size");
+    // save_build_positions cutoff cache (neighbor.rs:545-561)
+    g.min_skin = mn;
+    if (std::fabs(mx - mn) < 1e-15) {
+        const double cutoff = 2.0 * mn * skin_fraction;
+        g.uniform = true;
+        g.cutoff_sq = cutoff * cutoff;
+    } else {
+        g.uniform = false;
+    }
+    g.sf2 = skin_fraction * skin_fraction;
+    return g;
+}
+
+// expected periodic images per particle: prod over periodic axes of (1 + 2 gc / L) - 1
+double images_per_particle(const BoxP& box, double gc) {
+    double per = 1.0;
+    for (int d = 0; d < 3; d++)
+        if (box.periodic[d]) per *= 1.0 + std::min(1.0, 2.0 * gc / box.L[d]);
+    return per - 1.0;
+}
+
+void upload_runs(Ctx& c, const std::vector<int>& rf, const std::vector<int>& rl) {
+    c.run_first.resize(rf.size());
+    c.run_last.resize(rl.size());
+    MDD_HIP(hipMemcpy(c.run_first.p, rf.data(), sizeof(int) * rf.size(), hipMemcpyHostToDevice));
+    MDD_HIP(hipMemcpy(c.run_last.p, rl.data(), sizeof(int) * rl.size(), hipMemcpyHostToDevice));
+}
+
+// one geometry per system, cells and runs concatenated
+void ensemble_setup(Ctx& c) {
+    std::vector<int> rf, rl;
+    std::vector<int64_t> cell_off(size_t(c.nsys) + 1, 0);
+    double images = 0.0;
+    for (int s = 0; s < c.nsys; s++) {
+        SysP& S = c.sys_h[s];
+        const Geometry g =
+            make_geometry(S.box, c.sys_rmin[s], c.sys_rmax[s], c.skin_fraction, c.bin_min_size);
+        S.grid = g.grid;
+        S.cell_off = cell_off[s];
+        S.gc = g.ghost_cutoff;
+        S.cutoff_sq = g.cutoff_sq;
+        S.sf2 = g.sf2;
+        const double skin = (c.skin_fraction - 1.0) * g.min_skin;
+        S.thr2 = skin * skin;
+        S.uniform = g.uniform ? 1 : 0;
+        S.run_off = int(rf.size());
+        S.nruns = int(g.run_first.size());
+        S.run0 = g.run0;
+        rf.insert(rf.end(), g.run_first.begin(), g.run_first.end());
+        rl.insert(rl.end(), g.run_last.begin(), g.run_last.end());
+        cell_off[s + 1] = cell_off[s] + g.grid.ncells;
+        images += double(c.sys_first[s + 1] - c.sys_first[s]) * images_per_particle(S.box, S.gc);
+    }
+    if (cell_off[c.nsys] >= (int64_t(1) << 32))
+        throw ApiError(-5, "ensemble bin grids too large; raise bin_size");
+    c.grid.ncells = cell_off[c.nsys];
+    upload_runs(c, rf, rl);
+    c.sysp.resize(c.sys_h.size());
+    MDD_HIP(hipMemcpy(c.sysp.p, c.sys_h.data(), sizeof(SysP) * c.sys_h.size(),
+                      hipMemcpyHostToDevice));
+    c.sys_cell_off.resize(cell_off.size());
+    MDD_HIP(hipMemcpy(c.sys_cell_off.p, cell_off.data(), sizeof(int64_t) * cell_off.size(),
+                      hipMemcpyHostToDevice));
+    if (c.ghost_cap == 0) c.ghost_cap = int64_t(images * 1.25) + 1024;
+    c.setup_done = true;
+}
+
+}  // namespace
+
+void neighbor_setup(Ctx& c) {
     if (c.n <= 0) throw ApiError(-1, "neighbor_setup needs particles");
+    if (c.nsys > 0) {
+        ensemble_setup(c);
+        return;
+    }
     std::vector<double> cr(c.n);
     MDD_HIP(hipMemcpy(cr.data(), c.cutr.p, sizeof(double) * c.n, hipMemcpyDeviceToHost));
     double mx = 0.0, mn = 1.7976931348623157e308;
@@ -941,91 +1152,21 @@ void neighbor_setup(Ctx& c) {
         if (v > mx) mx = v;
         if (v < mn) mn = v;
     }
-    double max_cutoff = mx > 0.0 ? 2.0 * mx * c.skin_fraction : c.bin_min_size;
-    double buffer = mn < 1.7976931348623157e308 * 0.5 ? (c.skin_fraction - 1.0) * mn : 0.0;
-    double ghost_cut = max_cutoff + 2.0 * buffer;
-    if (ghost_cut < 0.0) ghost_cut = 0.0;
-    c.ghost_cutoff = ghost_cut;
-    c.max_cutoff = max_cutoff;
-    double bin_min = c.bin_min_size < max_cutoff ? max_cutoff : c.bin_min_size;
-    c.bin_min_size = bin_min;
-    // compute_bin_grid (neighbor.rs:290-371), comm_size = 1
-    int nint[3], sl[3];
-    for (int d = 0; d < 3; d++) {
-        double q = std::floor(c.box.L[d] / bin_min);
-        if (q < 1.0) q = 1.0;
-        nint[d] = int(q);
-        c.bin_size[d] = c.box.L[d] / double(nint[d]);
-    }
-    double gcut = c.ghost_cutoff > max_cutoff ? c.ghost_cutoff : max_cutoff;
-    for (int d = 0; d < 3; d++) {
-        int range = int(std::ceil(max_cutoff / c.bin_size[d]));
-        sl[d] = int(std::ceil(gcut / c.bin_size[d])) + range;
-        c.grid.n[d] = nint[d] + 2 * sl[d];
-        c.grid.origin[d] = c.box.lo[d] - c.bin_size[d] * double(sl[d]);
-        c.grid.inv_bs[d] = 1.0 / c.bin_size[d];
-    }
-    c.grid.ncells = int64_t(c.grid.n[0]) * c.grid.n[1] * c.grid.n[2];
-    if (c.grid.ncells > (int64_t(1) << 31))
-        throw ApiError(-5, "bin grid too large; raise bin_size");
-    double cutoff_sq = max_cutoff * max_cutoff;
-    c.fwd.clear();
-    c.full.clear();
-    const int ny = c.grid.n[1], nz = c.grid.n[2];
-    for (int dx = -sl[0]; dx <= sl[0]; dx++)
-        for (int dy = -sl[1]; dy <= sl[1]; dy++)
-            for (int dz = -sl[2]; dz <= sl[2]; dz++) {
-                int ax = std::abs(dx) - 1, ay = std::abs(dy) - 1, az = std::abs(dz) - 1;
-                double mdx = double(ax > 0 ? ax : 0) * c.bin_size[0];
-                double mdy = double(ay > 0 ? ay : 0) * c.bin_size[1];
-                double mdz = double(az > 0 ? az : 0) * c.bin_size[2];
-                if (mdx * mdx + mdy * mdy + mdz * mdz < cutoff_sq) {
-                    int off = dx * ny * nz + dy * nz + dz;
-                    c.full.push_back(off);
-                    if (off > 0) c.fwd.push_back(off);
-                }
-            }
-    c.grid.nfwd = int(c.fwd.size());
-    c.stencil.resize(c.fwd.size());
-    MDD_HIP(hipMemcpy(c.stencil.p, c.fwd.data(), sizeof(int) * c.fwd.size(),
-                      hipMemcpyHostToDevice));
-    // the full stencil as runs of consecutive cell offsets (contiguous bins)
-    std::sort(c.full.begin(), c.full.end());
-    std::vector<int> rf, rl;
-    c.run0 = -1;
-    for (size_t k = 0; k < c.full.size(); k++) {
-        if (k == 0 || c.full[k] != c.full[k - 1] + 1) {
-            rf.push_back(c.full[k]);
-            rl.push_back(c.full[k]);
-        } else {
-            rl.back() = c.full[k];
-        }
-        if (c.full[k] == 0) c.run0 = int(rf.size()) - 1;
-    }
-    if (rf.size() > size_t(kMaxRuns) || c.run0 < 0)
-        throw ApiError(-5, "bin stencil has too many cell runs; raise bin_size");
-    c.nruns = int(rf.size());
-    c.run_first.resize(rf.size());
-    c.run_last.resize(rl.size());
-    MDD_HIP(hipMemcpy(c.run_first.p, rf.data(), sizeof(int) * rf.size(), hipMemcpyHostToDevice));
-    MDD_HIP(hipMemcpy(c.run_last.p, rl.data(), sizeof(int) * rl.size(), hipMemcpyHostToDevice));
-    // save_build_positions cutoff cache (neighbor.rs:545-561)
-    c.min_skin = mn;
-    if (std::fabs(mx - mn) < 1e-15) {
-        double cutoff = 2.0 * mn * c.skin_fraction;
-        c.uniform = true;
-        c.cutoff_sq = cutoff * cutoff;
-    } else {
-        c.uniform = false;
-    }
-    c.sf2 = c.skin_fraction * c.skin_fraction;
-    if (c.ghost_cap == 0) {
-        // expected images per particle: prod over periodic axes of (1 + 2 gc / L) - 1
-        double per = 1.0;
-        for (int d = 0; d < 3; d++)
-            if (c.box.periodic[d]) per *= 1.0 + std::min(1.0, 2.0 * c.ghost_cutoff / c.box.L[d]);
-        c.ghost_cap = int64_t(double(c.n) * (per - 1.0) * 1.25) + 1024;
-    }
+    const Geometry g = make_geometry(c.box, mn, mx, c.skin_fraction, c.bin_min_size);
+    c.grid = g.grid;
+    c.max_cutoff = g.max_cutoff;
+    c.ghost_cutoff = g.ghost_cutoff;
+    for (int d = 0; d < 3; d++) c.bin_size[d] = g.bin_size[d];
+    c.full = g.full;
+    c.nruns = int(g.run_first.size());
+    c.run0 = g.run0;
+    upload_runs(c, g.run_first, g.run_last);
+    c.min_skin = g.min_skin;
+    c.uniform = g.uniform;
+    c.cutoff_sq = g.cutoff_sq;
+    c.sf2 = g.sf2;
+    if (c.ghost_cap == 0)
+        c.ghost_cap = int64_t(double(c.n) * images_per_particle(c.box, c.ghost_cutoff) * 1.25) + 1024;
     c.setup_done = true;
 }
 

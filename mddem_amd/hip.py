@@ -100,6 +100,10 @@ SIGNATURES = {
     "mdd_force_kernel_stats": (C.c_int, [_vp, _dp, _i64p]),
     "mdd_set_force_lanes": (C.c_int, [_vp, C.c_int]),
     "mdd_device_memory": (C.c_int, [C.c_int, _i64p, _i64p]),
+    "mdd_add_system": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_int64, _u32p, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mdd_nsystems": (C.c_int, [_vp]),
+    "mdd_system_range": (C.c_int, [_vp, C.c_int, _i64p, _i64p]),
 }
 
 
@@ -207,8 +211,37 @@ class System:
         self._c(self.L.mdd_upload_state(self.h, n, _p(tag, _u32p), _p(atype, _u32p), _p(pos),
                                         _p(vel), _p(om), _p(rad), _p(cr), _p(mass)))
 
+    def add_system(self, lo, hi, youngs, poisson, restitution, friction, pos, vel, mass,
+                   cutoff_radius, radius, omega=None, tag=None):
+        """Append one periodic DEM system to a batched ensemble."""
+        pos = _d(pos, (-1, 3))
+        n = pos.shape[0]
+        tag = np.ascontiguousarray(np.arange(n) if tag is None else tag, dtype=np.uint32)
+        vel = _d(vel, (-1, 3))
+        mass = _d(np.broadcast_to(mass, (n,)))
+        cr = _d(np.broadcast_to(cutoff_radius, (n,)))
+        rad = _d(np.broadcast_to(radius, (n,)))
+        om = None if omega is None else _d(omega, (-1, 3))
+        self._c(self.L.mdd_add_system(self.h, _p(_d(lo)), _p(_d(hi)), float(youngs),
+                                      float(poisson), float(restitution), float(friction), n,
+                                      _p(tag, _u32p), _p(pos), _p(vel), _p(om), _p(rad), _p(cr),
+                                      _p(mass)))
+        self._ensemble = True
+
+    def nsystems(self) -> int:
+        rc = self.L.mdd_nsystems(self.h)
+        self._c(min(rc, 0))
+        return rc
+
+    def system_range(self, k: int):
+        f, c = C.c_int64(0), C.c_int64(0)
+        self._c(self.L.mdd_system_range(self.h, int(k), C.byref(f), C.byref(c)))
+        return f.value, c.value
+
     @property
     def n(self):
+        if getattr(self, "_ensemble", False):
+            self.nsystems()  # commits staged systems
         return self.L.mdd_nlocal(self.h)
 
     def state(self):

@@ -230,6 +230,71 @@ def dem_gas(n: int = 500, box: float = 0.025, radius: float = 1e-3, speed: float
         steps=steps, density=dens)
 
 
+def gas_system(n: int, phi: float, restitution: float, friction: float, seed: int,
+               radius: float = 1e-3, speed: float = 0.5) -> Workload:
+    """One system of the granular-gas ensemble: a periodic cube of edge
+    L = (N 4/3 pi r^3 / phi)^(1/3), seeded lattice start, speed 0.5 m/s."""
+    st = Stream(seed)
+    L = (n * 4.0 / 3.0 * np.pi * radius ** 3 / phi) ** (1.0 / 3.0)
+    r = np.full(n, radius)
+    lo, hi = np.zeros(3), np.full(3, L)
+    pos = _fcc_sites(n, L, radius, st)
+    vel = _isotropic(n, speed, st)
+    dens = GLASS["density"]
+    t_r = float(np.min(rayleigh_time(GLASS["youngs"], GLASS["poisson"], dens, r)))
+    return Workload(
+        name=f"gas_{n}_{seed & 0xffff}", model="dem", lo=lo, hi=hi, periodic=(True, True, True),
+        pos=pos, vel=vel, mass=dem_mass(dens, r), cutoff_radius=r.copy(), radius=r,
+        omega=np.zeros((n, 3)), atype=np.zeros(n, np.uint32), dt=0.05 * t_r,
+        skin_fraction=1.1, bin_size=0.005,
+        materials=glass_materials(restitution=restitution, friction=friction), density=dens)
+
+
+def _fcc_sites(n, L, radius, stream):
+    """n non-overlapping spheres in a periodic cube: n sites drawn from the
+    FCC lattice (m_x x m_y x m_z cells tiling the box) with the widest
+    nearest-neighbor spacing, each jittered by less than half the slack."""
+    m = int(math.ceil((n / 4.0) ** (1.0 / 3.0)))
+    best = None
+    for mx in (m - 1, m):
+        for my in (m - 1, m):
+            for mz in (m - 1, m):
+                if min(mx, my, mz) < 1 or 4 * mx * my * mz < n:
+                    continue
+                ax, ay, az = L / mx, L / my, L / mz
+                nn = min(ax, ay, az, math.hypot(ax, ay) / 2, math.hypot(ax, az) / 2,
+                         math.hypot(ay, az) / 2)
+                if best is None or nn > best[0]:
+                    best = (nn, mx, my, mz)
+    nn, mx, my, mz = best
+    slack = nn - 2.002 * radius
+    if slack <= 0:
+        raise ValueError("volume fraction too high for lattice placement")
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.stack(np.meshgrid(np.arange(mx), np.arange(my), np.arange(mz), indexing="ij"),
+                     axis=-1).reshape(-1, 3)
+    sites = (cells[:, None, :] + basis[None, :, :]).reshape(-1, 3) * np.array([L / mx, L / my, L / mz])
+    keys = stream.uniform(sites.shape[0])
+    pick = np.sort(np.argsort(keys, kind="stable")[:n])
+    jitter = (stream.uniform(3 * n).reshape(n, 3) - 0.5) * (slack / math.sqrt(3.0))
+    return np.mod(sites[pick] + jitter + 0.25 * np.array([L / mx, L / my, L / mz]), L)
+
+
+def ensemble(nsys: int = 1024, seed: int | None = None, nmin: int = 200,
+             nmax: int = 20000) -> list:
+    """The batched granular-gas ensemble: N log-uniform in [nmin, nmax],
+    restitution U[0.7, 0.99], friction U[0.1, 0.6], volume fraction
+    U[0.05, 0.30], r = 1 mm; system k is seeded with base + k."""
+    base = SEED_BASE + 3 if seed is None else seed
+    u = Stream(base).uniform(4 * nsys).reshape(nsys, 4)
+    out = []
+    for k in range(nsys):
+        n = int(round(math.exp(math.log(nmin) + u[k, 0] * math.log(nmax / nmin))))
+        out.append(gas_system(n, 0.05 + 0.25 * u[k, 3], 0.7 + 0.29 * u[k, 1],
+                              0.1 + 0.5 * u[k, 2], seed=(base << 12) + k))
+    return out
+
+
 def _jittered_lattice(n, radius, lo, hi, stream, gap=1.1):
     """n sites on the coarsest cubic-ish lattice that fits the box, each site
     jittered inside the slack left by gap * 2 * r_max; sites are drawn
