@@ -368,3 +368,87 @@ def test_precision_report_small_lj():
     nve = P.nve_drift(w, steps=200, samples=8)
     assert nve["fp64"]["max_rel_drift"] < 1e-2
     assert nve["mixed"]["max_rel_drift"] <= 2.0 * nve["fp64"]["max_rel_drift"] + 1e-6
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+def test_rotation_composes_world_frame_turns(prec):
+    # closed form of initial_rotation / final_rotation (rotational.rs:59-100)
+    from tests.test_oracle import _axis_angle, _qmul
+    r, dt = 1e-3, 1e-3
+    m = dem_mass(2500.0, r)
+    inv_i = 1.0 / (0.4 * m * r * r)
+    s = hip.System(hip.MODEL_DEM, prec)
+    s.set_box((-1, -1, -1), (1, 1, 1), (False, False, False))
+    s.set_materials([8.7e9], [0.3], [0.95], [0.4])
+    s.set_dt(dt)
+    s.upload([[0, 0, 0]], [[0, 0, 0]], [m], [r], radius=[r], omega=[[30.0, 0, 0]])
+    for _ in range(40):
+        s.initial_rotation()
+    s.set_forces(np.zeros((1, 3)), np.array([[-30.0, 20.0, 0.0]]) * 2.0 / (dt * inv_i))
+    s.final_rotation()
+    s.set_forces(np.zeros((1, 3)), np.zeros((1, 3)))
+    for _ in range(25):
+        s.initial_rotation()
+    st = s.state()
+    want = _qmul(_axis_angle((0, 1, 0), 20.0 * 25 * dt), _axis_angle((1, 0, 0), 30.0 * 40 * dt))
+    tol = 1e-9 if prec == hip.FP64 else 1e-5
+    assert np.abs(st["quat"][0] - want).max() < tol
+    assert abs(st["omega"][0, 1] - 20.0) < 20.0 * tol
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+def test_plane_wall_hertz_uses_sphere_radius(prec):
+    r, d = K["hertz_radius"], K["hertz_delta"]
+    m = dem_mass(2500.0, r)
+    s = hip.System(hip.MODEL_DEM, prec)
+    s.set_box((-1, -1, -1), (1, 1, 1), (False, False, False))
+    s.set_materials([8.7e9], [0.3], [0.95], [0.4])
+    s.set_dt(1e-7)
+    s.add_wall((0, 0, 0), (0, 0, 1), 0)
+    s.upload([[0, 0, r - d]], [[0, 0, 0]], [m], [r], radius=[r])
+    s.set_forces(np.zeros((1, 3)), np.zeros((1, 3)))
+    s.wall_contact_force()
+    f, _ = s.forces()
+    tol = 1e-12 if prec == hip.FP64 else MIXED_TOL
+    assert abs(f[0, 2] / (math.sqrt(2.0) * K["hertz_force"]) - 1.0) < tol
+
+
+def test_rebuild_when_displacement_passes_the_skin_budget():
+    s = hip.System(hip.MODEL_LJ, hip.FP64)
+    s.set_box((0, 0, 0), (20, 20, 20), (False, False, False))
+    s.set_neighbor(1.2, 1.0, 0, True, 1000, True)
+    s.set_lj(1.0, 1.0, 2.5)
+    s.set_dt(0.01)
+    s.set_integrators(True, False)
+    s.upload([[5.0, 5.0, 5.0], [15.0, 15.0, 15.0]], [[0.7, 0, 0], [0, 0, 0]], [1.0, 1.0],
+             [1.0, 1.0])
+    s.run(1)
+    base = s.thermo()["rebuilds"]
+    steps = 0
+    while s.thermo()["rebuilds"] == base:
+        s.run(1)
+        steps += 1
+        assert steps < 100
+    assert steps == int(math.floor(0.2 / 0.007)) + 1
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+def test_granular_gas_fixture(prec):
+    import os
+    from tests.golden.make_cfg0 import load_state
+    w, _ = load_state()
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg0_step10.npz"))
+    s = hip.from_workload(w, prec, box_exit=True)
+    s.run(10)
+    st = s.state()
+    f, t = s.forces()
+    order = np.argsort(st["tag"])
+    tol = FP64_TOL if prec == hip.FP64 else MIXED_TOL
+    assert rel_err(f[order], ref["force"]) < tol
+    assert rel_err(t[order], ref["torque"]) < tol
+    go, gp, gc = s.pairs()
+    assert np.array_equal(canonical_pairs(go, gp, gc), ref["pairs"])
+    lo, hi, spring = s.history()
+    k = np.lexsort((hi, lo))
+    assert np.array_equal(lo[k], ref["hist_lo"]) and np.array_equal(hi[k], ref["hist_hi"])
+    assert rel_err(spring[k], ref["hist_spring"]) < (1e-12 if prec == hip.FP64 else 1e-6)

@@ -378,3 +378,157 @@ def test_reference_history_mode_matches_canonical_on_fixed_list():
 
 def test_rel_err_helper():
     assert rel_err([1, 2], [1, 2]) == 0.0
+
+
+def _free_nh(tau=0.5, t_target=1.0, heat=1.01, n=125, dt=0.005):
+    """Free LJ particles (40 sigma apart, far beyond the cutoff for the whole
+    run) under Nose-Hoover."""
+    o = Oracle(MODEL_LJ)
+    o.set_box((0, 0, 0), (200, 200, 200), (True, True, True))
+    o.set_neighbor(1.12, 1.0, 0, True, 1000, True)
+    o.set_lj(1.0, 1.0, 2.5)
+    o.set_dt(dt)
+    o.set_nose_hoover(t_target, tau)
+    o.set_integrators(False, False)
+    g = (np.arange(5) + 0.5) * 40.0
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    vel = W.maxwell_boltzmann(n, t_target * heat, 1.0, W.Stream(77))
+    o.set_atoms(pos, vel, np.ones(n), np.full(n, 1.25))
+    return o, 3.0 * n - 3.0
+
+
+def test_nose_hoover_small_oscillation_period():
+    # Free particles: dKE/dt = -2 (p/Q) KE, dp/dt = 2 KE - ndof T with
+    # Q = ndof T tau^2 (md_thermostat/src/lib.rs:195-353). Linearised about
+    # KE* = ndof T / 2 the deviation oscillates with omega^2 = 4 KE*/Q = 2/tau^2,
+    # period 2 pi tau / sqrt(2). A wrong sign diverges; a lost factor 2 moves
+    # the period by sqrt(2).
+    tau, dt = 0.5, 0.005
+    o, ndof = _free_nh(tau=tau, dt=dt)
+    temps = []
+    for _ in range(1000):
+        o.step()
+        temps.append(2.0 * o.compute_ke() / ndof - 1.0)
+    temps = np.asarray(temps)
+    assert np.abs(temps).max() < 0.05  # stays in the linear regime
+    s = np.sign(temps)
+    cross = np.nonzero(s[1:] * s[:-1] < 0)[0]
+    # linear interpolation of the zero crossings
+    tz = [(k + temps[k] / (temps[k] - temps[k + 1])) * dt for k in cross]
+    half_periods = np.diff(tz)
+    period = 2.0 * float(np.mean(half_periods))
+    assert len(half_periods) >= 4
+    assert abs(period / (2.0 * math.pi * tau / math.sqrt(2.0)) - 1.0) < 0.01
+
+
+def _spinner():
+    o = Oracle(MODEL_DEM)
+    o.set_box((-1, -1, -1), (1, 1, 1), (False, False, False))
+    o.set_materials([8.7e9], [0.3], [0.95], [0.4])
+    o.set_dt(1e-3)
+    m = dem_mass(2500.0, 1e-3)
+    o.set_atoms([[0, 0, 0]], [[0, 0, 0]], [m], [1e-3], radius=[1e-3], omega=[[30.0, 0, 0]])
+    return o, m
+
+
+def _axis_angle(axis, angle):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    return np.concatenate([[math.cos(angle / 2)], a * math.sin(angle / 2)])
+
+
+def _qmul(a, b):
+    w1, x1, y1, z1 = a
+    w2, x2, y2, z2 = b
+    return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+
+
+def test_rotation_composes_world_frame_turns():
+    # initial_rotation (rotational.rs:59-84): q <- dq(|w| dt, w^) * q with w
+    # in the world frame. 40 steps about x at 30 rad/s, then (after a torque
+    # half-kick that turns w onto y) 25 steps about y at 20 rad/s must give
+    # q = dq_y(20*25*dt) * dq_x(30*40*dt); the reversed product is far off.
+    o, m = _spinner()
+    dt, inv_i = 1e-3, 1.0 / (0.4 * m * 1e-6)
+    for _ in range(40):
+        o.initial_rotation()
+    # final_rotation: w += dt/2 * tau * inv_I; choose tau to land on (0, 20, 0)
+    o.set_forces(np.zeros((1, 3)), np.array([[-30.0, 20.0, 0.0]]) * 2.0 / (dt * inv_i))
+    o.final_rotation()
+    o.set_forces(np.zeros((1, 3)), np.zeros((1, 3)))
+    w = o.atoms()["omega"][0]
+    assert abs(w[0]) < 1e-12 and abs(w[1] - 20.0) < 1e-12
+    for _ in range(25):
+        o.initial_rotation()
+    q = o.atoms()["quat"][0]
+    want = _qmul(_axis_angle((0, 1, 0), 20.0 * 25 * dt), _axis_angle((1, 0, 0), 30.0 * 40 * dt))
+    wrong = _qmul(_axis_angle((1, 0, 0), 30.0 * 40 * dt), _axis_angle((0, 1, 0), 20.0 * 25 * dt))
+    assert np.abs(q - want).max() < 1e-9
+    assert np.abs(q - wrong).max() > 0.1
+    assert abs(np.dot(q, q) - 1.0) < 1e-12
+
+
+def test_plane_wall_hertz_uses_sphere_radius():
+    # wall_contact_force (dem_wall/src/lib.rs:862-1002): R* = r, m_r = m, so at
+    # rest F = 4/3 E* sqrt(r d) d: sqrt(2) times the sphere-sphere force of
+    # the same overlap (R* = r/2), which the Hertz closed form pins.
+    r, d = K["hertz_radius"], K["hertz_delta"]
+    o = Oracle(MODEL_DEM)
+    o.set_box((-1, -1, -1), (1, 1, 1), (False, False, False))
+    o.set_materials([8.7e9], [0.3], [0.95], [0.4])
+    o.set_dt(1e-7)
+    o.add_wall((0, 0, 0), (0, 0, 1), 0)
+    m = dem_mass(2500.0, r)
+    o.set_atoms([[0, 0, r - d]], [[0, 0, 0]], [m], [r], radius=[r])
+    o.zero_all_forces()
+    o.wall_contact_force()
+    fz = o.atoms()["force"][0, 2]
+    assert abs(fz / (math.sqrt(2.0) * K["hertz_force"]) - 1.0) < 1e-12
+    o.set_atoms([[0, 0, r + 1e-6]], [[0, 0, 0]], [m], [r], radius=[r])
+    o.zero_all_forces()
+    o.wall_contact_force()
+    assert o.atoms()["force"][0, 2] == 0.0
+
+
+def test_rebuild_when_displacement_passes_the_skin_budget():
+    # displacement_exceeded (neighbor.rs:564-601): rebuild once a particle
+    # has moved more than (sf - 1) * r_min since the last build (strictly).
+    o = Oracle(MODEL_LJ)
+    o.set_box((0, 0, 0), (20, 20, 20), (False, False, False))
+    o.set_neighbor(1.2, 1.0, 0, True, 1000, True)  # budget 0.2 * 1.0
+    o.set_lj(1.0, 1.0, 2.5)
+    o.set_dt(0.01)
+    o.set_integrators(True, False)
+    pos = [[5.0, 5.0, 5.0], [15.0, 15.0, 15.0]]
+    vel = [[0.7, 0.0, 0.0], [0.0, 0.0, 0.0]]  # 0.007 per step
+    o.set_atoms(pos, vel, [1.0, 1.0], [1.0, 1.0])
+    o.step()  # initial build
+    base = o.rebuilds
+    steps = 0
+    while o.rebuilds == base:
+        o.step()
+        steps += 1
+    # the first step drifts, then builds (decide_rebuild runs after the
+    # initial integration), so after k more steps the particle is 0.007 k
+    # from its build position: the first k with 0.007 k > 0.2 is 29
+    assert steps == int(math.floor(0.2 / 0.007)) + 1
+
+
+def test_granular_gas_fixture_is_reproduced():
+    # tests/golden/cfg0_step10.npz: 10 FP64 steps from the committed
+    # 500-sphere state (written by tests/golden/make_cfg0.py)
+    from tests.golden.make_cfg0 import load_state
+    import os
+    w, tags = load_state()
+    assert np.array_equal(tags, np.arange(w.n))
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg0_step10.npz"))
+    o = oracle_from_workload(w)
+    o.run(10)
+    a = o.atoms()
+    order = np.argsort(a["tag"])
+    assert np.array_equal(a["force"][order], ref["force"])
+    assert np.array_equal(a["torque"][order], ref["torque"])
+    assert np.array_equal(a["pos"][order], ref["pos"])
+    go, gp, gc = o.pairs()
+    assert np.array_equal(canonical_pairs(go, gp, gc), ref["pairs"])
+    assert len(ref["pairs"]) > 50 and len(ref["hist_lo"]) > 0
