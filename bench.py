@@ -32,28 +32,43 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True):
-    """Warm up (the first three quarters of the warmup steps time the
-    pair-force launch shapes, 2, 4 and 8 lanes per particle, and keep the
-    fastest), then time exactly `steps` steps."""
+def layout_name(lg):
+    return "cell tiles" if lg < 0 else f"rows, {1 << lg} lanes/particle"
+
+
+def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True, layout=None, graphs=True):
+    """Warm up (the last four fifths of the warmup steps try the pair-force
+    layouts: row kernels at 2, 4 and 8 lanes per particle and, for LJ, the
+    cell-tile kernel; the one with the shortest wall time per step is kept),
+    then time exactly `steps` steps."""
     from mddem_amd import hip
     s = hip.from_workload(w, precision, box_exit=False)
-    lanes_log2 = 2
-    trial = warmup // 4 if tune else 0
+    s.set_launch_graphs(graphs)
+    if layout is not None:
+        s.set_force_lanes(layout)
+        tune = False
+    layout = 2 if layout is None else layout
+    options = (-1, 1, 2, 3) if w.model == "lj" else (1, 2, 3)
+    trial = warmup // (len(options) + 1) if tune else 0
     if trial >= 100:
-        s.run(warmup - 3 * trial - 1)  # equilibrate first
+        s.run(warmup - len(options) * trial - 1)  # equilibrate first
         best = None
-        for lg in (1, 2, 3):
+        for lg in options:
             s.set_force_lanes(lg)
             s.run(trial // 4)  # relayout rebuild
             s.set_profiling(1)
+            s.synchronize()
+            t0 = time.perf_counter()
             s.run(trial - trial // 4)
+            s.synchronize()
+            per_step = (time.perf_counter() - t0) / (trial - trial // 4)
             ms, _ = s.force_kernel_stats()
-            log(f"[bench]   tune: {1 << lg} lanes/particle, pair-force kernel {ms * 1e3:.1f} us")
-            if best is None or ms < best[0]:
-                best = (ms, lg)
-        lanes_log2 = best[1]
-        s.set_force_lanes(lanes_log2)
+            log(f"[bench]   tune: {layout_name(lg)}: {per_step * 1e6:.1f} us/step, "
+                f"pair-force kernel {ms * 1e3:.1f} us")
+            if best is None or per_step < best[0]:
+                best = (per_step, lg)
+        layout = best[1]
+        s.set_force_lanes(layout)
         s.set_profiling(0)
         s.run(1)  # the last warmup step lays the list out for the chosen shape
     else:
@@ -68,10 +83,11 @@ def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True):
     t1 = time.perf_counter()
     th1 = s.thermo(energy=(w.model == "lj"))
     avg_ms, launches = s.force_kernel_stats()
+    rb_ms, rb_n = s.rebuild_stats()
     res = dict(seconds=t1 - t0, steps=steps, n=w.n, rebuilds=th1["rebuilds"] - th0["rebuilds"],
                npairs=th1["npairs"], force_ms=avg_ms, force_launches=launches,
                temperature=th1["temperature"], pe=th1["pe"], ke=th1["ke"],
-               lanes=1 << lanes_log2)
+               layout=layout_name(layout), tile=layout < 0, rebuild_ms=rb_ms, rebuilds_timed=rb_n)
     s.close()
     return res
 
@@ -159,6 +175,10 @@ def main():
     ap.add_argument("--ncell", type=int, default=40)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--fp64-steps", type=int, default=5000)
+    ap.add_argument("--graphs", choices=["on", "off"], default="on",
+                    help="issue neighbor rebuilds as captured HIP graphs")
+    ap.add_argument("--layout", choices=["auto", "tile", "1", "2", "3"], default="auto",
+                    help="pair-force layout: tuned, cell tiles, or rows at 2^k lanes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precision-report", type=int, default=2000,
@@ -183,14 +203,17 @@ def main():
     prec = hip.MIXED if args.precision == "mixed" else hip.FP64
     log(f"[bench] workload {w.name} n={w.n} precision={args.precision} "
         f"steps={args.steps} warmup={args.warmup}")
-    r = run_gpu(w, prec, args.steps, args.warmup)
+    layout = None if args.layout == "auto" else (-1 if args.layout == "tile" else int(args.layout))
+    graphs = args.graphs == "on"
+    r = run_gpu(w, prec, args.steps, args.warmup, layout=layout, graphs=graphs)
     value = r["n"] * r["steps"] / r["seconds"]
     log(f"[bench] {value:.4g} particle-steps/s, {r['rebuilds']} rebuilds, "
-        f"force kernel {r['force_ms'] * 1e3:.1f} us")
+        f"{r['layout']}, force kernel {r['force_ms'] * 1e3:.1f} us, "
+        f"rebuild {r['rebuild_ms'] * 1e3:.1f} us ({r['rebuilds_timed']} timed)")
 
     fp64 = None
     if prec == hip.MIXED and args.fp64_steps > 0:
-        r64 = run_gpu(w, hip.FP64, args.fp64_steps, args.warmup)
+        r64 = run_gpu(w, hip.FP64, args.fp64_steps, args.warmup, layout=layout, graphs=graphs)
         fp64 = r64["n"] * r64["steps"] / r64["seconds"]
         log(f"[bench] fp64 {fp64:.4g} particle-steps/s, force kernel "
             f"{r64['force_ms'] * 1e3:.1f} us")
@@ -201,7 +224,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles(w.name),
-                "kernel": "k_lj (pair force)", "bytes_per_launch": alg,
+                "kernel": ("k_lj_tile (pair force + final half step)" if r["tile"]
+                           else "k_lj (pair force)"), "bytes_per_launch": alg,
                 "avg_launch_us": round(r["force_ms"] * 1e3, 2)}
 
     precision = None
@@ -255,7 +279,8 @@ def main():
                                "T*=0.85 Nose-Hoover tau=1, dt=0.005, rc=2.5, skin 2.8",
                    "precision": args.precision, "rebuild": "displacement (lazy wrap)",
                    "rebuilds_in_timed_loop": r["rebuilds"],
-                   "force_lanes_per_particle": r["lanes"],
+                   "force_layout": r["layout"],
+                   "rebuild_us": round(r["rebuild_ms"] * 1e3, 1),
                    "pairs_per_atom_half": round(r["npairs"] / r["n"], 3)},
         "fp64_gpu_value": None if fp64 is None else round(fp64, 1),
         "mixed_over_fp64": None if fp64 is None else round(value / fp64, 3),

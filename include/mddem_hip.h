@@ -196,13 +196,28 @@ int mdd_set_profiling(mdd_ctx* ctx, int every);
 /* Average device time (ms) of the timed pair-force launches since the last
  * mdd_set_profiling, and the number of pair-force launches. */
 int mdd_force_kernel_stats(mdd_ctx* ctx, double* avg_ms, int64_t* launches);
+/* Average device time (ms) of the neighbor rebuilds timed since the last
+ * mdd_set_profiling (first rebuild launch to the end of its last kernel, on
+ * the context's stream), and how many were timed. */
+int mdd_rebuild_stats(mdd_ctx* ctx, double* avg_ms, int64_t* count);
 /* Free and total device memory of a device (hipMemGetInfo), for admission by
  * footprint. No reference counterpart: serving-loop support. */
 int mdd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
-/* Lanes per particle in the pair-force kernels, as log2 (0..3; default 2).
+/* Pair-force layout: log2 lanes per particle of the row kernels (0..3;
+ * default 2), or -1 for the LJ cell-tile kernel (16-bit rows into each bin
+ * cell's candidate list staged in LDS; LJ contexts only, others keep rows).
  * Changes the neighbor-row layout, so the next step rebuilds the list. No
  * reference counterpart: launch-shape tuning. */
 int mdd_set_force_lanes(mdd_ctx* ctx, int log2_lanes);
+/* Issue each neighbor rebuild (wrap, spatial sort, images, bins, rows) as one
+ * HIP graph launch (default on). The rebuild's launches are recorded with
+ * their arguments; a graph captured from a launch sequence with identical
+ * kernels, grids and argument bytes is replayed, otherwise the sequence is
+ * captured anew. Results are identical either way. No reference counterpart:
+ * launch-overhead tuning. */
+int mdd_set_launch_graphs(mdd_ctx* ctx, int enabled);
+/* Rebuilds issued by replaying a captured graph, and graphs captured. */
+int mdd_launch_graph_stats(mdd_ctx* ctx, int64_t* replays, int64_t* captures);
 
 #ifdef __cplusplus
 }

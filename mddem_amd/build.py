@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libmddem_hip.so")
 
-SOURCES = ["scan_sort.hip", "neighbor.hip", "forces.hip", "integrate.hip", "capi.hip"]
+SOURCES = ["scan_sort.hip", "neighbor.hip", "forces.hip", "lj_tile.hip", "integrate.hip", "capi.hip"]
 
 # -ffp-contract=off: no implicit fusion of a*b+c, so the FP64 path keeps the
 # reference's rounding (its explicit mul_add calls are written as fma()).

@@ -49,7 +49,6 @@ __global__ void k_status_init(mdd::Status* st) {
     st->nh_scale[0] = st->nh_scale[1] = 1.0;
 }
 
-__global__ void k_set_abort(mdd::Status* st, int v) { st->abort_step = v; }
 __global__ void k_set_pxi(mdd::Status* st, double v) { st->p_xi = v; }
 
 constexpr int kEventPool = 2048;
@@ -105,6 +104,14 @@ void harvest_profile(Ctx& c) {
         }
     }
     c.prof_pending = 0;
+    for (int64_t k = 0; k < c.rb_pending; k++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c.rb_ev[2 * k], c.rb_ev[2 * k + 1]) == hipSuccess) {
+            c.rb_ms_sum += ms;
+            c.rb_count++;
+        }
+    }
+    c.rb_pending = 0;
 }
 
 void check_device_error(Ctx& c) {
@@ -151,8 +158,8 @@ void run_steps(Ctx& c, int64_t nsteps) {
     // resume at the force pass of step ab after the device stopped there:
     // a rebuild request from the drift, or a rebuild that did not fit
     auto resume = [&](int ab) {
-        const bool retry = (c.status_host->overflow & 3u) != 0u;
-        k_set_abort<<<1, 1, 0, c.stream>>>(c.status.p, INT_MAX);
+        const bool retry = (c.status_host->overflow & 11u) != 0u;
+        // the rebuild's first kernel clears the abort
         if (retry) {
             mdd::rebuild_grow(c, *c.status_host);
             if (++c.retries > 6) throw ApiError(MDD_ERR_CAPACITY, "neighbor rebuild did not fit");
@@ -179,6 +186,7 @@ void run_steps(Ctx& c, int64_t nsteps) {
         while (c.step < end) {
             const int64_t s = c.step + 1;
             const bool forced = !c.list_valid || c.list_injected || c.list_lg != c.lg ||
+                                c.list_tile != c.want_tile() ||
                                 (c.sort_every > 0 && s % c.sort_every == 0);
             if (forced) {
                 // FullRebuild step: integrate, rebuild, force, integrate
@@ -451,6 +459,8 @@ int mdd_destroy(mdd_ctx* ctx) {
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     for (hipEvent_t e : c.ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c.rb_ev) (void)hipEventDestroy(e);
+    mdd::release_graphs(c);
     if (c.status_host) (void)hipHostFree(c.status_host);
     c.status_host = nullptr;
     hipStream_t s = c.stream;
@@ -822,6 +832,7 @@ int mdd_set_half_list(mdd_ctx* ctx, int64_t nlocal, const uint32_t* offsets,
         }
         c.list_valid = true;
         c.list_injected = true;
+        c.list_tile = false;
         c.list_lg = c.lg;
         c.npairs = offsets[c.n];
     });
@@ -842,6 +853,7 @@ int mdd_download_pairs(mdd_ctx* ctx, int64_t cap, uint32_t* owner_tag, uint32_t*
                        uint8_t* code, int64_t* n_out) {
     return guarded(ctx, [&](Ctx& c) {
         need_list(c);
+        if (c.list_tile) mdd::expand_tile_rows(c);
         MDD_HIP(hipStreamSynchronize(c.stream));
         const int64_t n = c.n, np = c.npad;
         auto nn = dl(c.nnb, size_t(n));
@@ -1046,12 +1058,24 @@ int mdd_set_profiling(mdd_ctx* ctx, int every) {
         if (c.prof_every && c.ev_pool.empty()) {
             c.ev_pool.resize(2 * kEventPool);
             for (auto& e : c.ev_pool) MDD_HIP(hipEventCreate(&e));
+            c.rb_ev.resize(128);
+            for (auto& e : c.rb_ev) MDD_HIP(hipEventCreate(&e));
         }
         c.prof_ms_sum = 0;
         c.prof_count = 0;
         c.prof_pending = 0;
         c.force_launches = 0;
         c.prof_ctr = 0;
+        c.rb_ms_sum = 0;
+        c.rb_count = 0;
+        c.rb_pending = 0;
+    });
+}
+
+int mdd_rebuild_stats(mdd_ctx* ctx, double* avg_ms, int64_t* count) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (avg_ms) *avg_ms = c.rb_count ? c.rb_ms_sum / double(c.rb_count) : 0.0;
+        if (count) *count = c.rb_count;
     });
 }
 
@@ -1074,11 +1098,30 @@ int mdd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
     return MDD_OK;
 }
 
+int mdd_set_launch_graphs(mdd_ctx* ctx, int enabled) {
+    return guarded(ctx, [&](Ctx& c) {
+        c.graphs_on = enabled != 0;
+        if (!c.graphs_on) {
+            MDD_HIP(hipStreamSynchronize(c.stream));
+            mdd::release_graphs(c);
+        }
+    });
+}
+
+int mdd_launch_graph_stats(mdd_ctx* ctx, int64_t* replays, int64_t* captures) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (replays) *replays = c.graph_hits;
+        if (captures) *captures = c.graph_captures;
+    });
+}
+
 int mdd_set_force_lanes(mdd_ctx* ctx, int log2_lanes) {
     return guarded(ctx, [&](Ctx& c) {
-        if (log2_lanes < 0 || log2_lanes > 3)
-            throw ApiError(MDD_ERR_INVALID, "log2 lanes per particle must be 0..3");
-        c.lg = log2_lanes;  // the next step rebuilds the list in the new layout
+        if (log2_lanes < -1 || log2_lanes > 3)
+            throw ApiError(MDD_ERR_INVALID, "log2 lanes per particle must be -1..3");
+        // the next step rebuilds the list in the new layout
+        c.tile_enabled = log2_lanes < 0;
+        if (log2_lanes >= 0) c.lg = log2_lanes;
     });
 }
 

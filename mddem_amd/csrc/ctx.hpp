@@ -7,8 +7,12 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <functional>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <type_traits>
 #include <vector>
 
 namespace mdd {
@@ -131,7 +135,8 @@ struct Status {
     int abort_step;          // first step whose drift needs a rebuild; INT_MAX none
     int error;               // 0 or an MDD_ERR_* code raised on the device
     unsigned overflow;       // bit 0: image capacity, bit 1: neighbor row capacity,
-                             // bit 2: particle outside a non-periodic face
+                             // bit 2: particle outside a non-periodic face,
+                             // bit 3: candidate list beyond 16-bit tile indices
     int max_row;             // longest full-list row at the last build
     int max_half;            // longest half-list row at the last build
     unsigned ghost_total;    // periodic images at the last build
@@ -158,6 +163,13 @@ inline void pair_constants(double Ei, double nui, double ei, double mui, double 
     out[3] = 1.0 / (2.0 * (2.0 - nui) * (1.0 + nui) / Ei + 2.0 * (2.0 - nuj) * (1.0 + nuj) / Ej);
 }
 
+// Device pointers whose hipFree waits until recorded launches are issued
+// (set by the recording thread, see LaunchLog).
+inline std::vector<void*>*& deferred_frees() {
+    static thread_local std::vector<void*>* list = nullptr;
+    return list;
+}
+
 template <class T>
 struct DBuf {
     T* p = nullptr;
@@ -167,7 +179,14 @@ struct DBuf {
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            // while launches are being recorded, a buffer they name stays
+            // allocated until they have been issued
+            if (deferred_frees())
+                deferred_frees()->push_back(p);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -182,6 +201,23 @@ struct DBuf {
         std::swap(p, o.p);
         std::swap(n, o.n);
     }
+};
+
+// Launch recording. While `on`, launch() does not launch: it appends the
+// kernel, its grid, block and argument bytes to `sig` and a closure that
+// issues the same launch to `calls`. A launch sequence whose signature
+// equals that of a captured graph is replayed as that graph; two sequences
+// with equal signatures launch the same kernels on the same arguments.
+struct LaunchLog {
+    bool on = false;
+    std::vector<uint8_t> sig;
+    std::vector<std::function<void(hipStream_t)>> calls;
+    std::vector<void*> frees;  // buffers released while recording
+};
+
+struct GraphEntry {
+    std::vector<uint8_t> sig;
+    hipGraphExec_t exec = nullptr;
 };
 
 struct Ctx {
@@ -272,6 +308,18 @@ struct Ctx {
     DBuf<uint32_t> bin_gidx;        // bin slot of a periodic image -> image index
     DBuf<uint32_t> gnbr, gcnt_img;  // per image: particles owning a pair with it
     DBuf<uint32_t> nmain;           // row length from the particle's own cell pass
+    // LJ tile rows: crow (16-bit candidate indices, tile_addr layout, count
+    // nmain) and xnbr (pairs owned through periodic images, p * maxnb + k,
+    // count nx)
+    // tile_enabled: the caller's choice (mdd_set_force_lanes); tile_blocked:
+    // a candidate list outgrew 16-bit indices
+    bool tile_enabled = false, tile_blocked = false, list_tile = false;
+    bool want_tile() const { return model == 0 && nsys == 0 && tile_enabled && !tile_blocked; }
+    bool last_force_fused = false;  // the last scheduled force pass also integrated
+    int tile_cap = 0;
+    DBuf<uint16_t> crow;
+    DBuf<uint32_t> xnbr, nx;
+    DBuf<double> ke_total;          // NH kinetic energy sum for the next pre kernel
     DBuf<uint32_t> row_blk;         // per-block row maxima of the last build
 
     // ---- full neighbor list: entry k of particle i at row_addr(i, k, npad, lg)
@@ -297,6 +345,17 @@ struct Ctx {
     int chunk = 8;
     int retries = 0;  // consecutive builds that did not fit
 
+    // ---- rebuild launch sequences captured as graphs
+    LaunchLog rec;
+    int row_nblk = 0;  // per-block row-maxima slots of the last build
+    // device time of whole rebuilds (first launch to the gate), when profiling
+    std::vector<hipEvent_t> rb_ev;
+    int64_t rb_pending = 0, rb_count = 0;
+    double rb_ms_sum = 0;
+    std::vector<GraphEntry> graphs;  // most recent last
+    bool graphs_on = true;
+    int64_t graph_hits = 0, graph_captures = 0;
+
     // ---- profiling of the pair-force kernel
     int prof_every = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -312,17 +371,23 @@ struct Ctx {
 void exclusive_scan(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n);
 void counting_sort(Ctx& c, const uint32_t* key, int64_t n, const uint32_t* n_dev, int64_t ncells,
                    uint32_t* cnt, uint32_t* start, uint32_t* placed, uint32_t* order,
-                   uint32_t* slot_of);
+                   uint32_t* slot_of, bool cnt_zero = false, const uint32_t* add = nullptr,
+                   uint32_t* out2 = nullptr);
 void fill_u32(Ctx& c, uint32_t* p, uint32_t v, int64_t n);
 void fill_f64(Ctx& c, double* p, double v, int64_t n);
 void fill_f32(Ctx& c, float* p, float v, int64_t n);
 // neighbor.hip
 void rebuild(Ctx& c);
 void rebuild_async(Ctx& c, int step, bool retry);
+void release_graphs(Ctx& c);
 bool rebuild_grow(Ctx& c, const Status& s);
 // forces.hip
 void launch_pair_forces(Ctx& c, int step, bool energy);
 void launch_walls_gravity(Ctx& c, bool walls, bool gravity);
+// lj_tile.hip
+void launch_lj_tile(Ctx& c, int step, bool energy, bool fuse);
+void launch_ke_reduce(Ctx& c, int step, int64_t count);
+void expand_tile_rows(Ctx& c);
 void upload_materials(Ctx& c);
 // integrate.hip
 void launch_pre(Ctx& c, int step, bool check_rebuild);
@@ -341,6 +406,34 @@ void thermo_reduce(Ctx& c, double out[8]);
 inline int blocks_for(int64_t n, int threads) {
     int64_t b = (n + threads - 1) / threads;
     return int(b > 0 ? b : 1);
+}
+
+template <typename V>
+inline void sig_put(std::vector<uint8_t>& s, const V& v) {
+    const uint8_t* b = reinterpret_cast<const uint8_t*>(&v);
+    s.insert(s.end(), b, b + sizeof(V));
+}
+
+// Launch kernel k on the context's stream, or record the launch (c.rec.on).
+template <typename... P, typename... A>
+inline void launch(Ctx& c, void (*k)(P...), dim3 grid, dim3 block, A&&... a) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    if (!c.rec.on) {
+        k<<<grid, block, 0, c.stream>>>(std::forward<A>(a)...);
+        return;
+    }
+    auto args = std::make_shared<std::tuple<std::decay_t<P>...>>(
+        static_cast<std::decay_t<P>>(std::forward<A>(a))...);
+    sig_put(c.rec.sig, reinterpret_cast<uintptr_t>(k));
+    const uint32_t dims[6] = {grid.x, grid.y, grid.z, block.x, block.y, block.z};
+    sig_put(c.rec.sig, dims);
+    std::apply([&](const auto&... v) { (sig_put(c.rec.sig, v), ...); }, *args);
+    c.rec.calls.push_back([k, grid, block, args](hipStream_t s) {
+        void* argv[sizeof...(P) + 1] = {};
+        size_t i = 0;
+        std::apply([&](auto&... v) { ((argv[i++] = static_cast<void*>(&v)), ...); }, *args);
+        MDD_HIP(hipLaunchKernel(reinterpret_cast<const void*>(k), grid, block, argv, 0, s));
+    });
 }
 
 }  // namespace mdd

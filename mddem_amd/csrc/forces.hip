@@ -17,6 +17,11 @@
 // and per-particle accumulation in FP32.
 #include "ctx.hpp"
 
+// row entries per lane gathered together by the LJ row kernel
+#ifndef MDD_LJ_BATCH
+#define MDD_LJ_BATCH 4
+#endif
+
 namespace mdd {
 
 namespace {
@@ -151,26 +156,54 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
             pe += 0.5 * (r6 * (a.e12 * r6 - a.e6));
         }
     };
-    if (__ballot(img) == 0ull) {
-#pragma unroll 4
-        for (int k = t; k < cnt; k += G) {
-            const uint32_t e = row[int64_t(k >> LG) * stride];
-            const double4 xj = a.pos[entry_idx(e)];
-            pair(e, xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
+    // Batches of kB entries per lane: the batch's kB gathers are issued
+    // together, and the next batch's row entries are loaded while they are
+    // in flight, so a wave waits on memory once per batch.
+    constexpr int kB = MDD_LJ_BATCH;
+    uint32_t en[kB];
+#pragma unroll
+    for (int u = 0; u < kB; u++) {
+        const int k = t + u * G;
+        en[u] = k < cnt ? row[int64_t(k >> LG) * stride] : 0u;
+    }
+    const bool any_img = __ballot(img) != 0ull;
+    for (int kb = t; kb < cnt; kb += kB * G) {
+        uint32_t e[kB];
+        double px[kB], py[kB], pz[kB];
+#pragma unroll
+        for (int u = 0; u < kB; u++) {
+            e[u] = en[u];
+            px[u] = py[u] = pz[u] = 0.0;
+            if (kb + u * G < cnt) {
+                const double4 xj = a.pos[entry_idx(e[u])];
+                px[u] = xj.x;
+                py[u] = xj.y;
+                pz[u] = xj.z;
+            }
         }
-    } else {
-#pragma unroll 4
-        for (int k = t; k < cnt; k += G) {
-            const uint32_t e = row[int64_t(k >> LG) * stride];
-            const double4 xj = a.pos[entry_idx(e)];
-            const int code = entry_code(e);
-            const bool rev = entry_rev(e);
-            const double shx = image_shift(code_sx(code), a.L[0]);
-            const double shy = image_shift(code_sy(code), a.L[1]);
-            const double shz = image_shift(code_sz(code), a.L[2]);
-            pair(e, (xj.x + (rev ? 0.0 : shx)) - (xi.x + (rev ? shx : 0.0)),
-                 (xj.y + (rev ? 0.0 : shy)) - (xi.y + (rev ? shy : 0.0)),
-                 (xj.z + (rev ? 0.0 : shz)) - (xi.z + (rev ? shz : 0.0)));
+#pragma unroll
+        for (int u = 0; u < kB; u++) {
+            const int k = kb + (kB + u) * G;
+            en[u] = k < cnt ? row[int64_t(k >> LG) * stride] : 0u;
+        }
+        if (!any_img) {
+#pragma unroll
+            for (int u = 0; u < kB; u++)
+                if (kb + u * G < cnt) pair(e[u], px[u] - xi.x, py[u] - xi.y, pz[u] - xi.z);
+        } else {
+#pragma unroll
+            for (int u = 0; u < kB; u++) {
+                if (kb + u * G < cnt) {
+                    const int code = entry_code(e[u]);
+                    const bool rev = entry_rev(e[u]);
+                    const double shx = image_shift(code_sx(code), a.L[0]);
+                    const double shy = image_shift(code_sy(code), a.L[1]);
+                    const double shz = image_shift(code_sz(code), a.L[2]);
+                    pair(e[u], (px[u] + (rev ? 0.0 : shx)) - (xi.x + (rev ? shx : 0.0)),
+                         (py[u] + (rev ? 0.0 : shy)) - (xi.y + (rev ? shy : 0.0)),
+                         (pz[u] + (rev ? 0.0 : shz)) - (xi.z + (rev ? shz : 0.0)));
+                }
+            }
         }
     }
     fx = group_sum<G>(fx);
@@ -565,7 +598,13 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
         e1 = c.ev_pool[2 * c.prof_pending + 1];
         MDD_HIP(hipEventRecord(e0, c.stream));
     }
-    if (c.model == 0) {
+    c.last_force_fused = false;
+    if (c.model == 0 && c.list_tile) {
+        if (energy) c.pe.resize(c.npad);
+        const bool fuse = !energy && step >= 0;
+        launch_lj_tile(c, step, energy, fuse);
+        c.last_force_fused = fuse;
+    } else if (c.model == 0) {
         if (energy) {
             c.pe.resize(c.npad);
             a.pe = c.pe.p;

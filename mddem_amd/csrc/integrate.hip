@@ -380,6 +380,13 @@ IntArgs make_args(Ctx& c, int step, bool check) {
     c.partial.resize(kMaxPartials);
     a.partial = c.partial.p;
     a.nparts = post_blocks(c, true);
+    if (c.list_tile) {
+        // tile lists: the fused force pass leaves per-cell partials, summed
+        // into ke_total by launch_ke_reduce
+        c.ke_total.resize(1);
+        a.partial = c.ke_total.p;
+        a.nparts = 1;
+    }
     a.q_in = step >= 0 ? ((step - 1) & 1) : 0;
     a.q_out = step >= 0 ? (step & 1) : 1;
     a.ndof = c.nh_ndof;
@@ -425,7 +432,13 @@ void launch_pre(Ctx& c, int step, bool check_rebuild) {
 
 // scheduled post kernel of a step; its thermostat half uses slot step & 1
 void launch_post(Ctx& c, int step) {
+    if (c.last_force_fused) {
+        // the tile force pass integrated already; sum its KE partials
+        if (c.nh_on) launch_ke_reduce(c, step, c.grid.ncells);
+        return;
+    }
     IntArgs a = make_args(c, step, false);
+    a.partial = c.partial.p;
     a.rot = c.rot_on && c.model == 1;
     a.vv = c.vv_on;
     a.q_in = step & 1;
@@ -433,6 +446,7 @@ void launch_post(Ctx& c, int step) {
         post_impl<true>(c, a);
     else
         post_impl<false>(c, a);
+    if (c.nh_on && c.list_tile) launch_ke_reduce(c, step, post_blocks(c, true));
 }
 
 void launch_initial_integration(Ctx& c, int step, bool check_rebuild) {
@@ -462,8 +476,10 @@ void launch_final_rotation(Ctx& c) {
 // KE partials of the current velocities (thermostat start)
 void launch_ke_partials(Ctx& c) {
     IntArgs a = make_args(c, -1, false);
+    a.partial = c.partial.p;
     k_ke_partials<<<post_blocks(c, true), kThreads, 0, c.stream>>>(a);
     MDD_HIP(hipGetLastError());
+    if (c.list_tile) launch_ke_reduce(c, -1, post_blocks(c, true));
 }
 
 void launch_nh_seed(Ctx& c, int parity) {
@@ -496,7 +512,9 @@ void launch_nh_post_standalone(Ctx& c) {
     k_nh_seed<<<1, 1, 0, c.stream>>>(a, 0, 0);
     MDD_HIP(hipGetLastError());
     a.q_in = 0;
+    a.partial = c.partial.p;
     post_impl<true>(c, a);
+    if (c.list_tile) launch_ke_reduce(c, -1, post_blocks(c, true));
     launch_nh_finish(c, 0);
 }
 

@@ -23,14 +23,11 @@
 // [cell_off, cell_off + ncells) and its own stencil runs and cutoffs.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
-#include "ctx.hpp"
+#include "cells.hpp"
 
 namespace mdd {
-
-void counting_sort(Ctx& c, const uint32_t* key, int64_t n, const uint32_t* n_dev, int64_t ncells,
-                   uint32_t* cnt, uint32_t* start, uint32_t* placed, uint32_t* order,
-                   uint32_t* slot_of);
 
 namespace {
 
@@ -44,11 +41,19 @@ __device__ inline uint32_t cell_of(const GridP& g, double x, double y, double z)
     return uint32_t(cx * g.n[1] * g.n[2] + cy * g.n[2] + cz);
 }
 
-__global__ void k_status_reset_build(Status* st) {
-    st->overflow = 0u;
-    st->max_row = 0;
-    st->max_half = 0;
-    st->ghost_total = 0u;
+// Start of a rebuild: the step sequence resumes (abort cleared), the build
+// status is reset and the sort's per-cell counts are zeroed.
+__global__ void k_rebuild_begin(Status* st, uint32_t* __restrict__ cnt, int64_t ncells) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < ncells;
+         i += int64_t(gridDim.x) * blockDim.x)
+        cnt[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->abort_step = INT_MAX;
+        st->overflow = 0u;
+        st->max_row = 0;
+        st->max_half = 0;
+        st->ghost_total = 0u;
+    }
 }
 
 // pbc wrap (wrap_periodic, domain.rs:396-407) + bin key of every particle
@@ -227,11 +232,6 @@ __global__ void k_ghost_emit(int64_t n, const double4* __restrict__ pos, BoxP bo
 }
 
 // combined bin starts: particles of a cell first (already sorted), then images
-__global__ void k_bin_start(int64_t ncells, const uint32_t* __restrict__ lstart,
-                            const uint32_t* __restrict__ gstart, uint32_t* __restrict__ bstart) {
-    int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (c <= ncells) bstart[c] = lstart[c] + gstart[c];
-}
 
 // binned copies of position / neighbor skin radius and the encoded entry
 __global__ void k_bin_fill(int64_t n, int64_t ghost_cap, const uint32_t* __restrict__ ng_live,
@@ -286,41 +286,15 @@ struct CellArgs {
     uint32_t* __restrict__ nhalf;
     uint32_t* __restrict__ gnbr;
     uint32_t* __restrict__ gcnt;
+    Status* st;
     const SysP* __restrict__ sysp;
     const int64_t* __restrict__ sys_cell_off;
     int nsys;
+    // tile rows (LJ): 16-bit candidate indices, laid out per cell by tile_addr
+    uint16_t* __restrict__ crow;
+    const uint32_t* __restrict__ lstart;  // first particle of each cell
+    int tile_cap;
 };
-
-// Blocks are dispatched round-robin over the 8 XCDs; map them so that each
-// XCD works on one contiguous range of cells, whose neighbourhoods share
-// that XCD's L2.
-__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    const uint32_t x = b & 7u, i = b >> 3, q = nb >> 3, r = nb & 7u;
-    return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + i;
-}
-
-// lanes above / below lane `rel` of a 64-lane chunk
-__device__ inline uint64_t lanes_above(int rel) {
-    if (rel < 0) return ~0ull;
-    if (rel >= 63) return 0ull;
-    return ~0ull << (rel + 1);
-}
-__device__ inline uint64_t lanes_below(int rel) {
-    if (rel <= 0) return 0ull;
-    if (rel >= 64) return ~0ull;
-    return (1ull << rel) - 1ull;
-}
-
-__device__ inline double readlane_f64(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(b), l));
-    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(b >> 32), l));
-    return __longlong_as_double(static_cast<long long>((uint64_t(hi) << 32) | lo));
-}
-
-__device__ inline uint32_t lanes_before(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-}
 
 // Pair cutoff of the cell's system: uniform (r2 < cut2) or per pair
 // (r2 < (s_i + s_j)^2 sf2), as neighbor.rs:938 and 1003-1004 choose it.
@@ -352,10 +326,19 @@ struct Window {
 // Pair tests of one particle of the cell (flat index fs relative to the
 // window) against the window. Candidates before it in bin order own the
 // pair with it (entry flagged), candidates after it are pairs it owns.
-template <bool UNIFORM>
+//
+// TILE: the row is written as 16-bit indices into the cell's candidate list
+// (flat index win + 64 c + lane) at tile_addr(s0, cap, M, Gt, m, k), for the
+// tile force kernel; tg = (s0, M, Gt, m, 65536 / Gt rounded up).
+struct TileMember {
+    uint32_t s0, M, G, m, ginv;
+};
+
+template <bool UNIFORM, bool TILE>
 __device__ inline void test_particle(const CellArgs& a, const CellCut& cc,
                                      const Window<UNIFORM>& W, int nch, int fs, const double4& xi,
-                                     uint32_t q, uint32_t& cnt, uint32_t& nrev) {
+                                     uint32_t q, uint32_t& cnt, uint32_t& nrev, uint32_t win,
+                                     const TileMember& tm) {
     const int lane = int(threadIdx.x & 63);
     const uint32_t G = 1u << a.lg;
 #pragma unroll
@@ -370,11 +353,19 @@ __device__ inline void test_particle(const CellArgs& a, const CellCut& cc,
                 const uint64_t before = __ballot(f < fs);
                 const uint32_t k = cnt + lanes_before(pass);
                 if (__builtin_amdgcn_inverse_ballot_w64(pass) && k < uint32_t(a.maxnb)) {
-                    // row_addr with 24-bit multiplies (npad < 2^24 is checked by the host)
-                    const uint32_t at =
-                        ((__umul24(k >> a.lg, uint32_t(a.npad)) + q) << a.lg) + (k & (G - 1u));
-                    a.nbr[at] = __builtin_amdgcn_inverse_ballot_w64(before) ? (W.e[c] | kRevBit)
-                                                                            : W.e[c];
+                    if (TILE) {
+                        const uint32_t kq = (k * tm.ginv) >> 16;  // k / G (k < 2^15)
+                        const uint32_t at = tm.s0 * uint32_t(a.tile_cap) + kq * (tm.M * tm.G) +
+                                            tm.m * tm.G + (k - kq * tm.G);
+                        a.crow[at] = uint16_t(win + uint32_t(f));
+                    } else {
+                        // row_addr with 24-bit multiplies (npad < 2^24 is checked by the host)
+                        const uint32_t at = ((__umul24(k >> a.lg, uint32_t(a.npad)) + q) << a.lg) +
+                                            (k & (G - 1u));
+                        a.nbr[at] = __builtin_amdgcn_inverse_ballot_w64(before)
+                                        ? (W.e[c] | kRevBit)
+                                        : W.e[c];
+                    }
                 }
                 cnt += uint32_t(__popcll(pass));
                 nrev += uint32_t(__popcll(pass & before));
@@ -423,7 +414,7 @@ __device__ inline void test_image(const CellArgs& a, const CellCut& cc,
 //
 // In a batched ensemble (BATCH) the cell space is shared: the wave finds its
 // cell's system and uses that system's cell range, stencil runs and cutoff.
-template <bool UNIFORM, bool BATCH>
+template <bool UNIFORM, bool BATCH, bool TILE>
 __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
     const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const int lane = int(threadIdx.x & 63);
@@ -478,6 +469,14 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
     // flat indices of this cell's first slot and of the slot after it
     const int f_i0 = int(i0 - base0), f_i1 = int(i1 - base0);
     const uint32_t ni = i1 - i0;
+    TileMember tm{0u, 0u, 1u, 0u, 65536u};
+    if (TILE) {
+        tm.s0 = a.lstart[C];
+        tm.M = a.lstart[C + 1] - tm.s0;
+        tm.G = uint32_t(tile_lanes(tm.M));
+        tm.ginv = (65536u + tm.G - 1u) / tm.G;
+        if (total > 65535u) atomicOr(&a.st->overflow, 8u);  // tile index range
+    }
     for (uint32_t ib = 0; ib < ni; ib += 64) {
         const uint32_t nib = ni - ib < 64u ? ni - ib : 64u;
         uint32_t cntv = 0, revv = 0;  // lane l: member ib + l
@@ -537,7 +536,9 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
                 uint32_t cnt = __builtin_amdgcn_readlane(cntv, int(ii));
                 uint32_t nrev = __builtin_amdgcn_readlane(revv, int(ii));
                 const int fs = f_i0 + int(ib + ii) - shift;
-                test_particle<UNIFORM>(a, cc, W, nch, fs, xi, entry_idx(ei), cnt, nrev);
+                if (TILE) tm.m = ib + ii;
+                test_particle<UNIFORM, TILE>(a, cc, W, nch, fs, xi, entry_idx(ei), cnt, nrev, win,
+                                             tm);
                 if (uint32_t(lane) == ii) {
                     cntv = cnt;
                     revv = nrev;
@@ -570,7 +571,7 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
 
 // Row length of every particle: its own cell pass plus the pairs owned
 // through each of its periodic images. Row maxima go to per-block slots
-// (blk[3 b .. 3 b + 2]); k_row_finish reduces them, so no two blocks contend
+// (blk[2 b], blk[2 b + 1]); k_rows_gate reduces them, so no two blocks contend
 // for one address.
 __global__ void __launch_bounds__(256) k_row_totals(int64_t n, const uint32_t* __restrict__ goff,
                                                     const uint32_t* __restrict__ ng_live,
@@ -578,16 +579,20 @@ __global__ void __launch_bounds__(256) k_row_totals(int64_t n, const uint32_t* _
                                                     const uint32_t* __restrict__ nmain,
                                                     const uint32_t* __restrict__ nhalf,
                                                     uint32_t* __restrict__ nnb,
-                                                    uint32_t* __restrict__ blk) {
+                                                    uint32_t* __restrict__ blk,
+                                                    uint32_t* __restrict__ nx) {
     __shared__ uint32_t lds[2][4];
     const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     uint32_t cnt = 0, nh = 0;
     if (q < n) {
         cnt = nmain[q];
         nh = nhalf[q];
+        uint32_t xs = 0;
         if (*ng_live != 0u)
-            for (uint32_t g = goff[q]; g < goff[q + 1]; g++) cnt += gcnt[g];
+            for (uint32_t g = goff[q]; g < goff[q + 1]; g++) xs += gcnt[g];
+        cnt += xs;
         nnb[q] = cnt;
+        if (nx) nx[q] = xs;  // tile rows: image-owned pairs kept apart
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -612,36 +617,6 @@ __global__ void __launch_bounds__(256) k_row_totals(int64_t n, const uint32_t* _
     }
 }
 
-__global__ void __launch_bounds__(256) k_row_finish(int nblk, const uint32_t* __restrict__ blk,
-                                                    int maxnb, Status* st) {
-    __shared__ uint32_t lds[2][4];
-    uint32_t mc = 0, mh = 0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        mc = blk[2 * b] > mc ? blk[2 * b] : mc;
-        mh = blk[2 * b + 1] > mh ? blk[2 * b + 1] : mh;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t x = __shfl_xor(mc, o, 64), y = __shfl_xor(mh, o, 64);
-        mc = x > mc ? x : mc;
-        mh = y > mh ? y : mh;
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        lds[0][w] = mc;
-        lds[1][w] = mh;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < int(blockDim.x >> 6); k++) {
-            mc = lds[0][k] > mc ? lds[0][k] : mc;
-            mh = lds[1][k] > mh ? lds[1][k] : mh;
-        }
-        st->max_row = int(mc);
-        st->max_half = int(mh);
-        if (mc > uint32_t(maxnb)) atomicOr(&st->overflow, 2u);
-    }
-}
 
 // Copy each image's scratch row to the end of its particle's row: after the
 // particle's own pass and the rows of the particle's earlier images.
@@ -654,20 +629,26 @@ __global__ void __launch_bounds__(256) k_copy_image_rows(int64_t ghost_cap, int6
                                                          const uint32_t* __restrict__ gcnt,
                                                          const uint32_t* __restrict__ nmain,
                                                          const uint32_t* __restrict__ gnbr,
-                                                         uint32_t* __restrict__ nbr) {
+                                                         uint32_t* __restrict__ nbr,
+                                                         uint32_t* __restrict__ xnbr) {
     const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     const int64_t g = gt / kCopyLanes;
     const uint32_t t = uint32_t(gt % kCopyLanes);
     if (g >= ghost_cap || g >= int64_t(*ng_live)) return;
     const uint32_t p = groot[g];
-    uint32_t off = nmain[p];
+    // tile rows (xnbr given): a separate row per particle, p * maxnb + k
+    uint32_t off = xnbr ? 0u : nmain[p];
     for (uint32_t h = goff[p]; h < uint32_t(g); h++) off += gcnt[h];
     uint32_t c = gcnt[g];
     if (c > uint32_t(maxnb)) c = uint32_t(maxnb);
     for (uint32_t k = t; k < c; k += kCopyLanes) {
         const uint32_t kk = off + k;
-        if (kk < uint32_t(maxnb))
-            nbr[row_addr(p, int(kk), npad, lg)] = gnbr[size_t(g) * uint32_t(maxnb) + k];
+        if (kk >= uint32_t(maxnb)) continue;
+        const uint32_t e = gnbr[size_t(g) * uint32_t(maxnb) + k];
+        if (xnbr)
+            xnbr[size_t(p) * uint32_t(maxnb) + kk] = e;
+        else
+            nbr[row_addr(p, int(kk), npad, lg)] = e;
     }
 }
 
@@ -727,16 +708,56 @@ void neighbor_setup(Ctx& c);
 
 namespace {
 
-__global__ void k_status_reset_attempt(Status* st) {
-    st->overflow &= 4u;  // keep the fixed-boundary exit seen by the wrap
-    st->max_row = 0;
-    st->max_half = 0;
-    st->ghost_total = 0u;
+// Start of one build attempt (a retry repeats it with larger capacities).
+__global__ void k_attempt_begin(Status* st, uint32_t* __restrict__ cnt, int64_t ncells) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < ncells;
+         i += int64_t(gridDim.x) * blockDim.x)
+        cnt[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->abort_step = INT_MAX;
+        st->overflow &= 4u;  // keep the fixed-boundary exit seen by the wrap
+        st->max_row = 0;
+        st->max_half = 0;
+        st->ghost_total = 0u;
+    }
 }
 
-// a build that did not fit stops the step sequence at this step
-__global__ void k_overflow_gate(Status* st, int step) {
-    if (st->overflow != 0u) atomicMin(&st->abort_step, step);
+// End of a build attempt: row maxima from the per-block slots into the
+// status, and a build that did not fit stops the step sequence at `step`.
+__global__ void __launch_bounds__(256) k_rows_gate(int nblk, const uint32_t* __restrict__ blk,
+                                                   int maxnb, Status* st, int step) {
+    __shared__ uint32_t lds[2][4];
+    uint32_t mc = 0, mh = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        mc = blk[2 * b] > mc ? blk[2 * b] : mc;
+        mh = blk[2 * b + 1] > mh ? blk[2 * b + 1] : mh;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t x = __shfl_xor(mc, o, 64), y = __shfl_xor(mh, o, 64);
+        mc = x > mc ? x : mc;
+        mh = y > mh ? y : mh;
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[0][w] = mc;
+        lds[1][w] = mh;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < int(blockDim.x >> 6); k++) {
+            mc = lds[0][k] > mc ? lds[0][k] : mc;
+            mh = lds[1][k] > mh ? lds[1][k] : mh;
+        }
+        st->max_row = int(mc);
+        st->max_half = int(mh);
+        uint32_t ov = st->overflow;
+        if (mc > uint32_t(maxnb)) {
+            ov |= 2u;
+            st->overflow = ov;
+        }
+        if (ov != 0u && step < st->abort_step) st->abort_step = step;
+    }
 }
 
 // pbc, spatial sort of the particles, and the previous list kept as the
@@ -761,16 +782,18 @@ void rebuild_prepare(Ctx& c) {
     // reallocated while kernels that use it are queued
     c.scan_tmp.resize(size_t((std::max<int64_t>(ncells, n) + 2047) / 2048 + 2));
 
-    k_status_reset_build<<<1, 1, 0, c.stream>>>(c.status.p);
+    launch(c, k_rebuild_begin, blocks_for(ncells, 256) < 512 ? blocks_for(ncells, 256) : 512, 256,
+           c.status.p, c.cnt.p, ncells);
     // pbc + keys
-    k_wrap_key<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, c.pos.p, c.image.p, c.box, c.grid,
+    launch(c, k_wrap_key, blocks_for(n, 256), 256, n, c.pos.p, c.image.p, c.box, c.grid,
                                                           c.key.p, c.status.p, c.sysp_or_null(),
                                                           c.psys.p);
     // sort_atoms_by_bin: stable counting sort of particles by bin
     counting_sort(c, c.key.p, n, nullptr, ncells, c.cnt.p, c.start.p, c.placed.p, c.order.p,
-                  nullptr);
+                  nullptr, true);
     // permute every particle array into the alternates, then swap
-    PermArgs a{};
+    PermArgs a;
+    std::memset(&a, 0, sizeof(a));  // padding too: the bytes are a launch signature
     a.n = n;
     a.npad = c.npad;
     a.order = c.order.p;
@@ -807,7 +830,7 @@ void rebuild_prepare(Ctx& c) {
         a.t64b = dem ? c.t64b.p : nullptr;
     }
     a.dem = dem;
-    k_permute<<<blocks_for(n, 256), 256, 0, c.stream>>>(a);
+    launch(c, k_permute, blocks_for(n, 256), 256, a);
     MDD_HIP(hipGetLastError());
     c.pos.swap(c.pos2);
     c.vel.swap(c.vel2);
@@ -863,6 +886,14 @@ void rebuild_attempt(Ctx& c) {
     if (int64_t(c.maxnb) * c.npad >= (int64_t(1) << 31) || c.npad >= (int64_t(1) << 24))
         throw ApiError(-5, "neighbor list too large for 32-bit row indexing");
     c.nbr.resize(size_t(c.maxnb) * c.npad);
+    // LJ: rows as 16-bit indices into each cell's candidate list (tile force kernel)
+    const bool tile = c.want_tile();
+    c.tile_cap = (c.maxnb + 15 + 7) / 8 * 8;  // >= maxnb + 15: rows of any G <= 16 fit
+    if (tile) {
+        c.crow.resize(size_t(c.tile_cap) * size_t(n));
+        c.xnbr.resize(size_t(c.maxnb) * size_t(n));
+        c.nx.resize(size_t(n));
+    }
     c.gnbr.resize(size_t(c.ghost_cap) * c.maxnb);
     c.gcnt_img.resize(c.ghost_cap);
     c.nnb.resize(c.npad);
@@ -876,27 +907,28 @@ void rebuild_attempt(Ctx& c) {
     }
     uint32_t* ng_live = c.ghost_key.p + c.ghost_cap;
 
-    k_status_reset_attempt<<<1, 1, 0, c.stream>>>(c.status.p);
+    launch(c, k_attempt_begin, blocks_for(ncells, 256) < 512 ? blocks_for(ncells, 256) : 512, 256,
+           c.status.p, c.cnt.p, ncells);
     // borders: periodic images and their bins
-    k_ghost_count<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, c.pos.p, c.box, c.ghost_cutoff,
+    launch(c, k_ghost_count, blocks_for(n, 256), 256, n, c.pos.p, c.box, c.ghost_cutoff,
                                                              c.gcnt.p, c.sysp_or_null(), c.psys.p);
     exclusive_scan(c, c.gcnt.p, c.goff.p, n);
-    k_ghost_emit<<<blocks_for(n, 256), 256, 0, c.stream>>>(
+    launch(c, k_ghost_emit, blocks_for(n, 256), 256,
         n, c.pos.p, c.box, c.ghost_cutoff, c.grid, c.goff.p, c.ghost_cap, c.ghost_root.p,
         c.ghost_code.p, c.ghost_pos.p, c.ghost_key.p, ng_live, c.status.p, c.sysp_or_null(),
         c.psys.p);
     // images binned after the particles of each cell (bin_neighbor_list
     // steps 1-2b: counting sort in index order, images have the larger indices)
+    // bin_start = particle start + image start of every cell, from the same scan
     counting_sort(c, c.ghost_key.p, c.ghost_cap, ng_live, ncells, c.cnt.p, c.gstart.p, c.placed.p,
-                  c.ghost_order.p, nullptr);
-    k_bin_start<<<blocks_for(ncells + 1, 256), 256, 0, c.stream>>>(ncells, c.start.p, c.gstart.p,
-                                                                     c.bin_start.p);
-    k_bin_fill<<<blocks_for(nall_max, 256), 256, 0, c.stream>>>(
+                  c.ghost_order.p, nullptr, true, c.start.p, c.bin_start.p);
+    launch(c, k_bin_fill, blocks_for(nall_max, 256), 256,
         n, c.ghost_cap, ng_live, c.key.p, c.start.p, c.gstart.p, c.bin_start.p, c.ghost_order.p,
         c.ghost_key.p, c.pos.p, c.cutr.p, c.ghost_pos.p, c.ghost_root.p, c.ghost_code.p,
         c.bin_pos.p, c.bin_entry.p, c.bin_gidx.p, c.slot_of.p);
     // rows: one wavefront per cell, then image-owned pairs appended
-    CellArgs ca{};
+    CellArgs ca;
+    std::memset(&ca, 0, sizeof(ca));
     ca.ncells = ncells;
     ca.npad = c.npad;
     ca.maxnb = c.maxnb;
@@ -919,24 +951,34 @@ void rebuild_attempt(Ctx& c) {
     ca.sysp = c.sysp_or_null();
     ca.sys_cell_off = c.sys_cell_off.p;
     ca.nsys = c.nsys;
+    ca.st = c.status.p;
+    ca.crow = tile ? c.crow.p : nullptr;
+    ca.lstart = c.start.p;
+    ca.tile_cap = c.tile_cap;
     const int cblocks = blocks_for(ncells, kCellWaves);
     if (c.nsys > 0)
-        k_build_cells<false, true><<<cblocks, 256, 0, c.stream>>>(ca);
+        launch(c, k_build_cells<false, true, false>, cblocks, 256, ca);
+    else if (tile && c.uniform)
+        launch(c, k_build_cells<true, false, true>, cblocks, 256, ca);
+    else if (tile)
+        launch(c, k_build_cells<false, false, true>, cblocks, 256, ca);
     else if (c.uniform)
-        k_build_cells<true, false><<<cblocks, 256, 0, c.stream>>>(ca);
+        launch(c, k_build_cells<true, false, false>, cblocks, 256, ca);
     else
-        k_build_cells<false, false><<<cblocks, 256, 0, c.stream>>>(ca);
+        launch(c, k_build_cells<false, false, false>, cblocks, 256, ca);
     const int tblocks = blocks_for(n, 256);
     c.row_blk.resize(size_t(2) * tblocks);
-    k_row_totals<<<tblocks, 256, 0, c.stream>>>(n, c.goff.p, ng_live, c.gcnt_img.p, c.nmain.p,
-                                                c.nhalf.p, c.nnb.p, c.row_blk.p);
-    k_row_finish<<<1, 256, 0, c.stream>>>(tblocks, c.row_blk.p, c.maxnb, c.status.p);
-    k_copy_image_rows<<<blocks_for(c.ghost_cap * kCopyLanes, 256), 256, 0, c.stream>>>(
+    launch(c, k_row_totals, tblocks, 256, n, c.goff.p, ng_live, c.gcnt_img.p, c.nmain.p,
+                                                c.nhalf.p, c.nnb.p, c.row_blk.p,
+                                                tile ? c.nx.p : nullptr);
+    c.row_nblk = tblocks;
+    launch(c, k_copy_image_rows, blocks_for(c.ghost_cap * kCopyLanes, 256), 256,
         c.ghost_cap, c.npad, c.maxnb, c.lg, ng_live, c.ghost_root.p, c.goff.p, c.gcnt_img.p,
-        c.nmain.p, c.gnbr.p, c.nbr.p);
+        c.nmain.p, c.gnbr.p, c.nbr.p, tile ? c.xnbr.p : nullptr);
     MDD_HIP(hipGetLastError());
+    c.list_tile = tile;
     if (dem) {
-        k_remap_history<<<blocks_for(n, 256), 256, 0, c.stream>>>(
+        launch(c, k_remap_history, blocks_for(n, 256), 256,
             n, c.npad, c.maxnb, c.lg, c.nbr.p, c.nnb.p, c.tag.p, c.order.p, c.npad_old,
             c.maxnb_old, c.lg_old, c.nbr_old.p, c.nnb_old.p, c.tag2.p, c.hist_old.p,
             c.cmask_old.p, c.hist.p, c.cmask.p, c.remap_have_old ? 1 : 0);
@@ -965,6 +1007,10 @@ bool rebuild_grow(Ctx& c, const Status& s) {
         if (c.model == 1 && c.maxnb > 64) c.maxnb = s.max_row <= 64 ? 64 : s.max_row;
         again = true;
     }
+    if (s.overflow & 8u) {
+        c.tile_blocked = true;  // a candidate list beyond 16-bit indices
+        again = true;
+    }
     return again;
 }
 
@@ -974,6 +1020,8 @@ void rebuild(Ctx& c) {
     bool fitted = false;
     for (int attempt = 0; attempt < 6 && !fitted; attempt++) {
         rebuild_attempt(c);
+        k_rows_gate<<<1, 256, 0, c.stream>>>(c.row_nblk, c.row_blk.p, c.maxnb, c.status.p,
+                                             INT_MAX);
         MDD_HIP(hipMemcpyAsync(c.status_host, c.status.p, sizeof(Status), hipMemcpyDeviceToHost,
                                c.stream));
         MDD_HIP(hipStreamSynchronize(c.stream));
@@ -989,14 +1037,97 @@ void rebuild(Ctx& c) {
 // rebuild request) and the host repeats the build with larger capacities,
 // starting from the same sorted state (retry = true), when it next reads
 // the status.
-void rebuild_async(Ctx& c, int step, bool retry) {
-    if (!retry) {
-        rebuild_prepare(c);
-        c.rebuilds++;
+namespace {
+
+// Issue the recorded launches: replay the captured graph whose signature is
+// the same, or capture them as a new graph (at most kGraphs kept; the two
+// alternating buffer sets of successive rebuilds give two signatures).
+constexpr size_t kGraphs = 4;
+
+void issue_recorded(Ctx& c) {
+    for (size_t k = 0; k < c.graphs.size(); k++) {
+        if (c.graphs[k].sig == c.rec.sig) {
+            MDD_HIP(hipGraphLaunch(c.graphs[k].exec, c.stream));
+            c.graph_hits++;
+            std::rotate(c.graphs.begin() + long(k), c.graphs.begin() + long(k) + 1,
+                        c.graphs.end());
+            return;
+        }
     }
-    rebuild_attempt(c);
-    k_overflow_gate<<<1, 1, 0, c.stream>>>(c.status.p, step);
+    MDD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+        for (auto& f : c.rec.calls) f(c.stream);
+    } catch (...) {
+        if (hipStreamEndCapture(c.stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    MDD_HIP(hipStreamEndCapture(c.stream, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MDD_HIP(e);
+    MDD_HIP(hipGraphLaunch(ex, c.stream));
+    c.graph_captures++;
+    if (c.graphs.size() >= kGraphs) {
+        (void)hipGraphExecDestroy(c.graphs.front().exec);
+        c.graphs.erase(c.graphs.begin());
+    }
+    c.graphs.push_back(GraphEntry{c.rec.sig, ex});
+}
+
+void end_recording(Ctx& c) {
+    c.rec.on = false;
+    deferred_frees() = nullptr;
+}
+
+}  // namespace
+
+void release_graphs(Ctx& c) {
+    for (auto& g : c.graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c.graphs.clear();
+}
+
+// One rebuild. A first attempt (not a retry) is recorded and issued as a
+// graph: its ~30 launches reach the device at once instead of one host
+// launch at a time.
+void rebuild_async(Ctx& c, int step, bool retry) {
+    const bool timed = c.prof_every > 0 && c.rb_pending < int64_t(c.rb_ev.size() / 2);
+    if (timed) MDD_HIP(hipEventRecord(c.rb_ev[2 * c.rb_pending], c.stream));
+    const bool record = !retry && c.graphs_on && c.setup_done;
+    if (record) {
+        c.rec.sig.clear();
+        c.rec.calls.clear();
+        c.rec.frees.clear();
+        c.rec.on = true;
+        deferred_frees() = &c.rec.frees;
+    }
+    try {
+        if (!retry) {
+            rebuild_prepare(c);
+            c.rebuilds++;
+        }
+        rebuild_attempt(c);
+    } catch (...) {
+        if (record) end_recording(c);
+        throw;
+    }
+    if (record) {
+        end_recording(c);
+        issue_recorded(c);
+        c.rec.calls.clear();
+        if (!c.rec.frees.empty()) {
+            // buffers replaced during the rebuild: free them once the
+            // launches that still name them have run
+            MDD_HIP(hipStreamSynchronize(c.stream));
+            for (void* p : c.rec.frees) (void)hipFree(p);
+            c.rec.frees.clear();
+        }
+    }
+    k_rows_gate<<<1, 256, 0, c.stream>>>(c.row_nblk, c.row_blk.p, c.maxnb, c.status.p, step);
     MDD_HIP(hipGetLastError());
+    if (timed) MDD_HIP(hipEventRecord(c.rb_ev[2 * c.rb_pending++ + 1], c.stream));
 }
 
 namespace {

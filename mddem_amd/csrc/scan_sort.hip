@@ -114,6 +114,89 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const uint32_t* __r
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sums[nb];
 }
 
+// Exclusive scan of in[0..n) by one workgroup, for arrays up to
+// kOneBlockScan elements: out[0..n] (out[n] = total). zero_in: in[i] is
+// cleared once read (a counting sort's count array becomes its cursor).
+// add/out2: out2[i] = out[i] + add[i] as well (bin starts: particles plus
+// images of each cell).
+constexpr int kOneThreads = 1024;
+constexpr int kOneItems = 8;
+constexpr int kOneTile = kOneThreads * kOneItems;
+constexpr int64_t kOneBlockScan = int64_t(1) << 17;
+
+// Tiles of kOneTile elements pass through LDS: read and written coalesced,
+// scanned as kOneItems contiguous elements per thread.
+__global__ void __launch_bounds__(kOneThreads) k_scan_one(uint32_t* __restrict__ in,
+                                                          uint32_t* __restrict__ out, int64_t n,
+                                                          int zero_in,
+                                                          const uint32_t* __restrict__ add,
+                                                          uint32_t* __restrict__ out2) {
+    __shared__ uint4 tile4[kOneTile / 4];
+    __shared__ uint32_t wsum[kOneThreads / 64];
+    uint32_t* tile = reinterpret_cast<uint32_t*>(tile4);
+    const int tid = int(threadIdx.x), lane = tid & 63, w = tid >> 6;
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < n; base += kOneTile) {
+#pragma unroll
+        for (int k = 0; k < kOneItems; k++) {
+            const int64_t i = base + int64_t(k) * kOneThreads + tid;
+            uint32_t v = 0u;
+            if (i < n) {
+                v = in[i];
+                if (zero_in) in[i] = 0u;
+            }
+            tile[k * kOneThreads + tid] = v;
+        }
+        __syncthreads();
+        const uint4 a = tile4[2 * tid], b = tile4[2 * tid + 1];
+        const uint32_t s = (a.x + a.y + a.z + a.w) + (b.x + b.y + b.z + b.w);
+        const uint32_t inc = wave_incl_scan(s);
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint32_t off = carry, tot = 0;
+#pragma unroll
+        for (int k = 0; k < kOneThreads / 64; k++) {
+            const uint32_t x = wsum[k];
+            if (k < w) off += x;
+            tot += x;
+        }
+        uint4 ra, rb;
+        ra.x = off + inc - s;
+        ra.y = ra.x + a.x;
+        ra.z = ra.y + a.y;
+        ra.w = ra.z + a.z;
+        rb.x = ra.w + a.w;
+        rb.y = rb.x + b.x;
+        rb.z = rb.y + b.y;
+        rb.w = rb.z + b.z;
+        tile4[2 * tid] = ra;
+        tile4[2 * tid + 1] = rb;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kOneItems; k++) {
+            const int64_t i = base + int64_t(k) * kOneThreads + tid;
+            if (i < n) {
+                const uint32_t r = tile[k * kOneThreads + tid];
+                out[i] = r;
+                if (out2) out2[i] = r + add[i];
+            }
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[n] = carry;
+        if (out2) out2[n] = carry + add[n];
+    }
+}
+
+// out[i] = a[i] + b[i], i < n
+__global__ void k_add_u32(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                          uint32_t* __restrict__ out, int64_t n) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i < n) out[i] = a[i] + b[i];
+}
+
 // Runs of equal keys inside a wavefront: the lane of the run head (h) and
 // of the run tail (t). Invalid lanes form runs of their own.
 __device__ inline void wave_runs(uint32_t key, bool valid, int& h, int& t) {
@@ -198,46 +281,57 @@ int fill_blocks(int64_t n) {
 
 void fill_u32(Ctx& c, uint32_t* p, uint32_t v, int64_t n) {
     if (n <= 0) return;
-    k_fill_u32<<<fill_blocks(n), 256, 0, c.stream>>>(p, v, n);
+    launch(c, k_fill_u32, fill_blocks(n), 256, p, v, n);
     MDD_HIP(hipGetLastError());
 }
 void fill_f64(Ctx& c, double* p, double v, int64_t n) {
     if (n <= 0) return;
-    k_fill_f64<<<fill_blocks(n), 256, 0, c.stream>>>(p, v, n);
+    launch(c, k_fill_f64, fill_blocks(n), 256, p, v, n);
     MDD_HIP(hipGetLastError());
 }
 void fill_f32(Ctx& c, float* p, float v, int64_t n) {
     if (n <= 0) return;
-    k_fill_f32<<<fill_blocks(n), 256, 0, c.stream>>>(p, v, n);
+    launch(c, k_fill_f32, fill_blocks(n), 256, p, v, n);
     MDD_HIP(hipGetLastError());
 }
 
 // out[0..n] = exclusive scan of in[0..n), out[n] = total
 void exclusive_scan(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n) {
+    if (n <= kOneBlockScan) {
+        launch(c, k_scan_one, 1, kOneThreads, const_cast<uint32_t*>(in), out, n, 0,
+               static_cast<const uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr));
+        return;
+    }
     int64_t nb = (n + kScanTile - 1) / kScanTile;
     if (nb < 1) nb = 1;
     c.scan_tmp.resize(size_t(nb + 2));
-    k_scan_reduce<<<int(nb), kScanThreads, 0, c.stream>>>(in, n, c.scan_tmp.p);
-    k_scan_sums<<<1, kScanThreads, 0, c.stream>>>(c.scan_tmp.p, nb);
-    k_scan_apply<<<int(nb), kScanThreads, 0, c.stream>>>(in, out, n, c.scan_tmp.p, nb);
+    launch(c, k_scan_reduce, int(nb), kScanThreads, in, n, c.scan_tmp.p);
+    launch(c, k_scan_sums, 1, kScanThreads, c.scan_tmp.p, nb);
+    launch(c, k_scan_apply, int(nb), kScanThreads, in, out, n, c.scan_tmp.p, nb);
     MDD_HIP(hipGetLastError());
 }
 
 // Stable counting sort of indices 0..n-1 by key in [0, ncells). The element
 // count is n, or *n_dev (<= n) when n_dev is given. start: ncells+1;
 // order: n (slot -> index); slot_of: n (index -> slot) or null; cnt: ncells
-// scratch; placed: n scratch.
+// scratch, already zero when cnt_zero; placed: n scratch. With add given,
+// out2[c] = start[c] + add[c] for c <= ncells as well.
 void counting_sort(Ctx& c, const uint32_t* key, int64_t n, const uint32_t* n_dev, int64_t ncells,
                    uint32_t* cnt, uint32_t* start, uint32_t* placed, uint32_t* order,
-                   uint32_t* slot_of) {
-    fill_u32(c, cnt, 0u, ncells);
-    if (n > 0) k_count<<<blocks_for(n, 256), 256, 0, c.stream>>>(key, n, n_dev, cnt);
-    exclusive_scan(c, cnt, start, ncells);
-    fill_u32(c, cnt, 0u, ncells);
+                   uint32_t* slot_of, bool cnt_zero, const uint32_t* add, uint32_t* out2) {
+    if (!cnt_zero) fill_u32(c, cnt, 0u, ncells);
+    if (n > 0) launch(c, k_count, blocks_for(n, 256), 256, key, n, n_dev, cnt);
+    if (ncells <= kOneBlockScan) {
+        // scan and cursor reset in one pass
+        launch(c, k_scan_one, 1, kOneThreads, cnt, start, ncells, 1, add, out2);
+    } else {
+        exclusive_scan(c, cnt, start, ncells);
+        fill_u32(c, cnt, 0u, ncells);
+        if (add) launch(c, k_add_u32, blocks_for(ncells + 1, 256), 256, start, add, out2, ncells + 1);
+    }
     if (n > 0) {
-        k_scatter<<<blocks_for(n, 256), 256, 0, c.stream>>>(key, n, n_dev, start, cnt, placed);
-        k_rank<<<blocks_for(n, 256), 256, 0, c.stream>>>(key, n, n_dev, start, placed, order,
-                                                          slot_of);
+        launch(c, k_scatter, blocks_for(n, 256), 256, key, n, n_dev, start, cnt, placed);
+        launch(c, k_rank, blocks_for(n, 256), 256, key, n, n_dev, start, placed, order, slot_of);
     }
     MDD_HIP(hipGetLastError());
 }

@@ -98,7 +98,10 @@ SIGNATURES = {
     "mdd_synchronize": (C.c_int, [_vp]),
     "mdd_set_profiling": (C.c_int, [_vp, C.c_int]),
     "mdd_force_kernel_stats": (C.c_int, [_vp, _dp, _i64p]),
+    "mdd_rebuild_stats": (C.c_int, [_vp, _dp, _i64p]),
     "mdd_set_force_lanes": (C.c_int, [_vp, C.c_int]),
+    "mdd_set_launch_graphs": (C.c_int, [_vp, C.c_int]),
+    "mdd_launch_graph_stats": (C.c_int, [_vp, _i64p, _i64p]),
     "mdd_device_memory": (C.c_int, [C.c_int, _i64p, _i64p]),
     "mdd_add_system": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                  C.c_int64, _u32p, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -361,8 +364,26 @@ class System:
         self._c(self.L.mdd_force_kernel_stats(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def rebuild_stats(self):
+        """(average device ms of a timed rebuild, rebuilds timed)."""
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        self._c(self.L.mdd_rebuild_stats(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def set_force_lanes(self, log2_lanes: int):
+        """Row kernels at 2^log2_lanes lanes per particle (0..3), or the LJ
+        cell-tile kernel (-1)."""
         self._c(self.L.mdd_set_force_lanes(self.h, int(log2_lanes)))
+
+    def set_launch_graphs(self, enabled: bool):
+        self._c(self.L.mdd_set_launch_graphs(self.h, int(bool(enabled))))
+
+    def launch_graph_stats(self):
+        """(rebuilds replayed from a captured graph, graphs captured)."""
+        r, k = C.c_int64(0), C.c_int64(0)
+        self._c(self.L.mdd_launch_graph_stats(self.h, C.byref(r), C.byref(k)))
+        return r.value, k.value
 
 
 def device_memory(device: int = 0):
