@@ -83,11 +83,12 @@ def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True, layout=Non
     t1 = time.perf_counter()
     th1 = s.thermo(energy=(w.model == "lj"))
     avg_ms, launches = s.force_kernel_stats()
-    rb_ms, rb_n = s.rebuild_stats()
+    rb_ms, rb_gap_ms, rb_n = s.rebuild_stats()
     res = dict(seconds=t1 - t0, steps=steps, n=w.n, rebuilds=th1["rebuilds"] - th0["rebuilds"],
                npairs=th1["npairs"], force_ms=avg_ms, force_launches=launches,
                temperature=th1["temperature"], pe=th1["pe"], ke=th1["ke"],
-               layout=layout_name(layout), tile=layout < 0, rebuild_ms=rb_ms, rebuilds_timed=rb_n)
+               layout=layout_name(layout), tile=layout < 0, rebuild_ms=rb_ms, rebuild_gap_ms=rb_gap_ms,
+               rebuilds_timed=rb_n)
     s.close()
     return res
 
@@ -209,7 +210,8 @@ def main():
     value = r["n"] * r["steps"] / r["seconds"]
     log(f"[bench] {value:.4g} particle-steps/s, {r['rebuilds']} rebuilds, "
         f"{r['layout']}, force kernel {r['force_ms'] * 1e3:.1f} us, "
-        f"rebuild {r['rebuild_ms'] * 1e3:.1f} us ({r['rebuilds_timed']} timed)")
+        f"rebuild {r['rebuild_ms'] * 1e3:.1f} us + {r['rebuild_gap_ms'] * 1e3:.1f} us idle before "
+        f"({r['rebuilds_timed']} timed)")
 
     fp64 = None
     if prec == hip.MIXED and args.fp64_steps > 0:
@@ -281,6 +283,7 @@ def main():
                    "rebuilds_in_timed_loop": r["rebuilds"],
                    "force_layout": r["layout"],
                    "rebuild_us": round(r["rebuild_ms"] * 1e3, 1),
+                   "idle_before_rebuild_us": round(r["rebuild_gap_ms"] * 1e3, 1),
                    "pairs_per_atom_half": round(r["npairs"] / r["n"], 3)},
         "fp64_gpu_value": None if fp64 is None else round(fp64, 1),
         "mixed_over_fp64": None if fp64 is None else round(value / fp64, 3),

@@ -198,8 +198,10 @@ int mdd_set_profiling(mdd_ctx* ctx, int every);
 int mdd_force_kernel_stats(mdd_ctx* ctx, double* avg_ms, int64_t* launches);
 /* Average device time (ms) of the neighbor rebuilds timed since the last
  * mdd_set_profiling (first rebuild launch to the end of its last kernel, on
- * the context's stream), and how many were timed. */
-int mdd_rebuild_stats(mdd_ctx* ctx, double* avg_ms, int64_t* count);
+ * the context's stream), the average idle gap before a rebuild (end of the
+ * step chunk that asked for it to its first launch: the host round trip),
+ * and how many rebuilds were timed. */
+int mdd_rebuild_stats(mdd_ctx* ctx, double* avg_ms, double* avg_gap_ms, int64_t* count);
 /* Free and total device memory of a device (hipMemGetInfo), for admission by
  * footprint. No reference counterpart: serving-loop support. */
 int mdd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);

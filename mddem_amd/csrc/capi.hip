@@ -14,6 +14,7 @@
 // scheduled by the host.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -106,9 +107,14 @@ void harvest_profile(Ctx& c) {
     c.prof_pending = 0;
     for (int64_t k = 0; k < c.rb_pending; k++) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c.rb_ev[2 * k], c.rb_ev[2 * k + 1]) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, c.rb_ev[3 * k + 1], c.rb_ev[3 * k + 2]) == hipSuccess) {
             c.rb_ms_sum += ms;
             c.rb_count++;
+        }
+        if (c.rb_has_gap[size_t(k)] &&
+            hipEventElapsedTime(&ms, c.rb_ev[3 * k], c.rb_ev[3 * k + 1]) == hipSuccess) {
+            c.rb_gap_ms_sum += ms;
+            c.rb_gap_count++;
         }
     }
     c.rb_pending = 0;
@@ -208,6 +214,10 @@ void run_steps(Ctx& c, int64_t nsteps) {
                 mdd::launch_pre(c, st, true);
                 mdd::launch_pair_forces(c, st, false);
                 mdd::launch_post(c, st);
+            }
+            if (c.prof_every > 0 && c.rb_mark) {
+                MDD_HIP(hipEventRecord(c.rb_mark, c.stream));
+                c.rb_chunk_mark = true;
             }
             const int ab = check_status();
             if (ab != INT_MAX) {
@@ -460,6 +470,7 @@ int mdd_destroy(mdd_ctx* ctx) {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     for (hipEvent_t e : c.ev_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : c.rb_ev) (void)hipEventDestroy(e);
+    if (c.rb_mark) (void)hipEventDestroy(c.rb_mark);
     mdd::release_graphs(c);
     if (c.status_host) (void)hipHostFree(c.status_host);
     c.status_host = nullptr;
@@ -1058,8 +1069,10 @@ int mdd_set_profiling(mdd_ctx* ctx, int every) {
         if (c.prof_every && c.ev_pool.empty()) {
             c.ev_pool.resize(2 * kEventPool);
             for (auto& e : c.ev_pool) MDD_HIP(hipEventCreate(&e));
-            c.rb_ev.resize(128);
+            c.rb_ev.resize(3 * 64);
+            c.rb_has_gap.assign(64, 0);
             for (auto& e : c.rb_ev) MDD_HIP(hipEventCreate(&e));
+            MDD_HIP(hipEventCreate(&c.rb_mark));
         }
         c.prof_ms_sum = 0;
         c.prof_count = 0;
@@ -1067,14 +1080,19 @@ int mdd_set_profiling(mdd_ctx* ctx, int every) {
         c.force_launches = 0;
         c.prof_ctr = 0;
         c.rb_ms_sum = 0;
+        c.rb_gap_ms_sum = 0;
         c.rb_count = 0;
+        c.rb_gap_count = 0;
         c.rb_pending = 0;
+        c.rb_chunk_mark = false;
     });
 }
 
-int mdd_rebuild_stats(mdd_ctx* ctx, double* avg_ms, int64_t* count) {
+int mdd_rebuild_stats(mdd_ctx* ctx, double* avg_ms, double* avg_gap_ms, int64_t* count) {
     return guarded(ctx, [&](Ctx& c) {
         if (avg_ms) *avg_ms = c.rb_count ? c.rb_ms_sum / double(c.rb_count) : 0.0;
+        if (avg_gap_ms)
+            *avg_gap_ms = c.rb_gap_count ? c.rb_gap_ms_sum / double(c.rb_gap_count) : 0.0;
         if (count) *count = c.rb_count;
     });
 }

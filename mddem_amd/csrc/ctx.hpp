@@ -316,6 +316,7 @@ struct Ctx {
     bool tile_enabled = false, tile_blocked = false, list_tile = false;
     bool want_tile() const { return model == 0 && nsys == 0 && tile_enabled && !tile_blocked; }
     bool last_force_fused = false;  // the last scheduled force pass also integrated
+    int64_t fused_parts = 0;        // its kinetic-energy partials
     int tile_cap = 0;
     DBuf<uint16_t> crow;
     DBuf<uint32_t> xnbr, nx;
@@ -348,10 +349,15 @@ struct Ctx {
     // ---- rebuild launch sequences captured as graphs
     LaunchLog rec;
     int row_nblk = 0;  // per-block row-maxima slots of the last build
-    // device time of whole rebuilds (first launch to the gate), when profiling
+    // device time of whole rebuilds (first launch to the gate) and of the
+    // idle gap before them (end of the step chunk that asked for the rebuild
+    // to its first launch), when profiling: events 3k, 3k+1, 3k+2
     std::vector<hipEvent_t> rb_ev;
-    int64_t rb_pending = 0, rb_count = 0;
-    double rb_ms_sum = 0;
+    std::vector<char> rb_has_gap;
+    hipEvent_t rb_mark = nullptr;  // end of the last step chunk
+    bool rb_chunk_mark = false;
+    int64_t rb_pending = 0, rb_count = 0, rb_gap_count = 0;
+    double rb_ms_sum = 0, rb_gap_ms_sum = 0;
     std::vector<GraphEntry> graphs;  // most recent last
     bool graphs_on = true;
     int64_t graph_hits = 0, graph_captures = 0;

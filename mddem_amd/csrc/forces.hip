@@ -604,6 +604,7 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
         const bool fuse = !energy && step >= 0;
         launch_lj_tile(c, step, energy, fuse);
         c.last_force_fused = fuse;
+        c.fused_parts = c.grid.ncells;
     } else if (c.model == 0) {
         if (energy) {
             c.pe.resize(c.npad);

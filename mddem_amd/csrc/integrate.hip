@@ -381,8 +381,8 @@ IntArgs make_args(Ctx& c, int step, bool check) {
     a.partial = c.partial.p;
     a.nparts = post_blocks(c, true);
     if (c.list_tile) {
-        // tile lists: the fused force pass leaves per-cell partials, summed
-        // into ke_total by launch_ke_reduce
+        // tile lists: the fused force pass (or the post kernel) leaves
+        // partials, summed into ke_total by launch_ke_reduce
         c.ke_total.resize(1);
         a.partial = c.ke_total.p;
         a.nparts = 1;
@@ -433,8 +433,8 @@ void launch_pre(Ctx& c, int step, bool check_rebuild) {
 // scheduled post kernel of a step; its thermostat half uses slot step & 1
 void launch_post(Ctx& c, int step) {
     if (c.last_force_fused) {
-        // the tile force pass integrated already; sum its KE partials
-        if (c.nh_on) launch_ke_reduce(c, step, c.grid.ncells);
+        // the force pass integrated already; sum its KE partials
+        if (c.nh_on) launch_ke_reduce(c, step, c.fused_parts);
         return;
     }
     IntArgs a = make_args(c, step, false);

@@ -1093,8 +1093,13 @@ void release_graphs(Ctx& c) {
 // graph: its ~30 launches reach the device at once instead of one host
 // launch at a time.
 void rebuild_async(Ctx& c, int step, bool retry) {
-    const bool timed = c.prof_every > 0 && c.rb_pending < int64_t(c.rb_ev.size() / 2);
-    if (timed) MDD_HIP(hipEventRecord(c.rb_ev[2 * c.rb_pending], c.stream));
+    const bool timed = c.prof_every > 0 && c.rb_pending < int64_t(c.rb_ev.size() / 3);
+    if (timed) {
+        MDD_HIP(hipEventRecord(c.rb_ev[3 * c.rb_pending + 1], c.stream));
+        c.rb_has_gap[size_t(c.rb_pending)] = c.rb_chunk_mark ? 1 : 0;
+        if (c.rb_chunk_mark) std::swap(c.rb_ev[3 * c.rb_pending], c.rb_mark);
+    }
+    c.rb_chunk_mark = false;
     const bool record = !retry && c.graphs_on && c.setup_done;
     if (record) {
         c.rec.sig.clear();
@@ -1127,7 +1132,7 @@ void rebuild_async(Ctx& c, int step, bool retry) {
     }
     k_rows_gate<<<1, 256, 0, c.stream>>>(c.row_nblk, c.row_blk.p, c.maxnb, c.status.p, step);
     MDD_HIP(hipGetLastError());
-    if (timed) MDD_HIP(hipEventRecord(c.rb_ev[2 * c.rb_pending++ + 1], c.stream));
+    if (timed) MDD_HIP(hipEventRecord(c.rb_ev[3 * c.rb_pending++ + 2], c.stream));
 }
 
 namespace {

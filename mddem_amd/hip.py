@@ -98,7 +98,7 @@ SIGNATURES = {
     "mdd_synchronize": (C.c_int, [_vp]),
     "mdd_set_profiling": (C.c_int, [_vp, C.c_int]),
     "mdd_force_kernel_stats": (C.c_int, [_vp, _dp, _i64p]),
-    "mdd_rebuild_stats": (C.c_int, [_vp, _dp, _i64p]),
+    "mdd_rebuild_stats": (C.c_int, [_vp, _dp, _dp, _i64p]),
     "mdd_set_force_lanes": (C.c_int, [_vp, C.c_int]),
     "mdd_set_launch_graphs": (C.c_int, [_vp, C.c_int]),
     "mdd_launch_graph_stats": (C.c_int, [_vp, _i64p, _i64p]),
@@ -365,11 +365,12 @@ class System:
         return ms.value, n.value
 
     def rebuild_stats(self):
-        """(average device ms of a timed rebuild, rebuilds timed)."""
-        ms = C.c_double(0)
+        """(average device ms of a timed rebuild, average idle ms before one,
+        rebuilds timed)."""
+        ms, gap = C.c_double(0), C.c_double(0)
         n = C.c_int64(0)
-        self._c(self.L.mdd_rebuild_stats(self.h, C.byref(ms), C.byref(n)))
-        return ms.value, n.value
+        self._c(self.L.mdd_rebuild_stats(self.h, C.byref(ms), C.byref(gap), C.byref(n)))
+        return ms.value, gap.value, n.value
 
     def set_force_lanes(self, log2_lanes: int):
         """Row kernels at 2^log2_lanes lanes per particle (0..3), or the LJ
