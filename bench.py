@@ -32,6 +32,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def kernel_name(lg, mixed):
+    """Name of the pair-force kernel a layout runs, as the profiler shows it."""
+    r = "float" if mixed else "double"
+    return f"k_lj_tile<{r},false,true>" if lg < 0 else f"k_lj<{r},false,{lg}>"
+
+
 def layout_name(lg):
     return "cell tiles" if lg < 0 else f"rows, {1 << lg} lanes/particle"
 
@@ -87,7 +93,8 @@ def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True, layout=Non
     res = dict(seconds=t1 - t0, steps=steps, n=w.n, rebuilds=th1["rebuilds"] - th0["rebuilds"],
                npairs=th1["npairs"], force_ms=avg_ms, force_launches=launches,
                temperature=th1["temperature"], pe=th1["pe"], ke=th1["ke"],
-               layout=layout_name(layout), tile=layout < 0, rebuild_ms=rb_ms, rebuild_gap_ms=rb_gap_ms,
+               layout=layout_name(layout), tile=layout < 0, rebuild_ms=rb_ms,
+               kernel_name=kernel_name(layout, precision == hip.MIXED), rebuild_gap_ms=rb_gap_ms,
                rebuilds_timed=rb_n)
     s.close()
     return res
@@ -159,13 +166,15 @@ def cpu_baseline(w, seconds_budget=12.0):
                        f"{w.n}-atom workload on 1 host thread")
 
 
-def traffic_from_profiles(name):
+def traffic_from_profiles(name, kernel):
+    """Calibrated HBM bytes per launch of `kernel` from the committed counter
+    summary for this workload, or None when it covers another kernel."""
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
-    return d.get("hbm_bytes_per_launch")
+    return d.get("hbm_bytes_per_launch") if d.get("kernel") == kernel else None
 
 
 def main():
@@ -178,8 +187,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=5000)
     ap.add_argument("--graphs", choices=["on", "off"], default="on",
                     help="issue neighbor rebuilds as captured HIP graphs")
-    ap.add_argument("--layout", choices=["auto", "tile", "1", "2", "3"], default="auto",
-                    help="pair-force layout: tuned, cell tiles, or rows at 2^k lanes")
+    ap.add_argument("--layout", choices=["auto", "tile", "1", "2", "3"], default="3",
+                    help="pair-force layout: tuned during warmup, cell tiles, or rows at "
+                         "2^k lanes per particle (default 8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precision-report", type=int, default=2000,
@@ -225,7 +235,7 @@ def main():
     achieved = alg / (r["force_ms"] * 1e-3) / 1e9 if r["force_ms"] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(w.name),
+                "traffic": traffic_from_profiles(w.name, r["kernel_name"]),
                 "kernel": ("k_lj_tile (pair force + final half step)" if r["tile"]
                            else "k_lj (pair force)"), "bytes_per_launch": alg,
                 "avg_launch_us": round(r["force_ms"] * 1e3, 2)}

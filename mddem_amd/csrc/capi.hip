@@ -313,25 +313,25 @@ void pack_upload(Ctx& c, int64_t n, const uint32_t* tag, const uint32_t* type, c
             hq[i] = make_double4(1.0, 0.0, 0.0, 0.0);
         }
     }
-    MDD_HIP(hipMemcpy(c.pos.p, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
-    MDD_HIP(hipMemcpy(c.vel.p, hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
-    MDD_HIP(hipMemcpy(c.tag.p, ht.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-    MDD_HIP(hipMemcpy(c.type.p, hty.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-    MDD_HIP(hipMemcpy(c.mass.p, mass, sizeof(double) * n, hipMemcpyHostToDevice));
-    MDD_HIP(hipMemcpy(c.cutr.p, cutr, sizeof(double) * n, hipMemcpyHostToDevice));
-    MDD_HIP(hipMemset(c.image.p, 0, sizeof(int4) * np));
-    MDD_HIP(hipMemcpy(c.build_pos.p, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    h2d(c, c.pos.p, hp.data(), sizeof(double4) * n);
+    h2d(c, c.vel.p, hv.data(), sizeof(double4) * n);
+    h2d(c, c.tag.p, ht.data(), sizeof(uint32_t) * n);
+    h2d(c, c.type.p, hty.data(), sizeof(uint32_t) * n);
+    h2d(c, c.mass.p, mass, sizeof(double) * n);
+    h2d(c, c.cutr.p, cutr, sizeof(double) * n);
+    dzero(c, c.image.p, sizeof(int4) * np);
+    h2d(c, c.build_pos.p, hp.data(), sizeof(double4) * n);
     if (mixed)
-        MDD_HIP(hipMemset(c.f32.p, 0, sizeof(float) * 3 * np));
+        dzero(c, c.f32.p, sizeof(float) * 3 * np);
     else
-        MDD_HIP(hipMemset(c.f64.p, 0, sizeof(double) * 3 * np));
+        dzero(c, c.f64.p, sizeof(double) * 3 * np);
     if (dem) {
-        MDD_HIP(hipMemcpy(c.omega.p, hw.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
-        MDD_HIP(hipMemcpy(c.quat.p, hq.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        h2d(c, c.omega.p, hw.data(), sizeof(double4) * n);
+        h2d(c, c.quat.p, hq.data(), sizeof(double4) * n);
         if (mixed)
-            MDD_HIP(hipMemset(c.t32.p, 0, sizeof(float) * 3 * np));
+            dzero(c, c.t32.p, sizeof(float) * 3 * np);
         else
-            MDD_HIP(hipMemset(c.t64.p, 0, sizeof(double) * 3 * np));
+            dzero(c, c.t64.p, sizeof(double) * 3 * np);
     }
     c.list_valid = false;
     c.list_injected = false;
@@ -392,15 +392,15 @@ void commit_systems(Ctx& c) {
     pack_upload(c, n, tag.data(), type.data(), pos.data(), vel.data(), omega.data(),
                 radius.data(), cutr.data(), mass.data());
     c.psys.resize(size_t(c.npad));
-    MDD_HIP(hipMemcpy(c.psys.p, psys.data(), sizeof(uint32_t) * size_t(n), hipMemcpyHostToDevice));
+    h2d(c, c.psys.p, psys.data(), sizeof(uint32_t) * size_t(n));
     c.nsys = ns;
     c.setup_done = false;
 }
 
 template <class T>
-std::vector<T> dl(const mdd::DBuf<T>& b, size_t n) {
+std::vector<T> dl(Ctx& c, const mdd::DBuf<T>& b, size_t n) {
     std::vector<T> h(n);
-    if (n) MDD_HIP(hipMemcpy(h.data(), b.p, sizeof(T) * n, hipMemcpyDeviceToHost));
+    if (n) d2h(c, h.data(), b.p, sizeof(T) * n);
     return h;
 }
 
@@ -447,7 +447,7 @@ int mdd_create(mdd_ctx** out, const mdd_params* p) {
     c.lj_e12 = 4.0;
     c.lj_e6 = 4.0;
     int rc = guarded(ctx, [&](Ctx& cc) {
-        MDD_HIP(hipStreamCreate(&cc.stream));
+        MDD_HIP(hipStreamCreateWithFlags(&cc.stream, hipStreamNonBlocking));
         cc.status.resize(1);
         MDD_HIP(hipHostMalloc(reinterpret_cast<void**>(&cc.status_host), sizeof(mdd::Status),
                               hipHostMallocDefault));
@@ -680,11 +680,11 @@ int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, do
         MDD_HIP(hipStreamSynchronize(c.stream));
         const int64_t n = c.n;
         if (tag) {
-            auto t = dl(c.tag, n);
+            auto t = dl(c, c.tag, n);
             std::copy(t.begin(), t.end(), tag);
         }
         if (pos) {
-            auto h = dl(c.pos, n);
+            auto h = dl(c, c.pos, n);
             for (int64_t i = 0; i < n; i++) {
                 pos[3 * i] = h[i].x;
                 pos[3 * i + 1] = h[i].y;
@@ -692,7 +692,7 @@ int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, do
             }
         }
         if (vel) {
-            auto h = dl(c.vel, n);
+            auto h = dl(c, c.vel, n);
             for (int64_t i = 0; i < n; i++) {
                 vel[3 * i] = h[i].x;
                 vel[3 * i + 1] = h[i].y;
@@ -701,7 +701,7 @@ int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, do
         }
         if (omega) {
             if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "no angular state");
-            auto h = dl(c.omega, n);
+            auto h = dl(c, c.omega, n);
             for (int64_t i = 0; i < n; i++) {
                 omega[3 * i] = h[i].x;
                 omega[3 * i + 1] = h[i].y;
@@ -710,7 +710,7 @@ int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, do
         }
         if (quat) {
             if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "no orientation state");
-            auto h = dl(c.quat, n);
+            auto h = dl(c, c.quat, n);
             for (int64_t i = 0; i < n; i++) {
                 quat[4 * i] = h[i].x;
                 quat[4 * i + 1] = h[i].y;
@@ -719,7 +719,7 @@ int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, do
             }
         }
         if (image) {
-            auto h = dl(c.image, n);
+            auto h = dl(c, c.image, n);
             for (int64_t i = 0; i < n; i++) {
                 image[3 * i] = h[i].x;
                 image[3 * i + 1] = h[i].y;
@@ -740,11 +740,11 @@ int mdd_download_forces(mdd_ctx* ctx, double* force, double* torque) {
             if (tq && c.model != MDD_MODEL_DEM)
                 throw ApiError(MDD_ERR_INVALID, "no torque state");
             if (mixed) {
-                auto h = dl(tq ? c.t32 : c.f32, size_t(3 * np));
+                auto h = dl(c, tq ? c.t32 : c.f32, size_t(3 * np));
                 for (int64_t i = 0; i < n; i++)
                     for (int d = 0; d < 3; d++) out[3 * i + d] = double(h[d * np + i]);
             } else {
-                auto h = dl(tq ? c.t64 : c.f64, size_t(3 * np));
+                auto h = dl(c, tq ? c.t64 : c.f64, size_t(3 * np));
                 for (int64_t i = 0; i < n; i++)
                     for (int d = 0; d < 3; d++) out[3 * i + d] = h[d * np + i];
             }
@@ -768,14 +768,12 @@ int mdd_upload_forces(mdd_ctx* ctx, const double* force, const double* torque) {
                 std::vector<float> h(size_t(3 * np), 0.f);
                 for (int64_t i = 0; i < n; i++)
                     for (int d = 0; d < 3; d++) h[d * np + i] = float(in[3 * i + d]);
-                MDD_HIP(hipMemcpy((tq ? c.t32 : c.f32).p, h.data(), sizeof(float) * h.size(),
-                                  hipMemcpyHostToDevice));
+                h2d(c, (tq ? c.t32 : c.f32).p, h.data(), sizeof(float) * h.size());
             } else {
                 std::vector<double> h(size_t(3 * np), 0.0);
                 for (int64_t i = 0; i < n; i++)
                     for (int d = 0; d < 3; d++) h[d * np + i] = in[3 * i + d];
-                MDD_HIP(hipMemcpy((tq ? c.t64 : c.f64).p, h.data(), sizeof(double) * h.size(),
-                                  hipMemcpyHostToDevice));
+                h2d(c, (tq ? c.t64 : c.f64).p, h.data(), sizeof(double) * h.size());
             }
         };
         put(force, false);
@@ -833,13 +831,13 @@ int mdd_set_half_list(mdd_ctx* ctx, int64_t nlocal, const uint32_t* offsets,
         c.nbr.resize(hn.size());
         c.nnb.resize(np);
         c.nhalf.resize(np);
-        MDD_HIP(hipMemcpy(c.nbr.p, hn.data(), sizeof(uint32_t) * hn.size(), hipMemcpyHostToDevice));
-        MDD_HIP(hipMemcpy(c.nnb.p, cnt.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice));
-        MDD_HIP(hipMemcpy(c.nhalf.p, half.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice));
+        h2d(c, c.nbr.p, hn.data(), sizeof(uint32_t) * hn.size());
+        h2d(c, c.nnb.p, cnt.data(), sizeof(uint32_t) * np);
+        h2d(c, c.nhalf.p, half.data(), sizeof(uint32_t) * np);
         if (c.model == MDD_MODEL_DEM) {
             c.hist.resize(size_t(3) * mx * np);
             c.cmask.resize(np);
-            MDD_HIP(hipMemset(c.cmask.p, 0, sizeof(uint64_t) * np));
+            dzero(c, c.cmask.p, sizeof(uint64_t) * np);
         }
         c.list_valid = true;
         c.list_injected = true;
@@ -854,7 +852,7 @@ int64_t mdd_npairs(mdd_ctx* ctx) {
     int64_t total = 0;
     int rc = guarded(ctx, [&](Ctx& c) {
         MDD_HIP(hipStreamSynchronize(c.stream));
-        auto h = dl(c.nhalf, size_t(c.n));
+        auto h = dl(c, c.nhalf, size_t(c.n));
         for (uint32_t v : h) total += v;
     });
     return rc == MDD_OK ? total : rc;
@@ -867,9 +865,9 @@ int mdd_download_pairs(mdd_ctx* ctx, int64_t cap, uint32_t* owner_tag, uint32_t*
         if (c.list_tile) mdd::expand_tile_rows(c);
         MDD_HIP(hipStreamSynchronize(c.stream));
         const int64_t n = c.n, np = c.npad;
-        auto nn = dl(c.nnb, size_t(n));
-        auto nb = dl(c.nbr, size_t(c.maxnb) * np);
-        auto tg = dl(c.tag, size_t(n));
+        auto nn = dl(c, c.nnb, size_t(n));
+        auto nb = dl(c, c.nbr, size_t(c.maxnb) * np);
+        auto tg = dl(c, c.tag, size_t(n));
         int64_t k = 0;
         for (int64_t i = 0; i < n; i++) {
             const int cnt = std::min<int>(int(nn[i]), c.maxnb);
@@ -990,11 +988,11 @@ int mdd_download_history(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* 
         MDD_HIP(hipStreamSynchronize(c.stream));
         const int64_t n = c.n, np = c.npad;
         const int64_t cs = int64_t(c.maxnb) * np;
-        auto nn = dl(c.nnb, size_t(n));
-        auto nb = dl(c.nbr, size_t(cs));
-        auto tg = dl(c.tag, size_t(n));
-        auto mk = dl(c.cmask, size_t(n));
-        auto hs = dl(c.hist, size_t(3 * cs));
+        auto nn = dl(c, c.nnb, size_t(n));
+        auto nb = dl(c, c.nbr, size_t(cs));
+        auto tg = dl(c, c.tag, size_t(n));
+        auto mk = dl(c, c.cmask, size_t(n));
+        auto hs = dl(c, c.hist, size_t(3 * cs));
         int64_t k = 0;
         for (int64_t i = 0; i < n; i++) {
             const int cnt = std::min<int>(int(nn[i]), c.maxnb);
@@ -1037,12 +1035,12 @@ int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out) {
         out->granular_temperature = s[4] > 0 ? s[5] / (3.0 * s[4]) : 0.0;
         out->p_xi = c.status_host->p_xi;
         if (c.list_valid) {
-            auto h = dl(c.nhalf, size_t(c.n));
+            auto h = dl(c, c.nhalf, size_t(c.n));
             int64_t t = 0;
             for (uint32_t v : h) t += v;
             out->npairs = t;
             if (c.model == MDD_MODEL_DEM) {
-                auto mk = dl(c.cmask, size_t(c.n));
+                auto mk = dl(c, c.cmask, size_t(c.n));
                 int64_t nc = 0;
                 for (int64_t i = 0; i < c.n; i++) nc += __builtin_popcountll(mk[i]);
                 out->ncontacts = nc / 2;
@@ -1051,7 +1049,7 @@ int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out) {
         if (with_energy && c.model == MDD_MODEL_LJ && c.list_valid) {
             mdd::launch_pair_forces(c, -1, true);
             MDD_HIP(hipStreamSynchronize(c.stream));
-            auto h = dl(c.pe, size_t(c.n));
+            auto h = dl(c, c.pe, size_t(c.n));
             double pe = 0.0;
             for (double v : h) pe += v;
             out->pe = pe;

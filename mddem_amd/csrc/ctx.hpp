@@ -414,6 +414,26 @@ inline int blocks_for(int64_t n, int threads) {
     return int(b > 0 ? b : 1);
 }
 
+// Copies and fills on the context's own stream, finished before returning.
+// The stream is non-blocking, so nothing here may rely on the legacy
+// stream's implicit ordering (and other threads' legacy-stream work never
+// waits on this stream, e.g. while it records a graph).
+inline void h2d(Ctx& c, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    MDD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+    MDD_HIP(hipStreamSynchronize(c.stream));
+}
+inline void d2h(Ctx& c, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    MDD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    MDD_HIP(hipStreamSynchronize(c.stream));
+}
+inline void dzero(Ctx& c, void* dst, size_t bytes) {
+    if (!bytes) return;
+    MDD_HIP(hipMemsetAsync(dst, 0, bytes, c.stream));
+    MDD_HIP(hipStreamSynchronize(c.stream));
+}
+
 template <typename V>
 inline void sig_put(std::vector<uint8_t>& s, const V& v) {
     const uint8_t* b = reinterpret_cast<const uint8_t*>(&v);

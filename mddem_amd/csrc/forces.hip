@@ -580,7 +580,7 @@ void upload_materials(Ctx& c) {
         h[3 * nm * nm + k] = c.mat.geff[k];
     }
     c.mat_dev.resize(h.size());
-    MDD_HIP(hipMemcpy(c.mat_dev.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    h2d(c, c.mat_dev.p, h.data(), sizeof(double) * h.size());
 }
 
 // one pair-force pass (Force phase). step < 0: ungated standalone call.

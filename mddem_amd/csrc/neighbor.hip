@@ -1054,19 +1054,28 @@ void issue_recorded(Ctx& c) {
             return;
         }
     }
-    MDD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    // capture; if the runtime refuses it, the launches go out one by one and
+    // this context stops recording (the results are the same either way)
     hipGraph_t g = nullptr;
-    try {
-        for (auto& f : c.rec.calls) f(c.stream);
-    } catch (...) {
-        if (hipStreamEndCapture(c.stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
-        throw;
-    }
-    MDD_HIP(hipStreamEndCapture(c.stream, &g));
     hipGraphExec_t ex = nullptr;
-    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    MDD_HIP(e);
+    bool captured = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (captured) {
+        try {
+            for (auto& f : c.rec.calls) f(c.stream);
+        } catch (const HipError&) {
+            captured = false;
+        }
+        const hipError_t ee = hipStreamEndCapture(c.stream, &g);
+        captured = captured && ee == hipSuccess && g != nullptr;
+        if (captured) captured = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) (void)hipGraphDestroy(g);
+    }
+    (void)hipGetLastError();  // a refused capture leaves its error behind
+    if (!captured) {
+        c.graphs_on = false;
+        for (auto& f : c.rec.calls) f(c.stream);
+        return;
+    }
     MDD_HIP(hipGraphLaunch(ex, c.stream));
     c.graph_captures++;
     if (c.graphs.size() >= kGraphs) {
@@ -1230,8 +1239,8 @@ double images_per_particle(const BoxP& box, double gc) {
 void upload_runs(Ctx& c, const std::vector<int>& rf, const std::vector<int>& rl) {
     c.run_first.resize(rf.size());
     c.run_last.resize(rl.size());
-    MDD_HIP(hipMemcpy(c.run_first.p, rf.data(), sizeof(int) * rf.size(), hipMemcpyHostToDevice));
-    MDD_HIP(hipMemcpy(c.run_last.p, rl.data(), sizeof(int) * rl.size(), hipMemcpyHostToDevice));
+    h2d(c, c.run_first.p, rf.data(), sizeof(int) * rf.size());
+    h2d(c, c.run_last.p, rl.data(), sizeof(int) * rl.size());
 }
 
 // one geometry per system, cells and runs concatenated
@@ -1264,11 +1273,9 @@ void ensemble_setup(Ctx& c) {
     c.grid.ncells = cell_off[c.nsys];
     upload_runs(c, rf, rl);
     c.sysp.resize(c.sys_h.size());
-    MDD_HIP(hipMemcpy(c.sysp.p, c.sys_h.data(), sizeof(SysP) * c.sys_h.size(),
-                      hipMemcpyHostToDevice));
+    h2d(c, c.sysp.p, c.sys_h.data(), sizeof(SysP) * c.sys_h.size());
     c.sys_cell_off.resize(cell_off.size());
-    MDD_HIP(hipMemcpy(c.sys_cell_off.p, cell_off.data(), sizeof(int64_t) * cell_off.size(),
-                      hipMemcpyHostToDevice));
+    h2d(c, c.sys_cell_off.p, cell_off.data(), sizeof(int64_t) * cell_off.size());
     if (c.ghost_cap == 0) c.ghost_cap = int64_t(images * 1.25) + 1024;
     c.setup_done = true;
 }
@@ -1282,7 +1289,7 @@ void neighbor_setup(Ctx& c) {
         return;
     }
     std::vector<double> cr(c.n);
-    MDD_HIP(hipMemcpy(cr.data(), c.cutr.p, sizeof(double) * c.n, hipMemcpyDeviceToHost));
+    d2h(c, cr.data(), c.cutr.p, sizeof(double) * c.n);
     double mx = 0.0, mn = 1.7976931348623157e308;
     for (double v : cr) {
         if (v > mx) mx = v;

@@ -99,7 +99,9 @@ def state_digest(s) -> dict:
     h.update(np.ascontiguousarray(st["pos"][order]).tobytes())
     h.update(np.ascontiguousarray(st["vel"][order]).tobytes())
     th = s.thermo()
-    return dict(sha256=h.hexdigest(), ke=th["ke"], rebuilds=th["rebuilds"], step=th["step"])
+    replays, captures = s.launch_graph_stats()
+    return dict(sha256=h.hexdigest(), ke=th["ke"], rebuilds=th["rebuilds"], step=th["step"],
+                graph_replays=replays, graph_captures=captures)
 
 
 def run_job(job: Job, precision) -> dict:

@@ -56,3 +56,27 @@ def test_jobs_are_identical_alone_and_in_the_mix():
     for j, a in zip(jobs, alone):
         assert j.digest["sha256"] == a["sha256"], j.jid
         assert j.digest["step"] == j.steps
+
+
+@pytest.mark.gpu
+def test_graph_rebuilds_with_uploads_and_copies_in_other_threads():
+    # later jobs upload their state while earlier ones capture and replay
+    # rebuild graphs on their own streams; every job must finish, match its
+    # run alone bit for bit, and have replayed rebuild graphs
+    from mddem_amd import hip
+    jobs = []
+    for k in range(10):
+        if k % 2:
+            jobs.append(serve.Job(k, "lj", 4000 + 2000 * k, 500, 0.005 * k,
+                                  dict(rho=0.85, temperature=1.0, seed=40 + k)))
+        else:
+            jobs.append(serve.Job(k, "dem", 3000 + 1000 * k, 700, 0.005 * k, dict(seed=40 + k)))
+    for j in jobs:
+        j.workload = serve.build_workload(j)
+    alone = [serve.run_job(j, hip.MIXED) for j in jobs]
+    rep = serve.Server(hbm_budget=16 << 30, slots=10).run(jobs)
+    assert rep["failed"] == 0, [j.error for j in jobs if j.error]
+    assert rep["max_concurrent"] >= 3
+    for j, a in zip(jobs, alone):
+        assert j.digest["sha256"] == a["sha256"], j.jid
+    assert sum(j.digest["graph_replays"] for j in jobs) > 0
