@@ -211,6 +211,11 @@ int mdd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
  * Changes the neighbor-row layout, so the next step rebuilds the list. No
  * reference counterpart: launch-shape tuning. */
 int mdd_set_force_lanes(mdd_ctx* ctx, int log2_lanes);
+/* Neighbor-row build kernel: 0 (default) picks by grid occupancy, 1 one
+ * wavefront per bin cell, 2 one lane per particle or image (for sparse grids
+ * with fewer particles than cells; not used for batched ensembles or LJ cell
+ * tiles). Both produce identical rows. No reference counterpart: tuning. */
+int mdd_set_build_kernel(mdd_ctx* ctx, int mode);
 /* Issue each neighbor rebuild (wrap, spatial sort, images, bins, rows) as one
  * HIP graph launch (default on). The rebuild's launches are recorded with
  * their arguments; a graph captured from a launch sequence with identical

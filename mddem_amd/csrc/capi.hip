@@ -161,6 +161,28 @@ void run_steps(Ctx& c, int64_t nsteps) {
         mdd::launch_nh_seed(c, int(c.step & 1));
     }
     const int64_t end = c.step + nsteps;
+    // Without a thermostat nothing global happens between a step's final
+    // half step and the next step's first: the final half is deferred and
+    // issued together with the next first half (one pass over the state).
+    const bool defer = !c.nh_on;
+    if (c.post_pending) {
+        mdd::launch_post(c, int(c.step));
+        c.post_pending = false;
+    }
+    auto first_half = [&](int s, bool check) {
+        if (c.post_pending) {
+            mdd::launch_post_pre(c, s, check);
+            c.post_pending = false;
+        } else {
+            mdd::launch_pre(c, s, check);
+        }
+    };
+    auto second_half = [&](int s) {
+        if (defer && !c.last_force_fused)
+            c.post_pending = true;
+        else
+            mdd::launch_post(c, s);
+    };
     // resume at the force pass of step ab after the device stopped there:
     // a rebuild request from the drift, or a rebuild that did not fit
     auto resume = [&](int ab) {
@@ -174,7 +196,7 @@ void run_steps(Ctx& c, int64_t nsteps) {
         }
         mdd::rebuild_async(c, ab, retry);
         mdd::launch_pair_forces(c, ab, false);
-        mdd::launch_post(c, ab);
+        second_half(ab);
         if (!retry) {
             const int64_t ran = int64_t(ab) - c.step;
             if (ran > 0) c.chunk = int(std::max<int64_t>(2, ran + ran / 4 + 1));
@@ -196,10 +218,10 @@ void run_steps(Ctx& c, int64_t nsteps) {
                                 (c.sort_every > 0 && s % c.sort_every == 0);
             if (forced) {
                 // FullRebuild step: integrate, rebuild, force, integrate
-                mdd::launch_pre(c, int(s), false);
+                first_half(int(s), false);
                 mdd::rebuild_async(c, int(s), false);
                 mdd::launch_pair_forces(c, int(s), false);
-                mdd::launch_post(c, int(s));
+                second_half(int(s));
                 c.step = s;
                 continue;
             }
@@ -211,9 +233,9 @@ void run_steps(Ctx& c, int64_t nsteps) {
             if (K < 1) K = 1;
             for (int64_t k = 1; k <= K; k++) {
                 const int st = int(c.step + k);
-                mdd::launch_pre(c, st, true);
+                first_half(st, true);
                 mdd::launch_pair_forces(c, st, false);
-                mdd::launch_post(c, st);
+                second_half(st);
             }
             if (c.prof_every > 0 && c.rb_mark) {
                 MDD_HIP(hipEventRecord(c.rb_mark, c.stream));
@@ -234,6 +256,10 @@ void run_steps(Ctx& c, int64_t nsteps) {
         resume(ab);
     }
     c.nghost = c.status_host->ghost_total;
+    if (c.post_pending) {
+        mdd::launch_post(c, int(c.step));
+        c.post_pending = false;
+    }
     // post half of the last step's thermostat update -> Status::p_xi, ke
     if (c.nh_on) mdd::launch_nh_finish(c, int(c.step & 1));
     sync_status(c);
@@ -1112,6 +1138,13 @@ int mdd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
     if (free_bytes) *free_bytes = int64_t(f);
     if (total_bytes) *total_bytes = int64_t(t);
     return MDD_OK;
+}
+
+int mdd_set_build_kernel(mdd_ctx* ctx, int mode) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (mode < 0 || mode > 2) throw ApiError(MDD_ERR_INVALID, "build kernel mode must be 0..2");
+        c.build_mode = mode;
+    });
 }
 
 int mdd_set_launch_graphs(mdd_ctx* ctx, int enabled) {

@@ -317,6 +317,13 @@ struct Ctx {
     bool want_tile() const { return model == 0 && nsys == 0 && tile_enabled && !tile_blocked; }
     bool last_force_fused = false;  // the last scheduled force pass also integrated
     int64_t fused_parts = 0;        // its kinetic-energy partials
+    // the final half step of step `step` is deferred into the next step's
+    // first pass (runs without a thermostat)
+    bool post_pending = false;
+    // neighbor build kernel: 0 chosen by occupancy, 1 one wavefront per cell,
+    // 2 one lane per member slot (sparse grids)
+    int build_mode = 0;
+    DBuf<uint32_t> bin_cell;  // cell of every bin slot (sparse build)
     int tile_cap = 0;
     DBuf<uint16_t> crow;
     DBuf<uint32_t> xnbr, nx;
@@ -397,6 +404,7 @@ void expand_tile_rows(Ctx& c);
 void upload_materials(Ctx& c);
 // integrate.hip
 void launch_pre(Ctx& c, int step, bool check_rebuild);
+void launch_post_pre(Ctx& c, int step, bool check_rebuild);
 void launch_post(Ctx& c, int step);
 void launch_initial_integration(Ctx& c, int step, bool check_rebuild);
 void launch_final_integration(Ctx& c);

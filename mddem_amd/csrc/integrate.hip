@@ -137,17 +137,10 @@ __device__ inline void rebuild_check(const IntArgs& a, int64_t i, const double4&
         atomicMin(&a.st->abort_step, a.step);
 }
 
-// initial_rotation body (rotational.rs:59-84)
-__device__ inline void rotate_initial(const IntArgs& a, int64_t i, double tx, double ty,
-                                      double tz) {
-    double4 w = a.omega[i];
-    const double ii = w.w;
-    if (ii == 0.0) return;
+// orientation update of initial_rotation with the kicked omega w
+// (rotational.rs:70-83)
+__device__ inline void rotate_quat(const IntArgs& a, int64_t i, const double4& w) {
     const double dt = a.dt;
-    w.x += 0.5 * dt * tx * ii;
-    w.y += 0.5 * dt * ty * ii;
-    w.z += 0.5 * dt * tz * ii;
-    a.omega[i] = w;
     const double omega_mag = sqrt(w.x * w.x + w.y * w.y + w.z * w.z);
     const double angle = omega_mag * dt;
     if (angle > 1e-14) {
@@ -164,6 +157,20 @@ __device__ inline void rotate_initial(const IntArgs& a, int64_t i, double tx, do
         q.w = a0 * b.w + a1 * b.z - a2 * b.y + a3 * b.x;
         a.quat[i] = q;
     }
+}
+
+// initial_rotation body (rotational.rs:59-84)
+__device__ inline void rotate_initial(const IntArgs& a, int64_t i, double tx, double ty,
+                                      double tz) {
+    double4 w = a.omega[i];
+    const double ii = w.w;
+    if (ii == 0.0) return;
+    const double dt = a.dt;
+    w.x += 0.5 * dt * tx * ii;
+    w.y += 0.5 * dt * ty * ii;
+    w.z += 0.5 * dt * tz * ii;
+    a.omega[i] = w;
+    rotate_quat(a, i, w);
 }
 
 // final_rotation body (rotational.rs:87-100)
@@ -280,6 +287,50 @@ __global__ void __launch_bounds__(kThreads) k_post(IntArgs a) {
     if (!NH) return;
     const double bs = block_sum(ke, lds4);
     if (threadIdx.x == 0) a.partial[blockIdx.x] = bs;
+}
+
+// The final half step of step s - 1 and the first half step of step s in
+// one pass, for runs without a thermostat (nothing global happens between
+// them): Verlet kick (final_integration), kick + drift (initial_integration),
+// final_rotation then initial_rotation on the same omega, then the rebuild
+// decision. Each particle's arithmetic is the same as the two kernels'.
+template <typename F>
+__global__ void __launch_bounds__(kThreads) k_post_pre(IntArgs a) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (a.step >= 0 && a.st->abort_step < a.step) return;
+    if (i == 0) a.st->overlap_count = 0;
+    if (i >= a.n) return;
+    double4 p = a.pos[i];
+    if (a.vv) {
+        double4 v = a.vel[i];
+        double fx, fy, fz;
+        load_f<F>(a, i, fx, fy, fz);
+        const double h = 0.5 * a.dt * v.w;
+        v.x += h * fx;
+        v.y += h * fy;
+        v.z += h * fz;
+        kick_drift(a, v, p, fx, fy, fz);
+        a.vel[i] = v;
+        a.pos[i] = p;
+    }
+    if (a.rot) {
+        double tx, ty, tz;
+        load_t<F>(a, i, tx, ty, tz);
+        double4 w = a.omega[i];
+        const double ii = w.w;
+        if (ii != 0.0) {
+            const double dt = a.dt;
+            w.x += 0.5 * dt * tx * ii;  // final_rotation of step s - 1
+            w.y += 0.5 * dt * ty * ii;
+            w.z += 0.5 * dt * tz * ii;
+            w.x += 0.5 * dt * tx * ii;  // initial_rotation of step s
+            w.y += 0.5 * dt * ty * ii;
+            w.z += 0.5 * dt * tz * ii;
+            a.omega[i] = w;
+            rotate_quat(a, i, w);
+        }
+    }
+    if (a.check) rebuild_check(a, i, p);
 }
 
 // KE partials of the current velocities, laid out like the post kernel's
@@ -428,6 +479,20 @@ void launch_pre(Ctx& c, int step, bool check_rebuild) {
         pre_impl<true>(c, a);
     else
         pre_impl<false>(c, a);
+}
+
+// the deferred final half step of step - 1 with the first half of `step`
+// (runs without a thermostat)
+void launch_post_pre(Ctx& c, int step, bool check_rebuild) {
+    IntArgs a = make_args(c, step, check_rebuild);
+    a.rot = c.rot_on && c.model == 1;
+    a.vv = c.vv_on;
+    const int nb = blocks_for(c.n, kThreads);
+    if (c.precision == 1)
+        k_post_pre<float><<<nb, kThreads, 0, c.stream>>>(a);
+    else
+        k_post_pre<double><<<nb, kThreads, 0, c.stream>>>(a);
+    MDD_HIP(hipGetLastError());
 }
 
 // scheduled post kernel of a step; its thermostat half uses slot step & 1

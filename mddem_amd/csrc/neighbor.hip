@@ -243,7 +243,7 @@ __global__ void k_bin_fill(int64_t n, int64_t ghost_cap, const uint32_t* __restr
                            const double4* __restrict__ gpos, const uint32_t* __restrict__ groot,
                            const uint8_t* __restrict__ gcode, double4* __restrict__ bin_pos,
                            uint32_t* __restrict__ bin_entry, uint32_t* __restrict__ bin_gidx,
-                           uint32_t* __restrict__ slot_local) {
+                           uint32_t* __restrict__ slot_local, uint32_t* __restrict__ bin_cell) {
     const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (t < n) {
         const uint32_t c = key[t];
@@ -252,6 +252,7 @@ __global__ void k_bin_fill(int64_t n, int64_t ghost_cap, const uint32_t* __restr
         bin_pos[s] = make_double4(p.x, p.y, p.z, cutr[t]);
         bin_entry[s] = enc_entry(uint32_t(t), kCodeNone, false);
         slot_local[t] = s;
+        if (bin_cell) bin_cell[s] = c;
         return;
     }
     const int64_t gs = t - n;
@@ -264,6 +265,7 @@ __global__ void k_bin_fill(int64_t n, int64_t ghost_cap, const uint32_t* __restr
     bin_pos[s] = make_double4(q.x, q.y, q.z, cutr[r]);
     bin_entry[s] = enc_entry(r, gcode[g], false);
     bin_gidx[s] = g;
+    if (bin_cell) bin_cell[s] = c;
 }
 
 constexpr int kCellWaves = 4;  // cells (one per wavefront) per 256-thread block
@@ -566,6 +568,84 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
                 a.gcnt[mg] = cntv;
             }
         }
+    }
+}
+
+// The same rows for sparse grids (most cells empty or nearly so, where a
+// wavefront per cell idles): one lane per member slot, walking its cell's
+// stencil runs in bin order. Candidate order, the ownership flags and the
+// counts are those of k_build_cells, so the rows are identical.
+template <bool UNIFORM>
+__global__ void __launch_bounds__(256) k_build_sparse(CellArgs a,
+                                                      const uint32_t* __restrict__ bin_cell,
+                                                      const uint32_t* __restrict__ ng_live,
+                                                      int64_t n) {
+    const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (s >= n + int64_t(*ng_live)) return;
+    const int64_t C = int64_t(bin_cell[s]);
+    const uint32_t i0 = a.bin_start[C], i1 = a.bin_start[C + 1];
+    const CellCut cc{a.cutoff_sq, a.sf2, UNIFORM ? 1 : 0};
+    // flat index of the cell's first slot: the runs before run0, then the
+    // cells of run0 before C
+    uint32_t f_i0 = 0;
+    for (int r = 0; r < a.nruns; r++) {
+        int64_t lo = C + a.run_first[r], hi = C + a.run_last[r];
+        if (lo < 0) lo = 0;
+        if (hi > a.ncells - 1) hi = a.ncells - 1;
+        if (lo > hi) continue;
+        if (r == a.run0) {
+            f_i0 += i0 - a.bin_start[lo];
+            break;
+        }
+        f_i0 += a.bin_start[hi + 1] - a.bin_start[lo];
+    }
+    const uint32_t e = a.bin_entry[s];
+    const double4 xi = a.bin_pos[s];
+    const bool image = entry_code(e) != kCodeNone;
+    const uint32_t fs = f_i0 + (uint32_t(s) - i0), f_end = f_i0 + (i1 - i0);
+    const uint32_t q = entry_idx(e);
+    const uint32_t G = 1u << a.lg;
+    uint32_t* out = nullptr;
+    uint32_t tagbits = 0u;
+    if (image) {
+        out = a.gnbr + size_t(a.bin_gidx[s]) * uint32_t(a.maxnb);
+        tagbits = (uint32_t(entry_code(e)) << kIdxBits) | kRevBit;
+    }
+    uint32_t cnt = 0, nrev = 0, flat = 0;
+    for (int r = 0; r < a.nruns; r++) {
+        int64_t lo = C + a.run_first[r], hi = C + a.run_last[r];
+        if (lo < 0) lo = 0;
+        if (hi > a.ncells - 1) hi = a.ncells - 1;
+        if (lo > hi) continue;
+        const uint32_t b0 = a.bin_start[lo], b1 = a.bin_start[hi + 1];
+        for (uint32_t t = b0; t < b1; t++) {
+            const uint32_t f = flat + (t - b0);
+            const uint32_t et = a.bin_entry[t];
+            const bool particle = entry_code(et) == kCodeNone;
+            if (image ? (!particle || f >= f_end) : (f == fs || !(particle || f >= f_i0)))
+                continue;
+            const double4 pt = a.bin_pos[t];
+            const double dx = pt.x - xi.x, dy = pt.y - xi.y, dz = pt.z - xi.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (!within<UNIFORM>(cc, r2, xi.w, pt.w)) continue;
+            const uint32_t k = cnt++;
+            if (image) {
+                if (k < uint32_t(a.maxnb)) out[k] = entry_idx(et) | tagbits;
+            } else {
+                const bool before = f < fs;
+                if (before) nrev++;
+                if (k < uint32_t(a.maxnb))
+                    a.nbr[((size_t(k >> a.lg) * size_t(a.npad) + q) << a.lg) + (k & (G - 1u))] =
+                        before ? (et | kRevBit) : et;
+            }
+        }
+        flat += b1 - b0;
+    }
+    if (image) {
+        a.gcnt[a.bin_gidx[s]] = cnt;
+    } else {
+        a.nnb[q] = cnt;
+        a.nhalf[q] = cnt - nrev;
     }
 }
 
@@ -880,6 +960,10 @@ void rebuild_attempt(Ctx& c) {
     c.bin_entry.resize(nall_max);
     c.bin_gidx.resize(nall_max);
     c.slot_of.resize(n);
+    // sparse grids (fewer particles than cells): the per-member build kernel
+    const bool sparse = c.nsys == 0 && !c.want_tile() &&
+                        (c.build_mode == 2 || (c.build_mode == 0 && n < ncells));
+    if (sparse) c.bin_cell.resize(nall_max);
     const int G = 1 << c.lg;
     if (c.maxnb < G) c.maxnb = G;
     c.maxnb = (c.maxnb + G - 1) / G * G;
@@ -925,8 +1009,9 @@ void rebuild_attempt(Ctx& c) {
     launch(c, k_bin_fill, blocks_for(nall_max, 256), 256,
         n, c.ghost_cap, ng_live, c.key.p, c.start.p, c.gstart.p, c.bin_start.p, c.ghost_order.p,
         c.ghost_key.p, c.pos.p, c.cutr.p, c.ghost_pos.p, c.ghost_root.p, c.ghost_code.p,
-        c.bin_pos.p, c.bin_entry.p, c.bin_gidx.p, c.slot_of.p);
-    // rows: one wavefront per cell, then image-owned pairs appended
+        c.bin_pos.p, c.bin_entry.p, c.bin_gidx.p, c.slot_of.p, sparse ? c.bin_cell.p : nullptr);
+    // rows: one wavefront per cell (or one lane per member on sparse grids),
+    // then image-owned pairs appended
     CellArgs ca;
     std::memset(&ca, 0, sizeof(ca));
     ca.ncells = ncells;
@@ -956,7 +1041,15 @@ void rebuild_attempt(Ctx& c) {
     ca.lstart = c.start.p;
     ca.tile_cap = c.tile_cap;
     const int cblocks = blocks_for(ncells, kCellWaves);
-    if (c.nsys > 0)
+    if (sparse && c.uniform)
+        launch(c, k_build_sparse<true>, blocks_for(nall_max, 256), 256, ca,
+               static_cast<const uint32_t*>(c.bin_cell.p), static_cast<const uint32_t*>(ng_live),
+               n);
+    else if (sparse)
+        launch(c, k_build_sparse<false>, blocks_for(nall_max, 256), 256, ca,
+               static_cast<const uint32_t*>(c.bin_cell.p), static_cast<const uint32_t*>(ng_live),
+               n);
+    else if (c.nsys > 0)
         launch(c, k_build_cells<false, true, false>, cblocks, 256, ca);
     else if (tile && c.uniform)
         launch(c, k_build_cells<true, false, true>, cblocks, 256, ca);

@@ -101,6 +101,7 @@ SIGNATURES = {
     "mdd_rebuild_stats": (C.c_int, [_vp, _dp, _dp, _i64p]),
     "mdd_set_force_lanes": (C.c_int, [_vp, C.c_int]),
     "mdd_set_launch_graphs": (C.c_int, [_vp, C.c_int]),
+    "mdd_set_build_kernel": (C.c_int, [_vp, C.c_int]),
     "mdd_launch_graph_stats": (C.c_int, [_vp, _i64p, _i64p]),
     "mdd_device_memory": (C.c_int, [C.c_int, _i64p, _i64p]),
     "mdd_add_system": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -376,6 +377,10 @@ class System:
         """Row kernels at 2^log2_lanes lanes per particle (0..3), or the LJ
         cell-tile kernel (-1)."""
         self._c(self.L.mdd_set_force_lanes(self.h, int(log2_lanes)))
+
+    def set_build_kernel(self, mode: int):
+        """0: by grid occupancy, 1: one wavefront per cell, 2: one lane per member."""
+        self._c(self.L.mdd_set_build_kernel(self.h, int(mode)))
 
     def set_launch_graphs(self, enabled: bool):
         self._c(self.L.mdd_set_launch_graphs(self.h, int(bool(enabled))))
