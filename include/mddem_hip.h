@@ -133,7 +133,18 @@ int mdd_system_range(mdd_ctx* ctx, int sys, int64_t* first, int64_t* count);
 int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, double* omega,
                        double* quat, int32_t* image);
 int mdd_download_forces(mdd_ctx* ctx, double* force, double* torque);
+/* per-particle constants in the current particle order (Atom::atom_type,
+ * Atom::mass, DemAtom::radius, Atom::cutoff_radius; atom.rs:289-305,
+ * dem_atom/src/lib.rs:531-557), for restart snapshots; any pointer may be
+ * NULL (radius is 0 for LJ) */
+int mdd_download_properties(mdd_ctx* ctx, uint32_t* type, double* mass, double* radius,
+                            double* cutoff_radius);
 int mdd_upload_forces(mdd_ctx* ctx, const double* force, const double* torque);
+/* orientation quaternions n*4 (w,x,y,z) in the current particle order (the
+ * upload order until the first step), as read_restart restores
+ * DemAtom::quaternion (mddem_print/src/lib.rs:1238-1241); uploads start at
+ * [1,0,0,0] as insert_single_particle does (dem_atom/src/insert.rs:350) */
+int mdd_upload_quaternions(mdd_ctx* ctx, const double* quat);
 
 /* ---- systems (one call = one reference system) ------------------------------ */
 /* neighbor_setup + compute_bin_grid (crates/mddem_core/src/neighbor.rs:480-541, 290-371) */

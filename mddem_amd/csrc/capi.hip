@@ -699,6 +699,31 @@ int mdd_system_range(mdd_ctx* ctx, int sys, int64_t* first, int64_t* count) {
 
 int64_t mdd_nlocal(const mdd_ctx* ctx) { return ctx ? ctx->c.n : -1; }
 
+int mdd_download_properties(mdd_ctx* ctx, uint32_t* type, double* mass, double* radius,
+                            double* cutoff_radius) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        const int64_t n = c.n;
+        if (type) {
+            auto h = dl(c, c.type, n);
+            std::copy(h.begin(), h.end(), type);
+        }
+        if (mass) {
+            auto h = dl(c, c.mass, n);
+            std::copy(h.begin(), h.end(), mass);
+        }
+        if (cutoff_radius) {
+            auto h = dl(c, c.cutr, n);
+            std::copy(h.begin(), h.end(), cutoff_radius);
+        }
+        if (radius) {
+            auto h = dl(c, c.pos, n);
+            for (int64_t i = 0; i < n; i++) radius[i] = h[i].w;
+        }
+    });
+}
+
 int mdd_download_state(mdd_ctx* ctx, uint32_t* tag, double* pos, double* vel, double* omega,
                        double* quat, int32_t* image) {
     return guarded(ctx, [&](Ctx& c) {
@@ -804,6 +829,20 @@ int mdd_upload_forces(mdd_ctx* ctx, const double* force, const double* torque) {
         };
         put(force, false);
         put(torque, true);
+    });
+}
+
+int mdd_upload_quaternions(mdd_ctx* ctx, const double* quat) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_state(c);
+        if (!quat) throw ApiError(MDD_ERR_INVALID, "null quaternions");
+        if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "no orientation state");
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        const int64_t n = c.n;
+        std::vector<double4> h(static_cast<size_t>(n));
+        for (int64_t i = 0; i < n; i++)
+            h[i] = make_double4(quat[4 * i], quat[4 * i + 1], quat[4 * i + 2], quat[4 * i + 3]);
+        h2d(c, c.quat.p, h.data(), sizeof(double4) * size_t(n));
     });
 }
 

@@ -101,6 +101,8 @@ SIGNATURES = {
     "mdd_rebuild_stats": (C.c_int, [_vp, _dp, _dp, _i64p]),
     "mdd_set_force_lanes": (C.c_int, [_vp, C.c_int]),
     "mdd_set_launch_graphs": (C.c_int, [_vp, C.c_int]),
+    "mdd_download_properties": (C.c_int, [_vp, _u32p, _dp, _dp, _dp]),
+    "mdd_upload_quaternions": (C.c_int, [_vp, _dp]),
     "mdd_set_build_kernel": (C.c_int, [_vp, C.c_int]),
     "mdd_launch_graph_stats": (C.c_int, [_vp, _i64p, _i64p]),
     "mdd_device_memory": (C.c_int, [C.c_int, _i64p, _i64p]),
@@ -260,6 +262,19 @@ class System:
                                           _p(out["vel"]), _p(out.get("omega")),
                                           _p(out.get("quat")), _p(out["image"], _i32p)))
         return out
+
+    def upload_quaternions(self, quat):
+        q = np.ascontiguousarray(quat, dtype=np.float64).reshape(self.n, 4)
+        self._c(self.L.mdd_upload_quaternions(self.h, _p(q)))
+
+    def properties(self):
+        """(type, mass, radius, cutoff_radius) in the current particle order."""
+        n = self.n
+        typ = np.zeros(n, np.uint32)
+        mass, rad, cutr = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._c(self.L.mdd_download_properties(self.h, _p(typ, _u32p), _p(mass), _p(rad),
+                                               _p(cutr)))
+        return typ, mass, rad, cutr
 
     def forces(self):
         n = self.n

@@ -1,0 +1,170 @@
+"""The reference's particle file formats: CSV ingest (insert.rs:652-776),
+RestartData in JSON and bincode (mddem_print/src/lib.rs:298-375, 1016-1253)
+and GranularTemp.txt lines (granular_temp.rs:128-133)."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from mddem_amd import particles_io as io
+from mddem_amd import workloads as W
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _write(tmp_path, text, name="p.csv"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_csv_header_comments_and_defaults(tmp_path):
+    path = _write(tmp_path, "x,y,z\n# a comment\n\n1.0, 2.0, 3.0\n4,5,6e-3\n")
+    p = io.read_csv_particles(path, density=2500.0, radius=1e-3, first_tag=7)
+    assert np.array_equal(p["tag"], [7, 8])
+    assert np.array_equal(p["pos"], [[1.0, 2.0, 3.0], [4.0, 5.0, 6e-3]])
+    assert np.array_equal(p["vel"], np.zeros((2, 3)))
+    assert np.array_equal(p["radius"], [1e-3, 1e-3])
+    assert np.array_equal(p["cutoff_radius"], p["radius"])
+    assert np.array_equal(p["quat"], [[1, 0, 0, 0], [1, 0, 0, 0]])
+
+
+def test_csv_header_only_on_the_first_line(tmp_path):
+    # a later line starting with a letter is data, and it fails to parse
+    path = _write(tmp_path, "1,2,3\nx,y,z\n")
+    with pytest.raises(io.FormatError, match=r"Failed to parse x \(column 0\) at line 2"):
+        io.read_csv_particles(path, density=1.0, radius=1.0)
+
+
+def test_csv_columns_radius_velocity_and_types(tmp_path):
+    text = "id,t,r,x,y,z,vx,vy,vz\n0,2,0.5,1,2,3,0.1,0.2,0.3\n1,5,0.25,4,5,6,-1,-2,-3\n"
+    path = _write(tmp_path, text)
+    cols = dict(x=3, y=4, z=5, radius=2, vx=6, vy=7, vz=8, atom_type=1)
+    p = io.read_csv_particles(path, density=1000.0, material=4, columns=cols,
+                              type_map={2: 1})
+    assert np.array_equal(p["type"], [1, 4])  # type 5 is not mapped: the default material
+    assert np.array_equal(p["radius"], [0.5, 0.25])
+    assert np.array_equal(p["vel"], [[0.1, 0.2, 0.3], [-1, -2, -3]])
+
+
+def test_csv_missing_radius_and_bad_field(tmp_path):
+    path = _write(tmp_path, "1,2,3\n")
+    with pytest.raises(io.FormatError, match="No radius column or default radius at line 1"):
+        io.read_csv_particles(path, density=1.0)
+    path = _write(tmp_path, "1,2,abc\n", "q.csv")
+    with pytest.raises(io.FormatError, match=r"Failed to parse z \(column 2\) at line 1"):
+        io.read_csv_particles(path, density=1.0, radius=1.0)
+    path = _write(tmp_path, "1,2\n", "r.csv")
+    with pytest.raises(io.FormatError, match=r"Failed to parse z \(column 2\)"):
+        io.read_csv_particles(path, density=1.0, radius=1.0)
+
+
+def test_csv_mass_and_inertia_follow_insertion():
+    # insert_single_particle: m = rho*4/3*pi*r^3 (as workloads.dem_mass), 1/I = 1/(0.4 m r^2)
+    path = os.path.join(GOLDEN, "cfg0_state.csv")
+    cols = dict(x=1, y=2, z=3, radius=4, vx=5, vy=6, vz=7)
+    p = io.read_csv_particles(path, density=2500.0, columns=cols)
+    raw = np.loadtxt(path, delimiter=",", skiprows=1)
+    assert len(p["tag"]) == 500
+    assert np.array_equal(p["pos"], raw[:, 1:4])
+    assert np.array_equal(p["vel"], raw[:, 5:8])
+    assert np.array_equal(p["mass"], W.dem_mass(2500.0, raw[:, 4]))
+    r = raw[:, 4]
+    m = W.dem_mass(2500.0, r)
+    assert np.array_equal(p["inv_inertia"], 1.0 / (0.4 * m * r * r))
+    # a 1 mm glass sphere weighs 1.0471975511965976e-5 kg
+    assert p["mass"][0] == 2500.0 * 4.0 / 3.0 * np.pi * 1e-9
+
+
+def test_csv_write_read_round_trip(tmp_path):
+    st = W.Stream(5)
+    pos = st.uniform(30).reshape(10, 3)
+    vel = st.uniform(30).reshape(10, 3) - 0.5
+    rad = 1e-3 * (0.8 + 0.4 * st.uniform(10))
+    path = str(tmp_path / "w.csv")
+    io.write_csv_particles(path, pos, rad, vel)
+    p = io.read_csv_particles(path, density=2500.0, columns=io.CSV_WRITTEN_COLUMNS)
+    assert np.array_equal(p["pos"], pos) and np.array_equal(p["vel"], vel)
+    assert np.array_equal(p["radius"], rad)
+
+
+def _restart(n=3, dem=True):
+    st = W.Stream(11)
+    d = io._empty_restart()
+    d.update(natoms=n, total_cycle=1234, dt=1.5e-7, tag=np.arange(n, dtype=np.uint32) * 3,
+             atom_type=np.zeros(n, np.uint32), mass=st.uniform(n), cutoff_radius=st.uniform(n))
+    for k in ("pos", "vel", "force"):
+        for a in "xyz":
+            d[f"{k}_{a}"] = st.uniform(n)
+    if dem:
+        q = st.uniform(4 * n).reshape(n, 4)
+        d["atom_data_buffers"] = [io.pack_dem(st.uniform(n), np.full(n, 2500.0), st.uniform(n),
+                                              q, st.uniform(3 * n).reshape(n, 3))]
+    return d
+
+
+def _same(a, b):
+    for name, kind in io._RESTART_FIELDS:
+        x, y = a[name], b[name]
+        if kind == "vvf64":
+            assert len(x) == len(y)
+            for u, v in zip(x, y):
+                assert np.array_equal(u, v)
+        else:
+            assert np.array_equal(np.asarray(x), np.asarray(y)), name
+
+
+def test_bincode_layout():
+    # bincode 1, default options: fields in order, u64 LE lengths, LE values
+    d = _restart(n=2)
+    raw = io.encode_bincode(d)
+    assert struct.unpack_from("<Q", raw, 0)[0] == 2          # natoms
+    assert struct.unpack_from("<Q", raw, 8)[0] == 1234       # total_cycle (usize)
+    assert struct.unpack_from("<d", raw, 16)[0] == 1.5e-7    # dt
+    assert struct.unpack_from("<Q", raw, 24)[0] == 2         # tag: Vec<u32>
+    assert struct.unpack_from("<2I", raw, 32) == (0, 3)
+    # tag, atom_type: 8 + 4 n each; 11 Vec<f64>: 8 + 8 n each; the buffers:
+    # 8 + (8 + 8 * 17 n); 9 empty legacy Vec<f64> and the empty quaternion Vec
+    n = 2
+    size = 24 + 2 * (8 + 4 * n) + 11 * (8 + 8 * n) + 8 + (8 + 8 * 17 * n) + 10 * 8
+    assert len(raw) == size
+    off = 24 + 2 * (8 + 4 * n) + 11 * (8 + 8 * n)
+    assert struct.unpack_from("<QQ", raw, off) == (1, 17 * n)
+
+
+def test_restart_round_trips(tmp_path):
+    for dem in (True, False):
+        d = _restart(dem=dem)
+        for fmt in ("bincode", "json"):
+            path = str(tmp_path / io.restart_file_name(1234, 0, fmt))
+            io.write_restart(path, d, fmt)
+            _same(io.read_restart(path), d)
+
+
+def test_restart_json_defaults_and_errors(tmp_path):
+    import json
+    d = _restart()
+    text = json.loads(io.encode_json(d))
+    for k in io._LEGACY:
+        del text[k]  # #[serde(default)] fields may be absent
+    _same(io.decode_json(json.dumps(text)), d)
+    del text["mass"]
+    with pytest.raises(io.FormatError, match="lacks field 'mass'"):
+        io.decode_json(json.dumps(text))
+    raw = io.encode_bincode(d)
+    with pytest.raises(io.FormatError, match="trailing"):
+        io.decode_bincode(raw + b"\0")
+    with pytest.raises(io.FormatError, match="ends early"):
+        io.decode_bincode(raw[:-1])
+    with pytest.raises(io.FormatError, match="multiple of 17"):
+        io.unpack_dem(np.zeros(16))
+
+
+def test_rust_exponent_format_and_granular_temp_line():
+    assert io.rust_exp(0.0, 6) == "0.000000e0"
+    assert io.rust_exp(1234.5, 6) == "1.234500e3"
+    assert io.rust_exp(1.5e-5, 6) == "1.500000e-5"
+    assert io.rust_exp(-2.5e-7, 10) == "-2.5000000000e-7"
+    line = io.granular_temp_line(100, 1.466980e-7, 0.04, 0.5, (3.0, 4.0, 0.0))
+    assert line == "100 1.466980e-5 4.0000000000e-2 5.0000000000e-1 5.0000000000e0"
