@@ -58,6 +58,10 @@ typedef struct {
     double momentum[3];    /* sum m v */
     double granular_temperature; /* sum m |v - vbar|^2 / (3 M)
                                     (crates/dem_granular/src/granular_temp.rs:44-134) */
+    double virial[6];      /* LJ pair virial sum d (x) F: xx yy zz xy xz yz
+                              (md_lj/src/lib.rs:510-585; computed on request) */
+    double pressure;       /* rho T - tr(W) / (3 V) + P_tail (md_measure/src/lib.rs:
+                              605-651, tails md_lj/src/lib.rs:397-443; on request) */
 } mdd_thermo;
 
 /* ---- lifetime (App / resources) ---------------------------------------- */

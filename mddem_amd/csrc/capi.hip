@@ -582,6 +582,8 @@ int mdd_set_lj(mdd_ctx* ctx, double epsilon, double sigma, double cutoff) {
         c.lj_cut2 = cs * cs;
         c.lj_e12 = 4.0 * epsilon * sigma6 * sigma6;
         c.lj_e6 = 4.0 * epsilon * sigma6;
+        c.lj_eps = epsilon;
+        c.lj_sigma = sigma;
     });
 }
 
@@ -1118,6 +1120,27 @@ int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out) {
             double pe = 0.0;
             for (double v : h) pe += v;
             out->pe = pe;
+            auto w = dl(c, c.vir, size_t(6) * size_t(c.npad));
+            for (int k = 0; k < 6; k++) {
+                double t = 0.0;
+                for (int64_t i = 0; i < c.n; i++) t += w[size_t(k) * size_t(c.npad) + size_t(i)];
+                out->virial[k] = t;
+            }
+            // compute_pressure (md_measure/src/lib.rs:605-651) with the tail
+            // of setup_lj_tails (md_lj/src/lib.rs:397-443, one type)
+            const double n = double(c.n);
+            const double vol = c.box.L[0] * c.box.L[1] * c.box.L[2];
+            const double rho = n / vol;
+            const double rc = std::sqrt(c.lj_cut2);
+            const double sigma3 = (c.lj_sigma * c.lj_sigma) * c.lj_sigma;
+            const double rc3 = (rc * rc) * rc;
+            const double rc9 = (rc3 * rc3) * rc3;
+            const double sigma9 = (sigma3 * sigma3) * sigma3;
+            const double pi = 3.141592653589793;
+            const double p_tail = 0.0 + 1.0 * (16.0 / 3.0) * pi * rho * rho * c.lj_eps * sigma3 *
+                                            (2.0 * sigma9 / (3.0 * rc9) - sigma3 / rc3);
+            const double trace = out->virial[0] + out->virial[1] + out->virial[2];
+            out->pressure = rho * out->temperature - trace / (3.0 * vol) + p_tail;
         }
     });
 }

@@ -233,6 +233,7 @@ struct Ctx {
     int box_exit_rebuild = 1;
     MatP mat{};
     double lj1 = 0, lj2 = 0, lj_cut2 = 0, lj_e12 = 0, lj_e6 = 0;
+    double lj_eps = 1.0, lj_sigma = 1.0;  // for the tail corrections
     int nh_on = 0;
     double nh_T = 0, nh_tau = 1, nh_ndof = 0, nh_Q = 1;
     bool nh_ready = false;
@@ -329,6 +330,7 @@ struct Ctx {
     DBuf<uint32_t> xnbr, nx;
     DBuf<double> ke_total;          // NH kinetic energy sum for the next pre kernel
     DBuf<uint32_t> row_blk;         // per-block row maxima of the last build
+    DBuf<double> vir;               // energy passes: per-particle virial, 6 x npad
 
     // ---- full neighbor list: entry k of particle i at row_addr(i, k, npad, lg)
     DBuf<uint32_t> nbr, nbr_old;

@@ -61,6 +61,7 @@ struct ForceArgs {
     double* t64;
     float* t32;
     double* pe;
+    double* vir;  // energy passes: per-particle half share of sum d (x) F, 6 x npad
     Status* st;
     int step;
     const SysP* sysp;     // batched ensemble: per-system box and materials, or null
@@ -135,6 +136,7 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     if (cnt > a.maxnb) cnt = a.maxnb;
     R fx = R(0), fy = R(0), fz = R(0);
     double pe = 0.0;
+    double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // virial (xx yy zz xy xz yz)
     const R lj1 = R(a.lj1), lj2 = R(a.lj2);
     const uint32_t* row = a.nbr + (i << LG) + t;
     const int64_t stride = a.npad << LG;
@@ -154,6 +156,15 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
         if (ENERGY && in) {
             const double r6 = double(r6inv);
             pe += 0.5 * (r6 * (a.e12 * r6 - a.e6));
+            // virial (md_lj/src/lib.rs:555-561): d (x) F per pair, half on
+            // each side of the full rows (both sides see the same d (x) F)
+            const double sd = double(s);
+            w[0] += 0.5 * (dx * (sd * dx));
+            w[1] += 0.5 * (dy * (sd * dy));
+            w[2] += 0.5 * (dz * (sd * dz));
+            w[3] += 0.5 * (dx * (sd * dy));
+            w[4] += 0.5 * (dx * (sd * dz));
+            w[5] += 0.5 * (dy * (sd * dz));
         }
     };
     // Batches of kB entries per lane: the batch's kB gathers are issued
@@ -209,7 +220,11 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     fx = group_sum<G>(fx);
     fy = group_sum<G>(fy);
     fz = group_sum<G>(fz);
-    if (ENERGY) pe = group_sum<G>(pe);
+    if (ENERGY) {
+        pe = group_sum<G>(pe);
+#pragma unroll
+        for (int k = 0; k < 6; k++) w[k] = group_sum<G>(w[k]);
+    }
     if (t != 0) return;
     if constexpr (sizeof(R) == 8) {
         a.f64[i] = fx;
@@ -220,7 +235,11 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
         a.f32[a.npad + i] = fy;
         a.f32[2 * a.npad + i] = fz;
     }
-    if (ENERGY) a.pe[i] = pe;
+    if (ENERGY) {
+        a.pe[i] = pe;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a.vir[k * a.npad + i] = w[k];
+    }
 }
 
 template <typename R>
@@ -537,6 +556,7 @@ ForceArgs make_args(Ctx& c, int step) {
     a.t64 = c.t64.p;
     a.t32 = c.t32.p;
     a.pe = c.pe.p;
+    a.vir = c.vir.p;
     a.st = c.status.p;
     a.step = step;
     a.sysp = c.sysp_or_null();
@@ -600,7 +620,10 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
     }
     c.last_force_fused = false;
     if (c.model == 0 && c.list_tile) {
-        if (energy) c.pe.resize(c.npad);
+        if (energy) {
+            c.pe.resize(c.npad);
+            c.vir.resize(size_t(6) * c.npad);
+        }
         const bool fuse = !energy && step >= 0;
         launch_lj_tile(c, step, energy, fuse);
         c.last_force_fused = fuse;
@@ -608,7 +631,9 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
     } else if (c.model == 0) {
         if (energy) {
             c.pe.resize(c.npad);
+            c.vir.resize(size_t(6) * c.npad);
             a.pe = c.pe.p;
+            a.vir = c.vir.p;
             if (mixed)
                 launch_lj<float, true>(c, a);
             else

@@ -55,6 +55,7 @@ struct TileArgs {
     double* f64;
     float* f32;
     double* pe;
+    double* vir;  // energy passes: per-particle half share of sum d (x) F, 6 x npad
     Status* st;
     int step;
     int vv, nh, q;  // fused post: Verlet half kick, thermostat, scale slot
@@ -73,9 +74,15 @@ __device__ inline float rcp_t<float>(float x) {
     return __builtin_amdgcn_rcpf(x);
 }
 
+// energy passes: pair energy and virial (xx yy zz xy xz yz) of one particle
+struct EAcc {
+    double pe = 0.0;
+    double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+};
+
 template <typename R, bool ENERGY>
 __device__ inline void lj_pair(const TileArgs& a, double dx, double dy, double dz, R& fx, R& fy,
-                               R& fz, double& pe) {
+                               R& fz, EAcc& pe) {
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const bool in = r2 < a.cut2;
     const R r2inv = rcp_t<R>(R(r2));
@@ -87,7 +94,15 @@ __device__ inline void lj_pair(const TileArgs& a, double dx, double dy, double d
     fz = fma(s, R(dz), fz);
     if (ENERGY && in) {
         const double r6 = double(r6inv);
-        pe += 0.5 * (r6 * (a.e12 * r6 - a.e6));
+        pe.pe += 0.5 * (r6 * (a.e12 * r6 - a.e6));
+        // virial (md_lj/src/lib.rs:555-561), half on each side of the full rows
+        const double sd = double(s);
+        pe.w[0] += 0.5 * (dx * (sd * dx));
+        pe.w[1] += 0.5 * (dy * (sd * dy));
+        pe.w[2] += 0.5 * (dz * (sd * dz));
+        pe.w[3] += 0.5 * (dx * (sd * dy));
+        pe.w[4] += 0.5 * (dx * (sd * dz));
+        pe.w[5] += 0.5 * (dy * (sd * dz));
     }
 }
 
@@ -123,7 +138,7 @@ template <typename R, bool ENERGY>
 __device__ inline void row_pairs_tile(const TileArgs& a, const TileLds& sh, const uint16_t* row,
                                       uint32_t stride, uint32_t G, uint32_t t, uint32_t cnt,
                                       const double4& xi, uint32_t (&idx)[kJ], R& fx, R& fy, R& fz,
-                                      double& pe) {
+                                      EAcc& pe) {
     for (uint32_t jb = 0;;) {
 #pragma unroll
         for (int u = 0; u < kJ; u++) {
@@ -144,7 +159,7 @@ template <typename R, bool ENERGY>
 __device__ inline void row_pairs_global(const TileArgs& a, const CellRuns& Rn,
                                               const uint16_t* row, uint32_t stride, uint32_t G,
                                               uint32_t t, uint32_t cnt, const double4& xi, R& fx,
-                                              R& fy, R& fz, double& pe) {
+                                              R& fy, R& fz, EAcc& pe) {
     uint32_t j = 0;
     for (uint32_t k = t; k < cnt; k += G, j++) {
         double yx, yy, yz;
@@ -207,7 +222,7 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_tile(TileArgs a) {
     double ke = 0.0;
     for (;;) {
         R fx = R(0), fy = R(0), fz = R(0);
-        double pe = 0.0;
+        EAcc pe;
         if (act) {
             const uint16_t* row = rows + m * G;
             if (tiled)
@@ -234,13 +249,21 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_tile(TileArgs a) {
                 const R ox = __shfl(fx, lane + int(off), 64);
                 const R oy = __shfl(fy, lane + int(off), 64);
                 const R oz = __shfl(fz, lane + int(off), 64);
-                double op = 0.0;
-                if (ENERGY) op = __shfl(pe, lane + int(off), 64);
+                EAcc op;
+                if (ENERGY) {
+                    op.pe = __shfl(pe.pe, lane + int(off), 64);
+#pragma unroll
+                    for (int k = 0; k < 6; k++) op.w[k] = __shfl(pe.w[k], lane + int(off), 64);
+                }
                 if ((t & (2u * off - 1u)) == 0u && t + off < G) {
                     fx += ox;
                     fy += oy;
                     fz += oz;
-                    if (ENERGY) pe += op;
+                    if (ENERGY) {
+                        pe.pe += op.pe;
+#pragma unroll
+                        for (int k = 0; k < 6; k++) pe.w[k] += op.w[k];
+                    }
                 }
             }
         }
@@ -254,7 +277,11 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_tile(TileArgs a) {
                 a.f32[a.npad + q] = fy;
                 a.f32[2 * a.npad + q] = fz;
             }
-            if (ENERGY) a.pe[q] = pe;
+            if (ENERGY) {
+                a.pe[q] = pe.pe;
+#pragma unroll
+                for (int k = 0; k < 6; k++) a.vir[k * a.npad + q] = pe.w[k];
+            }
             if (FUSE) {
                 double4 v = a.vel[q];
                 const double h = 0.5 * a.dt * v.w;
@@ -398,6 +425,7 @@ TileArgs tile_args(Ctx& c, int step) {
     a.f64 = c.f64.p;
     a.f32 = c.f32.p;
     a.pe = c.pe.p;
+    a.vir = c.vir.p;
     a.st = c.status.p;
     a.step = step;
     a.vv = c.vv_on;

@@ -43,11 +43,13 @@ class Thermo(C.Structure):
     _fields_ = [("step", C.c_int64), ("rebuilds", C.c_int64), ("npairs", C.c_int64),
                 ("nghost", C.c_int64), ("ncontacts", C.c_int64), ("ke", C.c_double),
                 ("temperature", C.c_double), ("pe", C.c_double), ("p_xi", C.c_double),
-                ("momentum", C.c_double * 3), ("granular_temperature", C.c_double)]
+                ("momentum", C.c_double * 3), ("granular_temperature", C.c_double),
+                ("virial", C.c_double * 6), ("pressure", C.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
         d["momentum"] = list(self.momentum)
+        d["virial"] = list(self.virial)
         return d
 
 

@@ -112,7 +112,7 @@ struct orc_sys {
     int gravity_on;
     double gravity[3];
     /* LJ (md_lj lib.rs:230-239) */
-    double lj1, lj2, lj_cut2;
+    double lj1, lj2, lj_cut2, lj_eps, lj_sigma;
     /* Nose-Hoover (md_thermostat lib.rs:128-139) */
     int nh_on;
     double nh_T, nh_tau, nh_ndof, nh_Q, p_xi;
@@ -394,6 +394,8 @@ void orc_set_lj(orc_sys* s, double epsilon, double sigma, double cutoff) {
     s->lj2 = 24.0 * epsilon * sigma6;
     double c = cutoff * sigma;
     s->lj_cut2 = c * c;
+    s->lj_eps = epsilon;
+    s->lj_sigma = sigma;
 }
 
 void orc_get_lj(const orc_sys* s, double out[3]) {
@@ -1316,6 +1318,69 @@ void orc_lj_force(orc_sys* s) {
         s->force[i][1] = fi1;
         s->force[i][2] = fi2;
     }
+}
+
+void orc_lj_virial(const orc_sys* s, double w[6]) {
+    /* virial accumulation of lj_force (md_lj/src/lib.rs:510-585) over the
+     * newton half list: W_ab = sum d_a F_b, d = x_j - x_i, F = -fpair d;
+     * order xx, yy, zz, xy, xz, yz */
+    double lj1 = s->lj1, lj2 = s->lj2, cut2 = s->lj_cut2;
+    for (int k = 0; k < 6; k++) w[k] = 0.0;
+    for (int i = 0; i < s->nlocal; i++) {
+        double pix = s->pos[i][0], piy = s->pos[i][1], piz = s->pos[i][2];
+        for (uint32_t m = s->offsets[i]; m < s->offsets[i + 1]; m++) {
+            int j = (int)s->indices[m];
+            double dx = s->pos[j][0] - pix, dy = s->pos[j][1] - piy, dz = s->pos[j][2] - piz;
+            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (r2 >= cut2) continue;
+            double r2inv = 1.0 / r2;
+            double r6inv = r2inv * r2inv * r2inv;
+            double fpair = r2inv * r6inv * fma(lj1, r6inv, -lj2);
+            double fx = -fpair * dx, fy = -fpair * dy, fz = -fpair * dz;
+            w[0] += dx * fx;
+            w[1] += dy * fy;
+            w[2] += dz * fz;
+            w[3] += dx * fy;
+            w[4] += dx * fz;
+            w[5] += dy * fz;
+        }
+    }
+}
+
+void orc_lj_tails(const orc_sys* s, double* e_tail, double* p_tail) {
+    /* setup_lj_tails (md_lj/src/lib.rs:397-443), one type (weight 1) */
+    const double pi = 3.141592653589793;
+    double n = (double)s->nlocal;
+    double v = s->size[0] * s->size[1] * s->size[2]; /* Domain::volume (domain.rs:153) */
+    double rho = n / v;
+    double rc = sqrt(s->lj_cut2);
+    double sigma3 = (s->lj_sigma * s->lj_sigma) * s->lj_sigma;
+    double rc3 = (rc * rc) * rc;
+    double rc9 = (rc3 * rc3) * rc3;
+    double sigma6 = sigma3 * sigma3;
+    double sigma9 = sigma6 * sigma3;
+    double w = 1.0 * 1.0;
+    *e_tail = 0.0 + w * (8.0 / 3.0) * pi * n * rho * s->lj_eps * sigma3 *
+                        (sigma9 / (3.0 * rc9) - sigma3 / rc3);
+    *p_tail = 0.0 + w * (16.0 / 3.0) * pi * rho * rho * s->lj_eps * sigma3 *
+                        (2.0 * sigma9 / (3.0 * rc9) - sigma3 / rc3);
+}
+
+double orc_pressure(const orc_sys* s, const double w[6], double p_tail) {
+    /* compute_pressure (md_measure/src/lib.rs:605-651):
+     * P = rho T - tr(W) / (3 V) + P_tail, T = 2 KE / (3 N - 3) */
+    double n = (double)s->nlocal;
+    double v = s->size[0] * s->size[1] * s->size[2];
+    double rho = n / v;
+    double ke = 0.0;
+    for (int i = 0; i < s->nlocal; i++)
+        ke += s->mass[i] * (s->vel[i][0] * s->vel[i][0] + s->vel[i][1] * s->vel[i][1] +
+                            s->vel[i][2] * s->vel[i][2]);
+    ke *= 0.5;
+    double ndof = 3.0 * n - 3.0;
+    double temp = ndof > 0.0 ? 2.0 * ke / ndof : 0.0;
+    double trace = w[0] + w[1] + w[2];
+    return rho * temp - trace / (3.0 * v) + p_tail;
 }
 
 /* ---- integrators ------------------------------------------------------------ */

@@ -94,6 +94,9 @@ int orc_hertz_mindlin_contact_force(orc_sys* s);      /* contact.rs:103-615 */
 void orc_wall_contact_force(orc_sys* s);              /* dem_wall lib.rs:848-1002 */
 void orc_apply_gravity(orc_sys* s);                   /* mddem_fixes lib.rs:756-761 */
 void orc_lj_force(orc_sys* s);                        /* md_lj lib.rs:488-633 */
+void orc_lj_virial(const orc_sys* s, double w[6]);   /* md_lj lib.rs:510-585 */
+void orc_lj_tails(const orc_sys* s, double* e_tail, double* p_tail); /* md_lj 397-443 */
+double orc_pressure(const orc_sys* s, const double w[6], double p_tail); /* md_measure 605-651 */
 void orc_reverse_send_force(orc_sys* s);              /* comm.rs:579-656 */
 void orc_initial_integration(orc_sys* s);             /* mddem_verlet lib.rs:97-123 */
 void orc_final_integration(orc_sys* s);               /* mddem_verlet lib.rs:136-154 */

@@ -78,6 +78,9 @@ def lib():
             "orc_wall_contact_force": (None, [vp]),
             "orc_apply_gravity": (None, [vp]),
             "orc_lj_force": (None, [vp]),
+            "orc_lj_virial": (None, [vp, _dp]),
+            "orc_lj_tails": (None, [vp, _dp, _dp]),
+            "orc_pressure": (C.c_double, [vp, _dp, C.c_double]),
             "orc_reverse_send_force": (None, [vp]),
             "orc_initial_integration": (None, [vp]),
             "orc_final_integration": (None, [vp]),
@@ -164,6 +167,20 @@ class Oracle:
 
     def set_lj(self, epsilon=1.0, sigma=1.0, cutoff=2.5):
         self.L.orc_set_lj(self.h, epsilon, sigma, cutoff)
+
+    def lj_virial(self):
+        """W (xx, yy, zz, xy, xz, yz) over the current half list."""
+        w = np.zeros(6)
+        self.L.orc_lj_virial(self.h, _ptr(w))
+        return w
+
+    def lj_tails(self):
+        e, p = C.c_double(0), C.c_double(0)
+        self.L.orc_lj_tails(self.h, C.byref(e), C.byref(p))
+        return e.value, p.value
+
+    def pressure(self, w, p_tail):
+        return self.L.orc_pressure(self.h, _ptr(_d(w)), float(p_tail))
 
     def lj_coeffs(self):
         out = np.zeros(3)

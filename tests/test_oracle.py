@@ -222,6 +222,57 @@ def test_lj_coefficients_and_zero_force_radius():
     assert abs(o.atoms()["force"][1, 0]) < 1e-12
 
 
+@pytest.mark.parametrize("r", [0.95, 1.0, 1.3, 2.2])
+def test_lj_virial_two_body(r):
+    # md_lj/src/lib.rs:510-585: W = d (x) F_i; one pair along a unit
+    # direction u: W = -r F(r) u (x) u with F(r) = 24 (2 r^-13 - r^-7)
+    fr = 24.0 * (2.0 * r ** -13 - r ** -7)
+    for u in (np.array([1.0, 0.0, 0.0]), np.array([1.0, 1.0, 0.0]) / math.sqrt(2.0),
+              np.array([1.0, -2.0, 2.0]) / 3.0):
+        o = Oracle(MODEL_LJ)
+        o.set_lj(1.0, 1.0, 2.5)
+        o.set_atoms([[0, 0, 0], list(r * u)], np.zeros((2, 3)), [1, 1], [0.5, 0.5])
+        o.set_half_list([0, 1, 1], [1])
+        w = o.lj_virial()
+        want = -r * fr * np.array([u[0] * u[0], u[1] * u[1], u[2] * u[2], u[0] * u[1],
+                                   u[0] * u[2], u[1] * u[2]])
+        assert np.allclose(w, want, rtol=1e-12, atol=1e-12 * abs(r * fr))
+    if r == 1.0:
+        assert w[0] + w[1] + w[2] == pytest.approx(-24.0, rel=1e-14)
+
+
+def test_lj_virial_vanishes_at_the_force_zero():
+    o = Oracle(MODEL_LJ)
+    o.set_lj(1.0, 1.0, 2.5)
+    r = K["lj_zero_force_r"]
+    o.set_atoms([[0, 0, 0], [r, 0, 0]], np.zeros((2, 3)), [1, 1], [0.5, 0.5])
+    o.set_half_list([0, 1, 1], [1])
+    assert abs(o.lj_virial()[0]) < 1e-12
+
+
+def test_lj_tails_and_ideal_gas_pressure():
+    # setup_lj_tails (md_lj/src/lib.rs:397-443) closed forms, and
+    # compute_pressure (md_measure/src/lib.rs:605-651) with no pairs: rho T + P_tail
+    n, L = 500, 10.0
+    o = Oracle(MODEL_LJ)
+    o.set_box((0, 0, 0), (L, L, L), (True, True, True))
+    o.set_lj(1.0, 1.0, 2.5)
+    st = W.Stream(3)
+    vel = st.uniform(3 * n).reshape(n, 3) - 0.5
+    o.set_atoms(st.uniform(3 * n).reshape(n, 3) * L, vel, np.ones(n), np.full(n, 1.0))
+    e_tail, p_tail = o.lj_tails()
+    rho, rc = n / L ** 3, 2.5
+    assert e_tail == pytest.approx(8 * math.pi * n * rho / 3 * (1 / (3 * rc ** 9) - 1 / rc ** 3),
+                                   rel=1e-14)
+    assert p_tail == pytest.approx(16 * math.pi * rho ** 2 / 3 * (2 / (3 * rc ** 9) - 1 / rc ** 3),
+                                   rel=1e-14)
+    temp = np.sum(vel * vel) / (3 * n - 3)
+    assert o.pressure(np.zeros(6), p_tail) == pytest.approx(rho * temp + p_tail, rel=1e-13)
+    # a virial trace of -3 V adds 1 to the pressure
+    assert o.pressure([-3 * L ** 3, 0, 0, 0, 0, 0], 0.0) == pytest.approx(rho * temp + 1.0,
+                                                                          rel=1e-13)
+
+
 def test_trig30_bin_list_matches_brute_force():
     # neighbor.rs:1164-1266
     pos = np.array(K["trig30_pos"])
