@@ -92,8 +92,9 @@ __device__ inline uint32_t flat_to_bin(const CellRuns& R, uint32_t f) {
 
 // The tile force kernel works a cell with kTileWaves wavefronts; wave w
 // takes the cell's particles [w mw, (w + 1) mw), mw = ceil(M / kTileWaves),
-// and G lanes share a particle, G = 64 / mw capped at 16 (1 for mw >= 64).
-constexpr uint32_t kTileWaves = 4;
+// and G lanes share a particle, G = 64 / mw capped at 16 (1 for mw >= 64);
+// a wave with more than 64 / G particles takes them in passes.
+constexpr uint32_t kTileWaves = 1;
 
 __device__ __host__ inline int tile_lanes(uint32_t M) {
     const uint32_t mw = (M + kTileWaves - 1u) / kTileWaves;

@@ -1,15 +1,14 @@
 // LJ pair forces on cell tiles, with the step's final integration fused in.
 //
-// One workgroup of kTileWaves wavefronts per bin cell. The cell's candidate
-// list (its stencil cells' particles and periodic images, in bin order) is
-// loaded once per step into LDS: current positions, images shifted by
-// exactly +-L as the reference's ghost copies are (comm.rs:280). Each
-// particle of the cell then reads its row of 16-bit indices into that list,
-// so a pair costs LDS reads instead of a gather through L1. Wave w takes the
-// cell's particles [w mw, (w + 1) mw) and G lanes share a particle
-// (tile_lanes). A lane's first kJ row entries are loaded before the tile,
-// so the two gathers are in flight together and the pair loop runs from
-// registers and LDS.
+// One wavefront per bin cell. The cell's candidate list (its stencil cells'
+// particles and periodic images, in bin order) is loaded once per step into
+// LDS: current positions, images shifted by exactly +-L as the reference's
+// ghost copies are (comm.rs:280). Each particle of the cell then reads its
+// row of 16-bit indices into that list, so a pair costs LDS reads instead of
+// a gather through L1. G lanes share a particle (tile_lanes), 64 / G
+// particles per pass. A lane's first kJ row entries are loaded before the
+// tile, so the two gathers are in flight together; entries past the row
+// point at a far-away slot, so the pair loop runs in branch-free groups.
 //
 // Every tile entry evaluates d = y - x_i with y the candidate's (shifted)
 // position. For a pair the particle owns that is the reference's owner-frame
@@ -30,8 +29,10 @@ namespace mdd {
 namespace {
 
 constexpr uint32_t kTileThreads = 64u * kTileWaves;
-constexpr int kTile = 1024;  // candidates staged in LDS (24 KB); longer lists read HBM
-constexpr int kJ = 16;       // row entries per lane held in registers per batch
+constexpr int kTile = 768;  // candidates staged in LDS (18 KB); longer lists read HBM
+constexpr int kJ = 32;      // row entries per lane held in registers per batch
+constexpr int kSlots = kTile / 64;  // tile slots per lane
+constexpr int kMaxTileRuns = 32;    // stencil runs held in the LDS run table
 static_assert(kTile % kTileThreads == 0, "tile slots per thread");
 
 struct TileArgs {
@@ -118,22 +119,25 @@ __device__ inline void candidate_global(const TileArgs& a, const CellRuns& Rn, u
 }
 
 struct TileLds {
-    double x[kTile], y[kTile], z[kTile];
-    double ke[kTileWaves];
+    // slot kTile is a candidate far away from every particle: rows are padded
+    // with it, so a padded pair is out of range and adds exactly zero
+    double x[kTile + 1], y[kTile + 1], z[kTile + 1];
+    uint32_t run_excl[kMaxTileRuns + 1], run_base[kMaxTileRuns];
 };
 
-// the lane's row entries jb .. jb + kJ - 1 (entry k = j G + t, j = jb + u)
+// the lane's row entries jb .. jb + kJ - 1 (entry k = j G + t, j = jb + u);
+// entries past the row point at the far-away slot
 __device__ inline void load_batch(const uint16_t* row, uint32_t stride, uint32_t jb, uint32_t G,
                                   uint32_t t, uint32_t cnt, uint32_t (&idx)[kJ]) {
 #pragma unroll
     for (int u = 0; u < kJ; u++) {
         const uint32_t j = jb + uint32_t(u);
-        idx[u] = j * G + t < cnt ? uint32_t(row[size_t(j) * stride]) : 0u;
+        idx[u] = j * G + t < cnt ? uint32_t(row[size_t(j) * stride]) : uint32_t(kTile);
     }
 }
 
-// pairs of the lane's row from the tile: batches of kJ entries, the first
-// one preloaded
+// pairs of the lane's row from the tile, in groups of 4 (their LDS reads go
+// out together); a batch of kJ entries is preloaded, further ones load here
 template <typename R, bool ENERGY>
 __device__ inline void row_pairs_tile(const TileArgs& a, const TileLds& sh, const uint16_t* row,
                                       uint32_t stride, uint32_t G, uint32_t t, uint32_t cnt,
@@ -141,8 +145,10 @@ __device__ inline void row_pairs_tile(const TileArgs& a, const TileLds& sh, cons
                                       EAcc& pe) {
     for (uint32_t jb = 0;;) {
 #pragma unroll
-        for (int u = 0; u < kJ; u++) {
-            if ((jb + uint32_t(u)) * G + t < cnt) {
+        for (int u0 = 0; u0 < kJ; u0 += 4) {
+            if (!__any((jb + uint32_t(u0)) * G + t < cnt)) break;
+#pragma unroll
+            for (int u = u0; u < u0 + 4; u++) {
                 const uint32_t id = idx[u];
                 lj_pair<R, ENERGY>(a, sh.x[id] - xi.x, sh.y[id] - xi.y, sh.z[id] - xi.z, fx, fy,
                                    fz, pe);
@@ -157,9 +163,9 @@ __device__ inline void row_pairs_tile(const TileArgs& a, const TileLds& sh, cons
 // the same with every candidate read from HBM (a list longer than the tile)
 template <typename R, bool ENERGY>
 __device__ inline void row_pairs_global(const TileArgs& a, const CellRuns& Rn,
-                                              const uint16_t* row, uint32_t stride, uint32_t G,
-                                              uint32_t t, uint32_t cnt, const double4& xi, R& fx,
-                                              R& fy, R& fz, EAcc& pe) {
+                                        const uint16_t* row, uint32_t stride, uint32_t G,
+                                        uint32_t t, uint32_t cnt, const double4& xi, R& fx,
+                                        R& fy, R& fz, EAcc& pe) {
     uint32_t j = 0;
     for (uint32_t k = t; k < cnt; k += G, j++) {
         double yx, yy, yz;
@@ -168,35 +174,77 @@ __device__ inline void row_pairs_global(const TileArgs& a, const CellRuns& Rn,
     }
 }
 
+// The cell's candidate positions into LDS. Lane l fills slots l + 64 j; the
+// run holding each slot is found with a cursor over the run table (slots
+// increase with j), then all bin-entry loads and all position gathers go
+// out together.
+__device__ inline void fill_tile(const TileArgs& a, TileLds& sh, const CellRuns& Rn, int lane) {
+    if (lane < Rn.nruns) {
+        sh.run_excl[lane] = Rn.excl;
+        sh.run_base[lane] = Rn.rbase;
+    }
+    if (lane == 0) {
+        sh.run_excl[Rn.nruns] = Rn.total;
+        sh.x[kTile] = 1e30;
+        sh.y[kTile] = 1e30;
+        sh.z[kTile] = 1e30;
+    }
+    __syncthreads();
+    uint32_t bin[kSlots];
+    int r = 0;
+#pragma unroll
+    for (int j = 0; j < kSlots; j++) {
+        const uint32_t f = uint32_t(lane + 64 * j);
+        bin[j] = 0u;
+        if (f < Rn.total) {
+            while (f >= sh.run_excl[r + 1]) r++;
+            bin[j] = f + sh.run_base[r];
+        }
+    }
+    uint32_t e[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; j++)
+        e[j] = uint32_t(lane + 64 * j) < Rn.total ? a.bin_entry[bin[j]] : 0u;
+#pragma unroll
+    for (int j = 0; j < kSlots; j++) {
+        const uint32_t f = uint32_t(lane + 64 * j);
+        if (f < Rn.total) {
+            const int code = entry_code(e[j]);
+            const double4 p = a.pos[entry_idx(e[j])];
+            sh.x[f] = p.x + image_shift(code_sx(code), a.L[0]);
+            sh.y[f] = p.y + image_shift(code_sy(code), a.L[1]);
+            sh.z[f] = p.z + image_shift(code_sz(code), a.L[2]);
+        }
+    }
+    __syncthreads();
+}
+
 template <typename R, bool ENERGY, bool FUSE>
 __global__ void __launch_bounds__(kTileThreads) k_lj_tile(TileArgs a) {
     __shared__ TileLds sh;
-    const uint32_t tid = threadIdx.x;
-    const int lane = int(tid & 63u);
-    const uint32_t w = uint32_t(__builtin_amdgcn_readfirstlane(int(tid >> 6)));
+    const int lane = int(threadIdx.x & 63u);
     const int64_t C = int64_t(xcd_block(blockIdx.x, gridDim.x));
     if (a.step >= 0 && a.st->abort_step <= a.step) return;
     const uint32_t s0 = a.lstart[C], M = a.lstart[C + 1] - s0;
     if (M == 0) {
-        if (FUSE && a.nh && tid == 0) a.partial[C] = 0.0;
+        if (FUSE && a.nh && lane == 0) a.partial[C] = 0.0;
         return;
     }
     const CellRuns Rn =
         cell_runs(C, 0, a.ncells - 1, a.run_first, a.run_last, a.nruns, a.run0, a.bin_start);
-    const bool tiled = Rn.total <= uint32_t(kTile);
+    const bool tiled = Rn.total <= uint32_t(kTile) && Rn.nruns <= kMaxTileRuns;
     const uint32_t G = uint32_t(tile_lanes(M));
     const uint32_t ginv = (65536u + G - 1u) / G;
     const uint32_t sub = (uint32_t(lane) * ginv) >> 16;  // lane / G
     const uint32_t t = uint32_t(lane) - sub * G;
     const uint32_t per = 64u / G;
-    const uint32_t mw = (M + kTileWaves - 1u) / kTileWaves;
-    const uint32_t mend = min(M, (w + 1u) * mw);
     const uint32_t stride = M * G;
     const uint16_t* rows = a.crow + size_t(s0) * uint32_t(a.cap) + t;
-    // first pass: this lane's particle, count and first row batch
-    uint32_t mb = w * mw;
-    uint32_t m = mb + sub;
-    bool act = sub < per && m < mend;
+    // first pass: this lane's particle, count and first row batch (issued
+    // before the tile's gathers, so both are in flight together)
+    uint32_t mb = 0;
+    uint32_t m = sub;
+    bool act = sub < per && m < M;
     uint32_t q = s0 + m, cnt = 0u;
     double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
     uint32_t idx[kJ];
@@ -204,21 +252,10 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_tile(TileArgs a) {
         cnt = min(a.ncr[q], uint32_t(a.maxnb));
         xi = a.pos[q];
     }
-    if (tiled) load_batch(rows + m * G, stride, 0u, G, t, cnt, idx);
     if (tiled) {
-#pragma unroll
-        for (int sl = 0; sl < kTile / int(kTileThreads); sl++) {
-            const uint32_t f = uint32_t(sl) * kTileThreads + tid;
-            if (f < Rn.total) {
-                double yx, yy, yz;
-                candidate_global(a, Rn, f, yx, yy, yz);
-                sh.x[f] = yx;
-                sh.y[f] = yy;
-                sh.z[f] = yz;
-            }
-        }
+        load_batch(rows + m * G, stride, 0u, G, t, cnt, idx);
+        fill_tile(a, sh, Rn, lane);
     }
-    __syncthreads();
     double ke = 0.0;
     for (;;) {
         R fx = R(0), fy = R(0), fz = R(0);
@@ -305,10 +342,10 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_tile(TileArgs a) {
             }
         }
         mb += per;
-        if (mb >= mend) break;
-        // further passes (cells of more than 64 kTileWaves particles)
+        if (mb >= M) break;
+        // further passes (cells of more than 64 particles)
         m = mb + sub;
-        act = sub < per && m < mend;
+        act = sub < per && m < M;
         q = s0 + m;
         cnt = 0u;
         if (act) {
@@ -320,14 +357,7 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_tile(TileArgs a) {
     if (FUSE && a.nh) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) ke += __shfl_xor(ke, o, 64);
-        if (lane == 0) sh.ke[w] = ke;
-        __syncthreads();
-        if (tid == 0) {
-            double s = 0.0;
-#pragma unroll
-            for (uint32_t k = 0; k < kTileWaves; k++) s += sh.ke[k];
-            a.partial[C] = s;
-        }
+        if (lane == 0) a.partial[C] = ke;
     }
 }
 
