@@ -198,8 +198,16 @@ void run_steps(Ctx& c, int64_t nsteps) {
         mdd::launch_pair_forces(c, ab, false);
         second_half(ab);
         if (!retry) {
-            const int64_t ran = int64_t(ab) - c.step;
-            if (ran > 0) c.chunk = int(std::max<int64_t>(2, ran + ran / 4 + 1));
+            // the next chunk ends a little past the expected next rebuild:
+            // a few idle launches cost less than a chunk that ends too early
+            // (one more host round trip)
+            const int64_t ran = int64_t(ab) - c.last_rebuild;
+            if (ran > 0) {
+                c.rb_interval = c.rb_interval > 0.0 ? 0.5 * c.rb_interval + 0.5 * double(ran)
+                                                    : double(ran);
+                c.chunk = int(std::min(1024.0, std::ceil(1.15 * c.rb_interval) + 1.0));
+            }
+            c.last_rebuild = ab;
         }
         c.step = ab;
     };
@@ -223,6 +231,7 @@ void run_steps(Ctx& c, int64_t nsteps) {
                 mdd::launch_pair_forces(c, int(s), false);
                 second_half(int(s));
                 c.step = s;
+                c.last_rebuild = s;
                 continue;
             }
             int64_t K = std::min<int64_t>(c.chunk, end - c.step);
@@ -248,7 +257,17 @@ void run_steps(Ctx& c, int64_t nsteps) {
                 c.step += K;
                 c.retries = 0;
                 c.nghost = c.status_host->ghost_total;
-                c.chunk = std::min(c.chunk * 2, 1024);
+                // no rebuild yet: the rest of the expected interval; past it,
+                // short chunks that grow with the steps run since the rebuild;
+                // with no interval known yet, doubling chunks
+                const double since = double(c.step - c.last_rebuild);
+                const double rest = 1.15 * c.rb_interval - since;
+                if (c.rb_interval <= 0.0)
+                    c.chunk = std::min(c.chunk * 2, 1024);
+                else if (rest >= 1.0)
+                    c.chunk = int(std::ceil(rest)) + 1;
+                else
+                    c.chunk = int(std::min(1024.0, std::max(2.0, std::ceil(0.25 * since))));
             }
         }
         const int ab = check_status();
@@ -367,6 +386,8 @@ void pack_upload(Ctx& c, int64_t n, const uint32_t* tag, const uint32_t* type, c
     c.rebuilds = 0;
     c.nghost = 0;
     c.chunk = 8;
+    c.last_rebuild = 0;
+    c.rb_interval = 0.0;
     k_status_init<<<1, 1, 0, c.stream>>>(c.status.p);
     MDD_HIP(hipStreamSynchronize(c.stream));
 }

@@ -33,6 +33,32 @@ __device__ inline double readlane_f64(double v, int l) {
     return __longlong_as_double(static_cast<long long>((uint64_t(hi) << 32) | lo));
 }
 
+// Runs of equal keys inside a wavefront: the lane of the run head (h) and
+// of the run tail (t). Invalid lanes form runs of their own.
+__device__ inline void wave_runs(uint32_t key, bool valid, int& h, int& t) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t kp = __shfl_up(key, 1, 64);
+    const int vp = __shfl_up(int(valid), 1, 64);
+    const uint32_t kn = __shfl_down(key, 1, 64);
+    const int vn = __shfl_down(int(valid), 1, 64);
+    const bool head = lane == 0 || !valid || !vp || kp != key;
+    const bool tail = lane == 63 || !valid || !vn || kn != key;
+    int hv = head ? lane : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int x = __shfl_up(hv, d, 64);
+        if (lane >= d && x > hv) hv = x;
+    }
+    int tv = tail ? lane : 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int x = __shfl_down(tv, d, 64);
+        if (lane + d < 64 && x < tv) tv = x;
+    }
+    h = hv;
+    t = tv;
+}
+
 __device__ inline uint32_t lanes_before(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }

@@ -72,6 +72,24 @@ __host__ __device__ inline int64_t row_addr(int64_t p, int k, int64_t npad, int 
 __host__ __device__ inline int code_sx(int code) { return code / 9 - 1; }
 __host__ __device__ inline int code_sy(int code) { return (code / 3) % 3 - 1; }
 __host__ __device__ inline int code_sz(int code) { return code % 3 - 1; }
+// The three digits of an image code with the divisions by 9 and by 3 done
+// as multiply-shifts (exact for codes below 27, checked below), and the
+// shifts as s * L: exactly +L, -L or +0.0, the values image_shift gives.
+__host__ __device__ constexpr int code_q9(int code) { return (code * 57) >> 9; }
+__host__ __device__ constexpr int code_q3(int code) { return (code * 11) >> 5; }
+__host__ __device__ inline void code_shifts(int code, double Lx, double Ly, double Lz,
+                                            double& sx, double& sy, double& sz) {
+    const int q9 = code_q9(code), q3 = code_q3(code);
+    sx = double(q9 - 1) * Lx;
+    sy = double(q3 - 3 * q9 - 1) * Ly;
+    sz = double(code - 3 * q3 - 1) * Lz;
+}
+constexpr bool code_digits_exact() {
+    for (int c = 0; c < 27; c++)
+        if (code_q9(c) != c / 9 || code_q3(c) != c / 3) return false;
+    return true;
+}
+static_assert(code_digits_exact(), "multiply-shift digits of image codes");
 
 // Periodic image displacement per image code: exactly +L, -L or 0.0 per axis,
 // so x + shift reproduces the reference ghost coordinate fl(x +- L).
@@ -353,6 +371,10 @@ struct Ctx {
     int64_t step = 0, rebuilds = 0;
     int64_t sort_counter = 0;
     int chunk = 8;
+    // chunk sizing: step of the last rebuild and the smoothed number of steps
+    // between rebuilds (0 until two rebuilds were seen)
+    int64_t last_rebuild = 0;
+    double rb_interval = 0.0;
     int retries = 0;  // consecutive builds that did not fit
 
     // ---- rebuild launch sequences captured as graphs
@@ -387,7 +409,7 @@ void exclusive_scan(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n);
 void counting_sort(Ctx& c, const uint32_t* key, int64_t n, const uint32_t* n_dev, int64_t ncells,
                    uint32_t* cnt, uint32_t* start, uint32_t* placed, uint32_t* order,
                    uint32_t* slot_of, bool cnt_zero = false, const uint32_t* add = nullptr,
-                   uint32_t* out2 = nullptr);
+                   uint32_t* out2 = nullptr, bool counted = false);
 void fill_u32(Ctx& c, uint32_t* p, uint32_t v, int64_t n);
 void fill_f64(Ctx& c, double* p, double v, int64_t n);
 void fill_f32(Ctx& c, float* p, float v, int64_t n);

@@ -205,14 +205,15 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
 #pragma unroll
             for (int u = 0; u < kB; u++) {
                 if (kb + u * G < cnt) {
-                    const int code = entry_code(e[u]);
-                    const bool rev = entry_rev(e[u]);
-                    const double shx = image_shift(code_sx(code), a.L[0]);
-                    const double shy = image_shift(code_sy(code), a.L[1]);
-                    const double shz = image_shift(code_sz(code), a.L[2]);
-                    pair(e[u], (px[u] + (rev ? 0.0 : shx)) - (xi.x + (rev ? shx : 0.0)),
-                         (py[u] + (rev ? 0.0 : shy)) - (xi.y + (rev ? shy : 0.0)),
-                         (pz[u] + (rev ? 0.0 : shz)) - (xi.z + (rev ? shz : 0.0)));
+                    double shx, shy, shz;
+                    code_shifts(entry_code(e[u]), a.L[0], a.L[1], a.L[2], shx, shy, shz);
+                    // the shift goes on the non-owner: kept on the partner
+                    // (keep = 1) unless this particle is the non-owner; the
+                    // products and differences below are exact (+-L or +-0)
+                    const double keep = entry_rev(e[u]) ? 0.0 : 1.0;
+                    const double jx = keep * shx, jy = keep * shy, jz = keep * shz;
+                    pair(e[u], (px[u] + jx) - (xi.x + (shx - jx)), (py[u] + jy) - (xi.y + (shy - jy)),
+                         (pz[u] + jz) - (xi.z + (shz - jz)));
                 }
             }
         }
@@ -338,9 +339,8 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
             const int code = entry_code(e);
             const bool rev = entry_rev(e);
             const double4 pj = a.pos[j];
-            const double shx = image_shift(code_sx(code), Lx);
-            const double shy = image_shift(code_sy(code), Ly);
-            const double shz = image_shift(code_sz(code), Lz);
+            double shx, shy, shz;
+            code_shifts(code, Lx, Ly, Lz, shx, shy, shz);
             // owner frame: d = partner image - owner; r1 owner radius
             double dx, dy, dz, r1, r2;
             if (rev) {

@@ -113,9 +113,11 @@ __device__ inline void candidate_global(const TileArgs& a, const CellRuns& Rn, u
     const uint32_t e = a.bin_entry[flat_to_bin(Rn, f)];
     const int code = entry_code(e);
     const double4 p = a.pos[entry_idx(e)];
-    yx = p.x + image_shift(code_sx(code), a.L[0]);
-    yy = p.y + image_shift(code_sy(code), a.L[1]);
-    yz = p.z + image_shift(code_sz(code), a.L[2]);
+    double sx, sy, sz;
+    code_shifts(code, a.L[0], a.L[1], a.L[2], sx, sy, sz);
+    yx = p.x + sx;
+    yy = p.y + sy;
+    yz = p.z + sz;
 }
 
 struct TileLds {
@@ -211,9 +213,11 @@ __device__ inline void fill_tile(const TileArgs& a, TileLds& sh, const CellRuns&
         if (f < Rn.total) {
             const int code = entry_code(e[j]);
             const double4 p = a.pos[entry_idx(e[j])];
-            sh.x[f] = p.x + image_shift(code_sx(code), a.L[0]);
-            sh.y[f] = p.y + image_shift(code_sy(code), a.L[1]);
-            sh.z[f] = p.z + image_shift(code_sz(code), a.L[2]);
+            double sx, sy, sz;
+            code_shifts(code, a.L[0], a.L[1], a.L[2], sx, sy, sz);
+            sh.x[f] = p.x + sx;
+            sh.y[f] = p.y + sy;
+            sh.z[f] = p.z + sz;
         }
     }
     __syncthreads();
@@ -272,9 +276,8 @@ __global__ void __launch_bounds__(kTileThreads) k_lj_tile(TileArgs a) {
                 const uint32_t e = a.xnbr[size_t(q) * uint32_t(a.maxnb) + k];
                 const int code = entry_code(e);
                 const double4 pj = a.pos[entry_idx(e)];
-                const double sx = image_shift(code_sx(code), a.L[0]);
-                const double sy = image_shift(code_sy(code), a.L[1]);
-                const double sz = image_shift(code_sz(code), a.L[2]);
+                double sx, sy, sz;
+                code_shifts(code, a.L[0], a.L[1], a.L[2], sx, sy, sz);
                 lj_pair<R, ENERGY>(a, pj.x - (xi.x + sx), pj.y - (xi.y + sy), pj.z - (xi.z + sz),
                                    fx, fy, fz, pe);
             }

@@ -56,46 +56,56 @@ __global__ void k_rebuild_begin(Status* st, uint32_t* __restrict__ cnt, int64_t 
     }
 }
 
-// pbc wrap (wrap_periodic, domain.rs:396-407) + bin key of every particle
-// Batched ensembles pass the per-system table: each particle then uses its
-// own system's box and grid, and its key is offset into the shared cell space.
+// pbc wrap (wrap_periodic, domain.rs:396-407) + bin key of every particle,
+// counted per cell for the sort (cnt zeroed by k_rebuild_begin; equal keys
+// of a wavefront are counted by one atomic). Batched ensembles pass the
+// per-system table: each particle then uses its own system's box and grid,
+// and its key is offset into the shared cell space.
 __global__ void k_wrap_key(int64_t n, double4* __restrict__ pos, int4* __restrict__ image,
                            BoxP box, GridP g, uint32_t* __restrict__ key, Status* st,
-                           const SysP* __restrict__ sysp, const uint32_t* __restrict__ psys) {
-    int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (i >= n) return;
-    int64_t coff = 0;
-    if (sysp) {
-        const SysP& S = sysp[psys[i]];
-        box = S.box;
-        g = S.grid;
-        coff = S.cell_off;
-    }
-    double4 p = pos[i];
-    int4 im = image[i];
-    double x[3] = {p.x, p.y, p.z};
-    int m[3] = {im.x, im.y, im.z};
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        if (box.periodic[d]) {
-            double high = box.lo[d] + box.L[d];
-            if (x[d] < box.lo[d]) {
-                x[d] += box.L[d];
-                m[d] += -1;
-            } else if (x[d] >= high) {
-                x[d] -= box.L[d];
-                m[d] += 1;
-            }
-        } else if (x[d] < box.lo[d] || x[d] >= box.hi[d]) {
-            atomicOr(&st->overflow, 4u);  // left a fixed boundary
+                           const SysP* __restrict__ sysp, const uint32_t* __restrict__ psys,
+                           uint32_t* __restrict__ cnt) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t kc = 0u;
+    if (valid) {
+        int64_t coff = 0;
+        if (sysp) {
+            const SysP& S = sysp[psys[i]];
+            box = S.box;
+            g = S.grid;
+            coff = S.cell_off;
         }
+        double4 p = pos[i];
+        int4 im = image[i];
+        double x[3] = {p.x, p.y, p.z};
+        int m[3] = {im.x, im.y, im.z};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (box.periodic[d]) {
+                double high = box.lo[d] + box.L[d];
+                if (x[d] < box.lo[d]) {
+                    x[d] += box.L[d];
+                    m[d] += -1;
+                } else if (x[d] >= high) {
+                    x[d] -= box.L[d];
+                    m[d] += 1;
+                }
+            } else if (x[d] < box.lo[d] || x[d] >= box.hi[d]) {
+                atomicOr(&st->overflow, 4u);  // left a fixed boundary
+            }
+        }
+        p.x = x[0];
+        p.y = x[1];
+        p.z = x[2];
+        pos[i] = p;
+        image[i] = make_int4(m[0], m[1], m[2], 0);
+        kc = uint32_t(coff + cell_of(g, x[0], x[1], x[2]));
+        key[i] = kc;
     }
-    p.x = x[0];
-    p.y = x[1];
-    p.z = x[2];
-    pos[i] = p;
-    image[i] = make_int4(m[0], m[1], m[2], 0);
-    key[i] = uint32_t(coff + cell_of(g, x[0], x[1], x[2]));
+    int h, t;
+    wave_runs(kc, valid, h, t);
+    if (valid && int(threadIdx.x & 63) == t) atomicAdd(&cnt[kc], uint32_t(t - h + 1));
 }
 
 struct PermArgs {
@@ -160,10 +170,24 @@ __device__ inline void image_flags(const BoxP& box, double gc, const double x[3]
     }
 }
 
+// Start of one build attempt (a retry repeats it with larger capacities):
+// status reset and the image sort's per-cell counts zeroed, fused with the
+// image count of every particle.
 __global__ void k_ghost_count(int64_t n, const double4* __restrict__ pos, BoxP box, double gc,
                               uint32_t* __restrict__ gcnt, const SysP* __restrict__ sysp,
-                              const uint32_t* __restrict__ psys) {
+                              const uint32_t* __restrict__ psys, Status* st,
+                              uint32_t* __restrict__ cnt, int64_t ncells) {
+    for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < ncells;
+         c += int64_t(gridDim.x) * blockDim.x)
+        cnt[c] = 0u;
     int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i == 0) {
+        st->abort_step = INT_MAX;
+        st->overflow &= 4u;  // keep the fixed-boundary exit seen by the wrap
+        st->max_row = 0;
+        st->max_half = 0;
+        st->ghost_total = 0u;
+    }
     if (i >= n) return;
     if (sysp) {
         const SysP& S = sysp[psys[i]];
@@ -506,16 +530,24 @@ __global__ void __launch_bounds__(256) k_build_cells(CellArgs a) {
                 for (int c = 0; c < kNch; c++)
                     if (win + uint32_t(c * 64 + lane) >= pr) base[c] = br;
             }
+            // every chunk's loads are issued before any is used (lanes past
+            // the list read slot 0 and are cleared), so the window costs one
+            // memory latency rather than one per chunk
+            double4 qv[kNch];
+            uint32_t ev[kNch];
+#pragma unroll
+            for (int c = 0; c < kNch; c++) {
+                const uint32_t f = win + uint32_t(c * 64 + lane);
+                const uint32_t src = f < total ? f + base[c] : 0u;
+                qv[c] = a.bin_pos[src];
+                ev[c] = a.bin_entry[src];
+            }
 #pragma unroll
             for (int c = 0; c < kNch; c++) {
                 const uint32_t f = win + uint32_t(c * 64 + lane);
                 const bool valid = f < total;
-                double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
-                uint32_t e = 0u;
-                if (valid) {
-                    q = a.bin_pos[f + base[c]];
-                    e = a.bin_entry[f + base[c]];
-                }
+                const double4 q = valid ? qv[c] : make_double4(0.0, 0.0, 0.0, 0.0);
+                const uint32_t e = valid ? ev[c] : 0u;
                 W.x[c] = q.x;
                 W.y[c] = q.y;
                 W.z[c] = q.z;
@@ -788,20 +820,6 @@ void neighbor_setup(Ctx& c);
 
 namespace {
 
-// Start of one build attempt (a retry repeats it with larger capacities).
-__global__ void k_attempt_begin(Status* st, uint32_t* __restrict__ cnt, int64_t ncells) {
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < ncells;
-         i += int64_t(gridDim.x) * blockDim.x)
-        cnt[i] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->abort_step = INT_MAX;
-        st->overflow &= 4u;  // keep the fixed-boundary exit seen by the wrap
-        st->max_row = 0;
-        st->max_half = 0;
-        st->ghost_total = 0u;
-    }
-}
-
 // End of a build attempt: row maxima from the per-block slots into the
 // status, and a build that did not fit stops the step sequence at `step`.
 __global__ void __launch_bounds__(256) k_rows_gate(int nblk, const uint32_t* __restrict__ blk,
@@ -867,10 +885,10 @@ void rebuild_prepare(Ctx& c) {
     // pbc + keys
     launch(c, k_wrap_key, blocks_for(n, 256), 256, n, c.pos.p, c.image.p, c.box, c.grid,
                                                           c.key.p, c.status.p, c.sysp_or_null(),
-                                                          c.psys.p);
+                                                          c.psys.p, c.cnt.p);
     // sort_atoms_by_bin: stable counting sort of particles by bin
     counting_sort(c, c.key.p, n, nullptr, ncells, c.cnt.p, c.start.p, c.placed.p, c.order.p,
-                  nullptr, true);
+                  nullptr, true, nullptr, nullptr, true);
     // permute every particle array into the alternates, then swap
     PermArgs a;
     std::memset(&a, 0, sizeof(a));  // padding too: the bytes are a launch signature
@@ -991,11 +1009,10 @@ void rebuild_attempt(Ctx& c) {
     }
     uint32_t* ng_live = c.ghost_key.p + c.ghost_cap;
 
-    launch(c, k_attempt_begin, blocks_for(ncells, 256) < 512 ? blocks_for(ncells, 256) : 512, 256,
-           c.status.p, c.cnt.p, ncells);
     // borders: periodic images and their bins
     launch(c, k_ghost_count, blocks_for(n, 256), 256, n, c.pos.p, c.box, c.ghost_cutoff,
-                                                             c.gcnt.p, c.sysp_or_null(), c.psys.p);
+                                                             c.gcnt.p, c.sysp_or_null(), c.psys.p,
+                                                             c.status.p, c.cnt.p, ncells);
     exclusive_scan(c, c.gcnt.p, c.goff.p, n);
     launch(c, k_ghost_emit, blocks_for(n, 256), 256,
         n, c.pos.p, c.box, c.ghost_cutoff, c.grid, c.goff.p, c.ghost_cap, c.ghost_root.p,

@@ -6,7 +6,7 @@
 // runs of equal keys inside a wavefront, since the input is mostly sorted
 // already), then every element computes its index rank inside its cell in
 // parallel and is written to its final slot, so the result is deterministic.
-#include "ctx.hpp"
+#include "cells.hpp"
 
 namespace mdd {
 
@@ -73,27 +73,18 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* __
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-// single block: exclusive scan of nb block sums in place; sums[nb] = total
-__global__ void __launch_bounds__(kScanThreads) k_scan_sums(uint32_t* sums, int64_t nb) {
-    __shared__ uint32_t lds4[4];
-    uint32_t carry = 0;
-    for (int64_t base = 0; base < nb; base += kScanThreads) {
-        int64_t i = base + threadIdx.x;
-        uint32_t v = i < nb ? sums[i] : 0u;
-        uint32_t tot;
-        uint32_t ex = block_excl_scan(v, lds4, &tot);
-        if (i < nb) sums[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) sums[nb] = carry;
-}
-
+// Second pass: each block first sums the block totals before it (a few
+// thousand at most, read from L2), so no separate scan of the totals runs.
 __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const uint32_t* __restrict__ in,
                                                              uint32_t* __restrict__ out,
                                                              int64_t n,
                                                              const uint32_t* __restrict__ sums,
                                                              int64_t nb) {
     __shared__ uint32_t lds4[4];
+    uint32_t before = 0;
+    for (int64_t j = threadIdx.x; j < int64_t(blockIdx.x); j += kScanThreads) before += sums[j];
+    uint32_t off;
+    block_excl_scan(before, lds4, &off);
     const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
@@ -104,14 +95,14 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const uint32_t* __r
         s += v[k];
     }
     uint32_t tot;
-    uint32_t run = block_excl_scan(s, lds4, &tot) + sums[blockIdx.x];
+    uint32_t run = block_excl_scan(s, lds4, &tot) + off;
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
         int64_t i = base + k;
         if (i < n) out[i] = run;
         run += v[k];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sums[nb];
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = off + tot;
 }
 
 // Exclusive scan of in[0..n) by one workgroup, for arrays up to
@@ -125,7 +116,9 @@ constexpr int kOneTile = kOneThreads * kOneItems;
 constexpr int64_t kOneBlockScan = int64_t(1) << 17;
 
 // Tiles of kOneTile elements pass through LDS: read and written coalesced,
-// scanned as kOneItems contiguous elements per thread.
+// scanned as kOneItems contiguous elements per thread. The next tile's
+// loads are issued before the current tile is scanned, so one global
+// latency per tile overlaps the LDS work instead of adding to it.
 __global__ void __launch_bounds__(kOneThreads) k_scan_one(uint32_t* __restrict__ in,
                                                           uint32_t* __restrict__ out, int64_t n,
                                                           int zero_in,
@@ -135,18 +128,27 @@ __global__ void __launch_bounds__(kOneThreads) k_scan_one(uint32_t* __restrict__
     __shared__ uint32_t wsum[kOneThreads / 64];
     uint32_t* tile = reinterpret_cast<uint32_t*>(tile4);
     const int tid = int(threadIdx.x), lane = tid & 63, w = tid >> 6;
-    uint32_t carry = 0;
-    for (int64_t base = 0; base < n; base += kOneTile) {
+    uint32_t v[kOneItems], ad[kOneItems];
+    auto fetch = [&](int64_t base) {
 #pragma unroll
         for (int k = 0; k < kOneItems; k++) {
             const int64_t i = base + int64_t(k) * kOneThreads + tid;
-            uint32_t v = 0u;
-            if (i < n) {
-                v = in[i];
-                if (zero_in) in[i] = 0u;
-            }
-            tile[k * kOneThreads + tid] = v;
+            v[k] = i < n ? in[i] : 0u;
+            ad[k] = out2 && i < n ? add[i] : 0u;
         }
+    };
+    fetch(0);
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < n; base += kOneTile) {
+        uint32_t adc[kOneItems];
+#pragma unroll
+        for (int k = 0; k < kOneItems; k++) {
+            const int64_t i = base + int64_t(k) * kOneThreads + tid;
+            tile[k * kOneThreads + tid] = v[k];
+            if (zero_in && i < n) in[i] = 0u;
+            adc[k] = ad[k];
+        }
+        if (base + kOneTile < n) fetch(base + kOneTile);
         __syncthreads();
         const uint4 a = tile4[2 * tid], b = tile4[2 * tid + 1];
         const uint32_t s = (a.x + a.y + a.z + a.w) + (b.x + b.y + b.z + b.w);
@@ -178,7 +180,7 @@ __global__ void __launch_bounds__(kOneThreads) k_scan_one(uint32_t* __restrict__
             if (i < n) {
                 const uint32_t r = tile[k * kOneThreads + tid];
                 out[i] = r;
-                if (out2) out2[i] = r + add[i];
+                if (out2) out2[i] = r + adc[k];
             }
         }
         carry += tot;
@@ -195,32 +197,6 @@ __global__ void k_add_u32(const uint32_t* __restrict__ a, const uint32_t* __rest
                           uint32_t* __restrict__ out, int64_t n) {
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (i < n) out[i] = a[i] + b[i];
-}
-
-// Runs of equal keys inside a wavefront: the lane of the run head (h) and
-// of the run tail (t). Invalid lanes form runs of their own.
-__device__ inline void wave_runs(uint32_t key, bool valid, int& h, int& t) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t kp = __shfl_up(key, 1, 64);
-    const int vp = __shfl_up(int(valid), 1, 64);
-    const uint32_t kn = __shfl_down(key, 1, 64);
-    const int vn = __shfl_down(int(valid), 1, 64);
-    const bool head = lane == 0 || !valid || !vp || kp != key;
-    const bool tail = lane == 63 || !valid || !vn || kn != key;
-    int hv = head ? lane : 0;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int x = __shfl_up(hv, d, 64);
-        if (lane >= d && x > hv) hv = x;
-    }
-    int tv = tail ? lane : 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int x = __shfl_down(tv, d, 64);
-        if (lane + d < 64 && x < tv) tv = x;
-    }
-    h = hv;
-    t = tv;
 }
 
 // count per key; n taken from n_dev when given (device-side element count)
@@ -306,7 +282,6 @@ void exclusive_scan(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n) {
     if (nb < 1) nb = 1;
     c.scan_tmp.resize(size_t(nb + 2));
     launch(c, k_scan_reduce, int(nb), kScanThreads, in, n, c.scan_tmp.p);
-    launch(c, k_scan_sums, 1, kScanThreads, c.scan_tmp.p, nb);
     launch(c, k_scan_apply, int(nb), kScanThreads, in, out, n, c.scan_tmp.p, nb);
     MDD_HIP(hipGetLastError());
 }
@@ -314,13 +289,17 @@ void exclusive_scan(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n) {
 // Stable counting sort of indices 0..n-1 by key in [0, ncells). The element
 // count is n, or *n_dev (<= n) when n_dev is given. start: ncells+1;
 // order: n (slot -> index); slot_of: n (index -> slot) or null; cnt: ncells
-// scratch, already zero when cnt_zero; placed: n scratch. With add given,
-// out2[c] = start[c] + add[c] for c <= ncells as well.
+// scratch, already zero when cnt_zero, or already holding the per-key counts
+// when counted (the kernel that wrote the keys counted them); placed: n
+// scratch. With add given, out2[c] = start[c] + add[c] for c <= ncells as well.
 void counting_sort(Ctx& c, const uint32_t* key, int64_t n, const uint32_t* n_dev, int64_t ncells,
                    uint32_t* cnt, uint32_t* start, uint32_t* placed, uint32_t* order,
-                   uint32_t* slot_of, bool cnt_zero, const uint32_t* add, uint32_t* out2) {
-    if (!cnt_zero) fill_u32(c, cnt, 0u, ncells);
-    if (n > 0) launch(c, k_count, blocks_for(n, 256), 256, key, n, n_dev, cnt);
+                   uint32_t* slot_of, bool cnt_zero, const uint32_t* add, uint32_t* out2,
+                   bool counted) {
+    if (!counted) {
+        if (!cnt_zero) fill_u32(c, cnt, 0u, ncells);
+        if (n > 0) launch(c, k_count, blocks_for(n, 256), 256, key, n, n_dev, cnt);
+    }
     if (ncells <= kOneBlockScan) {
         // scan and cursor reset in one pass
         launch(c, k_scan_one, 1, kOneThreads, cnt, start, ncells, 1, add, out2);
