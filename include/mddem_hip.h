@@ -88,6 +88,20 @@ int mdd_set_dt(mdd_ctx* ctx, double dt);
 /* [[dem.materials]] + MaterialTable::build_pair_tables (crates/dem_atom/src/lib.rs:389-515) */
 int mdd_set_materials(mdd_ctx* ctx, int n, const double* youngs, const double* poisson,
                       const double* restitution, const double* friction);
+/* Optional MaterialConfig fields (crates/dem_atom/src/lib.rs:89-118) and their
+ * pair tables (build_pair_tables, lib.rs:425-510): values holds 10 arrays of
+ * n, in this order: rolling_friction, twisting_friction, cohesion_energy,
+ * surface_energy, kn, kt, rolling_stiffness, rolling_damping,
+ * twisting_stiffness, twisting_damping. Call after mdd_set_materials (which
+ * clears them). Any nonzero value selects the contact kernel of the whole
+ * model family (contact.rs:103-615, 622-990) with 7 history values per
+ * contact; switching the history width drops the stored history. Single
+ * systems only (not with mdd_set_systems). */
+int mdd_set_material_extras(mdd_ctx* ctx, int n, const double* values);
+/* [materials] contact_model "hooke" (1) or "hertz" (0), adhesion_model "dmt"
+ * (1) or "jkr" (0), rolling_model and twisting_model "sds" (1) or "constant"
+ * (0) (crates/dem_atom/src/lib.rs:142-157; contact.rs:65-90 picks the kernel) */
+int mdd_set_contact_models(mdd_ctx* ctx, int hooke, int dmt, int rolling_sds, int twisting_sds);
 /* [lj] section, LJPairCoeffs::from_params (crates/md_lj/src/lib.rs:230-239) */
 int mdd_set_lj(mdd_ctx* ctx, double epsilon, double sigma, double cutoff);
 /* [thermostat] section, NoseHooverPlugin (crates/md_thermostat/src/lib.rs:166-243) */
@@ -199,6 +213,12 @@ int mdd_run(mdd_ctx* ctx, int64_t nsteps);
  * at most cap entries, total in *n */
 int mdd_download_history(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* tag_hi,
                          double* spring, int64_t* n);
+/* the same with all 7 history values per contact (the [f64; 7] of
+ * ContactHistoryStore, contact.rs:343-350): spring (3), rolling displacement
+ * (3), twist angle (1), in the owner's sign convention; zeros past the
+ * spring when no optional model is active */
+int mdd_download_history7(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* tag_hi,
+                          double* hist7, int64_t* n);
 
 /* ---- observables (print_thermo, mddem_print/src/lib.rs:595-700) -------------- */
 int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out);

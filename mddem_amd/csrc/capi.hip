@@ -589,6 +589,79 @@ int mdd_set_materials(mdd_ctx* ctx, int n, const double* E, const double* nu, co
                 c.mat.geff[i * n + j] = k[3];
             }
         mdd::upload_materials(c);
+        // optional properties belong to the previous table: cleared
+        std::memset(c.mext_host, 0, sizeof(c.mext_host));
+        if (c.mext_dev.p) {
+            c.mext_dev.resize(size_t(mdd::kNumExtras) * n * n);
+            h2d(c, c.mext_dev.p, c.mext_host, sizeof(double) * mdd::kNumExtras * n * n);
+        }
+        if (c.ext_on) mdd::set_contact_path(c, c.cm_hooke != 0);
+    });
+}
+
+int mdd_set_material_extras(mdd_ctx* ctx, int n, const double* values) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
+        if (c.mat.n == 0 || n != c.mat.n || !values)
+            throw ApiError(MDD_ERR_INVALID, "set the materials first, one value per material");
+        if (c.nsys > 0)
+            throw ApiError(MDD_ERR_INVALID, "optional contact models need a single system");
+        const int nn = n * n;
+        auto v = [&](int x, int i) { return values[x * n + i]; };
+        auto gmean = [](double a, double b) { return std::sqrt(a * b); };
+        auto gmean0 = [](double a, double b) {
+            return std::sqrt((a > 0.0 ? a : 0.0) * (b > 0.0 ? b : 0.0));
+        };
+        auto hmean = [](double a, double b, bool either) {
+            if (a > 0.0 && b > 0.0) return 2.0 * a * b / (a + b);
+            if (!either) return 0.0;
+            return a > 0.0 ? a : b;
+        };
+        // MaterialTable::build_pair_tables, optional fields (dem_atom/src/lib.rs:425-510)
+        bool any = false;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double* t = c.mext_host;
+                const int k = i * n + j;
+                t[0 * nn + k] = gmean(v(0, i), v(0, j));    // rolling friction
+                t[1 * nn + k] = gmean0(v(1, i), v(1, j));   // twisting friction
+                t[2 * nn + k] = gmean(v(2, i), v(2, j));    // cohesion energy
+                t[3 * nn + k] = gmean(v(3, i), v(3, j));    // surface energy
+                t[4 * nn + k] = hmean(v(4, i), v(4, j), false);  // Hooke k_n
+                t[5 * nn + k] = hmean(v(5, i), v(5, j), false);  // Hooke k_t
+                t[6 * nn + k] = hmean(v(6, i), v(6, j), true);   // SDS rolling stiffness
+                t[7 * nn + k] = gmean0(v(7, i), v(7, j));   // SDS rolling damping
+                t[8 * nn + k] = hmean(v(8, i), v(8, j), true);   // SDS twisting stiffness
+                t[9 * nn + k] = gmean0(v(9, i), v(9, j));   // SDS twisting damping
+            }
+        for (int x = 0; x < mdd::kNumExtras * nn; x++) any = any || c.mext_host[x] != 0.0;
+        c.mext_dev.resize(size_t(mdd::kNumExtras) * nn);
+        h2d(c, c.mext_dev.p, c.mext_host, sizeof(double) * mdd::kNumExtras * nn);
+        mdd::set_contact_path(c, any || c.cm_hooke);
+    });
+}
+
+int mdd_set_contact_models(mdd_ctx* ctx, int hooke, int dmt, int rolling_sds, int twisting_sds) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
+        if (c.nsys > 0)
+            throw ApiError(MDD_ERR_INVALID, "optional contact models need a single system");
+        c.cm_hooke = hooke ? 1 : 0;
+        c.cm_dmt = dmt ? 1 : 0;
+        c.cm_roll_sds = rolling_sds ? 1 : 0;
+        c.cm_twist_sds = twisting_sds ? 1 : 0;
+        if (c.mext_dev.p == nullptr) {
+            // no optional properties given: every table is zero
+            const int nn = c.mat.n * c.mat.n;
+            if (nn == 0) throw ApiError(MDD_ERR_INVALID, "set the materials first");
+            std::memset(c.mext_host, 0, sizeof(c.mext_host));
+            c.mext_dev.resize(size_t(mdd::kNumExtras) * nn);
+            h2d(c, c.mext_dev.p, c.mext_host, sizeof(double) * mdd::kNumExtras * nn);
+        }
+        bool any = c.cm_hooke != 0;
+        for (int x = 0; x < mdd::kNumExtras * c.mat.n * c.mat.n; x++)
+            any = any || c.mext_host[x] != 0.0;
+        mdd::set_contact_path(c, any);
     });
 }
 
@@ -923,7 +996,7 @@ int mdd_set_half_list(mdd_ctx* ctx, int64_t nlocal, const uint32_t* offsets,
         h2d(c, c.nnb.p, cnt.data(), sizeof(uint32_t) * np);
         h2d(c, c.nhalf.p, half.data(), sizeof(uint32_t) * np);
         if (c.model == MDD_MODEL_DEM) {
-            c.hist.resize(size_t(3) * mx * np);
+            c.hist.resize(size_t(c.hist_slots) * mx * np);
             c.cmask.resize(np);
             dzero(c, c.cmask.p, sizeof(uint64_t) * np);
         }
@@ -1095,6 +1168,43 @@ int mdd_download_history(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* 
                     if (tag_hi) tag_hi[k] = std::max(a, b);
                     if (spring)
                         for (int d = 0; d < 3; d++) spring[3 * k + d] = hs[d * cs + slot];
+                }
+                k++;
+            }
+        }
+        if (n_out) *n_out = k;
+    });
+}
+
+int mdd_download_history7(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* tag_hi,
+                          double* hist7, int64_t* n_out) {
+    return guarded(ctx, [&](Ctx& c) {
+        need_list(c);
+        if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        const int64_t n = c.n, np = c.npad;
+        const int64_t cs = int64_t(c.maxnb) * np;
+        const int ns = c.hist_slots;
+        auto nn = dl(c, c.nnb, size_t(n));
+        auto nb = dl(c, c.nbr, size_t(cs));
+        auto tg = dl(c, c.tag, size_t(n));
+        auto mk = dl(c, c.cmask, size_t(n));
+        auto hs = dl(c, c.hist, size_t(ns * cs));
+        int64_t k = 0;
+        for (int64_t i = 0; i < n; i++) {
+            const int cnt = std::min<int>(int(nn[i]), c.maxnb);
+            for (int m = 0; m < cnt; m++) {
+                if (!((mk[i] >> m) & 1ull)) continue;
+                const int64_t slot = mdd::row_addr(i, m, np, c.list_lg);
+                const uint32_t e = nb[slot];
+                if (mdd::entry_rev(e)) continue;
+                if (k < cap) {
+                    const uint32_t a = tg[i], b = tg[mdd::entry_idx(e)];
+                    if (tag_lo) tag_lo[k] = std::min(a, b);
+                    if (tag_hi) tag_hi[k] = std::max(a, b);
+                    if (hist7)
+                        for (int d = 0; d < 7; d++)
+                            hist7[7 * k + d] = d < ns ? hs[d * cs + slot] : 0.0;
                 }
                 k++;
             }

@@ -133,6 +133,7 @@ struct Wall {
 };
 constexpr int kMaxWalls = 8;
 constexpr int kMaxMat = 16;
+constexpr int kNumExtras = 10;  // optional per-material contact properties
 
 struct WallsP {
     int count;
@@ -357,10 +358,20 @@ struct Ctx {
     bool remap_have_old = false;
     int maxnb_old = 0;
     // DEM shear history per list slot, canonical sign (lower tag's frame)
-    DBuf<double> hist, hist_old;  // 3 * maxnb * npad, SoA by component
+    DBuf<double> hist, hist_old;  // hist_slots * maxnb * npad, SoA by component
     // bit k of cmask[i]: slot k of particle i was in contact at the last force pass
     DBuf<uint64_t> cmask, cmask_old;
     DBuf<double> mat_dev;         // beta | mu | E* | G*, each nmat x nmat
+    // optional contact models: pair tables of the optional material
+    // properties (rolling / twisting friction, cohesion, surface energy, Hooke
+    // and SDS stiffness and damping), each nmat x nmat, and the model choices
+    DBuf<double> mext_dev;
+    bool ext_on = false;
+    int cm_hooke = 0, cm_dmt = 0, cm_roll_sds = 0, cm_twist_sds = 0;
+    double mext_host[kNumExtras * kMaxMat * kMaxMat] = {};
+    // history values per contact: 3 (tangential spring) or, with the optional
+    // models, 7 (spring, rolling displacement, twist angle)
+    int hist_slots = 3;
     DBuf<double> pe;              // per-particle LJ energy (thermo only)
 
     // ---- run state
@@ -426,6 +437,16 @@ void launch_lj_tile(Ctx& c, int step, bool energy, bool fuse);
 void launch_ke_reduce(Ctx& c, int step, int64_t count);
 void expand_tile_rows(Ctx& c);
 void upload_materials(Ctx& c);
+// switch between the plain Hertz-Mindlin kernel and the optional-model kernel;
+// a change of history width drops the stored history and rebuilds the list
+inline void set_contact_path(Ctx& c, bool on) {
+    const int slots = on ? 7 : 3;
+    c.ext_on = on;
+    if (slots != c.hist_slots) {
+        c.hist_slots = slots;
+        c.list_valid = false;
+    }
+}
 // integrate.hip
 void launch_pre(Ctx& c, int step, bool check_rebuild);
 void launch_post_pre(Ctx& c, int step, bool check_rebuild);

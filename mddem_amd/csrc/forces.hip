@@ -32,6 +32,9 @@ constexpr double kSqrt53 = 0.9128709291752768;
 constexpr double kTangEps = 1e-30;
 constexpr double kLargeOverlap = 0.90;
 constexpr int kMaxOverlapWarnings = 500;
+// optional pair tables in ForceArgs::mext, each nm x nm (the C API's order)
+constexpr int kXRolling = 0, kXTwisting = 1, kXCohesion = 2, kXSurface = 3, kXKn = 4, kXKt = 5,
+              kXRollK = 6, kXRollG = 7, kXTwistK = 8, kXTwistG = 9;
 
 struct ForceArgs {
     int64_t n, npad;
@@ -49,6 +52,8 @@ struct ForceArgs {
     double lj1, lj2, cut2, e12, e6;
     const double* mat;
     int nm;
+    const double* mext;  // optional pair tables (kX* order), each nm x nm, or null
+    int hooke, dmt, roll_sds, twist_sds;
     double dt;
     double* hist;
     uint64_t* cmask;
@@ -496,6 +501,291 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
     store_force<R>(a, i, fx, fy, fz, tx, ty, tz);
 }
 
+// The contact model family on the same rows: Hertz-Mindlin with JKR / DMT
+// adhesion or SJKR cohesion (contact.rs:103-615) or, with the Hooke model,
+// hooke_contact_force (contact.rs:622-990); constant or SDS rolling resistance
+// and twisting friction (contact.rs:427-557) with their history slots (spring
+// 0-2, rolling 3-5, twist 6, canonical sign). Every pair quantity is FP64;
+// only the sums are kept in R. Pairs are evaluated in the owner frame as in
+// k_hertz, so both rows of a pair carry the same history.
+template <typename R, int LG>
+__global__ void __launch_bounds__(256) k_contact_ext(ForceArgs a) {
+    constexpr int G = 1 << LG;
+    const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const int64_t i = gt >> LG;
+    const int t = int(gt & (G - 1));
+    if (i >= a.n) return;
+    if (gated(a)) return;
+    const double PI = 3.141592653589793;
+    const double4 pi = a.pos[i];
+    const double4 vi = a.vel[i];
+    R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
+    const uint32_t ti = a.type[i];
+    const double Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
+    if (a.use_pairs) {
+        const double4 wi = a.omega[i];
+        const uint32_t tagi = a.tag[i];
+        int cnt = int(a.nnb[i]);
+        if (cnt > a.maxnb) cnt = a.maxnb;
+        const uint64_t mask_in = a.cmask[i];
+        uint64_t mask_out = 0ull;
+        const int nm = a.nm, nn = nm * nm;
+        const double dt = a.dt;
+        const int64_t cs = a.hist_cs;
+        for (int k = t; k < cnt; k += G) {
+            const int64_t slot = row_addr(i, k, a.npad, LG);
+            const uint32_t e = a.nbr[slot];
+            const uint32_t j = entry_idx(e);
+            const bool rev = entry_rev(e);
+            const double4 pj = a.pos[j];
+            double shx, shy, shz;
+            code_shifts(entry_code(e), Lx, Ly, Lz, shx, shy, shz);
+            double dx, dy, dz, r1, r2;
+            if (rev) {
+                dx = (pi.x + shx) - pj.x;
+                dy = (pi.y + shy) - pj.y;
+                dz = (pi.z + shz) - pj.z;
+                r1 = pj.w;
+                r2 = pi.w;
+            } else {
+                dx = (pj.x + shx) - pi.x;
+                dy = (pj.y + shy) - pi.y;
+                dz = (pj.z + shz) - pi.z;
+                r1 = pi.w;
+                r2 = pj.w;
+            }
+            const double dist_sq = dx * dx + dy * dy + dz * dz;
+            const double sum_r = r1 + r2;
+            const uint32_t tj = a.type[j];
+            const int mO = int(rev ? tj : ti), mP = int(rev ? ti : tj);
+            const int mij = mO * nm + mP;
+            const double r_eff = (r1 * r2) / sum_r;
+            const double e_eff = a.mat[2 * nn + mij];
+            const double surface = a.hooke ? 0.0 : a.mext[kXSurface * nn + mij];
+            double d_po = 0.0;  // JKR pull-off distance (contact.rs:163-171)
+            if (surface > 0.0 && !a.dmt)
+                d_po = cbrt(PI * PI * surface * surface * r_eff / (4.0 * e_eff * e_eff));
+            const double reach = sum_r + d_po;
+            if (dist_sq >= reach * reach) continue;
+            const double distance = sqrt(dist_sq);
+            if (distance == 0.0) continue;
+            const double r_min = r1 < r2 ? r1 : r2;
+            double delta = sum_r - distance;
+            if (0.5 * r_min < delta) delta = 0.5 * r_min;
+            if (a.hooke && delta <= 0.0) continue;
+            if (delta > 0.0 && !rev && distance / sum_r < kLargeOverlap) {
+                int c = atomicAdd(&a.st->overlap_count, 1) + 1;
+                if (c > kMaxOverlapWarnings) {
+                    atomicCAS(&a.st->error, 0, -3);
+                    if (a.step >= 0) atomicMin(&a.st->abort_step, a.step);
+                }
+            }
+            if (delta <= 0.0 && surface <= 0.0) continue;
+            const double4 vj = a.vel[j];
+            const double4 wj = a.omega[j];
+            const uint32_t tagj = a.tag[j];
+            const double4& vO = rev ? vj : vi;
+            const double4& vP = rev ? vi : vj;
+            const double4& wO = rev ? wj : wi;
+            const double4& wP = rev ? wi : wj;
+            const uint32_t tagO = rev ? tagj : tagi;
+            const uint32_t tagP = rev ? tagi : tagj;
+            const double inv_dist = 1.0 / distance;
+            const double nx = dx * inv_dist, ny = dy * inv_dist, nz = dz * inv_dist;
+            const double m_r = 1.0 / (vO.w + vP.w);
+            const double beta = a.mat[mij];
+            const double mu = a.mat[nn + mij];
+            const double cohesion = a.mext[kXCohesion * nn + mij];
+            const bool adhesion_only = surface > 0.0 && !a.dmt && delta <= 0.0;
+            double s_n = 0.0, k_n = 0.0, k_t = 0.0, gamma_n = 0.0;
+            if (a.hooke) {
+                k_n = a.mext[kXKn * nn + mij];
+                k_t = a.mext[kXKt * nn + mij];
+                gamma_n = 2.0 * beta * sqrt(k_n * m_r);
+            } else if (delta > 0.0) {
+                const double sdr = sqrt(delta * r_eff);
+                s_n = 2.0 * e_eff * sdr;
+                k_n = 4.0 / 3.0 * e_eff * sdr;
+                k_t = 8.0 * a.mat[3 * nn + mij] * sdr;
+            }
+            const double r1nx = r1 * nx, r1ny = r1 * ny, r1nz = r1 * nz;
+            const double vcix = vO.x + (wO.y * r1nz - wO.z * r1ny);
+            const double vciy = vO.y + (wO.z * r1nx - wO.x * r1nz);
+            const double vciz = vO.z + (wO.x * r1ny - wO.y * r1nx);
+            const double r2nx = r2 * nx, r2ny = r2 * ny, r2nz = r2 * nz;
+            const double vcjx = vP.x + (-wP.y * r2nz + wP.z * r2ny);
+            const double vcjy = vP.y + (-wP.z * r2nx + wP.x * r2nz);
+            const double vcjz = vP.z + (-wP.x * r2ny + wP.y * r2nx);
+            const double vrx = vcjx - vcix, vry = vcjy - vciy, vrz = vcjz - vciz;
+            const double v_n = vrx * nx + vry * ny + vrz * nz;
+            double f_n;
+            if (a.hooke) {
+                f_n = k_n * delta - gamma_n * v_n;
+                if (cohesion > 0.0)
+                    f_n -= cohesion * PI * delta * r_eff;
+                else if (!(f_n > 0.0))
+                    f_n = 0.0;
+            } else {
+                const double f_diss = 2.0 * beta * kSqrt53 * sqrt(s_n * m_r) * v_n;
+                if (surface > 0.0 && a.dmt) {
+                    f_n = k_n * delta - f_diss - 2.0 * PI * surface * r_eff;
+                } else if (surface > 0.0) {
+                    const double f_adh = 1.5 * PI * surface * r_eff;
+                    f_n = adhesion_only ? -f_adh : k_n * delta - f_diss - f_adh;
+                } else if (cohesion > 0.0) {
+                    f_n = k_n * delta - f_diss - cohesion * PI * delta * r_eff;
+                } else {
+                    f_n = k_n * delta - f_diss;
+                    if (!(f_n > 0.0)) f_n = 0.0;
+                }
+            }
+            // owner: -F_n, partner: +F_n
+            const double sn = rev ? 1.0 : -1.0;
+            fx += R(sn * (f_n * nx));
+            fy += R(sn * (f_n * ny));
+            fz += R(sn * (f_n * nz));
+            if (adhesion_only) continue;  // no tangential part, no history
+            const double vtx = vrx - v_n * nx, vty = vry - v_n * ny, vtz = vrz - v_n * nz;
+            const double sign = tagO < tagP ? 1.0 : -1.0;
+            double h[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if ((mask_in >> k) & 1ull) {
+#pragma unroll
+                for (int q = 0; q < 7; q++) h[q] = sign * a.hist[q * cs + slot];
+            }
+            double sx = h[0], sy = h[1], sz = h[2];
+            const double s_dot_n = sx * nx + sy * ny + sz * nz;
+            sx -= s_dot_n * nx;
+            sy -= s_dot_n * ny;
+            sz -= s_dot_n * nz;
+            sx += vtx * dt;
+            sy += vty * dt;
+            sz += vtz * dt;
+            const double s_mag = sqrt(sx * sx + sy * sy + sz * sz);
+            const double f_t_spring = k_t * s_mag;
+            const double f_t_max = mu * fabs(f_n);
+            if (f_t_spring > f_t_max && f_t_spring > kTangEps) {
+                const double sc = f_t_max / f_t_spring;
+                sx *= sc;
+                sy *= sc;
+                sz *= sc;
+            }
+            const double gamma_t = 2.0 * kSqrt53 * beta * sqrt(k_t * m_r);
+            double ftx = k_t * sx - gamma_t * vtx;
+            double fty = k_t * sy - gamma_t * vty;
+            double ftz = k_t * sz - gamma_t * vtz;
+            const double f_t_mag = sqrt(ftx * ftx + fty * fty + ftz * ftz);
+            if (f_t_mag > f_t_max && f_t_mag > kTangEps) {
+                const double sc = f_t_max / f_t_mag;
+                ftx *= sc;
+                fty *= sc;
+                ftz *= sc;
+            }
+            // owner: +F_t, torque (r1 n) x F_t; partner: -F_t, torque (r2 n) x F_t
+            const double rnx = rev ? r2nx : r1nx, rny = rev ? r2ny : r1ny, rnz = rev ? r2nz : r1nz;
+            fx += R(-sn * ftx);
+            fy += R(-sn * fty);
+            fz += R(-sn * ftz);
+            double ttx = rny * ftz - rnz * fty;
+            double tty = rnz * ftx - rnx * ftz;
+            double ttz = rnx * fty - rny * ftx;
+            // rolling resistance and twisting friction (contact.rs:427-557):
+            // +tau on the owner, -tau on the partner
+            const double orx = wO.x - wP.x, ory = wO.y - wP.y, orz = wO.z - wP.z;
+            const double or_n = orx * nx + ory * ny + orz * nz;
+            const double rlx = orx - or_n * nx, rly = ory - or_n * ny, rlz = orz - or_n * nz;
+            double rdx = h[3], rdy = h[4], rdz = h[5], tw = h[6];
+            const double mu_r = a.mext[kXRolling * nn + mij];
+            const double mu_tw = a.mext[kXTwisting * nn + mij];
+            const double st = rev ? -1.0 : 1.0;
+            if (mu_r > 0.0) {
+                const double roll_mag = sqrt(rlx * rlx + rly * rly + rlz * rlz);
+                const double tau_max = mu_r * fabs(f_n) * r_eff;
+                if (a.roll_sds) {
+                    const double k_roll = a.mext[kXRollK * nn + mij];
+                    const double g_roll = a.mext[kXRollG * nn + mij];
+                    const double rd_n = rdx * nx + rdy * ny + rdz * nz;
+                    rdx -= rd_n * nx;
+                    rdy -= rd_n * ny;
+                    rdz -= rd_n * nz;
+                    rdx += rlx * dt;
+                    rdy += rly * dt;
+                    rdz += rlz * dt;
+                    double trx = -k_roll * rdx - g_roll * rlx;
+                    double try_ = -k_roll * rdy - g_roll * rly;
+                    double trz = -k_roll * rdz - g_roll * rlz;
+                    const double tr_mag = sqrt(trx * trx + try_ * try_ + trz * trz);
+                    if (tr_mag > tau_max && tr_mag > kTangEps) {
+                        const double sc = tau_max / tr_mag;
+                        trx *= sc;
+                        try_ *= sc;
+                        trz *= sc;
+                        if (k_roll > kTangEps) {
+                            rdx = (trx + g_roll * rlx) / (-k_roll);
+                            rdy = (try_ + g_roll * rly) / (-k_roll);
+                            rdz = (trz + g_roll * rlz) / (-k_roll);
+                        }
+                    }
+                    ttx += st * trx;
+                    tty += st * try_;
+                    ttz += st * trz;
+                } else if (roll_mag > 1e-30) {
+                    const double inv_roll = tau_max / roll_mag;
+                    ttx += st * (-inv_roll * rlx);
+                    tty += st * (-inv_roll * rly);
+                    ttz += st * (-inv_roll * rlz);
+                }
+            }
+            if (mu_tw > 0.0) {
+                const double tau_max = mu_tw * fabs(f_n) * r_eff;
+                double tau = 0.0;
+                if (a.twist_sds) {
+                    const double k_tw = a.mext[kXTwistK * nn + mij];
+                    const double g_tw = a.mext[kXTwistG * nn + mij];
+                    tw += or_n * dt;
+                    tau = -k_tw * tw - g_tw * or_n;
+                    if (fabs(tau) > tau_max) {
+                        tau = (signbit(tau) ? -1.0 : 1.0) * tau_max;
+                        if (k_tw > kTangEps) tw = (tau + g_tw * or_n) / (-k_tw);
+                    }
+                } else if (fabs(or_n) > 1e-30) {
+                    tau = (or_n > 0.0 ? -1.0 : 1.0) * tau_max;
+                }
+                ttx += st * (tau * nx);
+                tty += st * (tau * ny);
+                ttz += st * (tau * nz);
+            }
+            tx += R(ttx);
+            ty += R(tty);
+            tz += R(ttz);
+            a.hist[slot] = sign * sx;
+            a.hist[cs + slot] = sign * sy;
+            a.hist[2 * cs + slot] = sign * sz;
+            a.hist[3 * cs + slot] = sign * rdx;
+            a.hist[4 * cs + slot] = sign * rdy;
+            a.hist[5 * cs + slot] = sign * rdz;
+            a.hist[6 * cs + slot] = sign * tw;
+            mask_out |= 1ull << k;
+        }
+        fx = group_sum<G>(fx);
+        fy = group_sum<G>(fy);
+        fz = group_sum<G>(fz);
+        tx = group_sum<G>(tx);
+        ty = group_sum<G>(ty);
+        tz = group_sum<G>(tz);
+        mask_out = group_or<G>(mask_out);
+        if (t == 0) a.cmask[i] = mask_out;
+    }
+    if (t != 0) return;
+    if (a.use_walls) wall_forces<R>(a, pi, vi, a.mass[i], ti, fx, fy, fz);
+    if (a.use_gravity) {
+        const double m = a.mass[i];
+        fx += R(m * a.g[0]);
+        fy += R(m * a.g[1]);
+        fz += R(m * a.g[2]);
+    }
+    store_force<R>(a, i, fx, fy, fz, tx, ty, tz);
+}
+
 // standalone wall / gravity system: adds to the current forces
 template <typename R>
 __global__ void k_walls_gravity(ForceArgs a) {
@@ -543,6 +833,11 @@ ForceArgs make_args(Ctx& c, int step) {
     a.e6 = c.lj_e6;
     a.mat = c.mat_dev.p;
     a.nm = c.mat.n;
+    a.mext = c.ext_on ? c.mext_dev.p : nullptr;
+    a.hooke = c.cm_hooke;
+    a.dmt = c.cm_dmt;
+    a.roll_sds = c.cm_roll_sds;
+    a.twist_sds = c.cm_twist_sds;
     a.dt = c.dt;
     a.hist = c.hist.p;
     a.cmask = c.cmask.p;
@@ -580,6 +875,15 @@ template <typename R>
 void launch_hertz(Ctx& c, const ForceArgs& a) {
     const int64_t threads = c.n << c.list_lg;
     const int nb = blocks_for(threads, 256);
+    if (c.ext_on) {
+        switch (c.list_lg) {
+            case 0: k_contact_ext<R, 0><<<nb, 256, 0, c.stream>>>(a); break;
+            case 1: k_contact_ext<R, 1><<<nb, 256, 0, c.stream>>>(a); break;
+            case 3: k_contact_ext<R, 3><<<nb, 256, 0, c.stream>>>(a); break;
+            default: k_contact_ext<R, 2><<<nb, 256, 0, c.stream>>>(a); break;
+        }
+        return;
+    }
     switch (c.list_lg) {
         case 0: k_hertz<R, 0><<<nb, 256, 0, c.stream>>>(a); break;
         case 1: k_hertz<R, 1><<<nb, 256, 0, c.stream>>>(a); break;

@@ -781,7 +781,7 @@ __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
                                 const double* __restrict__ hist_old,
                                 const uint64_t* __restrict__ cmask_old,
                                 double* __restrict__ hist, uint64_t* __restrict__ cmask,
-                                int have_old) {
+                                int have_old, int nslot) {
     int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (i >= n) return;
     const int64_t cs = int64_t(maxnb) * npad;
@@ -801,9 +801,7 @@ __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
                 if (!((om >> ko) & 1ull)) continue;
                 const int64_t so = row_addr(oi, ko, npad_old, lg_old);
                 if (tag_old[entry_idx(nbr_old[so])] == tp) {
-                    hist[s] = hist_old[so];
-                    hist[cs + s] = hist_old[cso + so];
-                    hist[2 * cs + s] = hist_old[2 * cso + so];
+                    for (int q = 0; q < nslot; q++) hist[q * cs + s] = hist_old[q * cso + so];
                     nm |= 1ull << k;
                     break;
                 }
@@ -1004,7 +1002,7 @@ void rebuild_attempt(Ctx& c) {
     if (dem) {
         if (c.maxnb > 64)
             throw ApiError(-5, "DEM neighbor rows above 64 entries are not supported");
-        c.hist.resize(size_t(3) * c.maxnb * c.npad);
+        c.hist.resize(size_t(c.hist_slots) * c.maxnb * c.npad);
         c.cmask.resize(c.npad);
     }
     uint32_t* ng_live = c.ghost_key.p + c.ghost_cap;
@@ -1091,7 +1089,7 @@ void rebuild_attempt(Ctx& c) {
         launch(c, k_remap_history, blocks_for(n, 256), 256,
             n, c.npad, c.maxnb, c.lg, c.nbr.p, c.nnb.p, c.tag.p, c.order.p, c.npad_old,
             c.maxnb_old, c.lg_old, c.nbr_old.p, c.nnb_old.p, c.tag2.p, c.hist_old.p,
-            c.cmask_old.p, c.hist.p, c.cmask.p, c.remap_have_old ? 1 : 0);
+            c.cmask_old.p, c.hist.p, c.cmask.p, c.remap_have_old ? 1 : 0, c.hist_slots);
         MDD_HIP(hipGetLastError());
     }
     c.list_valid = true;

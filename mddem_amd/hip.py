@@ -18,6 +18,10 @@ LIB_PATH = os.path.join(HERE, "libmddem_hip.so")
 
 MODEL_LJ, MODEL_DEM = 0, 1
 FP64, MIXED = 0, 1
+# optional per-material contact properties, in the order mdd_set_material_extras takes
+MATERIAL_EXTRAS = ("rolling_friction", "twisting_friction", "cohesion_energy", "surface_energy",
+                   "kn", "kt", "rolling_stiffness", "rolling_damping", "twisting_stiffness",
+                   "twisting_damping")
 ERRORS = {-1: "invalid", -2: "hip", -3: "overlap", -4: "capacity", -5: "unsupported",
           -6: "no gfx950 device"}
 
@@ -67,6 +71,8 @@ SIGNATURES = {
     "mdd_set_rebuild_on_box_exit": (C.c_int, [_vp, C.c_int]),
     "mdd_set_dt": (C.c_int, [_vp, C.c_double]),
     "mdd_set_materials": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
+    "mdd_set_material_extras": (C.c_int, [_vp, C.c_int, _dp]),
+    "mdd_set_contact_models": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mdd_set_lj": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
     "mdd_set_nose_hoover": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double]),
     "mdd_set_integrators": (C.c_int, [_vp, C.c_int, C.c_int]),
@@ -96,6 +102,7 @@ SIGNATURES = {
     "mdd_nh_post_final": (C.c_int, [_vp]),
     "mdd_run": (C.c_int, [_vp, C.c_int64]),
     "mdd_download_history": (C.c_int, [_vp, C.c_int64, _u32p, _u32p, _dp, _i64p]),
+    "mdd_download_history7": (C.c_int, [_vp, C.c_int64, _u32p, _u32p, _dp, _i64p]),
     "mdd_compute_thermo": (C.c_int, [_vp, C.c_int, C.POINTER(Thermo)]),
     "mdd_synchronize": (C.c_int, [_vp]),
     "mdd_set_profiling": (C.c_int, [_vp, C.c_int]),
@@ -146,6 +153,7 @@ class System:
         self.L = lib()
         self.model = model
         self.precision = precision
+        self.nmat = 0
         h = _vp()
         rc = self.L.mdd_create(C.byref(h), C.byref(Params(device, model, precision, capacity)))
         if rc != 0:
@@ -187,6 +195,20 @@ class System:
     def set_materials(self, youngs, poisson, restitution, friction):
         self._c(self.L.mdd_set_materials(self.h, len(youngs), _p(_d(youngs)), _p(_d(poisson)),
                                          _p(_d(restitution)), _p(_d(friction))))
+        self.nmat = len(youngs)
+
+    def set_material_extras(self, **props):
+        """Optional per-material contact properties by name (MATERIAL_EXTRAS),
+        one value per material; those not given are zero."""
+        n = self.nmat
+        vals = np.zeros((len(MATERIAL_EXTRAS), n))
+        for k, v in props.items():
+            vals[MATERIAL_EXTRAS.index(k)] = v
+        self._c(self.L.mdd_set_material_extras(self.h, n, _p(_d(vals))))
+
+    def set_contact_models(self, hooke=False, dmt=False, rolling_sds=False, twisting_sds=False):
+        self._c(self.L.mdd_set_contact_models(self.h, int(hooke), int(dmt), int(rolling_sds),
+                                              int(twisting_sds)))
 
     def set_lj(self, epsilon=1.0, sigma=1.0, cutoff=2.5):
         self._c(self.L.mdd_set_lj(self.h, epsilon, sigma, cutoff))
@@ -327,6 +349,18 @@ class System:
         s = np.zeros((m, 3))
         self._c(self.L.mdd_download_history(self.h, m, _p(a, _u32p), _p(b, _u32p), _p(s),
                                             C.byref(n)))
+        return a, b, s
+
+    def history7(self):
+        """History with all 7 values per contact: spring, rolling, twist."""
+        n = C.c_int64(0)
+        self._c(self.L.mdd_download_history7(self.h, 0, None, None, None, C.byref(n)))
+        m = n.value
+        a = np.zeros(m, np.uint32)
+        b = np.zeros(m, np.uint32)
+        s = np.zeros((m, 7))
+        self._c(self.L.mdd_download_history7(self.h, m, _p(a, _u32p), _p(b, _u32p), _p(s),
+                                             C.byref(n)))
         return a, b, s
 
     def lj_force(self):

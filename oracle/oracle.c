@@ -22,9 +22,13 @@
 #define LARGE_OVERLAP_WARN_THRESHOLD 0.90
 #define MAX_OVERLAP_WARNINGS 500
 
+/* per-contact history: tangential spring (3), rolling displacement (3) and
+ * twist angle (1), stored in the lower tag's frame (tangential.rs:45-53) */
+#define HSLOTS 7
+
 typedef struct {
     uint32_t tag;
-    double s[3];
+    double s[HSLOTS];
     int active;
 } hist_entry;
 
@@ -35,7 +39,7 @@ typedef struct {
 
 typedef struct {
     uint64_t key;
-    double s[3];
+    double s[HSLOTS];
     int used, active;
 } cslot;
 
@@ -107,6 +111,13 @@ struct orc_sys {
     double* friction_ij;
     double* e_eff_ij;
     double* g_eff_ij;
+    /* optional per-material properties and their pair tables, in the order
+     * of ORC_X_* (zero unless set) */
+    double extra[ORC_NX][ORC_MAX_MAT];
+    double* extra_ij[ORC_NX];
+    /* contact_model "hooke", adhesion_model "dmt", rolling_model "sds",
+     * twisting_model "sds" (dem_atom/src/lib.rs:142-157) */
+    int hooke, dmt, roll_sds, twist_sds;
     int nwalls;
     wall_t walls[ORC_MAX_WALL];
     int gravity_on;
@@ -173,15 +184,14 @@ static void hist_copy(hist_list* dst, const hist_list* src) {
     dst->n = src->n;
 }
 
-static void hist_push(hist_list* h, uint32_t tag, const double s[3], int active) {
+/* s holds ns values; the remaining slots start at zero */
+static void hist_push(hist_list* h, uint32_t tag, const double* s, int ns, int active) {
     if (h->n == h->cap) {
         h->cap = h->cap ? 2 * h->cap : 4;
         h->e = xrealloc(h->e, sizeof(hist_entry) * h->cap);
     }
     h->e[h->n].tag = tag;
-    h->e[h->n].s[0] = s[0];
-    h->e[h->n].s[1] = s[1];
-    h->e[h->n].s[2] = s[2];
+    for (int k = 0; k < HSLOTS; k++) h->e[h->n].s[k] = k < ns ? s[k] : 0.0;
     h->e[h->n].active = active;
     h->n++;
 }
@@ -246,7 +256,7 @@ static cslot* cmap_find(cmap* m, uint64_t key, int insert) {
             if (!insert) return NULL;
             sl->used = 1;
             sl->key = key;
-            sl->s[0] = sl->s[1] = sl->s[2] = 0.0;
+            for (int k = 0; k < HSLOTS; k++) sl->s[k] = 0.0;
             sl->active = 0;
             m->count++;
             return sl;
@@ -311,6 +321,7 @@ void orc_destroy(orc_sys* s) {
     free(s->sorted_atoms); free(s->atom_sorted_idx); free(s->sorted_pos);
     for (int k = 0; k < 6; k++) free(s->swaps[k].send);
     free(s->beta_ij); free(s->friction_ij); free(s->e_eff_ij); free(s->g_eff_ij);
+    for (int x = 0; x < ORC_NX; x++) free(s->extra_ij[x]);
     free(s->cm.slots);
     free(s);
 }
@@ -340,6 +351,36 @@ void orc_set_neighbor(orc_sys* s, double skin_fraction, double bin_size, int eve
 
 void orc_set_dt(orc_sys* s, double dt) { s->dt = dt; }
 
+static double gmean(double a, double b) { return sqrt(a * b); }
+static double gmean0(double a, double b) { return sqrt((a > 0.0 ? a : 0.0) * (b > 0.0 ? b : 0.0)); }
+/* harmonic mean when both are set; `either` keeps the one that is set */
+static double hmean(double a, double b, int either) {
+    if (a > 0.0 && b > 0.0) return 2.0 * a * b / (a + b);
+    if (!either) return 0.0;
+    return a > 0.0 ? a : b;
+}
+
+/* MaterialTable::build_pair_tables, optional fields (dem_atom/src/lib.rs:425-510) */
+static void build_extra_tables(orc_sys* s) {
+    int n = s->nmat;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            const int k = i * n + j;
+#define X(w) s->extra[w][i], s->extra[w][j]
+            s->extra_ij[ORC_X_ROLLING_FRICTION][k] = gmean(X(ORC_X_ROLLING_FRICTION));
+            s->extra_ij[ORC_X_COHESION_ENERGY][k] = gmean(X(ORC_X_COHESION_ENERGY));
+            s->extra_ij[ORC_X_SURFACE_ENERGY][k] = gmean(X(ORC_X_SURFACE_ENERGY));
+            s->extra_ij[ORC_X_TWISTING_FRICTION][k] = gmean0(X(ORC_X_TWISTING_FRICTION));
+            s->extra_ij[ORC_X_KN][k] = hmean(X(ORC_X_KN), 0);
+            s->extra_ij[ORC_X_KT][k] = hmean(X(ORC_X_KT), 0);
+            s->extra_ij[ORC_X_ROLLING_STIFFNESS][k] = hmean(X(ORC_X_ROLLING_STIFFNESS), 1);
+            s->extra_ij[ORC_X_ROLLING_DAMPING][k] = gmean0(X(ORC_X_ROLLING_DAMPING));
+            s->extra_ij[ORC_X_TWISTING_STIFFNESS][k] = hmean(X(ORC_X_TWISTING_STIFFNESS), 1);
+            s->extra_ij[ORC_X_TWISTING_DAMPING][k] = gmean0(X(ORC_X_TWISTING_DAMPING));
+#undef X
+        }
+}
+
 int orc_set_materials(orc_sys* s, int n, const double* E, const double* nu, const double* e,
                       const double* mu) {
     if (n <= 0 || n > ORC_MAX_MAT) {
@@ -354,6 +395,10 @@ int orc_set_materials(orc_sys* s, int n, const double* E, const double* nu, cons
         s->friction[i] = mu[i];
     }
     free(s->beta_ij); free(s->friction_ij); free(s->e_eff_ij); free(s->g_eff_ij);
+    for (int x = 0; x < ORC_NX; x++) {
+        free(s->extra_ij[x]);
+        s->extra_ij[x] = xrealloc(NULL, sizeof(double) * n * n);
+    }
     s->beta_ij = xrealloc(NULL, sizeof(double) * n * n);
     s->friction_ij = xrealloc(NULL, sizeof(double) * n * n);
     s->e_eff_ij = xrealloc(NULL, sizeof(double) * n * n);
@@ -372,7 +417,32 @@ int orc_set_materials(orc_sys* s, int n, const double* E, const double* nu, cons
             s->g_eff_ij[i * n + j] = 1.0 / (2.0 * (2.0 - nu_i) * (1.0 + nu_i) / E[i] +
                                             2.0 * (2.0 - nu_j) * (1.0 + nu_j) / E[j]);
         }
+    build_extra_tables(s);
     return 0;
+}
+
+int orc_set_material_extras(orc_sys* s, int n, const double* values) {
+    if (n != s->nmat) {
+        set_err(s, "material extras need one value per material");
+        return -1;
+    }
+    for (int x = 0; x < ORC_NX; x++)
+        for (int i = 0; i < n; i++) s->extra[x][i] = values[x * n + i];
+    build_extra_tables(s);
+    return 0;
+}
+
+int orc_get_extra_table(const orc_sys* s, int which, double* out) {
+    if (which < 0 || which >= ORC_NX || !s->extra_ij[which]) return -1;
+    memcpy(out, s->extra_ij[which], sizeof(double) * s->nmat * s->nmat);
+    return s->nmat;
+}
+
+void orc_set_contact_models(orc_sys* s, int hooke, int dmt, int rolling_sds, int twisting_sds) {
+    s->hooke = hooke != 0;
+    s->dmt = dmt != 0;
+    s->roll_sds = rolling_sds != 0;
+    s->twist_sds = twisting_sds != 0;
 }
 
 int orc_get_material_tables(const orc_sys* s, double* beta, double* friction, double* e_eff,
@@ -1077,13 +1147,116 @@ int64_t orc_get_pairs(const orc_sys* s, uint32_t* owner_tag, uint32_t* partner_t
 
 static double fmax0(double a) { return a > 0.0 ? a : 0.0; }
 
+/* rolling resistance and twisting friction of one contact (contact.rs:427-557,
+ * the same in the Hooke kernel 839-950); torques in the pair's frame are
+ * added to i and subtracted from j; rd / tw are the rolling displacement and
+ * twist angle in the local (i, j) frame, updated in place */
+static void roll_twist(orc_sys* s, int i, int j, int k, double nx, double ny, double nz,
+                       double f_n_mag, double r_eff, double* rd, double* tw) {
+    const double dt = s->dt;
+    double oix = s->omega[i][0], oiy = s->omega[i][1], oiz = s->omega[i][2];
+    double ojx = s->omega[j][0], ojy = s->omega[j][1], ojz = s->omega[j][2];
+    double orx = oix - ojx, ory = oiy - ojy, orz = oiz - ojz;
+    double or_dot_n = orx * nx + ory * ny + orz * nz;
+    double rollx = orx - or_dot_n * nx, rolly = ory - or_dot_n * ny, rollz = orz - or_dot_n * nz;
+    double mu_r = s->extra_ij[ORC_X_ROLLING_FRICTION][k];
+    double mu_tw = s->extra_ij[ORC_X_TWISTING_FRICTION][k];
+    if (mu_r > 0.0) {
+        double roll_mag = sqrt(rollx * rollx + rolly * rolly + rollz * rollz);
+        double trx = 0.0, try_ = 0.0, trz = 0.0;
+        int apply = 0;
+        if (s->roll_sds) {
+            double k_roll = s->extra_ij[ORC_X_ROLLING_STIFFNESS][k];
+            double g_roll = s->extra_ij[ORC_X_ROLLING_DAMPING][k];
+            double rd_dot_n = rd[0] * nx + rd[1] * ny + rd[2] * nz;
+            rd[0] -= rd_dot_n * nx;
+            rd[1] -= rd_dot_n * ny;
+            rd[2] -= rd_dot_n * nz;
+            rd[0] += rollx * dt;
+            rd[1] += rolly * dt;
+            rd[2] += rollz * dt;
+            trx = -k_roll * rd[0] - g_roll * rollx;
+            try_ = -k_roll * rd[1] - g_roll * rolly;
+            trz = -k_roll * rd[2] - g_roll * rollz;
+            double tr_mag = sqrt(trx * trx + try_ * try_ + trz * trz);
+            double tau_max = mu_r * fabs(f_n_mag) * r_eff;
+            if (tr_mag > tau_max && tr_mag > TANGENTIAL_EPSILON) {
+                double sc = tau_max / tr_mag;
+                trx *= sc;
+                try_ *= sc;
+                trz *= sc;
+                if (k_roll > TANGENTIAL_EPSILON) {
+                    rd[0] = (trx + g_roll * rollx) / (-k_roll);
+                    rd[1] = (try_ + g_roll * rolly) / (-k_roll);
+                    rd[2] = (trz + g_roll * rollz) / (-k_roll);
+                }
+            }
+            apply = 1;
+        } else if (roll_mag > 1e-30) {
+            double tau_mag = mu_r * fabs(f_n_mag) * r_eff;
+            double inv_roll = tau_mag / roll_mag;
+            trx = -inv_roll * rollx;
+            try_ = -inv_roll * rolly;
+            trz = -inv_roll * rollz;
+            apply = 1;
+        }
+        if (apply) {
+            s->torque[i][0] += trx;
+            s->torque[i][1] += try_;
+            s->torque[i][2] += trz;
+            s->torque[j][0] -= trx;
+            s->torque[j][1] -= try_;
+            s->torque[j][2] -= trz;
+        }
+    }
+    if (mu_tw > 0.0) {
+        double twist_vel = or_dot_n;
+        double ttx = 0.0, tty = 0.0, ttz = 0.0;
+        int apply = 0;
+        if (s->twist_sds) {
+            double k_tw = s->extra_ij[ORC_X_TWISTING_STIFFNESS][k];
+            double g_tw = s->extra_ij[ORC_X_TWISTING_DAMPING][k];
+            *tw += twist_vel * dt;
+            double tau_twist = -k_tw * *tw - g_tw * twist_vel;
+            double tau_max = mu_tw * fabs(f_n_mag) * r_eff;
+            if (fabs(tau_twist) > tau_max) {
+                /* f64::signum: +1 for +0.0, -1 for -0.0 */
+                tau_twist = (signbit(tau_twist) ? -1.0 : 1.0) * tau_max;
+                if (k_tw > TANGENTIAL_EPSILON) *tw = (tau_twist + g_tw * twist_vel) / (-k_tw);
+            }
+            ttx = tau_twist * nx;
+            tty = tau_twist * ny;
+            ttz = tau_twist * nz;
+            apply = 1;
+        } else if (fabs(twist_vel) > 1e-30) {
+            double tau = mu_tw * fabs(f_n_mag) * r_eff;
+            double sign_tw = twist_vel > 0.0 ? -1.0 : 1.0;
+            ttx = sign_tw * tau * nx;
+            tty = sign_tw * tau * ny;
+            ttz = sign_tw * tau * nz;
+            apply = 1;
+        }
+        if (apply) {
+            s->torque[i][0] += ttx;
+            s->torque[i][1] += tty;
+            s->torque[i][2] += ttz;
+            s->torque[j][0] -= ttx;
+            s->torque[j][1] -= tty;
+            s->torque[j][2] -= ttz;
+        }
+    }
+}
+
 int orc_hertz_mindlin_contact_force(orc_sys* s) {
-    /* hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615),
-     * surface_energy = cohesion = mu_r = mu_tw = 0, newton = true */
+    /* hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615) or,
+     * with contact_model "hooke", hooke_contact_force (contact.rs:622-990);
+     * newton = true, no bonds or rigid bodies */
+    const double PI = 3.141592653589793;
     int nlocal = s->nlocal;
     int nm = s->nmat;
     double dt = s->dt;
     int warnings = 0;
+    const int hooke = s->hooke;
     if (s->history_mode == ORC_HIST_REFERENCE) {
         for (int i = 0; i < nlocal; i++)
             for (int k = 0; k < s->hist[i].n; k++) s->hist[i].e[k].active = 0;
@@ -1100,15 +1273,24 @@ int orc_hertz_mindlin_contact_force(orc_sys* s) {
             double dist_sq = dx * dx + dy * dy + dz * dz;
             double sum_r = r1 + r2;
             int mi = (int)s->type[i], mj = (int)s->type[j];
+            const int kij = mi * nm + mj;
             double r_eff = (r1 * r2) / sum_r;
-            double e_eff = s->e_eff_ij[mi * nm + mj];
-            double interaction_r = sum_r + 0.0;
+            double e_eff = s->e_eff_ij[kij];
+            double surface_energy = hooke ? 0.0 : s->extra_ij[ORC_X_SURFACE_ENERGY][kij];
+            int use_dmt = s->dmt;
+            /* JKR pull-off distance (contact.rs:163-171) */
+            double delta_pulloff = 0.0;
+            if (surface_energy > 0.0 && !use_dmt)
+                delta_pulloff = cbrt(PI * PI * surface_energy * surface_energy * r_eff /
+                                     (4.0 * e_eff * e_eff));
+            double interaction_r = sum_r + delta_pulloff;
             if (dist_sq >= interaction_r * interaction_r) continue;
             double distance = sqrt(dist_sq);
             if (distance == 0.0) continue;
             double r_min = r1 < r2 ? r1 : r2;
             double delta = sum_r - distance;
             if (0.5 * r_min < delta) delta = 0.5 * r_min;
+            if (hooke && delta <= 0.0) continue;
             if (delta > 0.0 && distance / sum_r < LARGE_OVERLAP_WARN_THRESHOLD) {
                 warnings++;
                 s->overlap_total++;
@@ -1117,19 +1299,28 @@ int orc_hertz_mindlin_contact_force(orc_sys* s) {
                     return -2;
                 }
             }
-            if (delta <= 0.0) continue;
+            if (delta <= 0.0 && surface_energy <= 0.0) continue;
             double inv_dist = 1.0 / distance;
             double nx = dx * inv_dist, ny = dy * inv_dist, nz = dz * inv_dist;
-            double g_eff = s->g_eff_ij[mi * nm + mj];
+            double g_eff = s->g_eff_ij[kij];
             double inv_m_i = s->inv_mass[i] > 0.0 ? s->inv_mass[i] : 1.0 / s->mass[i];
             double inv_m_j = s->inv_mass[j] > 0.0 ? s->inv_mass[j] : 1.0 / s->mass[j];
             double m_r = 1.0 / (inv_m_i + inv_m_j);
-            double beta = s->beta_ij[mi * nm + mj];
-            double mu = s->friction_ij[mi * nm + mj];
-            double sdr = sqrt(delta * r_eff);
-            double s_n = 2.0 * e_eff * sdr;
-            double k_n = 4.0 / 3.0 * e_eff * sdr;
-            double k_t = 8.0 * g_eff * sdr;
+            double beta = s->beta_ij[kij];
+            double mu = s->friction_ij[kij];
+            double cohesion = s->extra_ij[ORC_X_COHESION_ENERGY][kij];
+            int jkr_adhesion_only = surface_energy > 0.0 && !use_dmt && delta <= 0.0;
+            double s_n = 0.0, k_n = 0.0, k_t = 0.0, gamma_n = 0.0;
+            if (hooke) {
+                k_n = s->extra_ij[ORC_X_KN][kij];
+                k_t = s->extra_ij[ORC_X_KT][kij];
+                gamma_n = 2.0 * beta * sqrt(k_n * m_r);
+            } else if (delta > 0.0) {
+                double sdr = sqrt(delta * r_eff);
+                s_n = 2.0 * e_eff * sdr;
+                k_n = 4.0 / 3.0 * e_eff * sdr;
+                k_t = 8.0 * g_eff * sdr;
+            }
             double oix = s->omega[i][0], oiy = s->omega[i][1], oiz = s->omega[i][2];
             double ojx = s->omega[j][0], ojy = s->omega[j][1], ojz = s->omega[j][2];
             double r1nx = r1 * nx, r1ny = r1 * ny, r1nz = r1 * nz;
@@ -1142,8 +1333,38 @@ int orc_hertz_mindlin_contact_force(orc_sys* s) {
             double vcjz = s->vel[j][2] + (-ojx * r2ny + ojy * r2nx);
             double vrx = vcjx - vcix, vry = vcjy - vciy, vrz = vcjz - vciz;
             double v_n = vrx * nx + vry * ny + vrz * nz;
-            double f_diss_n = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
-            double f_n_mag = fmax0(k_n * delta - f_diss_n);
+            double f_n_mag;
+            if (hooke) {
+                /* contact.rs:742-748 */
+                if (cohesion > 0.0) {
+                    double f_coh = cohesion * PI * delta * r_eff;
+                    f_n_mag = k_n * delta - gamma_n * v_n - f_coh;
+                } else {
+                    f_n_mag = fmax0(k_n * delta - gamma_n * v_n);
+                }
+            } else if (surface_energy > 0.0 && use_dmt) {
+                /* DMT (contact.rs:281-285) */
+                double f_dmt = 2.0 * PI * surface_energy * r_eff;
+                double f_diss_n = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+                f_n_mag = k_n * delta - f_diss_n - f_dmt;
+            } else if (surface_energy > 0.0) {
+                /* JKR (contact.rs:286-296) */
+                double f_adh = 1.5 * PI * surface_energy * r_eff;
+                if (jkr_adhesion_only) {
+                    f_n_mag = -f_adh;
+                } else {
+                    double f_diss_n = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+                    f_n_mag = k_n * delta - f_diss_n - f_adh;
+                }
+            } else if (cohesion > 0.0) {
+                /* SJKR (contact.rs:297-302) */
+                double f_diss_n = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+                double f_coh = cohesion * PI * delta * r_eff;
+                f_n_mag = k_n * delta - f_diss_n - f_coh;
+            } else {
+                double f_diss_n = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+                f_n_mag = fmax0(k_n * delta - f_diss_n);
+            }
             double fnx = f_n_mag * nx, fny = f_n_mag * ny, fnz = f_n_mag * nz;
             s->force[i][0] -= fnx;
             s->force[i][1] -= fny;
@@ -1151,10 +1372,13 @@ int orc_hertz_mindlin_contact_force(orc_sys* s) {
             s->force[j][0] += fnx;
             s->force[j][1] += fny;
             s->force[j][2] += fnz;
+            /* JKR adhesion-only: normal force only, no history touched
+             * (contact.rs:320-331: the entry is not marked active) */
+            if (jkr_adhesion_only) continue;
             double vtx = vrx - v_n * nx, vty = vry - v_n * ny, vtz = vrz - v_n * nz;
             uint32_t tag_i = s->tag[i], tag_j = s->tag[j];
             double sign = tag_i < tag_j ? 1.0 : -1.0;
-            double stored[3] = {0.0, 0.0, 0.0};
+            double stored[HSLOTS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
             hist_entry* he = NULL;
             cslot* cs = NULL;
             if (s->history_mode == ORC_HIST_REFERENCE) {
@@ -1216,13 +1440,17 @@ int orc_hertz_mindlin_contact_force(orc_sys* s) {
             s->torque[j][0] += tjx;
             s->torque[j][1] += tjy;
             s->torque[j][2] += tjz;
-            double ns[3] = {sign * sx, sign * sy, sign * sz};
+            double rd[3] = {sign * stored[3], sign * stored[4], sign * stored[5]};
+            double tw = sign * stored[6];
+            roll_twist(s, i, j, kij, nx, ny, nz, f_n_mag, r_eff, rd, &tw);
+            double ns[HSLOTS] = {sign * sx,    sign * sy,    sign * sz, sign * rd[0],
+                                 sign * rd[1], sign * rd[2], sign * tw};
             if (s->history_mode == ORC_HIST_REFERENCE) {
                 if (he) {
                     memcpy(he->s, ns, sizeof(ns));
                     he->active = 1;
                 } else {
-                    hist_push(&s->hist[i], tag_j, ns, 1);
+                    hist_push(&s->hist[i], tag_j, ns, HSLOTS, 1);
                 }
             } else {
                 memcpy(cs->s, ns, sizeof(ns));
@@ -1590,6 +1818,28 @@ int64_t orc_get_history(const orc_sys* s, uint32_t* owner_tag, uint32_t* partner
     return k;
 }
 
+int64_t orc_get_history7(const orc_sys* s, uint32_t* owner_tag, uint32_t* partner_tag,
+                         double* hist7) {
+    int64_t k = 0;
+    if (s->history_mode == ORC_HIST_CANONICAL) {
+        for (uint64_t m = 0; m < s->cm.cap; m++)
+            if (s->cm.slots[m].used) {
+                owner_tag[k] = (uint32_t)(s->cm.slots[m].key >> 32);
+                partner_tag[k] = (uint32_t)(s->cm.slots[m].key & 0xffffffffu);
+                memcpy(&hist7[HSLOTS * k], s->cm.slots[m].s, sizeof(double) * HSLOTS);
+                k++;
+            }
+        return k;
+    }
+    for (int i = 0; i < s->nlocal; i++)
+        for (int m = 0; m < s->hist[i].n; m++, k++) {
+            owner_tag[k] = s->tag[i];
+            partner_tag[k] = s->hist[i].e[m].tag;
+            memcpy(&hist7[HSLOTS * k], s->hist[i].e[m].s, sizeof(double) * HSLOTS);
+        }
+    return k;
+}
+
 int orc_set_history(orc_sys* s, int64_t n, const uint32_t* owner_tag, const uint32_t* partner_tag,
                     const double* spring) {
     if (s->history_mode == ORC_HIST_CANONICAL) {
@@ -1612,7 +1862,7 @@ int orc_set_history(orc_sys* s, int64_t n, const uint32_t* owner_tag, const uint
             set_err(s, "history owner tag not found");
             return -1;
         }
-        hist_push(&s->hist[found], partner_tag[k], &spring[3 * k], 0);
+        hist_push(&s->hist[found], partner_tag[k], &spring[3 * k], 3, 0);
     }
     return 0;
 }

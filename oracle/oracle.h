@@ -19,9 +19,9 @@
  *  - history_mode 1 keys shear history by the canonical pair (min tag,
  *    max tag) instead of per owner by partner tag
  *    (crates/dem_granular/src/contact.rs:358-364); mode 0 is the reference.
- *  - rolling/twisting history slots are not stored: every configuration on
- *    this path has mu_r = mu_tw = 0, where the reference leaves them zero
- *    (contact.rs:427-557).
+ *  - the contact force restates both reference kernels: Hertz-Mindlin
+ *    (contact.rs:103-615) and Hooke (contact.rs:622-990), chosen by the
+ *    global contact model as the plugin does (contact.rs:65-90).
  *  - only `every = 0` (displacement-driven rebuilds) is supported.
  */
 #ifndef MDDEM_ORACLE_H
@@ -54,6 +54,28 @@ int orc_set_materials(orc_sys* s, int n, const double* youngs, const double* poi
                       const double* restitution, const double* friction);
 int orc_get_material_tables(const orc_sys* s, double* beta, double* friction,
                             double* e_eff, double* g_eff);
+/* optional per-material properties (MaterialConfig, dem_atom/src/lib.rs:76-118),
+ * one array of n per ORC_X_* in this order; zero disables each. Builds their
+ * pair tables with the reference's mixing rules (lib.rs:417-511). */
+enum {
+    ORC_X_ROLLING_FRICTION = 0,
+    ORC_X_TWISTING_FRICTION,
+    ORC_X_COHESION_ENERGY,
+    ORC_X_SURFACE_ENERGY,
+    ORC_X_KN,
+    ORC_X_KT,
+    ORC_X_ROLLING_STIFFNESS,
+    ORC_X_ROLLING_DAMPING,
+    ORC_X_TWISTING_STIFFNESS,
+    ORC_X_TWISTING_DAMPING,
+    ORC_NX
+};
+int orc_set_material_extras(orc_sys* s, int n, const double* values /* ORC_NX * n */);
+/* pair table of one ORC_X_* property (nmat * nmat), returns nmat */
+int orc_get_extra_table(const orc_sys* s, int which, double* out);
+/* contact_model "hooke" (else "hertz"), adhesion_model "dmt" (else "jkr"),
+ * rolling_model / twisting_model "sds" (else "constant") */
+void orc_set_contact_models(orc_sys* s, int hooke, int dmt, int rolling_sds, int twisting_sds);
 void orc_set_lj(orc_sys* s, double epsilon, double sigma, double cutoff);
 void orc_get_lj(const orc_sys* s, double out[3]); /* lj1, lj2, cutoff2 */
 void orc_set_nose_hoover(orc_sys* s, int enabled, double temperature, double coupling);
@@ -90,7 +112,7 @@ void orc_forward_comm(orc_sys* s);                    /* comm.rs:367-424 */
 void orc_sort_atoms_by_bin(orc_sys* s);               /* neighbor.rs:686-794 */
 void orc_borders(orc_sys* s);                         /* comm.rs:449-570 */
 void orc_bin_neighbor_list(orc_sys* s);               /* neighbor.rs:810-1080 */
-int orc_hertz_mindlin_contact_force(orc_sys* s);      /* contact.rs:103-615 */
+int orc_hertz_mindlin_contact_force(orc_sys* s);      /* contact.rs:103-615, 622-990 */
 void orc_wall_contact_force(orc_sys* s);              /* dem_wall lib.rs:848-1002 */
 void orc_apply_gravity(orc_sys* s);                   /* mddem_fixes lib.rs:756-761 */
 void orc_lj_force(orc_sys* s);                        /* md_lj lib.rs:488-633 */
@@ -130,6 +152,10 @@ int64_t orc_get_history(const orc_sys* s, uint32_t* owner_tag, uint32_t* partner
                         double* spring);
 int orc_set_history(orc_sys* s, int64_t n, const uint32_t* owner_tag,
                     const uint32_t* partner_tag, const double* spring);
+/* the same with all 7 history values per entry: tangential spring (3),
+ * rolling displacement (3), twist angle (1) */
+int64_t orc_get_history7(const orc_sys* s, uint32_t* owner_tag, uint32_t* partner_tag,
+                         double* hist7);
 
 /* reference setup helpers (host-side precompute rows of the path) */
 double orc_dem_mass(double density, double radius);           /* insert.rs:340 */

@@ -52,6 +52,9 @@ def lib():
             "orc_set_dt": (None, [vp, C.c_double]),
             "orc_set_materials": (C.c_int, [vp, C.c_int, _dp, _dp, _dp, _dp]),
             "orc_get_material_tables": (C.c_int, [vp, _dp, _dp, _dp, _dp]),
+            "orc_set_material_extras": (C.c_int, [vp, C.c_int, _dp]),
+            "orc_get_extra_table": (C.c_int, [vp, C.c_int, _dp]),
+            "orc_set_contact_models": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
             "orc_set_lj": (None, [vp, C.c_double, C.c_double, C.c_double]),
             "orc_get_lj": (None, [vp, _dp]),
             "orc_set_nose_hoover": (None, [vp, C.c_int, C.c_double, C.c_double]),
@@ -101,6 +104,7 @@ def lib():
             "orc_get_pairs": (C.c_int64, [vp, _u32p, _u32p, _u8p]),
             "orc_history_count": (C.c_int64, [vp]),
             "orc_get_history": (C.c_int64, [vp, _u32p, _u32p, _dp]),
+            "orc_get_history7": (C.c_int64, [vp, _u32p, _u32p, _dp]),
             "orc_set_history": (C.c_int, [vp, C.c_int64, _u32p, _u32p, _dp]),
             "orc_dem_mass": (C.c_double, [C.c_double, C.c_double]),
             "orc_dem_inv_inertia": (C.c_double, [C.c_double, C.c_double]),
@@ -120,6 +124,12 @@ def _d(a):
 
 def _ptr(a, t=_dp):
     return None if a is None else a.ctypes.data_as(t)
+
+
+# optional per-material properties, in the order the C API takes them
+MATERIAL_EXTRAS = ("rolling_friction", "twisting_friction", "cohesion_energy", "surface_energy",
+                   "kn", "kt", "rolling_stiffness", "rolling_damping", "twisting_stiffness",
+                   "twisting_damping")
 
 
 class Oracle:
@@ -164,6 +174,25 @@ class Oracle:
         out = [np.zeros(n * n) for _ in range(4)]
         self.L.orc_get_material_tables(self.h, *[_ptr(o) for o in out])
         return {k: v.reshape(n, n) for k, v in zip(("beta", "friction", "e_eff", "g_eff"), out)}
+
+    def set_material_extras(self, **props):
+        """Optional per-material properties by name (MATERIAL_EXTRAS), one
+        value per material; properties not given are zero."""
+        n = self.L.orc_get_material_tables(self.h, None, None, None, None)
+        vals = np.zeros((len(MATERIAL_EXTRAS), n))
+        for k, v in props.items():
+            vals[MATERIAL_EXTRAS.index(k)] = v
+        self._check(self.L.orc_set_material_extras(self.h, n, _ptr(_d(vals))))
+
+    def extra_table(self, name):
+        n = self.L.orc_get_material_tables(self.h, None, None, None, None)
+        out = np.zeros(n * n)
+        self.L.orc_get_extra_table(self.h, MATERIAL_EXTRAS.index(name), _ptr(out))
+        return out.reshape(n, n)
+
+    def set_contact_models(self, hooke=False, dmt=False, rolling_sds=False, twisting_sds=False):
+        self.L.orc_set_contact_models(self.h, int(hooke), int(dmt), int(rolling_sds),
+                                      int(twisting_sds))
 
     def set_lj(self, epsilon=1.0, sigma=1.0, cutoff=2.5):
         self.L.orc_set_lj(self.h, epsilon, sigma, cutoff)
@@ -293,6 +322,15 @@ class Oracle:
         p = np.zeros(n, np.uint32)
         s = np.zeros((n, 3))
         self.L.orc_get_history(self.h, _ptr(o, _u32p), _ptr(p, _u32p), _ptr(s))
+        return o, p, s
+
+    def history7(self):
+        """History entries with all 7 values: spring (3), rolling (3), twist."""
+        n = self.L.orc_history_count(self.h)
+        o = np.zeros(n, np.uint32)
+        p = np.zeros(n, np.uint32)
+        s = np.zeros((n, 7))
+        self.L.orc_get_history7(self.h, _ptr(o, _u32p), _ptr(p, _u32p), _ptr(s))
         return o, p, s
 
     def set_history(self, owner, partner, spring):
