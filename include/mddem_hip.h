@@ -111,9 +111,34 @@ int mdd_set_nose_hoover(mdd_ctx* ctx, int enabled, double temperature, double co
 int mdd_set_integrators(mdd_ctx* ctx, int velocity_verlet, int rotation);
 /* [gravity] section, GravityPlugin (crates/mddem_fixes/src/lib.rs:735-761) */
 int mdd_set_gravity(mdd_ctx* ctx, int enabled, const double g[3]);
-/* [[wall]] plane entry, WallPlane (crates/dem_wall/src/lib.rs:302-357); static walls */
+/* [[wall]] entries (WallPlugin::build, crates/dem_wall/src/lib.rs:546-757).
+ * Walls share one index space in the order they are added (at most 8).
+ * Plane: WallPlane (lib.rs:302-357), the normal is normalised (a zero normal
+ * is an error), bounds may be NULL (unbounded). */
 int mdd_add_wall(mdd_ctx* ctx, const double point[3], const double normal[3], int material,
                  const double bound_lo[3], const double bound_hi[3]);
+/* type = "cylinder", WallCylinder (lib.rs:372-405): axis 0/1/2 = x/y/z,
+ * center (c0, c1) in the plane normal to the axis, axial range [lo, hi] */
+int mdd_add_cylinder_wall(mdd_ctx* ctx, int axis, double c0, double c1, double radius,
+                          double lo, double hi, int inside, int material);
+/* type = "sphere", WallSphere (lib.rs:411-428) */
+int mdd_add_sphere_wall(mdd_ctx* ctx, const double center[3], double radius, int inside,
+                        int material);
+/* plane wall motion (WallMotion, lib.rs:262-293): 0 static, 1 constant
+ * velocity (params: v[3]), 2 oscillate (amplitude, frequency), 3 servo
+ * (target_force, max_velocity, gain). The wall restarts from its configured
+ * point. Scheduled steps move the walls (wall_move, lib.rs:766-815). */
+int mdd_set_wall_motion(mdd_ctx* ctx, int wall, int motion, const double* params);
+/* Walls::deactivate_by_name (lib.rs:476-512), by index */
+int mdd_set_wall_active(mdd_ctx* ctx, int wall, int active);
+/* per wall: force_accumulator of the last force pass (sum of the scalar
+ * contact forces, kept in fixed point at 2^-32 N), current point[3] and
+ * velocity[3]; any output may be NULL; *count = number of walls */
+int mdd_get_walls(mdd_ctx* ctx, double* force, double* point, double* velocity, int* count);
+/* the wall systems one at a time (wall_move, lib.rs:766-815;
+ * wall_zero_force_accumulators, lib.rs:821-834) */
+int mdd_wall_move(mdd_ctx* ctx);
+int mdd_wall_zero_force_accumulators(mdd_ctx* ctx);
 int mdd_clear_walls(mdd_ctx* ctx);
 
 /* ---- state ------------------------------------------------------------------ */

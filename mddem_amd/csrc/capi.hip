@@ -706,26 +706,168 @@ int mdd_set_gravity(mdd_ctx* ctx, int enabled, const double g[3]) {
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// The wall table changes: the device copy is replaced and the current wall
+// state (points, velocities, time, force sums) is kept for walls that stay;
+// `fresh` walls start at their configured point and velocity.
+void sync_walls(Ctx& c, int fresh_from) {
+    mdd::WallsDyn d{};
+    if (c.wdyn.p) {
+        MDD_HIP(hipStreamSynchronize(c.stream));
+        d2h(c, &d, c.wdyn.p, sizeof(d));
+        MDD_HIP(hipStreamSynchronize(c.stream));
+    }
+    for (int k = fresh_from; k < c.walls.count; k++) {
+        const mdd::Wall& w = c.walls.w[k];
+        for (int q = 0; q < 3; q++) {
+            d.w[k].p[q] = w.p[q];
+            d.w[k].v[q] = w.motion == 1 ? w.v[q] : 0.0;
+        }
+        d.w[k].acc = 0ull;
+    }
+    c.walls_moving = false;
+    for (int k = 0; k < c.walls.count; k++)
+        c.walls_moving = c.walls_moving || (c.walls.w[k].kind == 0 && c.walls.w[k].motion != 0);
+    c.walls_dev.resize(1);
+    c.wdyn.resize(1);
+    h2d(c, c.walls_dev.p, &c.walls, sizeof(c.walls));
+    h2d(c, c.wdyn.p, &d, sizeof(d));
+}
+
+mdd::Wall& new_wall(Ctx& c, int material) {
+    if (c.walls.count >= mdd::kMaxWalls) throw ApiError(MDD_ERR_INVALID, "too many walls");
+    if (material < 0 || material >= mdd::kMaxMat)
+        throw ApiError(MDD_ERR_INVALID, "wall material out of range");
+    mdd::Wall& w = c.walls.w[c.walls.count];
+    std::memset(&w, 0, sizeof(w));
+    w.mat = material;
+    w.active = 1;
+    for (int d = 0; d < 3; d++) {
+        w.blo[d] = -1.0e300;
+        w.bhi[d] = 1.0e300;
+    }
+    return w;
+}
+
+mdd::Wall& wall_at(Ctx& c, int wall) {
+    if (wall < 0 || wall >= c.walls.count) throw ApiError(MDD_ERR_INVALID, "no such wall");
+    return c.walls.w[wall];
+}
+
+}  // namespace
+
+extern "C" {
+
 int mdd_add_wall(mdd_ctx* ctx, const double point[3], const double normal[3], int material,
                  const double blo[3], const double bhi[3]) {
     return guarded(ctx, [&](Ctx& c) {
-        if (c.walls.count >= mdd::kMaxWalls) throw ApiError(MDD_ERR_INVALID, "too many walls");
         if (!point || !normal) throw ApiError(MDD_ERR_INVALID, "null wall argument");
-        if (material < 0 || material >= mdd::kMaxMat)
-            throw ApiError(MDD_ERR_INVALID, "wall material out of range");
-        mdd::Wall& w = c.walls.w[c.walls.count++];
+        // WallPlugin::build normalises the plane normal (dem_wall/src/lib.rs:688-699)
+        const double mag =
+            std::sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+        if (!(mag > 1e-15)) throw ApiError(MDD_ERR_INVALID, "wall normal vector must be non-zero");
+        mdd::Wall& w = new_wall(c, material);
         for (int d = 0; d < 3; d++) {
             w.p[d] = point[d];
-            w.n[d] = normal[d];
-            w.blo[d] = blo ? blo[d] : -1.0e300;
-            w.bhi[d] = bhi ? bhi[d] : 1.0e300;
+            w.n[d] = normal[d] / mag;
+            if (blo) w.blo[d] = blo[d];
+            if (bhi) w.bhi[d] = bhi[d];
         }
-        w.mat = material;
+        c.walls.count++;
+        sync_walls(c, c.walls.count - 1);
+    });
+}
+
+int mdd_add_cylinder_wall(mdd_ctx* ctx, int axis, double c0, double c1, double radius,
+                          double lo, double hi, int inside, int material) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (axis < 0 || axis > 2) throw ApiError(MDD_ERR_INVALID, "cylinder axis must be 0, 1 or 2");
+        mdd::Wall& w = new_wall(c, material);
+        w.kind = 1;
+        w.axis = axis;
+        w.c[0] = c0;
+        w.c[1] = c1;
+        w.radius = radius;
+        w.lo = lo;
+        w.hi = hi;
+        w.inside = inside ? 1 : 0;
+        c.walls.count++;
+        sync_walls(c, c.walls.count - 1);
+    });
+}
+
+int mdd_add_sphere_wall(mdd_ctx* ctx, const double center[3], double radius, int inside,
+                        int material) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (!center) throw ApiError(MDD_ERR_INVALID, "null wall argument");
+        mdd::Wall& w = new_wall(c, material);
+        w.kind = 2;
+        for (int d = 0; d < 3; d++) w.c[d] = center[d];
+        w.radius = radius;
+        w.inside = inside ? 1 : 0;
+        c.walls.count++;
+        sync_walls(c, c.walls.count - 1);
+    });
+}
+
+int mdd_set_wall_motion(mdd_ctx* ctx, int wall, int motion, const double* params) {
+    return guarded(ctx, [&](Ctx& c) {
+        mdd::Wall& w = wall_at(c, wall);
+        if (w.kind != 0) throw ApiError(MDD_ERR_INVALID, "only plane walls move");
+        if (motion < 0 || motion > 3) throw ApiError(MDD_ERR_INVALID, "wall motion must be 0..3");
+        if (motion != 0 && !params) throw ApiError(MDD_ERR_INVALID, "null motion parameters");
+        w.motion = motion;
+        w.v[0] = w.v[1] = w.v[2] = 0.0;
+        w.a0 = w.a1 = w.a2 = 0.0;
+        if (motion == 1)
+            for (int d = 0; d < 3; d++) w.v[d] = params[d];
+        if (motion >= 2) {
+            w.a0 = params[0];
+            w.a1 = params[1];
+            if (motion == 3) w.a2 = params[2];
+        }
+        // the wall restarts from its configured point
+        sync_walls(c, wall);
+    });
+}
+
+int mdd_set_wall_active(mdd_ctx* ctx, int wall, int active) {
+    return guarded(ctx, [&](Ctx& c) {
+        wall_at(c, wall).active = active ? 1 : 0;
+        sync_walls(c, c.walls.count);
+    });
+}
+
+int mdd_get_walls(mdd_ctx* ctx, double* force, double* point, double* velocity, int* count) {
+    return guarded(ctx, [&](Ctx& c) {
+        mdd::WallsDyn d{};
+        if (c.wdyn.p) {
+            MDD_HIP(hipStreamSynchronize(c.stream));
+            d2h(c, &d, c.wdyn.p, sizeof(d));
+            MDD_HIP(hipStreamSynchronize(c.stream));
+        }
+        for (int k = 0; k < c.walls.count; k++) {
+            if (force)
+                force[k] = double(static_cast<long long>(d.w[k].acc)) / mdd::kWallAccScale;
+            for (int q = 0; q < 3; q++) {
+                if (point) point[3 * k + q] = d.w[k].p[q];
+                if (velocity) velocity[3 * k + q] = d.w[k].v[q];
+            }
+        }
+        if (count) *count = c.walls.count;
     });
 }
 
 int mdd_clear_walls(mdd_ctx* ctx) {
-    return guarded(ctx, [&](Ctx& c) { c.walls.count = 0; });
+    return guarded(ctx, [&](Ctx& c) {
+        c.walls.count = 0;
+        c.walls_moving = false;
+        c.wdyn.release();
+        c.walls_dev.release();
+    });
 }
 
 int mdd_upload_state(mdd_ctx* ctx, int64_t n, const uint32_t* tag, const uint32_t* type,
@@ -1072,6 +1214,14 @@ int mdd_wall_contact_force(mdd_ctx* ctx) {
         if (c.model != MDD_MODEL_DEM) throw ApiError(MDD_ERR_INVALID, "not a DEM context");
         mdd::launch_walls_gravity(c, true, false);
     });
+}
+
+int mdd_wall_move(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) { mdd::launch_wall_systems(c, true, false); });
+}
+
+int mdd_wall_zero_force_accumulators(mdd_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) { mdd::launch_wall_systems(c, false, true); });
 }
 
 int mdd_apply_gravity(mdd_ctx* ctx) {

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstdint>
 #include <functional>
 #include <memory>
@@ -127,11 +128,36 @@ struct SysP {
     double beta, mu, eeff, geff;  // single-material pair constants
 };
 
+// One wall (dem_wall/src/lib.rs:302-453). kind 0: plane through p with unit
+// normal n, active inside [blo, bhi]; 1: cylinder about axis through (c0, c1)
+// of radius `radius`, axial range [lo, hi]; 2: sphere about c of radius
+// `radius`. inside: particles live inside the curved surface. Plane motion
+// (WallMotion, lib.rs:262-293): 0 static, 1 constant velocity (v), 2
+// oscillate (a0 amplitude, a1 frequency), 3 servo (a0 target force, a1 max
+// velocity, a2 gain). p is the configured (origin) point; the current point
+// and velocity live on the device (WallsDyn).
 struct Wall {
     double p[3], n[3], blo[3], bhi[3];
     int mat;
+    int kind, active, motion;
+    double v[3], a0, a1, a2;
+    double c[3], radius, lo, hi;
+    int axis, inside;
 };
 constexpr int kMaxWalls = 8;
+// scalar contact force sums (force_accumulator) are kept in fixed point:
+// integer sums do not depend on the order the atomics land in
+constexpr double kWallAccScale = 4294967296.0;  // 2^32 per newton
+struct WallDyn {
+    double p[3], v[3];
+    unsigned long long acc;
+    unsigned long long pad;
+};
+struct WallsDyn {
+    double time;  // Walls::time
+    double pad;
+    WallDyn w[kMaxWalls];
+};
 constexpr int kMaxMat = 16;
 constexpr int kNumExtras = 10;  // optional per-material contact properties
 
@@ -139,6 +165,39 @@ struct WallsP {
     int count;
     Wall w[kMaxWalls];
 };
+
+// wall_move (dem_wall/src/lib.rs:766-815) for every plane wall, then the
+// force sums are cleared for the coming force pass (wall_zero_force_
+// accumulators, lib.rs:821-834). One thread runs it.
+__host__ __device__ inline void walls_advance(const WallsP& W, WallsDyn& D, double dt,
+                                              bool move = true, bool zero = true) {
+    const double PI = 3.141592653589793;
+    const double time = D.time;
+    for (int k = 0; k < W.count; k++) {
+        const Wall& w = W.w[k];
+        WallDyn& d = D.w[k];
+        if (move && w.kind == 0 && w.active) {
+            if (w.motion == 1) {
+                for (int q = 0; q < 3; q++) d.p[q] += d.v[q] * dt;
+            } else if (w.motion == 2) {
+                const double phase = 2.0 * PI * w.a1 * (time + dt);
+                const double disp = w.a0 * sin(phase);
+                for (int q = 0; q < 3; q++) d.p[q] = w.p[q] + disp * w.n[q];
+                const double vel_mag = w.a0 * 2.0 * PI * w.a1 * cos(phase);
+                for (int q = 0; q < 3; q++) d.v[q] = vel_mag * w.n[q];
+            } else if (w.motion == 3) {
+                const double measured = double(static_cast<long long>(d.acc)) / kWallAccScale;
+                double vel_mag = w.a2 * (w.a0 - measured);
+                if (vel_mag < -w.a1) vel_mag = -w.a1;
+                if (vel_mag > w.a1) vel_mag = w.a1;
+                for (int q = 0; q < 3; q++) d.v[q] = vel_mag * w.n[q];
+                for (int q = 0; q < 3; q++) d.p[q] += d.v[q] * dt;
+            }
+        }
+        if (zero) d.acc = 0ull;
+    }
+    if (move) D.time = time + dt;
+}
 
 // Material pair tables (beta, friction, E*, G*) in row-major nmat x nmat.
 struct MatP {
@@ -260,6 +319,9 @@ struct Ctx {
     int gravity_on = 0;
     double g[3] = {0, 0, 0};
     WallsP walls{};
+    DBuf<WallsP> walls_dev;   // device copy of the wall table (integrators)
+    DBuf<WallsDyn> wdyn;      // current wall points, velocities, force sums
+    bool walls_moving = false;
 
     // ---- batched ensemble (nsys > 0): particles of system s are
     // [sys_first[s], sys_first[s + 1]); per-particle system index psys
@@ -432,6 +494,8 @@ bool rebuild_grow(Ctx& c, const Status& s);
 // forces.hip
 void launch_pair_forces(Ctx& c, int step, bool energy);
 void launch_walls_gravity(Ctx& c, bool walls, bool gravity);
+// standalone wall_move / wall_zero_force_accumulators
+void launch_wall_systems(Ctx& c, bool move, bool zero);
 // lj_tile.hip
 void launch_lj_tile(Ctx& c, int step, bool energy, bool fuse);
 void launch_ke_reduce(Ctx& c, int step, int64_t count);

@@ -58,7 +58,8 @@ struct ForceArgs {
     double* hist;
     uint64_t* cmask;
     int64_t hist_cs;
-    WallsP walls;
+    const WallsP* walls;  // wall table (device copy)
+    WallsDyn* wdyn;       // current wall points / velocities and force sums
     int use_walls, use_gravity, use_pairs;
     double g[3];
     double* f64;
@@ -268,37 +269,155 @@ __device__ inline void store_force(const ForceArgs& a, int64_t i, R fx, R fy, R 
     }
 }
 
-// plane walls (dem_wall/src/lib.rs:862-1002): normal Hertz + damping, no
-// wall friction, static wall, mu_tw = 0
+// wall_contact_force (dem_wall/src/lib.rs:848-1241) for one particle: plane
+// walls (JKR / DMT adhesion, SJKR cohesion, twisting friction, moving
+// walls), then cylinder and sphere walls (Hertz, SJKR cohesion). FP64 pair
+// arithmetic; each wall's scalar force joins its fixed-point sum.
+__device__ inline double curved_wall_force(const ForceArgs& a, int k, double radius,
+                                           double mass, const double4& v, double delta,
+                                           double nx, double ny, double nz) {
+    const double PI = 3.141592653589793;
+    const int nn = a.nm * a.nm;
+    const double e_eff = a.mat[2 * nn + k];
+    const double sdr = sqrt(delta * radius);
+    const double k_n = 4.0 / 3.0 * e_eff * sdr;
+    const double s_n = 2.0 * e_eff * sdr;
+    const double v_n = v.x * nx + v.y * ny + v.z * nz;
+    const double beta = a.mat[k];
+    const double cohesion = a.mext ? a.mext[kXCohesion * nn + k] : 0.0;
+    const double f_diss = 2.0 * beta * kSqrt53 * sqrt(s_n * mass) * v_n;
+    if (cohesion > 0.0) return k_n * delta - f_diss - cohesion * PI * delta * radius;
+    const double f = k_n * delta - f_diss;
+    return f > 0.0 ? f : 0.0;
+}
+
+__device__ inline void wall_acc(const ForceArgs& a, int w, double f) {
+    atomicAdd(&a.wdyn->w[w].acc,
+              static_cast<unsigned long long>(__double2ll_rn(f * kWallAccScale)));
+}
+
 template <typename R>
 __device__ inline void wall_forces(const ForceArgs& a, const double4& p, const double4& v,
-                                   double mass, uint32_t ti, R& fx, R& fy, R& fz) {
-    const int nm = a.nm;
-    for (int w = 0; w < a.walls.count; w++) {
-        const Wall& W = a.walls.w[w];
+                                   const double4* om, double mass, uint32_t ti, R& fx, R& fy,
+                                   R& fz, R& tx, R& ty, R& tz) {
+    const double PI = 3.141592653589793;
+    const int nm = a.nm, nn = nm * nm;
+    const double radius = p.w;
+    const WallsP& WS = *a.walls;
+    for (int w = 0; w < WS.count; w++) {
+        const Wall& W = WS.w[w];
+        if (W.kind != 0 || !W.active) continue;
         if (!(p.x >= W.blo[0] && p.x <= W.bhi[0] && p.y >= W.blo[1] && p.y <= W.bhi[1] &&
               p.z >= W.blo[2] && p.z <= W.bhi[2]))
             continue;
-        const double dx = p.x - W.p[0], dy = p.y - W.p[1], dz = p.z - W.p[2];
+        const WallDyn& D = a.wdyn->w[w];
+        const double dx = p.x - D.p[0], dy = p.y - D.p[1], dz = p.z - D.p[2];
         const double distance = dx * W.n[0] + dy * W.n[1] + dz * W.n[2];
         if (distance <= 0.0) continue;
-        const double radius = p.w;
+        const int k = int(ti) * nm + W.mat;
+        const double e_eff = a.mat[2 * nn + k];
+        const double surface = a.mext ? a.mext[kXSurface * nn + k] : 0.0;
+        double d_po = 0.0;
+        if (surface > 0.0 && !a.dmt)
+            d_po = cbrt(PI * PI * surface * surface * radius / (4.0 * e_eff * e_eff));
         double delta = radius - distance;
         if (0.5 * radius < delta) delta = 0.5 * radius;
-        if (delta <= 0.0) continue;
-        const R e_eff = R(a.mat[2 * nm * nm + ti * nm + W.mat]);
-        const R beta = R(a.mat[ti * nm + W.mat]);
-        const R sdr = sqrt_r<R>(R(delta) * R(radius));
-        const R s_n = R(2.0) * e_eff * sdr;
-        const R k_n = R(4.0 / 3.0) * e_eff * sdr;
-        const R m_r = R(mass);
-        const double v_n = v.x * W.n[0] + v.y * W.n[1] + v.z * W.n[2];
-        const R f_diss = R(2.0) * beta * R(kSqrt53) * sqrt_r<R>(s_n * m_r) * R(v_n);
-        R f_net = k_n * R(delta) - f_diss;
-        if (!(f_net > R(0))) f_net = R(0);
-        fx += f_net * R(W.n[0]);
-        fy += f_net * R(W.n[1]);
-        fz += f_net * R(W.n[2]);
+        if (delta <= 0.0 && surface <= 0.0) continue;
+        if (delta < -d_po) continue;
+        const bool adhesion_only = surface > 0.0 && !a.dmt && delta <= 0.0;
+        double s_n = 0.0, k_n = 0.0;
+        if (delta > 0.0) {
+            const double sdr = sqrt(delta * radius);
+            s_n = 2.0 * e_eff * sdr;
+            k_n = 4.0 / 3.0 * e_eff * sdr;
+        }
+        const double v_n =
+            (v.x - D.v[0]) * W.n[0] + (v.y - D.v[1]) * W.n[1] + (v.z - D.v[2]) * W.n[2];
+        const double beta = a.mat[k];
+        const double cohesion = a.mext ? a.mext[kXCohesion * nn + k] : 0.0;
+        const double f_diss = 2.0 * beta * kSqrt53 * sqrt(s_n * mass) * v_n;
+        double f_net;
+        if (surface > 0.0 && a.dmt) {
+            f_net = k_n * delta - f_diss - 2.0 * PI * surface * radius;
+        } else if (surface > 0.0) {
+            const double f_adh = 1.5 * PI * surface * radius;
+            f_net = adhesion_only ? -f_adh : k_n * delta - f_diss - f_adh;
+        } else if (cohesion > 0.0) {
+            f_net = k_n * delta - f_diss - cohesion * PI * delta * radius;
+        } else {
+            f_net = k_n * delta - f_diss;
+            if (!(f_net > 0.0)) f_net = 0.0;
+        }
+        fx += R(f_net * W.n[0]);
+        fy += R(f_net * W.n[1]);
+        fz += R(f_net * W.n[2]);
+        const double mu_tw = a.mext ? a.mext[kXTwisting * nn + k] : 0.0;
+        if (delta > 0.0 && mu_tw > 0.0 && om) {
+            const double twist = om->x * W.n[0] + om->y * W.n[1] + om->z * W.n[2];
+            if (fabs(twist) > 1e-30) {
+                const double tau = (twist > 0.0 ? -1.0 : 1.0) * (mu_tw * fabs(f_net) * radius);
+                tx += R(tau * W.n[0]);
+                ty += R(tau * W.n[1]);
+                tz += R(tau * W.n[2]);
+            }
+        }
+        wall_acc(a, w, f_net);
+    }
+    for (int w = 0; w < WS.count; w++) {
+        const Wall& W = WS.w[w];
+        if (W.kind != 1 || !W.active) continue;
+        double axial, d0, d1;
+        if (W.axis == 0) {
+            axial = p.x;
+            d0 = p.y - W.c[0];
+            d1 = p.z - W.c[1];
+        } else if (W.axis == 1) {
+            axial = p.y;
+            d0 = p.x - W.c[0];
+            d1 = p.z - W.c[1];
+        } else {
+            axial = p.z;
+            d0 = p.x - W.c[0];
+            d1 = p.y - W.c[1];
+        }
+        if (axial < W.lo || axial > W.hi) continue;
+        const double radial = sqrt(d0 * d0 + d1 * d1);
+        if (radial < 1e-30) continue;
+        const double inv_r = 1.0 / radial;
+        const double delta0 =
+            W.inside ? radius - (W.radius - radial) : radius - (radial - W.radius);
+        const double n0 = W.inside ? -d0 * inv_r : d0 * inv_r;
+        const double n1 = W.inside ? -d1 * inv_r : d1 * inv_r;
+        const double nx = W.axis == 0 ? 0.0 : n0;
+        const double ny = W.axis == 0 ? n0 : (W.axis == 1 ? 0.0 : n1);
+        const double nz = W.axis == 2 ? 0.0 : n1;
+        if (delta0 <= 0.0) continue;
+        const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
+        const double f_net =
+            curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
+        fx += R(f_net * nx);
+        fy += R(f_net * ny);
+        fz += R(f_net * nz);
+        wall_acc(a, w, f_net);
+    }
+    for (int w = 0; w < WS.count; w++) {
+        const Wall& W = WS.w[w];
+        if (W.kind != 2 || !W.active) continue;
+        const double dx = p.x - W.c[0], dy = p.y - W.c[1], dz = p.z - W.c[2];
+        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+        if (dist < 1e-30) continue;
+        const double inv = 1.0 / dist;
+        const double sg = W.inside ? -1.0 : 1.0;
+        const double nx = sg * dx * inv, ny = sg * dy * inv, nz = sg * dz * inv;
+        const double delta0 = W.inside ? radius - (W.radius - dist) : radius - (dist - W.radius);
+        if (delta0 <= 0.0) continue;
+        const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
+        const double f_net =
+            curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
+        fx += R(f_net * nx);
+        fy += R(f_net * ny);
+        fz += R(f_net * nz);
+        wall_acc(a, w, f_net);
     }
 }
 
@@ -491,7 +610,9 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
         if (t == 0) a.cmask[i] = mask_out;
     }
     if (t != 0) return;
-    if (a.use_walls) wall_forces<R>(a, pi, vi, a.mass[i], ti, fx, fy, fz);
+    if (a.use_walls)
+        wall_forces<R>(a, pi, vi, a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
+                       ty, tz);
     if (a.use_gravity) {
         const double m = a.mass[i];
         fx += R(m * a.g[0]);
@@ -776,7 +897,9 @@ __global__ void __launch_bounds__(256) k_contact_ext(ForceArgs a) {
         if (t == 0) a.cmask[i] = mask_out;
     }
     if (t != 0) return;
-    if (a.use_walls) wall_forces<R>(a, pi, vi, a.mass[i], ti, fx, fy, fz);
+    if (a.use_walls)
+        wall_forces<R>(a, pi, vi, a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
+                       ty, tz);
     if (a.use_gravity) {
         const double m = a.mass[i];
         fx += R(m * a.g[0]);
@@ -797,7 +920,20 @@ __global__ void k_walls_gravity(ForceArgs a) {
     else
         f = a.f32;
     R fx = f[i], fy = f[a.npad + i], fz = f[2 * a.npad + i];
-    if (a.use_walls) wall_forces<R>(a, a.pos[i], a.vel[i], a.mass[i], a.type[i], fx, fy, fz);
+    R* t = nullptr;
+    if constexpr (sizeof(R) == 8)
+        t = a.t64;
+    else
+        t = a.t32;
+    R tx = t ? t[i] : R(0), ty = t ? t[a.npad + i] : R(0), tz = t ? t[2 * a.npad + i] : R(0);
+    if (a.use_walls)
+        wall_forces<R>(a, a.pos[i], a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i],
+                       a.type[i], fx, fy, fz, tx, ty, tz);
+    if (t) {
+        t[i] = tx;
+        t[a.npad + i] = ty;
+        t[2 * a.npad + i] = tz;
+    }
     if (a.use_gravity) {
         const double m = a.mass[i];
         fx += R(m * a.g[0]);
@@ -842,7 +978,8 @@ ForceArgs make_args(Ctx& c, int step) {
     a.hist = c.hist.p;
     a.cmask = c.cmask.p;
     a.hist_cs = int64_t(c.maxnb) * c.npad;
-    a.walls = c.walls;
+    a.walls = c.walls_dev.p;
+    a.wdyn = c.wdyn.p;
     a.g[0] = c.g[0];
     a.g[1] = c.g[1];
     a.g[2] = c.g[2];
@@ -963,6 +1100,19 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
         MDD_HIP(hipEventRecord(e1, c.stream));
         c.prof_pending++;
     }
+}
+
+namespace {
+__global__ void k_wall_systems(const WallsP* w, WallsDyn* d, double dt, int move, int zero) {
+    if (threadIdx.x == 0) walls_advance(*w, *d, dt, move != 0, zero != 0);
+}
+}  // namespace
+
+void launch_wall_systems(Ctx& c, bool move, bool zero) {
+    if (c.walls.count == 0 || !c.wdyn.p) return;
+    k_wall_systems<<<1, 64, 0, c.stream>>>(c.walls_dev.p, c.wdyn.p, c.dt, move ? 1 : 0,
+                                           zero ? 1 : 0);
+    MDD_HIP(hipGetLastError());
 }
 
 void launch_walls_gravity(Ctx& c, bool walls, bool gravity) {

@@ -56,6 +56,8 @@ struct IntArgs {
     double ndof, temp, q_mass;
     const SysP* sysp;       // batched ensemble: per-system box and threshold, or null
     const uint32_t* psys;
+    const WallsP* walls;    // walls (device copy) and their state, or null: scheduled
+    WallsDyn* wdyn;         // steps move them (PreInitialIntegration) and clear the sums
 };
 
 template <typename F>
@@ -207,7 +209,10 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     // uniform over the launch: only earlier kernels lower abort_step below step
     if (a.step >= 0 && a.st->abort_step < a.step) return;
-    if (i == 0) a.st->overlap_count = 0;
+    if (i == 0) {
+        a.st->overlap_count = 0;
+        if (a.wdyn && a.step >= 0) walls_advance(*a.walls, *a.wdyn, a.dt);
+    }
     double scale = 1.0;
     if (NH) {
         const double ke = 0.5 * sum_partials(a, lds4);
@@ -298,7 +303,10 @@ template <typename F>
 __global__ void __launch_bounds__(kThreads) k_post_pre(IntArgs a) {
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (a.step >= 0 && a.st->abort_step < a.step) return;
-    if (i == 0) a.st->overlap_count = 0;
+    if (i == 0) {
+        a.st->overlap_count = 0;
+        if (a.wdyn && a.step >= 0) walls_advance(*a.walls, *a.wdyn, a.dt);
+    }
     if (i >= a.n) return;
     double4 p = a.pos[i];
     if (a.vv) {
@@ -406,6 +414,10 @@ int post_blocks(const Ctx& c, bool nh) {
 
 IntArgs make_args(Ctx& c, int step, bool check) {
     IntArgs a{};
+    if (c.walls.count > 0 && c.wdyn.p) {
+        a.walls = c.walls_dev.p;
+        a.wdyn = c.wdyn.p;
+    }
     a.n = c.n;
     a.npad = c.npad;
     a.dt = c.dt;

@@ -9,6 +9,7 @@ the library is missing or the GPU is not a gfx950 the calls raise.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -79,6 +80,14 @@ SIGNATURES = {
     "mdd_set_gravity": (C.c_int, [_vp, C.c_int, _dp]),
     "mdd_add_wall": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, _dp]),
     "mdd_clear_walls": (C.c_int, [_vp]),
+    "mdd_add_cylinder_wall": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, C.c_int]),
+    "mdd_add_sphere_wall": (C.c_int, [_vp, _dp, C.c_double, C.c_int, C.c_int]),
+    "mdd_set_wall_motion": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "mdd_set_wall_active": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "mdd_get_walls": (C.c_int, [_vp, _dp, _dp, _dp, C.POINTER(C.c_int)]),
+    "mdd_wall_move": (C.c_int, [_vp]),
+    "mdd_wall_zero_force_accumulators": (C.c_int, [_vp]),
     "mdd_upload_state": (C.c_int, [_vp, C.c_int64, _u32p, _u32p, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mdd_nlocal": (C.c_int64, [_vp]),
     "mdd_download_state": (C.c_int, [_vp, _u32p, _dp, _dp, _dp, _dp, _i32p]),
@@ -371,6 +380,35 @@ class System:
 
     def wall_contact_force(self):
         self._c(self.L.mdd_wall_contact_force(self.h))
+
+    def add_cylinder_wall(self, axis, center, radius, lo=-math.inf, hi=math.inf, inside=False,
+                          material=0):
+        self._c(self.L.mdd_add_cylinder_wall(self.h, axis, center[0], center[1], radius, lo, hi,
+                                             int(inside), material))
+
+    def add_sphere_wall(self, center, radius, inside=False, material=0):
+        self._c(self.L.mdd_add_sphere_wall(self.h, _p(_d(center)), radius, int(inside),
+                                           material))
+
+    def set_wall_motion(self, wall, motion, params=(0.0, 0.0, 0.0)):
+        self._c(self.L.mdd_set_wall_motion(self.h, wall, motion, _p(_d(params))))
+
+    def set_wall_active(self, wall, active):
+        self._c(self.L.mdd_set_wall_active(self.h, wall, int(active)))
+
+    def walls(self):
+        """Per wall: last force pass' summed contact force, point, velocity."""
+        n = C.c_int(0)
+        self._c(self.L.mdd_get_walls(self.h, None, None, None, C.byref(n)))
+        acc, pt, vel = np.zeros(n.value), np.zeros((n.value, 3)), np.zeros((n.value, 3))
+        self._c(self.L.mdd_get_walls(self.h, _p(acc), _p(pt), _p(vel), C.byref(n)))
+        return dict(force=acc, point=pt, velocity=vel)
+
+    def wall_move(self):
+        self._c(self.L.mdd_wall_move(self.h))
+
+    def wall_zero_force_accumulators(self):
+        self._c(self.L.mdd_wall_zero_force_accumulators(self.h))
 
     def apply_gravity(self):
         self._c(self.L.mdd_apply_gravity(self.h))

@@ -57,9 +57,20 @@ typedef struct {
     int dim, sign;
 } swap_t;
 
+/* one wall (dem_wall/src/lib.rs:302-453): kind 0 plane, 1 cylinder, 2 sphere */
 typedef struct {
+    int kind, mat, active;
     double p[3], n[3], blo[3], bhi[3], v[3];
-    int mat;
+    /* plane motion (WallMotion, lib.rs:262-293): 0 static, 1 constant
+     * velocity, 2 oscillate (a0 amplitude, a1 frequency), 3 servo (a0 target
+     * force, a1 max velocity, a2 gain) */
+    int motion;
+    double a0, a1, a2, origin[3];
+    /* cylinder: axis, center (c[0], c[1]), radius, axial lo / hi; sphere:
+     * center c[0..2], radius */
+    int axis, inside;
+    double c[3], radius, lo, hi;
+    double acc; /* force_accumulator */
 } wall_t;
 
 struct orc_sys {
@@ -120,6 +131,7 @@ struct orc_sys {
     int hooke, dmt, roll_sds, twist_sds;
     int nwalls;
     wall_t walls[ORC_MAX_WALL];
+    double wall_time; /* Walls::time */
     int gravity_on;
     double gravity[3];
     /* LJ (md_lj lib.rs:230-239) */
@@ -492,17 +504,124 @@ void orc_set_gravity(orc_sys* s, int enabled, const double g[3]) {
 
 int orc_add_wall(orc_sys* s, const double point[3], const double normal[3], int material,
                  const double blo[3], const double bhi[3]) {
+    /* WallPlugin::build, plane walls (dem_wall/src/lib.rs:688-733): the
+     * normal is normalised, a zero normal is an error */
     if (s->nwalls >= ORC_MAX_WALL) return -1;
+    double mag = sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+    if (mag <= 1e-15) {
+        set_err(s, "wall normal vector must be non-zero");
+        return -1;
+    }
     wall_t* w = &s->walls[s->nwalls++];
+    memset(w, 0, sizeof(*w));
     for (int d = 0; d < 3; d++) {
         w->p[d] = point[d];
-        w->n[d] = normal[d];
-        w->blo[d] = blo[d];
-        w->bhi[d] = bhi[d];
-        w->v[d] = 0.0;
+        w->origin[d] = point[d];
+        w->n[d] = normal[d] / mag;
+        w->blo[d] = blo ? blo[d] : -INFINITY;
+        w->bhi[d] = bhi ? bhi[d] : INFINITY;
     }
+    w->kind = 0;
     w->mat = material;
+    w->active = 1;
+    return s->nwalls - 1;
+}
+
+int orc_add_cylinder_wall(orc_sys* s, int axis, double c0, double c1, double radius, double lo,
+                          double hi, int inside, int material) {
+    if (s->nwalls >= ORC_MAX_WALL || axis < 0 || axis > 2) return -1;
+    wall_t* w = &s->walls[s->nwalls++];
+    memset(w, 0, sizeof(*w));
+    w->kind = 1;
+    w->axis = axis;
+    w->c[0] = c0;
+    w->c[1] = c1;
+    w->radius = radius;
+    w->lo = lo;
+    w->hi = hi;
+    w->inside = inside != 0;
+    w->mat = material;
+    w->active = 1;
+    return s->nwalls - 1;
+}
+
+int orc_add_sphere_wall(orc_sys* s, const double center[3], double radius, int inside,
+                        int material) {
+    if (s->nwalls >= ORC_MAX_WALL) return -1;
+    wall_t* w = &s->walls[s->nwalls++];
+    memset(w, 0, sizeof(*w));
+    w->kind = 2;
+    for (int d = 0; d < 3; d++) w->c[d] = center[d];
+    w->radius = radius;
+    w->inside = inside != 0;
+    w->mat = material;
+    w->active = 1;
+    return s->nwalls - 1;
+}
+
+int orc_set_wall_motion(orc_sys* s, int wall, int motion, const double* params) {
+    if (wall < 0 || wall >= s->nwalls || s->walls[wall].kind != 0 || motion < 0 || motion > 3)
+        return -1;
+    wall_t* w = &s->walls[wall];
+    w->motion = motion;
+    w->v[0] = w->v[1] = w->v[2] = 0.0;
+    if (motion == 1)
+        for (int d = 0; d < 3; d++) w->v[d] = params[d];
+    if (motion >= 2) {
+        w->a0 = params[0];
+        w->a1 = params[1];
+        w->a2 = motion == 3 ? params[2] : 0.0;
+    }
     return 0;
+}
+
+int orc_set_wall_active(orc_sys* s, int wall, int active) {
+    if (wall < 0 || wall >= s->nwalls) return -1;
+    s->walls[wall].active = active != 0;
+    return 0;
+}
+
+int orc_get_walls(const orc_sys* s, double* force_acc, double* point, double* vel) {
+    for (int w = 0; w < s->nwalls; w++) {
+        if (force_acc) force_acc[w] = s->walls[w].acc;
+        for (int d = 0; d < 3; d++) {
+            if (point) point[3 * w + d] = s->walls[w].p[d];
+            if (vel) vel[3 * w + d] = s->walls[w].v[d];
+        }
+    }
+    return s->nwalls;
+}
+
+void orc_wall_move(orc_sys* s) {
+    /* wall_move (dem_wall/src/lib.rs:766-815), PreInitialIntegration */
+    const double PI = 3.141592653589793;
+    double dt = s->dt, time = s->wall_time;
+    for (int k = 0; k < s->nwalls; k++) {
+        wall_t* w = &s->walls[k];
+        if (w->kind != 0 || !w->active) continue;
+        if (w->motion == 1) {
+            for (int d = 0; d < 3; d++) w->p[d] += w->v[d] * dt;
+        } else if (w->motion == 2) {
+            double phase = 2.0 * PI * w->a1 * (time + dt);
+            double disp = w->a0 * sin(phase);
+            for (int d = 0; d < 3; d++) w->p[d] = w->origin[d] + disp * w->n[d];
+            double vel_mag = w->a0 * 2.0 * PI * w->a1 * cos(phase);
+            for (int d = 0; d < 3; d++) w->v[d] = vel_mag * w->n[d];
+        } else if (w->motion == 3) {
+            double error = w->a0 - w->acc;
+            double vel_mag = w->a2 * error;
+            if (vel_mag < -w->a1) vel_mag = -w->a1;
+            if (vel_mag > w->a1) vel_mag = w->a1;
+            for (int d = 0; d < 3; d++) w->v[d] = vel_mag * w->n[d];
+            for (int d = 0; d < 3; d++) w->p[d] += w->v[d] * dt;
+        }
+    }
+    s->wall_time += dt;
+}
+
+void orc_wall_zero_force_accumulators(orc_sys* s) {
+    /* wall_zero_force_accumulators (lib.rs:821-834), PreForce */
+    for (int k = 0; k < s->nwalls; k++) s->walls[k].acc = 0.0;
 }
 
 void orc_set_history_mode(orc_sys* s, int mode) { s->history_mode = mode; }
@@ -1472,12 +1591,36 @@ int orc_hertz_mindlin_contact_force(orc_sys* s) {
     return 0;
 }
 
+/* Hertz normal force of a curved wall with optional SJKR cohesion
+ * (cylinder and sphere walls, dem_wall/src/lib.rs:1080-1095, 1140-1155) */
+static double curved_wall_force(const orc_sys* s, int i, int mat, double delta, double nx,
+                                double ny, double nz) {
+    const double PI = 3.141592653589793;
+    int nm = s->nmat, k = (int)s->type[i] * nm + mat;
+    double radius = s->radius[i];
+    double r_eff = radius;
+    double e_eff = s->e_eff_ij[k];
+    double sdr = sqrt(delta * r_eff);
+    double k_n = 4.0 / 3.0 * e_eff * sdr;
+    double s_n = 2.0 * e_eff * sdr;
+    double m_r = s->mass[i];
+    double v_n = s->vel[i][0] * nx + s->vel[i][1] * ny + s->vel[i][2] * nz;
+    double beta = s->beta_ij[k];
+    double cohesion = s->extra_ij[ORC_X_COHESION_ENERGY][k];
+    double f_diss = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+    if (cohesion > 0.0) return k_n * delta - f_diss - cohesion * PI * delta * r_eff;
+    return fmax0(k_n * delta - f_diss);
+}
+
 void orc_wall_contact_force(orc_sys* s) {
-    /* wall_contact_force, plane walls (dem_wall/src/lib.rs:848-1002);
-     * surface_energy = cohesion = mu_tw = 0, static walls */
+    /* wall_contact_force (dem_wall/src/lib.rs:848-1241): plane walls with
+     * JKR / DMT / SJKR and twisting friction, then cylinder and sphere walls */
+    const double PI = 3.141592653589793;
     int nm = s->nmat;
     for (int w = 0; w < s->nwalls; w++) {
-        const wall_t* W = &s->walls[w];
+        wall_t* W = &s->walls[w];
+        if (W->kind != 0 || !W->active) continue;
+        double acc = 0.0;
         for (int i = 0; i < s->nlocal; i++) {
             double px = s->pos[i][0], py = s->pos[i][1], pz = s->pos[i][2];
             if (!(px >= W->blo[0] && px <= W->bhi[0] && py >= W->blo[1] && py <= W->bhi[1] &&
@@ -1487,27 +1630,152 @@ void orc_wall_contact_force(orc_sys* s) {
             double distance = dx * W->n[0] + dy * W->n[1] + dz * W->n[2];
             if (distance <= 0.0) continue;
             double radius = s->radius[i];
-            int mi = (int)s->type[i];
+            int k = (int)s->type[i] * nm + W->mat;
             double r_eff = radius;
-            double e_eff = s->e_eff_ij[mi * nm + W->mat];
+            double e_eff = s->e_eff_ij[k];
+            double surface = s->extra_ij[ORC_X_SURFACE_ENERGY][k];
+            int use_dmt = s->dmt;
+            double d_po = 0.0;
+            if (surface > 0.0 && !use_dmt)
+                d_po = cbrt(PI * PI * surface * surface * r_eff / (4.0 * e_eff * e_eff));
             double delta = radius - distance;
             if (0.5 * radius < delta) delta = 0.5 * radius;
-            if (delta <= 0.0) continue;
-            if (delta < -0.0) continue;
-            double sdr = sqrt(delta * r_eff);
-            double s_n = 2.0 * e_eff * sdr;
-            double k_n = 4.0 / 3.0 * e_eff * sdr;
+            if (delta <= 0.0 && surface <= 0.0) continue;
+            if (delta < -d_po) continue;
+            int adhesion_only = surface > 0.0 && !use_dmt && delta <= 0.0;
+            double s_n = 0.0, k_n = 0.0;
+            if (delta > 0.0) {
+                double sdr = sqrt(delta * r_eff);
+                s_n = 2.0 * e_eff * sdr;
+                k_n = 4.0 / 3.0 * e_eff * sdr;
+            }
             double m_r = s->mass[i];
             double vrx = s->vel[i][0] - W->v[0], vry = s->vel[i][1] - W->v[1],
                    vrz = s->vel[i][2] - W->v[2];
             double v_n = vrx * W->n[0] + vry * W->n[1] + vrz * W->n[2];
-            double beta = s->beta_ij[mi * nm + W->mat];
-            double f_diss = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
-            double f_net = fmax0(k_n * delta - f_diss);
+            double beta = s->beta_ij[k];
+            double cohesion = s->extra_ij[ORC_X_COHESION_ENERGY][k];
+            double f_net;
+            if (surface > 0.0 && use_dmt) {
+                double f_diss = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+                f_net = k_n * delta - f_diss - 2.0 * PI * surface * r_eff;
+            } else if (surface > 0.0) {
+                double f_adh = 1.5 * PI * surface * r_eff;
+                if (adhesion_only) {
+                    f_net = -f_adh;
+                } else {
+                    double f_diss = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+                    f_net = k_n * delta - f_diss - f_adh;
+                }
+            } else if (cohesion > 0.0) {
+                double f_diss = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+                f_net = k_n * delta - f_diss - cohesion * PI * delta * r_eff;
+            } else {
+                double f_diss = 2.0 * beta * SQRT_5_3 * sqrt(s_n * m_r) * v_n;
+                f_net = fmax0(k_n * delta - f_diss);
+            }
             s->force[i][0] += f_net * W->n[0];
             s->force[i][1] += f_net * W->n[1];
             s->force[i][2] += f_net * W->n[2];
+            if (delta > 0.0) {
+                double mu_tw = s->extra_ij[ORC_X_TWISTING_FRICTION][k];
+                if (mu_tw > 0.0) {
+                    double twist = s->omega[i][0] * W->n[0] + s->omega[i][1] * W->n[1] +
+                                   s->omega[i][2] * W->n[2];
+                    if (fabs(twist) > 1e-30) {
+                        double tau = mu_tw * fabs(f_net) * r_eff;
+                        double sign_tw = twist > 0.0 ? -1.0 : 1.0;
+                        s->torque[i][0] += sign_tw * tau * W->n[0];
+                        s->torque[i][1] += sign_tw * tau * W->n[1];
+                        s->torque[i][2] += sign_tw * tau * W->n[2];
+                    }
+                }
+            }
+            acc += f_net;
         }
+        W->acc += acc;
+    }
+    for (int w = 0; w < s->nwalls; w++) {
+        wall_t* W = &s->walls[w];
+        if (W->kind != 1 || !W->active) continue;
+        double acc = 0.0;
+        for (int i = 0; i < s->nlocal; i++) {
+            const double* pos = s->pos[i];
+            double radius = s->radius[i];
+            double axial, d0, d1;
+            if (W->axis == 0) {
+                axial = pos[0];
+                d0 = pos[1] - W->c[0];
+                d1 = pos[2] - W->c[1];
+            } else if (W->axis == 1) {
+                axial = pos[1];
+                d0 = pos[0] - W->c[0];
+                d1 = pos[2] - W->c[1];
+            } else {
+                axial = pos[2];
+                d0 = pos[0] - W->c[0];
+                d1 = pos[1] - W->c[1];
+            }
+            if (axial < W->lo || axial > W->hi) continue;
+            double radial = sqrt(d0 * d0 + d1 * d1);
+            if (radial < 1e-30) continue;
+            double inv_r = 1.0 / radial;
+            double delta, n0, n1;
+            if (W->inside) {
+                delta = radius - (W->radius - radial);
+                n0 = -d0 * inv_r;
+                n1 = -d1 * inv_r;
+            } else {
+                delta = radius - (radial - W->radius);
+                n0 = d0 * inv_r;
+                n1 = d1 * inv_r;
+            }
+            double nx, ny, nz;
+            if (W->axis == 0) {
+                nx = 0.0; ny = n0; nz = n1;
+            } else if (W->axis == 1) {
+                nx = n0; ny = 0.0; nz = n1;
+            } else {
+                nx = n0; ny = n1; nz = 0.0;
+            }
+            if (delta <= 0.0) continue;
+            if (0.5 * radius < delta) delta = 0.5 * radius;
+            double f_net = curved_wall_force(s, i, W->mat, delta, nx, ny, nz);
+            s->force[i][0] += f_net * nx;
+            s->force[i][1] += f_net * ny;
+            s->force[i][2] += f_net * nz;
+            acc += f_net;
+        }
+        W->acc += acc;
+    }
+    for (int w = 0; w < s->nwalls; w++) {
+        wall_t* W = &s->walls[w];
+        if (W->kind != 2 || !W->active) continue;
+        double acc = 0.0;
+        for (int i = 0; i < s->nlocal; i++) {
+            double dx = s->pos[i][0] - W->c[0], dy = s->pos[i][1] - W->c[1],
+                   dz = s->pos[i][2] - W->c[2];
+            double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist < 1e-30) continue;
+            double radius = s->radius[i];
+            double inv = 1.0 / dist;
+            double nx, ny, nz, delta;
+            if (W->inside) {
+                delta = radius - (W->radius - dist);
+                nx = -dx * inv; ny = -dy * inv; nz = -dz * inv;
+            } else {
+                delta = radius - (dist - W->radius);
+                nx = dx * inv; ny = dy * inv; nz = dz * inv;
+            }
+            if (delta <= 0.0) continue;
+            if (0.5 * radius < delta) delta = 0.5 * radius;
+            double f_net = curved_wall_force(s, i, W->mat, delta, nx, ny, nz);
+            s->force[i][0] += f_net * nx;
+            s->force[i][1] += f_net * ny;
+            s->force[i][2] += f_net * nz;
+            acc += f_net;
+        }
+        W->acc += acc;
     }
 }
 
@@ -1742,6 +2010,7 @@ int orc_step(orc_sys* s) {
     }
     /* PreInitialIntegration */
     if (s->nh_on) orc_nh_pre_initial(s);
+    if (s->model == ORC_MODEL_DEM) orc_wall_move(s);
     /* InitialIntegration */
     if (s->vv_on) orc_initial_integration(s);
     if (s->rot_on) orc_initial_rotation(s);
@@ -1757,6 +2026,8 @@ int orc_step(orc_sys* s) {
     if (s->state == ORC_STATE_FULL_REBUILD) orc_borders(s);
     /* Neighbor */
     if (s->state == ORC_STATE_FULL_REBUILD) orc_bin_neighbor_list(s);
+    /* PreForce */
+    if (s->model == ORC_MODEL_DEM) orc_wall_zero_force_accumulators(s);
     /* Force */
     if (s->model == ORC_MODEL_DEM) {
         if (orc_hertz_mindlin_contact_force(s) != 0) return -2;

@@ -81,8 +81,22 @@ void orc_get_lj(const orc_sys* s, double out[3]); /* lj1, lj2, cutoff2 */
 void orc_set_nose_hoover(orc_sys* s, int enabled, double temperature, double coupling);
 void orc_set_integrators(orc_sys* s, int velocity_verlet, int rotation);
 void orc_set_gravity(orc_sys* s, int enabled, const double g[3]);
+/* walls (dem_wall/src/lib.rs:302-453); each returns the wall index or -1.
+ * Plane: normal normalised, bounds may be NULL (unbounded). */
 int orc_add_wall(orc_sys* s, const double point[3], const double normal[3], int material,
                  const double bound_lo[3], const double bound_hi[3]);
+int orc_add_cylinder_wall(orc_sys* s, int axis, double c0, double c1, double radius, double lo,
+                          double hi, int inside, int material);
+int orc_add_sphere_wall(orc_sys* s, const double center[3], double radius, int inside,
+                        int material);
+/* plane motion: 0 static, 1 constant velocity (params v[3]), 2 oscillate
+ * (amplitude, frequency), 3 servo (target_force, max_velocity, gain) */
+int orc_set_wall_motion(orc_sys* s, int wall, int motion, const double* params);
+int orc_set_wall_active(orc_sys* s, int wall, int active);
+/* per wall: force accumulator, current point and velocity (3 each); any may be NULL */
+int orc_get_walls(const orc_sys* s, double* force_acc, double* point, double* vel);
+void orc_wall_move(orc_sys* s);                        /* dem_wall lib.rs:766-815 */
+void orc_wall_zero_force_accumulators(orc_sys* s);     /* dem_wall lib.rs:821-834 */
 void orc_set_history_mode(orc_sys* s, int mode);
 
 /* ---- state ---------------------------------------------------------------
@@ -113,7 +127,7 @@ void orc_sort_atoms_by_bin(orc_sys* s);               /* neighbor.rs:686-794 */
 void orc_borders(orc_sys* s);                         /* comm.rs:449-570 */
 void orc_bin_neighbor_list(orc_sys* s);               /* neighbor.rs:810-1080 */
 int orc_hertz_mindlin_contact_force(orc_sys* s);      /* contact.rs:103-615, 622-990 */
-void orc_wall_contact_force(orc_sys* s);              /* dem_wall lib.rs:848-1002 */
+void orc_wall_contact_force(orc_sys* s);              /* dem_wall lib.rs:848-1241 */
 void orc_apply_gravity(orc_sys* s);                   /* mddem_fixes lib.rs:756-761 */
 void orc_lj_force(orc_sys* s);                        /* md_lj lib.rs:488-633 */
 void orc_lj_virial(const orc_sys* s, double w[6]);   /* md_lj lib.rs:510-585 */

@@ -8,6 +8,7 @@ file:line each one restates.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -61,6 +62,14 @@ def lib():
             "orc_set_integrators": (None, [vp, C.c_int, C.c_int]),
             "orc_set_gravity": (None, [vp, C.c_int, _dp]),
             "orc_add_wall": (C.c_int, [vp, _dp, _dp, C.c_int, _dp, _dp]),
+            "orc_add_cylinder_wall": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                                C.c_double, C.c_double, C.c_int, C.c_int]),
+            "orc_add_sphere_wall": (C.c_int, [vp, _dp, C.c_double, C.c_int, C.c_int]),
+            "orc_set_wall_motion": (C.c_int, [vp, C.c_int, C.c_int, _dp]),
+            "orc_set_wall_active": (C.c_int, [vp, C.c_int, C.c_int]),
+            "orc_get_walls": (C.c_int, [vp, _dp, _dp, _dp]),
+            "orc_wall_move": (None, [vp]),
+            "orc_wall_zero_force_accumulators": (None, [vp]),
             "orc_set_history_mode": (None, [vp, C.c_int]),
             "orc_set_atoms": (C.c_int, [vp, C.c_int, _u32p, _u32p, _dp, _dp, _dp, _dp, _dp, _dp]),
             "orc_nlocal": (C.c_int, [vp]),
@@ -225,9 +234,35 @@ class Oracle:
     def set_gravity(self, g, enabled=True):
         self.L.orc_set_gravity(self.h, int(enabled), _ptr(_d(g)))
 
+    def _index(self, k):
+        if k < 0:
+            raise RuntimeError(self.L.orc_error(self.h).decode() or "wall rejected")
+        return k
+
     def add_wall(self, point, normal, material=0, bound_lo=(-1e300,) * 3, bound_hi=(1e300,) * 3):
-        self._check(self.L.orc_add_wall(self.h, _ptr(_d(point)), _ptr(_d(normal)), material,
-                                        _ptr(_d(bound_lo)), _ptr(_d(bound_hi))))
+        return self._index(self.L.orc_add_wall(self.h, _ptr(_d(point)), _ptr(_d(normal)),
+                                               material, _ptr(_d(bound_lo)), _ptr(_d(bound_hi))))
+
+    def add_cylinder_wall(self, axis, center, radius, lo=-math.inf, hi=math.inf, inside=False,
+                          material=0):
+        return self._index(self.L.orc_add_cylinder_wall(self.h, axis, center[0], center[1],
+                                                        radius, lo, hi, int(inside), material))
+
+    def add_sphere_wall(self, center, radius, inside=False, material=0):
+        return self._index(self.L.orc_add_sphere_wall(self.h, _ptr(_d(center)), radius,
+                                                      int(inside), material))
+
+    def set_wall_motion(self, wall, motion, params=(0.0, 0.0, 0.0)):
+        self._check(self.L.orc_set_wall_motion(self.h, wall, motion, _ptr(_d(params))))
+
+    def set_wall_active(self, wall, active):
+        self._check(self.L.orc_set_wall_active(self.h, wall, int(active)))
+
+    def walls(self):
+        n = self.L.orc_get_walls(self.h, None, None, None)
+        acc, pt, vel = np.zeros(n), np.zeros((n, 3)), np.zeros((n, 3))
+        self.L.orc_get_walls(self.h, _ptr(acc), _ptr(pt), _ptr(vel))
+        return dict(force=acc, point=pt, velocity=vel)
 
     def set_history_mode(self, mode):
         self.L.orc_set_history_mode(self.h, mode)
