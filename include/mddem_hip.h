@@ -112,7 +112,7 @@ int mdd_set_integrators(mdd_ctx* ctx, int velocity_verlet, int rotation);
 /* [gravity] section, GravityPlugin (crates/mddem_fixes/src/lib.rs:735-761) */
 int mdd_set_gravity(mdd_ctx* ctx, int enabled, const double g[3]);
 /* [[wall]] entries (WallPlugin::build, crates/dem_wall/src/lib.rs:546-757).
- * Walls share one index space in the order they are added (at most 8).
+ * Walls share one index space in the order they are added (at most 12).
  * Plane: WallPlane (lib.rs:302-357), the normal is normalised (a zero normal
  * is an error), bounds may be NULL (unbounded). */
 int mdd_add_wall(mdd_ctx* ctx, const double point[3], const double normal[3], int material,
@@ -124,6 +124,16 @@ int mdd_add_cylinder_wall(mdd_ctx* ctx, int axis, double c0, double c1, double r
 /* type = "sphere", WallSphere (lib.rs:411-428) */
 int mdd_add_sphere_wall(mdd_ctx* ctx, const double center[3], double radius, int inside,
                         int material);
+/* type = "region", WallRegion (lib.rs:435-447, forces 1170-1231): the
+ * region tree's leaf primitives in depth-first order (Union and Intersect
+ * both take the child surface nearest the particle, mddem_core/src/
+ * region.rs:282-309, so any tree flattens to its leaves), at most 8, each
+ * with 12 parameters (unused trailing ones zero):
+ *   0 block min[3] max[3]; 1 sphere center[3] radius;
+ *   2 cylinder center[2] radius axis lo hi; 3 cone center[2] axis rad_lo
+ *   rad_hi lo hi; 4 plane point[3] normal[3] */
+int mdd_add_region_wall(mdd_ctx* ctx, int nprim, const int* types, const double* params,
+                        int inside, int material);
 /* plane wall motion (WallMotion, lib.rs:262-293): 0 static, 1 constant
  * velocity (params: v[3]), 2 oscillate (amplitude, frequency), 3 servo
  * (target_force, max_velocity, gain). The wall restarts from its configured

@@ -813,6 +813,31 @@ int mdd_add_sphere_wall(mdd_ctx* ctx, const double center[3], double radius, int
     });
 }
 
+int mdd_add_region_wall(mdd_ctx* ctx, int nprim, const int* types, const double* params,
+                        int inside, int material) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (nprim < 1 || nprim > mdd::kMaxRegionPrims || !types || !params)
+            throw ApiError(MDD_ERR_INVALID, "a region wall takes 1..8 primitives");
+        for (int k = 0; k < nprim; k++) {
+            if (types[k] < 0 || types[k] > 4)
+                throw ApiError(MDD_ERR_INVALID, "region primitive type must be 0..4");
+            const int axis = types[k] == 2 ? int(params[12 * k + 3])
+                                           : (types[k] == 3 ? int(params[12 * k + 2]) : 0);
+            if (axis < 0 || axis > 2) throw ApiError(MDD_ERR_INVALID, "region axis must be 0..2");
+        }
+        mdd::Wall& w = new_wall(c, material);
+        w.kind = 3;
+        w.nprim = nprim;
+        for (int k = 0; k < nprim; k++) {
+            w.ptype[k] = types[k];
+            for (int q = 0; q < 12; q++) w.prm[k][q] = params[12 * k + q];
+        }
+        w.inside = inside ? 1 : 0;
+        c.walls.count++;
+        sync_walls(c, c.walls.count - 1);
+    });
+}
+
 int mdd_set_wall_motion(mdd_ctx* ctx, int wall, int motion, const double* params) {
     return guarded(ctx, [&](Ctx& c) {
         mdd::Wall& w = wall_at(c, wall);

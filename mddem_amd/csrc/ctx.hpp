@@ -131,11 +131,12 @@ struct SysP {
 // One wall (dem_wall/src/lib.rs:302-453). kind 0: plane through p with unit
 // normal n, active inside [blo, bhi]; 1: cylinder about axis through (c0, c1)
 // of radius `radius`, axial range [lo, hi]; 2: sphere about c of radius
-// `radius`. inside: particles live inside the curved surface. Plane motion
+// `radius`; 3: region surface. inside: particles live inside the curved surface. Plane motion
 // (WallMotion, lib.rs:262-293): 0 static, 1 constant velocity (v), 2
 // oscillate (a0 amplitude, a1 frequency), 3 servo (a0 target force, a1 max
 // velocity, a2 gain). p is the configured (origin) point; the current point
 // and velocity live on the device (WallsDyn).
+constexpr int kMaxRegionPrims = 8;
 struct Wall {
     double p[3], n[3], blo[3], bhi[3];
     int mat;
@@ -143,8 +144,15 @@ struct Wall {
     double v[3], a0, a1, a2;
     double c[3], radius, lo, hi;
     int axis, inside;
+    // kind 3, region surface (WallRegion, lib.rs:435-447): the region tree's
+    // leaf primitives in depth-first order, 12 parameters each (block: min,
+    // max; sphere: center, radius; cylinder: center[2], radius, axis, lo, hi;
+    // cone: center[2], axis, rad_lo, rad_hi, lo, hi; plane: point, normal)
+    int nprim;
+    int ptype[kMaxRegionPrims];
+    double prm[kMaxRegionPrims][12];
 };
-constexpr int kMaxWalls = 8;
+constexpr int kMaxWalls = 12;
 // scalar contact force sums (force_accumulator) are kept in fixed point:
 // integer sums do not depend on the order the atomics land in
 constexpr double kWallAccScale = 4294967296.0;  // 2^32 per newton

@@ -60,6 +60,8 @@ struct ForceArgs {
     int64_t hist_cs;
     const WallsP* walls;  // wall table (device copy)
     WallsDyn* wdyn;       // current wall points / velocities and force sums
+    int has_regions;      // region walls present: applied in their own pass
+    int regions_only;     // that pass: region walls (then gravity) only
     int use_walls, use_gravity, use_pairs;
     double g[3];
     double* f64;
@@ -291,6 +293,249 @@ __device__ inline double curved_wall_force(const ForceArgs& a, int k, double rad
     return f > 0.0 ? f : 0.0;
 }
 
+// Region::closest_point_on_surface (mddem_core/src/region.rs:260-772):
+// signed distance (positive outside), surface point, outward normal
+struct Surf {
+    double pt[3], nrm[3], d;
+};
+
+__device__ inline double clamp_d(double x, double lo, double hi) {
+    return x < lo ? lo : (x > hi ? hi : x);
+}
+
+// (axial, offsets from the centre) for an axis, and back
+__device__ inline void axis_split(const double* x, const double* c, int ax, double& a,
+                                  double& d0, double& d1) {
+    a = x[ax];
+    d0 = x[ax == 0 ? 1 : 0] - c[0];
+    d1 = x[ax == 2 ? 1 : 2] - c[1];
+}
+__device__ inline void axis_join(double a, double r0, double r1, const double* c, int ax,
+                                 double* out) {
+    out[ax] = a;
+    out[ax == 0 ? 1 : 0] = c[0] + r0;
+    out[ax == 2 ? 1 : 2] = c[1] + r1;
+}
+__device__ inline void axis_join_n(double a, double n0, double n1, int ax, double* out) {
+    out[ax] = a;
+    out[ax == 0 ? 1 : 0] = n0;
+    out[ax == 2 ? 1 : 2] = n1;
+}
+
+__device__ inline Surf surf_from(const double* pt, const double* nrm, double d) {
+    Surf r;
+    for (int q = 0; q < 3; q++) {
+        r.pt[q] = pt[q];
+        r.nrm[q] = nrm[q];
+    }
+    r.d = d;
+    return r;
+}
+
+// a point outside: the candidate's point, the normal along x - point
+__device__ inline Surf surf_outside(const double* x, const double* pt, const double* nrm) {
+    const double dx = x[0] - pt[0], dy = x[1] - pt[1], dz = x[2] - pt[2];
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist < 1e-30) return surf_from(pt, nrm, 0.0);
+    const double n[3] = {dx / dist, dy / dist, dz / dist};
+    return surf_from(pt, n, dist);
+}
+
+__device__ inline double norm_diff(const double* a, const double* b) {
+    const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ Surf surf_block(const double* x, const double* mn, const double* mx) {
+    const bool in = x[0] >= mn[0] && x[0] <= mx[0] && x[1] >= mn[1] && x[1] <= mx[1] &&
+                    x[2] >= mn[2] && x[2] <= mx[2];
+    if (!in) {
+        const double cl[3] = {clamp_d(x[0], mn[0], mx[0]), clamp_d(x[1], mn[1], mx[1]),
+                              clamp_d(x[2], mn[2], mx[2])};
+        const double up[3] = {0.0, 0.0, 1.0};
+        return surf_outside(x, cl, up);
+    }
+    double best = x[0] - mn[0];
+    int bi = 0;
+    for (int k = 1; k < 6; k++) {
+        const int axk = k >> 1;
+        const double dk = (k & 1) ? mx[axk] - x[axk] : x[axk] - mn[axk];
+        if (dk < best) {
+            best = dk;
+            bi = k;
+        }
+    }
+    const int ax = bi >> 1;
+    double pt[3] = {x[0], x[1], x[2]}, n[3] = {0.0, 0.0, 0.0};
+    pt[ax] = (bi & 1) ? mx[ax] : mn[ax];
+    n[ax] = (bi & 1) ? 1.0 : -1.0;
+    return surf_from(pt, n, -best);
+}
+
+__device__ Surf surf_sphere(const double* x, const double* c, double rad) {
+    const double dx = x[0] - c[0], dy = x[1] - c[1], dz = x[2] - c[2];
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist < 1e-30) {
+        const double pt[3] = {c[0] + rad, c[1], c[2]}, n[3] = {1.0, 0.0, 0.0};
+        return surf_from(pt, n, -rad);
+    }
+    const double inv = 1.0 / dist;
+    const double n[3] = {dx * inv, dy * inv, dz * inv};
+    const double pt[3] = {c[0] + n[0] * rad, c[1] + n[1] * rad, c[2] + n[2] * rad};
+    return surf_from(pt, n, dist - rad);
+}
+
+__device__ Surf surf_cylinder(const double* x, const double* c, double rad, int ax, double lo,
+                              double hi) {
+    double a, d0, d1;
+    axis_split(x, c, ax, a, d0, d1);
+    const double rd = sqrt(d0 * d0 + d1 * d1);
+    const double ac = clamp_d(a, lo, hi);
+    const bool a_in = a >= lo && a <= hi;
+    double cpt[3], cn[3], cdist;
+    if (rd < 1e-30) {
+        axis_join(ac, rad, 0.0, c, ax, cpt);
+        axis_join_n(0.0, 1.0, 0.0, ax, cn);
+        const double da = a < lo ? lo - a : a - hi;
+        cdist = a_in ? -rad : sqrt(rad * rad + da * da);
+    } else {
+        const double inv = 1.0 / rd;
+        axis_join(ac, d0 * inv * rad, d1 * inv * rad, c, ax, cpt);
+        axis_join_n(0.0, d0 * inv, d1 * inv, ax, cn);
+        if (a_in) {
+            cdist = rd - rad;
+        } else {
+            const double da = a < lo ? lo - a : a - hi;
+            const double dr = rd - rad;
+            cdist = sqrt(dr * dr + da * da) * ((dr < 0.0 && da == 0.0) ? -1.0 : 1.0);
+        }
+    }
+    const double cap_r = rd <= rad ? rd : rad;
+    double c0 = 0.0, c1 = 0.0;
+    if (!(rd < 1e-30)) {
+        const double inv = 1.0 / rd;
+        c0 = d0 * inv * cap_r;
+        c1 = d1 * inv * cap_r;
+    }
+    double lpt[3], ln[3], hpt[3], hn[3];
+    axis_join(lo, c0, c1, c, ax, lpt);
+    axis_join_n(-1.0, 0.0, 0.0, ax, ln);
+    axis_join(hi, c0, c1, c, ax, hpt);
+    axis_join_n(1.0, 0.0, 0.0, ax, hn);
+    const bool in = a_in && rd <= rad;
+    if (in) {
+        const double dc = fabs(rad - rd), dl = a - lo, dh = hi - a;
+        if (dc <= dl && dc <= dh) return surf_from(cpt, cn, rd - rad);
+        if (dl <= dh) return surf_from(lpt, ln, -(a - lo));
+        return surf_from(hpt, hn, -(hi - a));
+    }
+    const double el = norm_diff(x, lpt), eh = norm_diff(x, hpt);
+    const double lo_d = a < lo ? (a <= lo ? lo - a : el) : el;
+    const double hi_d = a > hi ? (a >= hi ? a - hi : eh) : eh;
+    const double k0 = fabs(cdist);
+    int best = 0;
+    double bk = k0;
+    if (lo_d < bk) {
+        best = 1;
+        bk = lo_d;
+    }
+    if (hi_d < bk) best = 2;
+    if (best == 0) return surf_outside(x, cpt, cn);
+    if (best == 1) return surf_outside(x, lpt, ln);
+    return surf_outside(x, hpt, hn);
+}
+
+__device__ Surf surf_cone(const double* x, const double* c, int ax, double rlo, double rhi,
+                          double lo, double hi) {
+    double a, d0, d1;
+    axis_split(x, c, ax, a, d0, d1);
+    const double rd = sqrt(d0 * d0 + d1 * d1);
+    const double h = hi - lo;
+    double n0 = 1.0, n1 = 0.0;
+    if (!(rd < 1e-30)) {
+        n0 = d0 / rd;
+        n1 = d1 / rd;
+    }
+    const bool a_in = a >= lo && a <= hi;
+    const double r_at =
+        fabs(h) < 1e-30 ? (rlo + rhi) * 0.5 : rlo + clamp_d((a - lo) / h, 0.0, 1.0) * (rhi - rlo);
+    const bool in = a_in && rd <= r_at;
+    // lateral surface: nearest point of the segment (lo, rlo)-(hi, rhi) in
+    // (axial, radial) coordinates
+    const double sa = hi - lo, sr = rhi - rlo;
+    const double s2 = sa * sa + sr * sr;
+    const double t = s2 < 1e-30 ? 0.5 : clamp_d(((a - lo) * sa + (rd - rlo) * sr) / s2, 0.0, 1.0);
+    const double pa = lo + t * sa, pr = rlo + t * sr;
+    const double da = a - pa, dr = rd - pr;
+    const double d2 = sqrt(da * da + dr * dr);
+    double cpt[3], cn[3];
+    axis_join(pa, n0 * pr, n1 * pr, c, ax, cpt);
+    double na = -sr, nr = sa;
+    const double nl = sqrt(na * na + nr * nr);
+    if (nl > 1e-30) {
+        na /= nl;
+        nr /= nl;
+    } else {
+        na = 0.0;
+        nr = 1.0;
+    }
+    axis_join_n(na, nr * n0, nr * n1, ax, cn);
+    const double rr = rd > 0.0 ? rd : 0.0;
+    const double lcr = rlo < rr ? rlo : rr, hcr = rhi < rr ? rhi : rr;
+    double lpt[3], ln[3], hpt[3], hn[3];
+    axis_join(lo, n0 * lcr, n1 * lcr, c, ax, lpt);
+    axis_join_n(-1.0, 0.0, 0.0, ax, ln);
+    axis_join(hi, n0 * hcr, n1 * hcr, c, ax, hpt);
+    axis_join_n(1.0, 0.0, 0.0, ax, hn);
+    if (in) {
+        const double dl = a - lo, dh = hi - a;
+        if (d2 <= dl && d2 <= dh) return surf_from(cpt, cn, -d2);
+        if (dl <= dh) return surf_from(lpt, ln, -dl);
+        return surf_from(hpt, hn, -dh);
+    }
+    const double el = norm_diff(x, lpt), eh = norm_diff(x, hpt);
+    int best = 0;
+    double bk = d2;
+    if (el < bk) {
+        best = 1;
+        bk = el;
+    }
+    if (eh < bk) best = 2;
+    if (best == 0) return surf_outside(x, cpt, cn);
+    if (best == 1) return surf_outside(x, lpt, ln);
+    return surf_outside(x, hpt, hn);
+}
+
+__device__ Surf surf_plane(const double* x, const double* p0, const double* nv) {
+    const double mag = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    double n[3] = {0.0, 0.0, 1.0};
+    if (mag > 1e-30)
+        for (int q = 0; q < 3; q++) n[q] = nv[q] / mag;
+    const double d = (x[0] - p0[0]) * n[0] + (x[1] - p0[1]) * n[1] + (x[2] - p0[2]) * n[2];
+    const double pt[3] = {x[0] - d * n[0], x[1] - d * n[1], x[2] - d * n[2]};
+    return surf_from(pt, n, d);
+}
+
+__device__ Surf surf_prim(int type, const double* q, const double* x) {
+    switch (type) {
+        case 0: return surf_block(x, q, q + 3);
+        case 1: return surf_sphere(x, q, q[3]);
+        case 2: return surf_cylinder(x, q, q[2], int(q[3]), q[4], q[5]);
+        case 3: return surf_cone(x, q, int(q[2]), q[3], q[4], q[5], q[6]);
+        default: return surf_plane(x, q, q + 3);
+    }
+}
+
+// nearest leaf surface, the first of equals (Union / Intersect, region.rs:282-309)
+__device__ Surf surf_region(const Wall& W, const double* x) {
+    Surf best = surf_prim(W.ptype[0], W.prm[0], x);
+    for (int k = 1; k < W.nprim; k++) {
+        const Surf r = surf_prim(W.ptype[k], W.prm[k], x);
+        if (fabs(r.d) < fabs(best.d)) best = r;
+    }
+    return best;
+}
+
 __device__ inline void wall_acc(const ForceArgs& a, int w, double f) {
     atomicAdd(&a.wdyn->w[w].acc,
               static_cast<unsigned long long>(__double2ll_rn(f * kWallAccScale)));
@@ -412,6 +657,34 @@ __device__ inline void wall_forces(const ForceArgs& a, const double4& p, const d
         const double delta0 = W.inside ? radius - (W.radius - dist) : radius - (dist - W.radius);
         if (delta0 <= 0.0) continue;
         const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
+        const double f_net =
+            curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
+        fx += R(f_net * nx);
+        fy += R(f_net * ny);
+        fz += R(f_net * nz);
+        wall_acc(a, w, f_net);
+    }
+}
+
+// region-surface walls (dem_wall/src/lib.rs:1170-1231), the last wall type a
+// particle sees; kept out of the pair kernels (a separate pass after them)
+template <typename R>
+__device__ void region_wall_forces(const ForceArgs& a, const double4& p, const double4& v,
+                                   double mass, uint32_t ti, R& fx, R& fy, R& fz) {
+    const int nm = a.nm;
+    const double radius = p.w;
+    const WallsP& WS = *a.walls;
+    for (int w = 0; w < WS.count; w++) {
+        const Wall& W = WS.w[w];
+        if (W.kind != 3 || !W.active) continue;
+        const double x[3] = {p.x, p.y, p.z};
+        const Surf sr = surf_region(W, x);
+        const double gap = W.inside ? -sr.d : sr.d;
+        const double delta0 = radius - gap;
+        if (delta0 <= 0.0) continue;
+        const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
+        const double sg = W.inside ? -1.0 : 1.0;
+        const double nx = sg * sr.nrm[0], ny = sg * sr.nrm[1], nz = sg * sr.nrm[2];
         const double f_net =
             curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
         fx += R(f_net * nx);
@@ -914,6 +1187,7 @@ template <typename R>
 __global__ void k_walls_gravity(ForceArgs a) {
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (i >= a.n) return;
+    if (gated(a)) return;
     R* f = nullptr;
     if constexpr (sizeof(R) == 8)
         f = a.f64;
@@ -926,9 +1200,12 @@ __global__ void k_walls_gravity(ForceArgs a) {
     else
         t = a.t32;
     R tx = t ? t[i] : R(0), ty = t ? t[a.npad + i] : R(0), tz = t ? t[2 * a.npad + i] : R(0);
-    if (a.use_walls)
-        wall_forces<R>(a, a.pos[i], a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i],
-                       a.type[i], fx, fy, fz, tx, ty, tz);
+    if (a.use_walls) {
+        if (!a.regions_only)
+            wall_forces<R>(a, a.pos[i], a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i],
+                           a.type[i], fx, fy, fz, tx, ty, tz);
+        if (a.has_regions) region_wall_forces<R>(a, a.pos[i], a.vel[i], a.mass[i], a.type[i], fx, fy, fz);
+    }
     if (t) {
         t[i] = tx;
         t[a.npad + i] = ty;
@@ -980,6 +1257,7 @@ ForceArgs make_args(Ctx& c, int step) {
     a.hist_cs = int64_t(c.maxnb) * c.npad;
     a.walls = c.walls_dev.p;
     a.wdyn = c.wdyn.p;
+    for (int k = 0; k < c.walls.count; k++) a.has_regions |= c.walls.w[k].kind == 3 ? 1 : 0;
     a.g[0] = c.g[0];
     a.g[1] = c.g[1];
     a.g[2] = c.g[2];
@@ -1089,10 +1367,24 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
         a.use_pairs = 1;
         a.use_walls = step >= 0 && c.walls.count > 0;
         a.use_gravity = step >= 0 && c.gravity_on;
+        // region walls come last among the walls and before gravity: their
+        // own pass follows the pair pass and adds gravity after them
+        const bool regions = a.use_walls && a.has_regions;
+        if (regions) a.use_gravity = 0;
         if (mixed)
             launch_hertz<float>(c, a);
         else
             launch_hertz<double>(c, a);
+        if (regions) {
+            ForceArgs b = a;
+            b.regions_only = 1;
+            b.use_gravity = c.gravity_on ? 1 : 0;
+            const int nb = blocks_for(c.n, 256);
+            if (mixed)
+                k_walls_gravity<float><<<nb, 256, 0, c.stream>>>(b);
+            else
+                k_walls_gravity<double><<<nb, 256, 0, c.stream>>>(b);
+        }
     }
     MDD_HIP(hipGetLastError());
     c.force_launches++;

@@ -33,6 +33,59 @@ _u8p = C.POINTER(C.c_uint8)
 _i64p = C.POINTER(C.c_int64)
 
 
+_AXES = {"x": 0, "y": 1, "z": 2}
+
+
+def flatten_region(region):
+    """Leaf primitives of a region tree (Region, mddem_core/src/region.rs:
+    40-78) in depth-first order, as (types, params[n, 12]) for
+    mdd_add_region_wall. A region is a dict like the reference's TOML:
+    {"type": "block", "min": [..], "max": [..]}, "sphere" (center, radius),
+    "cylinder" (center[2], radius, axis, lo, hi), "cone" (center[2], axis,
+    rad_lo, rad_hi, lo, hi), "plane" (point, normal), "union" / "intersect"
+    (regions: [...]); both combinations take the child surface nearest the
+    point, so a tree and its leaf list give the same surface."""
+    types, rows = [], []
+
+    def axis(r):
+        a = r.get("axis", "z")
+        return float(_AXES[a.lower()] if isinstance(a, str) else a)
+
+    def walk(r):
+        t = r["type"].lower()
+        q = np.zeros(12)
+        if t in ("union", "intersect"):
+            if not r["regions"]:
+                raise ValueError(f"{t} needs at least one region")
+            for child in r["regions"]:
+                walk(child)
+            return
+        if t == "block":
+            q[0:3], q[3:6] = r["min"], r["max"]
+            types.append(0)
+        elif t == "sphere":
+            q[0:3], q[3] = r["center"], r["radius"]
+            types.append(1)
+        elif t == "cylinder":
+            q[0:2], q[2], q[3], q[4], q[5] = r["center"], r["radius"], axis(r), r["lo"], r["hi"]
+            types.append(2)
+        elif t == "cone":
+            q[0:2], q[2] = r["center"], axis(r)
+            q[3], q[4], q[5], q[6] = r["rad_lo"], r["rad_hi"], r["lo"], r["hi"]
+            types.append(3)
+        elif t == "plane":
+            q[0:3], q[3:6] = r["point"], r["normal"]
+            types.append(4)
+        else:
+            raise ValueError(f"unknown region type {t!r}")
+        rows.append(q)
+
+    walk(region)
+    if len(types) > 8:
+        raise ValueError("a region wall takes at most 8 primitives")
+    return types, np.array(rows)
+
+
 class MddError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -84,6 +137,7 @@ SIGNATURES = {
                                         C.c_double, C.c_double, C.c_int, C.c_int]),
     "mdd_add_sphere_wall": (C.c_int, [_vp, _dp, C.c_double, C.c_int, C.c_int]),
     "mdd_set_wall_motion": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "mdd_add_region_wall": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _dp, C.c_int, C.c_int]),
     "mdd_set_wall_active": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mdd_get_walls": (C.c_int, [_vp, _dp, _dp, _dp, C.POINTER(C.c_int)]),
     "mdd_wall_move": (C.c_int, [_vp]),
@@ -388,6 +442,13 @@ class System:
 
     def add_sphere_wall(self, center, radius, inside=False, material=0):
         self._c(self.L.mdd_add_sphere_wall(self.h, _p(_d(center)), radius, int(inside),
+                                           material))
+
+    def add_region_wall(self, region, inside=False, material=0):
+        """A region-surface wall; `region` as flatten_region takes it."""
+        types, params = flatten_region(region)
+        t = (C.c_int * len(types))(*types)
+        self._c(self.L.mdd_add_region_wall(self.h, len(types), t, _p(_d(params)), int(inside),
                                            material))
 
     def set_wall_motion(self, wall, motion, params=(0.0, 0.0, 0.0)):

@@ -15,6 +15,7 @@
 
 #define ORC_MAX_MAT 64
 #define ORC_MAX_WALL 16
+#define ORC_MAX_PRIM 8
 /* dem_atom/src/lib.rs:63 */
 #define SQRT_5_3 0.9128709291752768
 /* dem_granular/src/lib.rs:93-108 */
@@ -71,6 +72,13 @@ typedef struct {
     int axis, inside;
     double c[3], radius, lo, hi;
     double acc; /* force_accumulator */
+    /* kind 3, region surface: leaf primitives of the region tree in
+     * depth-first order (unions and intersections both take the child
+     * surface nearest the point, region.rs:282-309, so any tree flattens to
+     * its leaves); 12 parameters each, see orc_add_region_wall */
+    int nprim;
+    int ptype[ORC_MAX_PRIM];
+    double prm[ORC_MAX_PRIM][12];
 } wall_t;
 
 struct orc_sys {
@@ -557,6 +565,336 @@ int orc_add_sphere_wall(orc_sys* s, const double center[3], double radius, int i
     w->mat = material;
     w->active = 1;
     return s->nwalls - 1;
+}
+
+int orc_add_region_wall(orc_sys* s, int nprim, const int* types, const double* params,
+                        int inside, int material) {
+    if (s->nwalls >= ORC_MAX_WALL || nprim < 1 || nprim > ORC_MAX_PRIM) return -1;
+    wall_t* w = &s->walls[s->nwalls++];
+    memset(w, 0, sizeof(*w));
+    w->kind = 3;
+    w->nprim = nprim;
+    for (int k = 0; k < nprim; k++) {
+        if (types[k] < 0 || types[k] > 4) {
+            s->nwalls--;
+            return -1;
+        }
+        w->ptype[k] = types[k];
+        memcpy(w->prm[k], params + 12 * k, sizeof(double) * 12);
+    }
+    w->inside = inside != 0;
+    w->mat = material;
+    w->active = 1;
+    return s->nwalls - 1;
+}
+
+/* ---- Region::closest_point_on_surface (mddem_core/src/region.rs:260-772) */
+typedef struct {
+    double point[3], normal[3], distance;
+} surf_t;
+
+static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+static void decompose(const double* pos, const double* c, int axis, double* ax, double* d0,
+                      double* d1) {
+    if (axis == 0) {
+        *ax = pos[0]; *d0 = pos[1] - c[0]; *d1 = pos[2] - c[1];
+    } else if (axis == 1) {
+        *ax = pos[1]; *d0 = pos[0] - c[0]; *d1 = pos[2] - c[1];
+    } else {
+        *ax = pos[2]; *d0 = pos[0] - c[0]; *d1 = pos[1] - c[1];
+    }
+}
+
+static void compose(double ax, double r0, double r1, const double* c, int axis, double* out) {
+    if (axis == 0) {
+        out[0] = ax; out[1] = c[0] + r0; out[2] = c[1] + r1;
+    } else if (axis == 1) {
+        out[0] = c[0] + r0; out[1] = ax; out[2] = c[1] + r1;
+    } else {
+        out[0] = c[0] + r0; out[1] = c[1] + r1; out[2] = ax;
+    }
+}
+
+static void compose_n(double ax, double n0, double n1, int axis, double* out) {
+    if (axis == 0) {
+        out[0] = ax; out[1] = n0; out[2] = n1;
+    } else if (axis == 1) {
+        out[0] = n0; out[1] = ax; out[2] = n1;
+    } else {
+        out[0] = n0; out[1] = n1; out[2] = ax;
+    }
+}
+
+static double dist3(const double* a, const double* b) {
+    double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+/* outside result: the candidate point, normal along pos - point */
+static surf_t outside_result(const double* pos, const double* point, const double* normal) {
+    surf_t r;
+    memcpy(r.point, point, sizeof(r.point));
+    double dx = pos[0] - point[0], dy = pos[1] - point[1], dz = pos[2] - point[2];
+    double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist < 1e-30) {
+        memcpy(r.normal, normal, sizeof(r.normal));
+        r.distance = 0.0;
+    } else {
+        r.normal[0] = dx / dist;
+        r.normal[1] = dy / dist;
+        r.normal[2] = dz / dist;
+        r.distance = dist;
+    }
+    return r;
+}
+
+static surf_t closest_block(const double* pos, const double* mn, const double* mx) {
+    /* region.rs:314-391 */
+    surf_t r;
+    double cl[3] = {clampd(pos[0], mn[0], mx[0]), clampd(pos[1], mn[1], mx[1]),
+                    clampd(pos[2], mn[2], mx[2])};
+    int inside = pos[0] >= mn[0] && pos[0] <= mx[0] && pos[1] >= mn[1] && pos[1] <= mx[1] &&
+                 pos[2] >= mn[2] && pos[2] <= mx[2];
+    if (!inside) {
+        double dx = pos[0] - cl[0], dy = pos[1] - cl[1], dz = pos[2] - cl[2];
+        double dist = sqrt(dx * dx + dy * dy + dz * dz);
+        memcpy(r.point, cl, sizeof(cl));
+        if (dist < 1e-30) {
+            r.normal[0] = 0.0; r.normal[1] = 0.0; r.normal[2] = 1.0;
+            r.distance = 0.0;
+        } else {
+            r.normal[0] = dx / dist; r.normal[1] = dy / dist; r.normal[2] = dz / dist;
+            r.distance = dist;
+        }
+        return r;
+    }
+    double d[6] = {pos[0] - mn[0], mx[0] - pos[0], pos[1] - mn[1],
+                   mx[1] - pos[1], pos[2] - mn[2], mx[2] - pos[2]};
+    int best = 0;
+    for (int k = 1; k < 6; k++)
+        if (d[k] < d[best]) best = k;
+    memcpy(r.point, pos, sizeof(r.point));
+    r.normal[0] = r.normal[1] = r.normal[2] = 0.0;
+    int axis = best / 2;
+    r.normal[axis] = (best & 1) ? 1.0 : -1.0;
+    r.point[axis] = (best & 1) ? mx[axis] : mn[axis];
+    r.distance = -d[best];
+    return r;
+}
+
+static surf_t closest_sphere(const double* pos, const double* c, double radius) {
+    /* region.rs:393-415 */
+    surf_t r;
+    double dx = pos[0] - c[0], dy = pos[1] - c[1], dz = pos[2] - c[2];
+    double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist < 1e-30) {
+        r.point[0] = c[0] + radius; r.point[1] = c[1]; r.point[2] = c[2];
+        r.normal[0] = 1.0; r.normal[1] = 0.0; r.normal[2] = 0.0;
+        r.distance = -radius;
+        return r;
+    }
+    double inv = 1.0 / dist;
+    double nx = dx * inv, ny = dy * inv, nz = dz * inv;
+    r.point[0] = c[0] + nx * radius; r.point[1] = c[1] + ny * radius; r.point[2] = c[2] + nz * radius;
+    r.normal[0] = nx; r.normal[1] = ny; r.normal[2] = nz;
+    r.distance = dist - radius;
+    return r;
+}
+
+static surf_t closest_cylinder(const double* pos, const double* c, double radius, int axis,
+                               double lo, double hi) {
+    /* region.rs:452-588 */
+    double ax, d0, d1;
+    decompose(pos, c, axis, &ax, &d0, &d1);
+    double rd = sqrt(d0 * d0 + d1 * d1);
+    double axc = clampd(ax, lo, hi);
+    int ax_in = ax >= lo && ax <= hi;
+    double cpt[3], cn[3], cdist;
+    if (rd < 1e-30) {
+        compose(axc, radius, 0.0, c, axis, cpt);
+        compose_n(0.0, 1.0, 0.0, axis, cn);
+        if (ax_in) {
+            cdist = -radius;
+        } else {
+            double da = ax < lo ? lo - ax : ax - hi;
+            cdist = sqrt(radius * radius + da * da);
+        }
+    } else {
+        double inv = 1.0 / rd, n0 = d0 * inv, n1 = d1 * inv;
+        compose(axc, n0 * radius, n1 * radius, c, axis, cpt);
+        compose_n(0.0, n0, n1, axis, cn);
+        if (ax_in) {
+            cdist = rd - radius;
+        } else {
+            double da = ax < lo ? lo - ax : ax - hi;
+            double dr = rd - radius;
+            cdist = sqrt(dr * dr + da * da) * ((dr < 0.0 && da == 0.0) ? -1.0 : 1.0);
+        }
+    }
+    double cap_r = rd <= radius ? rd : radius;
+    double c0 = 0.0, c1 = 0.0;
+    if (!(rd < 1e-30)) {
+        double inv = 1.0 / rd;
+        c0 = d0 * inv * cap_r;
+        c1 = d1 * inv * cap_r;
+    }
+    double lpt[3], ln[3], hpt[3], hn[3];
+    compose(lo, c0, c1, c, axis, lpt);
+    compose_n(-1.0, 0.0, 0.0, axis, ln);
+    compose(hi, c0, c1, c, axis, hpt);
+    compose_n(1.0, 0.0, 0.0, axis, hn);
+    double dl = dist3(pos, lpt), dh = dist3(pos, hpt);
+    double lo_dist = (ax < lo || (ax_in && rd <= radius)) ? (ax <= lo ? lo - ax : dl) : dl;
+    double hi_dist = (ax > hi || (ax_in && rd <= radius)) ? (ax >= hi ? ax - hi : dh) : dh;
+    surf_t r;
+    if (ax_in && rd <= radius) {
+        double dc = fabs(radius - rd), dlo = ax - lo, dhi = hi - ax;
+        if (dc <= dlo && dc <= dhi) {
+            memcpy(r.point, cpt, sizeof(cpt)); memcpy(r.normal, cn, sizeof(cn));
+            r.distance = rd - radius;
+        } else if (dlo <= dhi) {
+            memcpy(r.point, lpt, sizeof(lpt)); memcpy(r.normal, ln, sizeof(ln));
+            r.distance = -(ax - lo);
+        } else {
+            memcpy(r.point, hpt, sizeof(hpt)); memcpy(r.normal, hn, sizeof(hn));
+            r.distance = -(hi - ax);
+        }
+        return r;
+    }
+    double key[3] = {fabs(cdist), lo_dist, hi_dist};
+    int best = 0;
+    for (int k = 1; k < 3; k++)
+        if (key[k] < key[best]) best = k;
+    if (best == 0) return outside_result(pos, cpt, cn);
+    if (best == 1) return outside_result(pos, lpt, ln);
+    return outside_result(pos, hpt, hn);
+}
+
+static surf_t closest_cone(const double* pos, const double* c, int axis, double rlo, double rhi,
+                           double lo, double hi) {
+    /* region.rs:590-747 */
+    double ax, d0, d1;
+    decompose(pos, c, axis, &ax, &d0, &d1);
+    double rd = sqrt(d0 * d0 + d1 * d1);
+    double height = hi - lo;
+    double n0 = 1.0, n1 = 0.0;
+    if (!(rd < 1e-30)) {
+        n0 = d0 / rd;
+        n1 = d1 / rd;
+    }
+    int ax_in = ax >= lo && ax <= hi;
+    double r_at;
+    if (fabs(height) < 1e-30) {
+        r_at = (rlo + rhi) * 0.5;
+    } else {
+        double t = clampd((ax - lo) / height, 0.0, 1.0);
+        r_at = rlo + t * (rhi - rlo);
+    }
+    int inside = ax_in && rd <= r_at;
+    /* lateral surface: nearest point of the segment (lo, rlo)-(hi, rhi) in
+     * (axial, radial) coordinates */
+    double sda = hi - lo, sdr = rhi - rlo;
+    double sl2 = sda * sda + sdr * sdr;
+    double t = sl2 < 1e-30 ? 0.5 : clampd(((ax - lo) * sda + (rd - rlo) * sdr) / sl2, 0.0, 1.0);
+    double pa = lo + t * sda, pr = rlo + t * sdr;
+    double da = ax - pa, dr = rd - pr;
+    double d2 = sqrt(da * da + dr * dr);
+    double cpt[3], cn[3];
+    compose(pa, n0 * pr, n1 * pr, c, axis, cpt);
+    double na = -sdr, nr = sda;
+    double nl = sqrt(na * na + nr * nr);
+    if (nl > 1e-30) {
+        na /= nl;
+        nr /= nl;
+    } else {
+        na = 0.0;
+        nr = 1.0;
+    }
+    compose_n(na, nr * n0, nr * n1, axis, cn);
+    double csigned = inside ? -d2 : d2;
+    double rr = rd > 0.0 ? rd : 0.0;
+    double lcr = rlo < rr ? rlo : rr;
+    if (rlo < lcr) lcr = rlo;
+    double hcr = rhi < rr ? rhi : rr;
+    if (rhi < hcr) hcr = rhi;
+    double lpt[3], ln[3], hpt[3], hn[3];
+    compose(lo, n0 * lcr, n1 * lcr, c, axis, lpt);
+    compose_n(-1.0, 0.0, 0.0, axis, ln);
+    compose(hi, n0 * hcr, n1 * hcr, c, axis, hpt);
+    compose_n(1.0, 0.0, 0.0, axis, hn);
+    double dl = dist3(pos, lpt), dh = dist3(pos, hpt);
+    surf_t r;
+    if (inside) {
+        double dlo = ax - lo, dhi = hi - ax;
+        if (d2 <= dlo && d2 <= dhi) {
+            memcpy(r.point, cpt, sizeof(cpt)); memcpy(r.normal, cn, sizeof(cn));
+            r.distance = csigned;
+        } else if (dlo <= dhi) {
+            memcpy(r.point, lpt, sizeof(lpt)); memcpy(r.normal, ln, sizeof(ln));
+            r.distance = -dlo;
+        } else {
+            memcpy(r.point, hpt, sizeof(hpt)); memcpy(r.normal, hn, sizeof(hn));
+            r.distance = -dhi;
+        }
+        return r;
+    }
+    double key[3] = {d2, dl, dh};
+    int best = 0;
+    for (int k = 1; k < 3; k++)
+        if (key[k] < key[best]) best = k;
+    if (best == 0) return outside_result(pos, cpt, cn);
+    if (best == 1) return outside_result(pos, lpt, ln);
+    return outside_result(pos, hpt, hn);
+}
+
+static surf_t closest_plane(const double* pos, const double* pt, const double* nrm) {
+    /* region.rs:749-772 */
+    surf_t r;
+    double mag = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    double nx = 0.0, ny = 0.0, nz = 1.0;
+    if (mag > 1e-30) {
+        nx = nrm[0] / mag;
+        ny = nrm[1] / mag;
+        nz = nrm[2] / mag;
+    }
+    double dist = (pos[0] - pt[0]) * nx + (pos[1] - pt[1]) * ny + (pos[2] - pt[2]) * nz;
+    r.point[0] = pos[0] - dist * nx;
+    r.point[1] = pos[1] - dist * ny;
+    r.point[2] = pos[2] - dist * nz;
+    r.normal[0] = nx; r.normal[1] = ny; r.normal[2] = nz;
+    r.distance = dist;
+    return r;
+}
+
+static surf_t closest_prim(int type, const double* q, const double* pos) {
+    switch (type) {
+        case 0: return closest_block(pos, q, q + 3);
+        case 1: return closest_sphere(pos, q, q[3]);
+        case 2: return closest_cylinder(pos, q, q[2], (int)q[3], q[4], q[5]);
+        case 3: return closest_cone(pos, q, (int)q[2], q[3], q[4], q[5], q[6]);
+        default: return closest_plane(pos, q, q + 3);
+    }
+}
+
+/* a region wall's surface: the leaf with the smallest |distance| (the
+ * first of equals), as Union / Intersect choose (region.rs:282-309) */
+static surf_t closest_region(const wall_t* w, const double* pos) {
+    surf_t best = closest_prim(w->ptype[0], w->prm[0], pos);
+    for (int k = 1; k < w->nprim; k++) {
+        surf_t r = closest_prim(w->ptype[k], w->prm[k], pos);
+        if (fabs(r.distance) < fabs(best.distance)) best = r;
+    }
+    return best;
+}
+
+double orc_region_surface(const orc_sys* s, int wall, const double pos[3], double point[3],
+                          double normal[3]) {
+    const wall_t* w = &s->walls[wall];
+    surf_t r = closest_region(w, pos);
+    if (point) memcpy(point, r.point, sizeof(r.point));
+    if (normal) memcpy(normal, r.normal, sizeof(r.normal));
+    return r.distance;
 }
 
 int orc_set_wall_motion(orc_sys* s, int wall, int motion, const double* params) {
@@ -1769,6 +2107,28 @@ void orc_wall_contact_force(orc_sys* s) {
             }
             if (delta <= 0.0) continue;
             if (0.5 * radius < delta) delta = 0.5 * radius;
+            double f_net = curved_wall_force(s, i, W->mat, delta, nx, ny, nz);
+            s->force[i][0] += f_net * nx;
+            s->force[i][1] += f_net * ny;
+            s->force[i][2] += f_net * nz;
+            acc += f_net;
+        }
+        W->acc += acc;
+    }
+    /* region-surface walls (dem_wall/src/lib.rs:1170-1231) */
+    for (int w = 0; w < s->nwalls; w++) {
+        wall_t* W = &s->walls[w];
+        if (W->kind != 3 || !W->active) continue;
+        double acc = 0.0;
+        for (int i = 0; i < s->nlocal; i++) {
+            double radius = s->radius[i];
+            surf_t sr = closest_region(W, s->pos[i]);
+            double gap = W->inside ? -sr.distance : sr.distance;
+            double delta = radius - gap;
+            if (delta <= 0.0) continue;
+            if (0.5 * radius < delta) delta = 0.5 * radius;
+            double sg = W->inside ? -1.0 : 1.0;
+            double nx = sg * sr.normal[0], ny = sg * sr.normal[1], nz = sg * sr.normal[2];
             double f_net = curved_wall_force(s, i, W->mat, delta, nx, ny, nz);
             s->force[i][0] += f_net * nx;
             s->force[i][1] += f_net * ny;

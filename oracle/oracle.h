@@ -89,6 +89,21 @@ int orc_add_cylinder_wall(orc_sys* s, int axis, double c0, double c1, double rad
                           double hi, int inside, int material);
 int orc_add_sphere_wall(orc_sys* s, const double center[3], double radius, int inside,
                         int material);
+/* type = "region" (WallRegion, dem_wall/src/lib.rs:435-447): the leaf
+ * primitives of the region tree in depth-first order (Union and Intersect
+ * both take the child surface nearest the point, region.rs:282-309, so any
+ * tree flattens to its leaves), at most 8; 12 parameters each:
+ *   0 block:    min[3], max[3]
+ *   1 sphere:   center[3], radius
+ *   2 cylinder: center[2], radius, axis (0/1/2), lo, hi
+ *   3 cone:     center[2], axis, rad_lo, rad_hi, lo, hi
+ *   4 plane:    point[3], normal[3] */
+int orc_add_region_wall(orc_sys* s, int nprim, const int* types, const double* params,
+                        int inside, int material);
+/* Region::closest_point_on_surface of a region wall (region.rs:260-772):
+ * signed distance (positive outside), point and outward normal */
+double orc_region_surface(const orc_sys* s, int wall, const double pos[3], double point[3],
+                          double normal[3]);
 /* plane motion: 0 static, 1 constant velocity (params v[3]), 2 oscillate
  * (amplitude, frequency), 3 servo (target_force, max_velocity, gain) */
 int orc_set_wall_motion(orc_sys* s, int wall, int motion, const double* params);

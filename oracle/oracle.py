@@ -66,6 +66,8 @@ def lib():
                                                 C.c_double, C.c_double, C.c_int, C.c_int]),
             "orc_add_sphere_wall": (C.c_int, [vp, _dp, C.c_double, C.c_int, C.c_int]),
             "orc_set_wall_motion": (C.c_int, [vp, C.c_int, C.c_int, _dp]),
+            "orc_add_region_wall": (C.c_int, [vp, C.c_int, _ip, _dp, C.c_int, C.c_int]),
+            "orc_region_surface": (C.c_double, [vp, C.c_int, _dp, _dp, _dp]),
             "orc_set_wall_active": (C.c_int, [vp, C.c_int, C.c_int]),
             "orc_get_walls": (C.c_int, [vp, _dp, _dp, _dp]),
             "orc_wall_move": (None, [vp]),
@@ -251,6 +253,19 @@ class Oracle:
     def add_sphere_wall(self, center, radius, inside=False, material=0):
         return self._index(self.L.orc_add_sphere_wall(self.h, _ptr(_d(center)), radius,
                                                       int(inside), material))
+
+    def add_region_wall(self, types, params, inside=False, material=0):
+        """Leaf primitives (type codes and 12 parameters each, oracle.h)."""
+        t = (C.c_int * len(types))(*[int(x) for x in types])
+        q = _d(np.asarray(params, dtype=np.float64).reshape(len(types), 12))
+        return self._index(self.L.orc_add_region_wall(self.h, len(types), t, _ptr(q),
+                                                      int(inside), material))
+
+    def region_surface(self, wall, pos):
+        """(signed distance, closest point, outward normal) of a region wall."""
+        pt, nrm = np.zeros(3), np.zeros(3)
+        d = self.L.orc_region_surface(self.h, wall, _ptr(_d(pos)), _ptr(pt), _ptr(nrm))
+        return d, pt, nrm
 
     def set_wall_motion(self, wall, motion, params=(0.0, 0.0, 0.0)):
         self._check(self.L.orc_set_wall_motion(self.h, wall, motion, _ptr(_d(params))))

@@ -1,7 +1,8 @@
 """The wall family on the GPU against the checker (dem_wall/src/lib.rs:
 766-1241): plane walls with JKR / SJKR adhesion and twisting friction,
-cylinder and sphere walls, and moving plane walls (constant velocity,
-oscillation, servo) inside scheduled runs. A cloud of glass spheres is held
+cylinder, sphere and region-surface walls (a union of a block and a cone cut
+by a plane), and moving plane walls (constant velocity, oscillation, servo)
+inside scheduled runs. A cloud of glass spheres is held
 in a spherical container crossed by a cylinder, with an oscillating floor and
 a servo lid, so that every wall kind carries contacts."""
 import sys
@@ -43,6 +44,9 @@ def _cloud(n=300, seed=21, speed=0.6):
             continue
         if np.linalg.norm(p - OBSTACLE) < 0.002 + R + 1e-5:
             continue
+        # clear of the region wall's cone (inside its bounding cylinder)
+        if p[2] > 0.02 - R - 1e-4 and np.hypot(p[0] - 0.015, p[1] - 0.015) < 0.004 + R + 1e-4:
+            continue
         if all(np.linalg.norm(p - q) > 2.1 * R for q in pts):
             pts.append(p)
     w.pos = np.array(pts)
@@ -67,7 +71,27 @@ def _walls(sys_, oracle):
         s.set_wall_active(6, False)
         s.add_wall((0, 0, 0), (0, 1, 0))                                # 7 moves at 1 m/s
         s.set_wall_motion(7, 1, (0.0, 1.0, 0.0))
-    return 8
+        _add_region(s)                                                  # 8 region
+    return 9
+
+
+# a slab under the container and a cone hanging from above, as one
+# region-surface wall (spheres stay outside it)
+REGION = {"type": "union", "regions": [
+    {"type": "block", "min": [0.002, 0.002, 0.0], "max": [0.028, 0.028, 0.0031]},
+    {"type": "intersect", "regions": [
+        {"type": "cone", "center": [0.015, 0.015], "axis": "z", "rad_lo": 0.004,
+         "rad_hi": 0.001, "lo": 0.02, "hi": 0.028},
+        {"type": "plane", "point": [0.0, 0.0, 0.03], "normal": [0.0, 0.0, 1.0]}]},
+]}
+
+
+def _add_region(s):
+    if isinstance(s, hip.System):
+        s.add_region_wall(REGION)
+    else:
+        types, params = hip.flatten_region(REGION)
+        s.add_region_wall(types, params)
 
 
 EXTRAS = {
@@ -109,6 +133,13 @@ def _near_walls():
         ang = 2 * np.pi * st.uniform(1)[0]
         rr = 0.012 - R + d
         pts.append((0.01 + 0.002 * k, 0.015 + rr * np.cos(ang), 0.015 + rr * np.sin(ang)))
+        pts.append((jx, jy + 0.004, 0.0031 + R - d))        # region: slab top
+        # region: the cone's side (radius 0.0025 at z = 0.024, slope 3 / 8)
+        zc = 0.024
+        nrm = np.array([np.cos(ang), np.sin(ang), 3 / 8])
+        nrm /= np.linalg.norm(nrm)
+        base = np.array([0.015 + 0.0025 * np.cos(ang), 0.015 + 0.0025 * np.sin(ang), zc])
+        pts.append(tuple(base + nrm * (R - d)))
     return np.array(pts)
 
 
@@ -140,16 +171,20 @@ def test_wall_forces_match_oracle(extras):
     gw, ow = s.walls(), o.walls()
     assert np.abs(gw["force"] - ow["force"]).max() <= 1e-9 * np.abs(ow["force"]).max()
     # every wall but the deactivated one carries force
-    assert np.all(ow["force"][[0, 1, 2, 3, 4, 5, 7]] != 0.0) and ow["force"][6] == 0.0
+    assert np.all(ow["force"][[0, 1, 2, 3, 4, 5, 7, 8]] != 0.0) and ow["force"][6] == 0.0
 
 
 @pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
 @pytest.mark.parametrize("extras", ["plain", "cohesive"])
 def test_scheduled_runs_with_moving_walls_match_oracle(extras, prec):
     w, o, s = _pair(EXTRAS[extras], prec)
-    steps = 400
-    o.run(steps)
-    s.run(steps)
+    touched = False
+    for _ in range(4):
+        o.run(100)
+        s.run(100)
+        fw = o.walls()["force"]
+        touched = touched or fw[0] != 0.0 or fw[1] != 0.0
+        assert np.abs(s.walls()["force"] - fw).max() <= 1e-6 * max(1.0, np.abs(fw).max())
     ref = o.atoms()
     st = s.state()
     tol = 1e-9 if prec == hip.FP64 else 1e-4
@@ -161,8 +196,7 @@ def test_scheduled_runs_with_moving_walls_match_oracle(extras, prec):
     assert np.abs(gw["point"] - ow["point"]).max() < 1e-12
     assert np.abs(gw["velocity"] - ow["velocity"]).max() < (1e-9 if prec == hip.FP64 else 1e-5)
     assert ow["velocity"][3, 2] < 0  # the lid descends toward its target force
-    # contacts happened on the curved walls
-    assert ow["force"][0] != 0.0 or ow["force"][1] != 0.0
+    assert touched  # contacts happened on the curved walls
 
 
 def test_wall_runs_are_bitwise_deterministic():
