@@ -293,6 +293,19 @@ __device__ inline double curved_wall_force(const ForceArgs& a, int k, double rad
     return f > 0.0 ? f : 0.0;
 }
 
+// The wall table and the walls' current points and velocities do not change
+// while a force kernel runs (the host and the integrator kernels write them).
+// Read through the constant address space, their wavefront-uniform fields
+// come in scalar loads even though the kernel stores forces and wall sums;
+// through generic pointers every field was a dependent vector load per wall.
+typedef const __attribute__((address_space(4))) WallsP CWallsP;
+typedef const __attribute__((address_space(4))) Wall CWall;
+typedef const __attribute__((address_space(4))) WallDyn CWallDyn;
+__device__ inline CWallsP& wall_table(const ForceArgs& a) { return *(CWallsP*)a.walls; }
+__device__ inline CWallDyn& wall_state(const ForceArgs& a, int w) {
+    return *(CWallDyn*)&a.wdyn->w[w];
+}
+
 // Region::closest_point_on_surface (mddem_core/src/region.rs:260-772):
 // signed distance (positive outside), surface point, outward normal
 struct Surf {
@@ -527,18 +540,35 @@ __device__ Surf surf_prim(int type, const double* q, const double* x) {
 }
 
 // nearest leaf surface, the first of equals (Union / Intersect, region.rs:282-309)
-__device__ Surf surf_region(const Wall& W, const double* x) {
-    Surf best = surf_prim(W.ptype[0], W.prm[0], x);
-    for (int k = 1; k < W.nprim; k++) {
-        const Surf r = surf_prim(W.ptype[k], W.prm[k], x);
-        if (fabs(r.d) < fabs(best.d)) best = r;
+__device__ Surf surf_region(CWall& W, const double* x) {
+    Surf best;
+    for (int k = 0; k < W.nprim; k++) {
+        double q[12];
+        for (int j = 0; j < 12; j++) q[j] = W.prm[k][j];
+        const Surf r = surf_prim(W.ptype[k], q, x);
+        if (k == 0 || fabs(r.d) < fabs(best.d)) best = r;
     }
     return best;
 }
 
-__device__ inline void wall_acc(const ForceArgs& a, int w, double f) {
-    atomicAdd(&a.wdyn->w[w].acc,
-              static_cast<unsigned long long>(__double2ll_rn(f * kWallAccScale)));
+// a wall's force sum: the lanes of a wavefront in contact with it are summed
+// in fixed point by the scalar unit and the lowest active lane adds the total
+// (one atomic per wavefront, where thousands of spheres may rest on one wall)
+__device__ inline void wall_acc(const ForceArgs& a, int w, bool on, double f) {
+    const long long v = on ? __double2ll_rn(f * kWallAccScale) : 0ll;
+    uint64_t m = __ballot(v != 0ll);
+    if (m == 0ull) return;
+    const uint64_t act = __ballot(true);
+    long long sum = 0;
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1ull;
+        const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v)), l));
+        const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(uint64_t(v) >> 32)), l));
+        sum += static_cast<long long>((uint64_t(hi) << 32) | lo);
+    }
+    if (int(threadIdx.x & 63u) == __builtin_ctzll(act))
+        atomicAdd(&a.wdyn->w[w].acc, static_cast<unsigned long long>(sum));
 }
 
 template <typename R>
@@ -548,121 +578,135 @@ __device__ inline void wall_forces(const ForceArgs& a, const double4& p, const d
     const double PI = 3.141592653589793;
     const int nm = a.nm, nn = nm * nm;
     const double radius = p.w;
-    const WallsP& WS = *a.walls;
+    CWallsP& WS = wall_table(a);
     for (int w = 0; w < WS.count; w++) {
-        const Wall& W = WS.w[w];
+        CWall& W = WS.w[w];
         if (W.kind != 0 || !W.active) continue;
-        if (!(p.x >= W.blo[0] && p.x <= W.bhi[0] && p.y >= W.blo[1] && p.y <= W.bhi[1] &&
-              p.z >= W.blo[2] && p.z <= W.bhi[2]))
-            continue;
-        const WallDyn& D = a.wdyn->w[w];
-        const double dx = p.x - D.p[0], dy = p.y - D.p[1], dz = p.z - D.p[2];
-        const double distance = dx * W.n[0] + dy * W.n[1] + dz * W.n[2];
-        if (distance <= 0.0) continue;
-        const int k = int(ti) * nm + W.mat;
-        const double e_eff = a.mat[2 * nn + k];
-        const double surface = a.mext ? a.mext[kXSurface * nn + k] : 0.0;
-        double d_po = 0.0;
-        if (surface > 0.0 && !a.dmt)
-            d_po = cbrt(PI * PI * surface * surface * radius / (4.0 * e_eff * e_eff));
-        double delta = radius - distance;
-        if (0.5 * radius < delta) delta = 0.5 * radius;
-        if (delta <= 0.0 && surface <= 0.0) continue;
-        if (delta < -d_po) continue;
-        const bool adhesion_only = surface > 0.0 && !a.dmt && delta <= 0.0;
-        double s_n = 0.0, k_n = 0.0;
-        if (delta > 0.0) {
-            const double sdr = sqrt(delta * radius);
-            s_n = 2.0 * e_eff * sdr;
-            k_n = 4.0 / 3.0 * e_eff * sdr;
-        }
-        const double v_n =
-            (v.x - D.v[0]) * W.n[0] + (v.y - D.v[1]) * W.n[1] + (v.z - D.v[2]) * W.n[2];
-        const double beta = a.mat[k];
-        const double cohesion = a.mext ? a.mext[kXCohesion * nn + k] : 0.0;
-        const double f_diss = 2.0 * beta * kSqrt53 * sqrt(s_n * mass) * v_n;
-        double f_net;
-        if (surface > 0.0 && a.dmt) {
-            f_net = k_n * delta - f_diss - 2.0 * PI * surface * radius;
-        } else if (surface > 0.0) {
-            const double f_adh = 1.5 * PI * surface * radius;
-            f_net = adhesion_only ? -f_adh : k_n * delta - f_diss - f_adh;
-        } else if (cohesion > 0.0) {
-            f_net = k_n * delta - f_diss - cohesion * PI * delta * radius;
-        } else {
-            f_net = k_n * delta - f_diss;
-            if (!(f_net > 0.0)) f_net = 0.0;
-        }
-        fx += R(f_net * W.n[0]);
-        fy += R(f_net * W.n[1]);
-        fz += R(f_net * W.n[2]);
-        const double mu_tw = a.mext ? a.mext[kXTwisting * nn + k] : 0.0;
-        if (delta > 0.0 && mu_tw > 0.0 && om) {
-            const double twist = om->x * W.n[0] + om->y * W.n[1] + om->z * W.n[2];
-            if (fabs(twist) > 1e-30) {
-                const double tau = (twist > 0.0 ? -1.0 : 1.0) * (mu_tw * fabs(f_net) * radius);
-                tx += R(tau * W.n[0]);
-                ty += R(tau * W.n[1]);
-                tz += R(tau * W.n[2]);
+        double f_net = 0.0;
+        bool on = false;
+        do {
+            if (!(p.x >= W.blo[0] && p.x <= W.bhi[0] && p.y >= W.blo[1] && p.y <= W.bhi[1] &&
+                  p.z >= W.blo[2] && p.z <= W.bhi[2]))
+                break;
+            CWallDyn& D = wall_state(a, w);
+            const double dx = p.x - D.p[0], dy = p.y - D.p[1], dz = p.z - D.p[2];
+            const double distance = dx * W.n[0] + dy * W.n[1] + dz * W.n[2];
+            if (distance <= 0.0) break;
+            const int k = int(ti) * nm + W.mat;
+            const double e_eff = a.mat[2 * nn + k];
+            const double surface = a.mext ? a.mext[kXSurface * nn + k] : 0.0;
+            double d_po = 0.0;
+            if (surface > 0.0 && !a.dmt)
+                d_po = cbrt(PI * PI * surface * surface * radius / (4.0 * e_eff * e_eff));
+            double delta = radius - distance;
+            if (0.5 * radius < delta) delta = 0.5 * radius;
+            if (delta <= 0.0 && surface <= 0.0) break;
+            if (delta < -d_po) break;
+            const bool adhesion_only = surface > 0.0 && !a.dmt && delta <= 0.0;
+            double s_n = 0.0, k_n = 0.0;
+            if (delta > 0.0) {
+                const double sdr = sqrt(delta * radius);
+                s_n = 2.0 * e_eff * sdr;
+                k_n = 4.0 / 3.0 * e_eff * sdr;
             }
-        }
-        wall_acc(a, w, f_net);
+            const double v_n =
+                (v.x - D.v[0]) * W.n[0] + (v.y - D.v[1]) * W.n[1] + (v.z - D.v[2]) * W.n[2];
+            const double beta = a.mat[k];
+            const double cohesion = a.mext ? a.mext[kXCohesion * nn + k] : 0.0;
+            const double f_diss = 2.0 * beta * kSqrt53 * sqrt(s_n * mass) * v_n;
+            on = true;
+            if (surface > 0.0 && a.dmt) {
+                f_net = k_n * delta - f_diss - 2.0 * PI * surface * radius;
+            } else if (surface > 0.0) {
+                const double f_adh = 1.5 * PI * surface * radius;
+                f_net = adhesion_only ? -f_adh : k_n * delta - f_diss - f_adh;
+            } else if (cohesion > 0.0) {
+                f_net = k_n * delta - f_diss - cohesion * PI * delta * radius;
+            } else {
+                f_net = k_n * delta - f_diss;
+                if (!(f_net > 0.0)) f_net = 0.0;
+            }
+            fx += R(f_net * W.n[0]);
+            fy += R(f_net * W.n[1]);
+            fz += R(f_net * W.n[2]);
+            const double mu_tw = a.mext ? a.mext[kXTwisting * nn + k] : 0.0;
+            if (delta > 0.0 && mu_tw > 0.0 && om) {
+                const double twist = om->x * W.n[0] + om->y * W.n[1] + om->z * W.n[2];
+                if (fabs(twist) > 1e-30) {
+                    const double tau = (twist > 0.0 ? -1.0 : 1.0) * (mu_tw * fabs(f_net) * radius);
+                    tx += R(tau * W.n[0]);
+                    ty += R(tau * W.n[1]);
+                    tz += R(tau * W.n[2]);
+                }
+            }
+        } while (false);
+        wall_acc(a, w, on, f_net);
     }
     for (int w = 0; w < WS.count; w++) {
-        const Wall& W = WS.w[w];
+        CWall& W = WS.w[w];
         if (W.kind != 1 || !W.active) continue;
-        double axial, d0, d1;
-        if (W.axis == 0) {
-            axial = p.x;
-            d0 = p.y - W.c[0];
-            d1 = p.z - W.c[1];
-        } else if (W.axis == 1) {
-            axial = p.y;
-            d0 = p.x - W.c[0];
-            d1 = p.z - W.c[1];
-        } else {
-            axial = p.z;
-            d0 = p.x - W.c[0];
-            d1 = p.y - W.c[1];
-        }
-        if (axial < W.lo || axial > W.hi) continue;
-        const double radial = sqrt(d0 * d0 + d1 * d1);
-        if (radial < 1e-30) continue;
-        const double inv_r = 1.0 / radial;
-        const double delta0 =
-            W.inside ? radius - (W.radius - radial) : radius - (radial - W.radius);
-        const double n0 = W.inside ? -d0 * inv_r : d0 * inv_r;
-        const double n1 = W.inside ? -d1 * inv_r : d1 * inv_r;
-        const double nx = W.axis == 0 ? 0.0 : n0;
-        const double ny = W.axis == 0 ? n0 : (W.axis == 1 ? 0.0 : n1);
-        const double nz = W.axis == 2 ? 0.0 : n1;
-        if (delta0 <= 0.0) continue;
-        const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
-        const double f_net =
-            curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
-        fx += R(f_net * nx);
-        fy += R(f_net * ny);
-        fz += R(f_net * nz);
-        wall_acc(a, w, f_net);
+        double f_net = 0.0;
+        bool on = false;
+        do {
+            double axial, d0, d1;
+            if (W.axis == 0) {
+                axial = p.x;
+                d0 = p.y - W.c[0];
+                d1 = p.z - W.c[1];
+            } else if (W.axis == 1) {
+                axial = p.y;
+                d0 = p.x - W.c[0];
+                d1 = p.z - W.c[1];
+            } else {
+                axial = p.z;
+                d0 = p.x - W.c[0];
+                d1 = p.y - W.c[1];
+            }
+            if (axial < W.lo || axial > W.hi) break;
+            const double radial = sqrt(d0 * d0 + d1 * d1);
+            if (radial < 1e-30) break;
+            const double inv_r = 1.0 / radial;
+            const double delta0 =
+                W.inside ? radius - (W.radius - radial) : radius - (radial - W.radius);
+            const double n0 = W.inside ? -d0 * inv_r : d0 * inv_r;
+            const double n1 = W.inside ? -d1 * inv_r : d1 * inv_r;
+            const double nx = W.axis == 0 ? 0.0 : n0;
+            const double ny = W.axis == 0 ? n0 : (W.axis == 1 ? 0.0 : n1);
+            const double nz = W.axis == 2 ? 0.0 : n1;
+            if (delta0 <= 0.0) break;
+            const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
+            on = true;
+            f_net =
+                curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
+            fx += R(f_net * nx);
+            fy += R(f_net * ny);
+            fz += R(f_net * nz);
+        } while (false);
+        wall_acc(a, w, on, f_net);
     }
     for (int w = 0; w < WS.count; w++) {
-        const Wall& W = WS.w[w];
+        CWall& W = WS.w[w];
         if (W.kind != 2 || !W.active) continue;
-        const double dx = p.x - W.c[0], dy = p.y - W.c[1], dz = p.z - W.c[2];
-        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-        if (dist < 1e-30) continue;
-        const double inv = 1.0 / dist;
-        const double sg = W.inside ? -1.0 : 1.0;
-        const double nx = sg * dx * inv, ny = sg * dy * inv, nz = sg * dz * inv;
-        const double delta0 = W.inside ? radius - (W.radius - dist) : radius - (dist - W.radius);
-        if (delta0 <= 0.0) continue;
-        const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
-        const double f_net =
-            curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
-        fx += R(f_net * nx);
-        fy += R(f_net * ny);
-        fz += R(f_net * nz);
-        wall_acc(a, w, f_net);
+        double f_net = 0.0;
+        bool on = false;
+        do {
+            const double dx = p.x - W.c[0], dy = p.y - W.c[1], dz = p.z - W.c[2];
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist < 1e-30) break;
+            const double inv = 1.0 / dist;
+            const double sg = W.inside ? -1.0 : 1.0;
+            const double nx = sg * dx * inv, ny = sg * dy * inv, nz = sg * dz * inv;
+            const double delta0 = W.inside ? radius - (W.radius - dist) : radius - (dist - W.radius);
+            if (delta0 <= 0.0) break;
+            const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
+            on = true;
+            f_net =
+                curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
+            fx += R(f_net * nx);
+            fy += R(f_net * ny);
+            fz += R(f_net * nz);
+        } while (false);
+        wall_acc(a, w, on, f_net);
     }
 }
 
@@ -673,24 +717,29 @@ __device__ void region_wall_forces(const ForceArgs& a, const double4& p, const d
                                    double mass, uint32_t ti, R& fx, R& fy, R& fz) {
     const int nm = a.nm;
     const double radius = p.w;
-    const WallsP& WS = *a.walls;
+    CWallsP& WS = wall_table(a);
     for (int w = 0; w < WS.count; w++) {
-        const Wall& W = WS.w[w];
+        CWall& W = WS.w[w];
         if (W.kind != 3 || !W.active) continue;
-        const double x[3] = {p.x, p.y, p.z};
-        const Surf sr = surf_region(W, x);
-        const double gap = W.inside ? -sr.d : sr.d;
-        const double delta0 = radius - gap;
-        if (delta0 <= 0.0) continue;
-        const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
-        const double sg = W.inside ? -1.0 : 1.0;
-        const double nx = sg * sr.nrm[0], ny = sg * sr.nrm[1], nz = sg * sr.nrm[2];
-        const double f_net =
-            curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
-        fx += R(f_net * nx);
-        fy += R(f_net * ny);
-        fz += R(f_net * nz);
-        wall_acc(a, w, f_net);
+        double f_net = 0.0;
+        bool on = false;
+        do {
+            const double x[3] = {p.x, p.y, p.z};
+            const Surf sr = surf_region(W, x);
+            const double gap = W.inside ? -sr.d : sr.d;
+            const double delta0 = radius - gap;
+            if (delta0 <= 0.0) break;
+            const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
+            const double sg = W.inside ? -1.0 : 1.0;
+            const double nx = sg * sr.nrm[0], ny = sg * sr.nrm[1], nz = sg * sr.nrm[2];
+            on = true;
+            f_net =
+                curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
+            fx += R(f_net * nx);
+            fy += R(f_net * ny);
+            fz += R(f_net * nz);
+        } while (false);
+        wall_acc(a, w, on, f_net);
     }
 }
 
