@@ -96,7 +96,7 @@ int mdd_set_materials(mdd_ctx* ctx, int n, const double* youngs, const double* p
  * clears them). Any nonzero value selects the contact kernel of the whole
  * model family (contact.rs:103-615, 622-990) with 7 history values per
  * contact; switching the history width drops the stored history. Single
- * systems only (not with mdd_set_systems). */
+ * systems only (not with mdd_add_system). */
 int mdd_set_material_extras(mdd_ctx* ctx, int n, const double* values);
 /* [materials] contact_model "hooke" (1) or "hertz" (0), adhesion_model "dmt"
  * (1) or "jkr" (0), rolling_model and twisting_model "sds" (1) or "constant"
@@ -250,8 +250,9 @@ int mdd_download_history(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* 
                          double* spring, int64_t* n);
 /* the same with all 7 history values per contact (the [f64; 7] of
  * ContactHistoryStore, contact.rs:343-350): spring (3), rolling displacement
- * (3), twist angle (1), in the owner's sign convention; zeros past the
- * spring when no optional model is active */
+ * (3), twist angle (1), each stored as the reference stores it: sign * value,
+ * sign = +1 when the owning particle has the lower tag (contact.rs:337-339);
+ * zeros past the spring when no optional model is active */
 int mdd_download_history7(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t* tag_hi,
                           double* hist7, int64_t* n);
 

@@ -148,6 +148,8 @@ void run_steps(Ctx& c, int64_t nsteps) {
         throw ApiError(MDD_ERR_INVALID, "materials not set");
     if (c.nsys > 0 && c.walls.count > 0)
         throw ApiError(MDD_ERR_UNSUPPORTED, "walls in a batched ensemble");
+    if (c.nsys > 0 && c.ext_on)
+        throw ApiError(MDD_ERR_UNSUPPORTED, "optional contact models in a batched ensemble");
     if (c.every != 0)
         throw ApiError(MDD_ERR_UNSUPPORTED, "only displacement rebuilds (every = 0) are supported");
     if (c.step + nsteps > INT_MAX - 2) throw ApiError(MDD_ERR_INVALID, "step count too large");
