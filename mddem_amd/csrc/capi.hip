@@ -96,8 +96,13 @@ void sync_status(Ctx& c) {
     MDD_HIP(hipStreamSynchronize(c.stream));
 }
 
+// Called right after a status read. Timed force launches of steps at or
+// past the chunk's abort step returned at once (the step is redone after
+// the rebuild) and are not counted.
 void harvest_profile(Ctx& c) {
+    const int ab = c.status_host->abort_step;
     for (int64_t k = 0; k < c.prof_pending; k++) {
+        if (c.prof_step[size_t(k)] >= ab) continue;
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, c.ev_pool[2 * k], c.ev_pool[2 * k + 1]) == hipSuccess) {
             c.prof_ms_sum += ms;
@@ -1462,6 +1467,7 @@ int mdd_set_profiling(mdd_ctx* ctx, int every) {
         c.prof_every = every > 0 ? every : 0;
         if (c.prof_every && c.ev_pool.empty()) {
             c.ev_pool.resize(2 * kEventPool);
+            c.prof_step.assign(kEventPool, 0);
             for (auto& e : c.ev_pool) MDD_HIP(hipEventCreate(&e));
             c.rb_ev.resize(3 * 64);
             c.rb_has_gap.assign(64, 0);

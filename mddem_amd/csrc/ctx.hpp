@@ -482,6 +482,7 @@ struct Ctx {
     int64_t force_launches = 0;
     int64_t prof_ctr = 0;
     int64_t prof_pending = 0;
+    std::vector<int> prof_step;  // step of each pending timed launch
 };
 
 // ---- launchers implemented in the .hip files

@@ -1439,6 +1439,7 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
     c.force_launches++;
     if (prof) {
         MDD_HIP(hipEventRecord(e1, c.stream));
+        c.prof_step[size_t(c.prof_pending)] = step;
         c.prof_pending++;
     }
 }
