@@ -15,6 +15,7 @@
 // FP64 pair geometry (image shift, separation, cutoff test, overlap, normal,
 // relative velocity) and the FP64 shear spring, and does the force algebra
 // and per-particle accumulation in FP32.
+#include "cells.hpp"
 #include "ctx.hpp"
 
 // row entries per lane gathered together by the LJ row kernel
@@ -134,7 +135,9 @@ __device__ inline uint64_t group_or(uint64_t v) {
 template <typename R, bool ENERGY, int LG>
 __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     constexpr int G = 1 << LG;
-    const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    // XCD-aware block order: each XCD's L2 holds the positions of one
+    // contiguous (spatially sorted) range of particles and its neighbors
+    const int64_t gt = int64_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     const int64_t i = gt >> LG;
     const int t = int(gt & (G - 1));
     if (i >= a.n) return;
@@ -151,9 +154,9 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     // a row holds periodic-image entries only if its particle had images at
     // the last build; wavefronts without any take the shift-free loop
     const bool img = a.gcnt == nullptr || a.gcnt[i] != 0u;
-    auto pair = [&](uint32_t e, double dx, double dy, double dz) {
+    auto pair = [&](bool valid, double dx, double dy, double dz) {
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const bool in = r2 < a.cut2;
+        const bool in = valid && r2 < a.cut2;
         const R r2inv = rcp<R>(R(r2));
         const R r6inv = r2inv * r2inv * r2inv;
         const R fpair = r2inv * r6inv * fma(lj1, r6inv, -lj2);
@@ -177,7 +180,10 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     };
     // Batches of kB entries per lane: the batch's kB gathers are issued
     // together, and the next batch's row entries are loaded while they are
-    // in flight, so a wave waits on memory once per batch.
+    // in flight, so a wave waits on memory once per batch. The gathers are
+    // unconditional (an entry past the row is 0 and reads particle 0; its
+    // pair is masked out of the sum): a gather under a branch makes the
+    // compiler wait for it inside the branch, one gather at a time.
     constexpr int kB = MDD_LJ_BATCH;
     uint32_t en[kB];
 #pragma unroll
@@ -192,13 +198,10 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
 #pragma unroll
         for (int u = 0; u < kB; u++) {
             e[u] = en[u];
-            px[u] = py[u] = pz[u] = 0.0;
-            if (kb + u * G < cnt) {
-                const double4 xj = a.pos[entry_idx(e[u])];
-                px[u] = xj.x;
-                py[u] = xj.y;
-                pz[u] = xj.z;
-            }
+            const double4 xj = a.pos[entry_idx(e[u])];
+            px[u] = xj.x;
+            py[u] = xj.y;
+            pz[u] = xj.z;
         }
 #pragma unroll
         for (int u = 0; u < kB; u++) {
@@ -208,21 +211,19 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
         if (!any_img) {
 #pragma unroll
             for (int u = 0; u < kB; u++)
-                if (kb + u * G < cnt) pair(e[u], px[u] - xi.x, py[u] - xi.y, pz[u] - xi.z);
+                pair(kb + u * G < cnt, px[u] - xi.x, py[u] - xi.y, pz[u] - xi.z);
         } else {
 #pragma unroll
             for (int u = 0; u < kB; u++) {
-                if (kb + u * G < cnt) {
-                    double shx, shy, shz;
-                    code_shifts(entry_code(e[u]), a.L[0], a.L[1], a.L[2], shx, shy, shz);
-                    // the shift goes on the non-owner: kept on the partner
-                    // (keep = 1) unless this particle is the non-owner; the
-                    // products and differences below are exact (+-L or +-0)
-                    const double keep = entry_rev(e[u]) ? 0.0 : 1.0;
-                    const double jx = keep * shx, jy = keep * shy, jz = keep * shz;
-                    pair(e[u], (px[u] + jx) - (xi.x + (shx - jx)), (py[u] + jy) - (xi.y + (shy - jy)),
-                         (pz[u] + jz) - (xi.z + (shz - jz)));
-                }
+                double shx, shy, shz;
+                code_shifts(entry_code(e[u]), a.L[0], a.L[1], a.L[2], shx, shy, shz);
+                // the shift goes on the non-owner: kept on the partner
+                // (keep = 1) unless this particle is the non-owner; the
+                // products and differences below are exact (+-L or +-0)
+                const double keep = entry_rev(e[u]) ? 0.0 : 1.0;
+                const double jx = keep * shx, jy = keep * shy, jz = keep * shz;
+                pair(kb + u * G < cnt, (px[u] + jx) - (xi.x + (shx - jx)),
+                     (py[u] + jy) - (xi.y + (shy - jy)), (pz[u] + jz) - (xi.z + (shz - jz)));
             }
         }
     }
@@ -748,7 +749,9 @@ __device__ void region_wall_forces(const ForceArgs& a, const double4& p, const d
 template <typename R, int LG>
 __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
     constexpr int G = 1 << LG;
-    const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    // XCD-aware block order: each XCD's L2 holds the positions of one
+    // contiguous (spatially sorted) range of particles and its neighbors
+    const int64_t gt = int64_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     const int64_t i = gt >> LG;
     const int t = int(gt & (G - 1));
     if (i >= a.n) return;
@@ -954,7 +957,9 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
 template <typename R, int LG>
 __global__ void __launch_bounds__(256) k_contact_ext(ForceArgs a) {
     constexpr int G = 1 << LG;
-    const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    // XCD-aware block order: each XCD's L2 holds the positions of one
+    // contiguous (spatially sorted) range of particles and its neighbors
+    const int64_t gt = int64_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     const int64_t i = gt >> LG;
     const int t = int(gt & (G - 1));
     if (i >= a.n) return;
