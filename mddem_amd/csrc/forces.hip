@@ -573,9 +573,10 @@ __device__ inline void wall_acc(const ForceArgs& a, int w, bool on, double f) {
 }
 
 template <typename R>
-__device__ inline void wall_forces(const ForceArgs& a, const double4& p, const double4& v,
+__device__ inline void wall_forces(const ForceArgs& a, const double4& p, const double4* vp,
                                    const double4* om, double mass, uint32_t ti, R& fx, R& fy,
                                    R& fz, R& tx, R& ty, R& tz) {
+    // the velocity is read only by particles touching a wall
     const double PI = 3.141592653589793;
     const int nm = a.nm, nn = nm * nm;
     const double radius = p.w;
@@ -610,6 +611,7 @@ __device__ inline void wall_forces(const ForceArgs& a, const double4& p, const d
                 s_n = 2.0 * e_eff * sdr;
                 k_n = 4.0 / 3.0 * e_eff * sdr;
             }
+            const double4 v = *vp;
             const double v_n =
                 (v.x - D.v[0]) * W.n[0] + (v.y - D.v[1]) * W.n[1] + (v.z - D.v[2]) * W.n[2];
             const double beta = a.mat[k];
@@ -678,7 +680,7 @@ __device__ inline void wall_forces(const ForceArgs& a, const double4& p, const d
             const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
             on = true;
             f_net =
-                curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
+                curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, *vp, delta, nx, ny, nz);
             fx += R(f_net * nx);
             fy += R(f_net * ny);
             fz += R(f_net * nz);
@@ -702,7 +704,7 @@ __device__ inline void wall_forces(const ForceArgs& a, const double4& p, const d
             const double delta = 0.5 * radius < delta0 ? 0.5 * radius : delta0;
             on = true;
             f_net =
-                curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
+                curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, *vp, delta, nx, ny, nz);
             fx += R(f_net * nx);
             fy += R(f_net * ny);
             fz += R(f_net * nz);
@@ -714,7 +716,7 @@ __device__ inline void wall_forces(const ForceArgs& a, const double4& p, const d
 // region-surface walls (dem_wall/src/lib.rs:1170-1231), the last wall type a
 // particle sees; kept out of the pair kernels (a separate pass after them)
 template <typename R>
-__device__ void region_wall_forces(const ForceArgs& a, const double4& p, const double4& v,
+__device__ void region_wall_forces(const ForceArgs& a, const double4& p, const double4* vp,
                                    double mass, uint32_t ti, R& fx, R& fy, R& fz) {
     const int nm = a.nm;
     const double radius = p.w;
@@ -735,7 +737,7 @@ __device__ void region_wall_forces(const ForceArgs& a, const double4& p, const d
             const double nx = sg * sr.nrm[0], ny = sg * sr.nrm[1], nz = sg * sr.nrm[2];
             on = true;
             f_net =
-                curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, v, delta, nx, ny, nz);
+                curved_wall_force(a, int(ti) * nm + W.mat, radius, mass, *vp, delta, nx, ny, nz);
             fx += R(f_net * nx);
             fy += R(f_net * ny);
             fz += R(f_net * nz);
@@ -757,7 +759,6 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
     if (i >= a.n) return;
     if (gated(a)) return;
     const double4 pi = a.pos[i];
-    const double4 vi = a.vel[i];
     R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
     const uint32_t ti = a.type[i];
     // box lengths and (batched ensemble) the system's single-material constants
@@ -769,11 +770,12 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
         Lz = S->box.L[2];
     }
     if (a.use_pairs) {
-        const double4 wi = a.omega[i];
-        const uint32_t tagi = a.tag[i];
         int cnt = int(a.nnb[i]);
         if (cnt > a.maxnb) cnt = a.maxnb;
-        const uint64_t mask_in = a.cmask[i];
+        // contact bits only ever cover row slots: an empty row has none, so
+        // a particle without neighbors reads and writes no contact state
+        uint64_t mask_in = 0ull;
+        if (cnt > 0) mask_in = a.cmask[i];
         uint64_t mask_out = 0ull;
         const int nm = a.nm;
         const double* beta_t = a.mat;
@@ -821,9 +823,11 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
                 }
             }
             if (delta <= 0.0) continue;
-            const double4 vj = a.vel[j];
-            const double4 wj = a.omega[j];
-            const uint32_t tagj = a.tag[j];
+            // the particle's own velocity, spin and tag are read with the
+            // partner's, only for pairs in contact
+            const double4 vi = a.vel[i], vj = a.vel[j];
+            const double4 wi = a.omega[i], wj = a.omega[j];
+            const uint32_t tagi = a.tag[i], tagj = a.tag[j];
             const uint32_t tj = a.type[j];
             // owner (O) / partner (P) quantities
             const double4& vO = rev ? vj : vi;
@@ -932,11 +936,11 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
         ty = group_sum<G>(ty);
         tz = group_sum<G>(tz);
         mask_out = group_or<G>(mask_out);
-        if (t == 0) a.cmask[i] = mask_out;
+        if (t == 0 && mask_out != mask_in) a.cmask[i] = mask_out;
     }
     if (t != 0) return;
     if (a.use_walls)
-        wall_forces<R>(a, pi, vi, a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
+        wall_forces<R>(a, pi, &a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
                        ty, tz);
     if (a.use_gravity) {
         const double m = a.mass[i];
@@ -1225,7 +1229,7 @@ __global__ void __launch_bounds__(256) k_contact_ext(ForceArgs a) {
     }
     if (t != 0) return;
     if (a.use_walls)
-        wall_forces<R>(a, pi, vi, a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
+        wall_forces<R>(a, pi, &a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
                        ty, tz);
     if (a.use_gravity) {
         const double m = a.mass[i];
@@ -1256,9 +1260,9 @@ __global__ void k_walls_gravity(ForceArgs a) {
     R tx = t ? t[i] : R(0), ty = t ? t[a.npad + i] : R(0), tz = t ? t[2 * a.npad + i] : R(0);
     if (a.use_walls) {
         if (!a.regions_only)
-            wall_forces<R>(a, a.pos[i], a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i],
+            wall_forces<R>(a, a.pos[i], &a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i],
                            a.type[i], fx, fy, fz, tx, ty, tz);
-        if (a.has_regions) region_wall_forces<R>(a, a.pos[i], a.vel[i], a.mass[i], a.type[i], fx, fy, fz);
+        if (a.has_regions) region_wall_forces<R>(a, a.pos[i], &a.vel[i], a.mass[i], a.type[i], fx, fy, fz);
     }
     if (t) {
         t[i] = tx;
