@@ -15,6 +15,8 @@
 // FP64 pair geometry (image shift, separation, cutoff test, overlap, normal,
 // relative velocity) and the FP64 shear spring, and does the force algebra
 // and per-particle accumulation in FP32.
+#include <hip/hip_ext.h>
+
 #include "cells.hpp"
 #include "ctx.hpp"
 
@@ -1332,36 +1334,48 @@ ForceArgs make_args(Ctx& c, int step) {
     return a;
 }
 
+// A pair-force launch of nb blocks of 256 threads. Timing events, when
+// given, are attached to the dispatch itself (hipExtLaunchKernelGGL), so
+// they time the kernel alone, as a kernel trace does.
+template <typename K>
+void launch_force_kernel(Ctx& c, K kern, int nb, const ForceArgs& a, hipEvent_t e0,
+                         hipEvent_t e1) {
+    if (e0)
+        hipExtLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c.stream, e0, e1, 0, a);
+    else
+        kern<<<nb, 256, 0, c.stream>>>(a);
+}
+
 template <typename R, bool E>
-void launch_lj(Ctx& c, const ForceArgs& a) {
+void launch_lj(Ctx& c, const ForceArgs& a, hipEvent_t e0, hipEvent_t e1) {
     const int64_t threads = c.n << c.list_lg;
     const int nb = blocks_for(threads, 256);
     switch (c.list_lg) {
-        case 0: k_lj<R, E, 0><<<nb, 256, 0, c.stream>>>(a); break;
-        case 1: k_lj<R, E, 1><<<nb, 256, 0, c.stream>>>(a); break;
-        case 3: k_lj<R, E, 3><<<nb, 256, 0, c.stream>>>(a); break;
-        default: k_lj<R, E, 2><<<nb, 256, 0, c.stream>>>(a); break;
+        case 0: launch_force_kernel(c, k_lj<R, E, 0>, nb, a, e0, e1); break;
+        case 1: launch_force_kernel(c, k_lj<R, E, 1>, nb, a, e0, e1); break;
+        case 3: launch_force_kernel(c, k_lj<R, E, 3>, nb, a, e0, e1); break;
+        default: launch_force_kernel(c, k_lj<R, E, 2>, nb, a, e0, e1); break;
     }
 }
 
 template <typename R>
-void launch_hertz(Ctx& c, const ForceArgs& a) {
+void launch_hertz(Ctx& c, const ForceArgs& a, hipEvent_t e0, hipEvent_t e1) {
     const int64_t threads = c.n << c.list_lg;
     const int nb = blocks_for(threads, 256);
     if (c.ext_on) {
         switch (c.list_lg) {
-            case 0: k_contact_ext<R, 0><<<nb, 256, 0, c.stream>>>(a); break;
-            case 1: k_contact_ext<R, 1><<<nb, 256, 0, c.stream>>>(a); break;
-            case 3: k_contact_ext<R, 3><<<nb, 256, 0, c.stream>>>(a); break;
-            default: k_contact_ext<R, 2><<<nb, 256, 0, c.stream>>>(a); break;
+            case 0: launch_force_kernel(c, k_contact_ext<R, 0>, nb, a, e0, e1); break;
+            case 1: launch_force_kernel(c, k_contact_ext<R, 1>, nb, a, e0, e1); break;
+            case 3: launch_force_kernel(c, k_contact_ext<R, 3>, nb, a, e0, e1); break;
+            default: launch_force_kernel(c, k_contact_ext<R, 2>, nb, a, e0, e1); break;
         }
         return;
     }
     switch (c.list_lg) {
-        case 0: k_hertz<R, 0><<<nb, 256, 0, c.stream>>>(a); break;
-        case 1: k_hertz<R, 1><<<nb, 256, 0, c.stream>>>(a); break;
-        case 3: k_hertz<R, 3><<<nb, 256, 0, c.stream>>>(a); break;
-        default: k_hertz<R, 2><<<nb, 256, 0, c.stream>>>(a); break;
+        case 0: launch_force_kernel(c, k_hertz<R, 0>, nb, a, e0, e1); break;
+        case 1: launch_force_kernel(c, k_hertz<R, 1>, nb, a, e0, e1); break;
+        case 3: launch_force_kernel(c, k_hertz<R, 3>, nb, a, e0, e1); break;
+        default: launch_force_kernel(c, k_hertz<R, 2>, nb, a, e0, e1); break;
     }
 }
 
@@ -1390,11 +1404,16 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool prof = c.prof_every > 0 && step >= 0 && (c.prof_ctr++ % c.prof_every) == 0 &&
                       c.prof_pending < int64_t(c.ev_pool.size() / 2);
+    // the pass is one kernel except for cell tiles (staging) and region
+    // walls (a second pass): the events go on that kernel, else around all
+    const bool one_kernel = !(c.model == 0 && c.list_tile) &&
+                            !(c.model == 1 && step >= 0 && c.walls.count > 0 && a.has_regions);
     if (prof) {
         e0 = c.ev_pool[2 * c.prof_pending];
         e1 = c.ev_pool[2 * c.prof_pending + 1];
-        MDD_HIP(hipEventRecord(e0, c.stream));
+        if (!one_kernel) MDD_HIP(hipEventRecord(e0, c.stream));
     }
+    hipEvent_t k0 = one_kernel ? e0 : nullptr, k1 = one_kernel ? e1 : nullptr;
     c.last_force_fused = false;
     if (c.model == 0 && c.list_tile) {
         if (energy) {
@@ -1412,14 +1431,14 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
             a.pe = c.pe.p;
             a.vir = c.vir.p;
             if (mixed)
-                launch_lj<float, true>(c, a);
+                launch_lj<float, true>(c, a, k0, k1);
             else
-                launch_lj<double, true>(c, a);
+                launch_lj<double, true>(c, a, k0, k1);
         } else {
             if (mixed)
-                launch_lj<float, false>(c, a);
+                launch_lj<float, false>(c, a, k0, k1);
             else
-                launch_lj<double, false>(c, a);
+                launch_lj<double, false>(c, a, k0, k1);
         }
     } else {
         a.use_pairs = 1;
@@ -1430,9 +1449,9 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
         const bool regions = a.use_walls && a.has_regions;
         if (regions) a.use_gravity = 0;
         if (mixed)
-            launch_hertz<float>(c, a);
+            launch_hertz<float>(c, a, k0, k1);
         else
-            launch_hertz<double>(c, a);
+            launch_hertz<double>(c, a, k0, k1);
         if (regions) {
             ForceArgs b = a;
             b.regions_only = 1;
@@ -1447,7 +1466,7 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
     MDD_HIP(hipGetLastError());
     c.force_launches++;
     if (prof) {
-        MDD_HIP(hipEventRecord(e1, c.stream));
+        if (!one_kernel) MDD_HIP(hipEventRecord(e1, c.stream));
         c.prof_step[size_t(c.prof_pending)] = step;
         c.prof_pending++;
     }
