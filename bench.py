@@ -187,9 +187,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=5000)
     ap.add_argument("--graphs", choices=["on", "off"], default="on",
                     help="issue neighbor rebuilds as captured HIP graphs")
-    ap.add_argument("--layout", choices=["auto", "tile", "1", "2", "3"], default="3",
+    ap.add_argument("--layout", choices=["auto", "tile", "1", "2", "3"], default="2",
                     help="pair-force layout: tuned during warmup, cell tiles, or rows at "
-                         "2^k lanes per particle (default 8)")
+                         "2^k lanes per particle (default 4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precision-report", type=int, default=2000,

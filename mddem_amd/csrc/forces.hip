@@ -22,7 +22,7 @@
 
 // row entries per lane gathered together by the LJ row kernel
 #ifndef MDD_LJ_BATCH
-#define MDD_LJ_BATCH 4
+#define MDD_LJ_BATCH 3
 #endif
 
 namespace mdd {
