@@ -123,6 +123,11 @@ __device__ inline uint64_t group_or(uint64_t v) {
     return v;
 }
 
+// partner (j) and own (i) periodic shift of a row entry, 64 B per entry
+struct alignas(16) ShiftPair {
+    double j[3], i[3], pad[2];
+};
+
 // lj_force (md_lj/src/lib.rs:488-633): for each neighbor inside the cutoff,
 // fpair = r^-2 r^-6 (lj1 r^-6 - lj2), owner gets fma(-fpair, d, f),
 // partner fma(fpair, d, f), d = partner image - owner.
@@ -142,6 +147,27 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     const int64_t gt = int64_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     const int64_t i = gt >> LG;
     const int t = int(gt & (G - 1));
+    // The periodic shifts of a row entry, by its top six bits (image code
+    // and owner flag): the shift goes on the non-owner, so entry bits
+    // (code, rev) give the partner's shift j and this particle's shift i,
+    // one of them s = code_shifts(code) and the other +0 (exact values).
+    __shared__ ShiftPair shp[64];
+    if (threadIdx.x < 64u) {
+        const int code = int(threadIdx.x & 31u);
+        const bool rev = threadIdx.x >= 32u;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        if (code < 27) code_shifts(code, a.L[0], a.L[1], a.L[2], sx, sy, sz);
+        ShiftPair q;
+        q.j[0] = rev ? 0.0 : sx;
+        q.j[1] = rev ? 0.0 : sy;
+        q.j[2] = rev ? 0.0 : sz;
+        q.i[0] = rev ? sx : 0.0;
+        q.i[1] = rev ? sy : 0.0;
+        q.i[2] = rev ? sz : 0.0;
+        q.pad[0] = q.pad[1] = 0.0;
+        shp[threadIdx.x] = q;
+    }
+    __syncthreads();
     if (i >= a.n) return;
     if (gated(a)) return;
     const double4 xi = a.pos[i];
@@ -217,15 +243,9 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
         } else {
 #pragma unroll
             for (int u = 0; u < kB; u++) {
-                double shx, shy, shz;
-                code_shifts(entry_code(e[u]), a.L[0], a.L[1], a.L[2], shx, shy, shz);
-                // the shift goes on the non-owner: kept on the partner
-                // (keep = 1) unless this particle is the non-owner; the
-                // products and differences below are exact (+-L or +-0)
-                const double keep = entry_rev(e[u]) ? 0.0 : 1.0;
-                const double jx = keep * shx, jy = keep * shy, jz = keep * shz;
-                pair(kb + u * G < cnt, (px[u] + jx) - (xi.x + (shx - jx)),
-                     (py[u] + jy) - (xi.y + (shy - jy)), (pz[u] + jz) - (xi.z + (shz - jz)));
+                const ShiftPair& q = shp[e[u] >> kIdxBits];
+                pair(kb + u * G < cnt, (px[u] + q.j[0]) - (xi.x + q.i[0]),
+                     (py[u] + q.j[1]) - (xi.y + q.i[1]), (pz[u] + q.j[2]) - (xi.z + q.i[2]));
             }
         }
     }
