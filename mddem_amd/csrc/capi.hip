@@ -53,6 +53,12 @@ __global__ void k_status_init(mdd::Status* st) {
 __global__ void k_set_pxi(mdd::Status* st, double v) { st->p_xi = v; }
 
 constexpr int kEventPool = 2048;
+// a chunk after a rebuild runs to this multiple of the smoothed rebuild
+// interval, plus one step
+#ifndef MDD_CHUNK_OVERSHOOT
+#define MDD_CHUNK_OVERSHOOT 1.0
+#endif
+constexpr double kChunkOvershoot = MDD_CHUNK_OVERSHOOT;
 
 int fail(mdd_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->c.err = msg;
@@ -212,7 +218,7 @@ void run_steps(Ctx& c, int64_t nsteps) {
             if (ran > 0) {
                 c.rb_interval = c.rb_interval > 0.0 ? 0.5 * c.rb_interval + 0.5 * double(ran)
                                                     : double(ran);
-                c.chunk = int(std::min(1024.0, std::ceil(1.15 * c.rb_interval) + 1.0));
+                c.chunk = int(std::min(1024.0, std::ceil(kChunkOvershoot * c.rb_interval) + 1.0));
             }
             c.last_rebuild = ab;
         }
@@ -268,7 +274,7 @@ void run_steps(Ctx& c, int64_t nsteps) {
                 // short chunks that grow with the steps run since the rebuild;
                 // with no interval known yet, doubling chunks
                 const double since = double(c.step - c.last_rebuild);
-                const double rest = 1.15 * c.rb_interval - since;
+                const double rest = kChunkOvershoot * c.rb_interval - since;
                 if (c.rb_interval <= 0.0)
                     c.chunk = std::min(c.chunk * 2, 1024);
                 else if (rest >= 1.0)
