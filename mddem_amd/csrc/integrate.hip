@@ -171,7 +171,8 @@ __device__ inline void rotate_initial(const IntArgs& a, int64_t i, double tx, do
     w.x += 0.5 * dt * tx * ii;
     w.y += 0.5 * dt * ty * ii;
     w.z += 0.5 * dt * tz * ii;
-    a.omega[i] = w;
+    // a zero torque leaves the spin as it is: no store
+    if (tx != 0.0 || ty != 0.0 || tz != 0.0) a.omega[i] = w;
     rotate_quat(a, i, w);
 }
 
@@ -185,7 +186,7 @@ __device__ inline void rotate_final(const IntArgs& a, int64_t i, double tx, doub
     w.x += 0.5 * dt * tx * ii;
     w.y += 0.5 * dt * ty * ii;
     w.z += 0.5 * dt * tz * ii;
-    a.omega[i] = w;
+    if (tx != 0.0 || ty != 0.0 || tz != 0.0) a.omega[i] = w;
 }
 
 __device__ inline void kick_drift(const IntArgs& a, double4& v, double4& p, double fx, double fy,
@@ -334,7 +335,8 @@ __global__ void __launch_bounds__(kThreads) k_post_pre(IntArgs a) {
             w.x += 0.5 * dt * tx * ii;  // initial_rotation of step s
             w.y += 0.5 * dt * ty * ii;
             w.z += 0.5 * dt * tz * ii;
-            a.omega[i] = w;
+            // a zero torque leaves the spin as it is: no store
+            if (tx != 0.0 || ty != 0.0 || tz != 0.0) a.omega[i] = w;
             rotate_quat(a, i, w);
         }
     }
