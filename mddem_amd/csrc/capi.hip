@@ -1182,6 +1182,7 @@ int mdd_set_half_list(mdd_ctx* ctx, int64_t nlocal, const uint32_t* offsets,
         }
         c.list_valid = true;
         c.list_injected = true;
+        c.listed_ok = false;
         c.list_tile = false;
         c.list_lg = c.lg;
         c.npairs = offsets[c.n];

@@ -368,6 +368,10 @@ struct Ctx {
     int64_t npairs = 0;
     bool list_valid = false;
     bool list_injected = false;
+    // DEM: the particles with a nonempty row at the last rebuild and their
+    // count (the force pass at one lane per particle is split by them)
+    DBuf<uint32_t> listed, listed_n;
+    bool listed_ok = false;
 
     // ---- particle arrays (current order)
     DBuf<double4> pos, vel, omega, quat;  // pos.w radius, vel.w 1/m, omega.w 1/I

@@ -77,6 +77,8 @@ struct ForceArgs {
     int step;
     const SysP* sysp;     // batched ensemble: per-system box and materials, or null
     const uint32_t* psys;
+    const uint32_t* listed;    // DEM: particles with a nonempty row (last rebuild)
+    const uint32_t* listed_n;  // their count
 };
 
 __device__ inline bool gated(const ForceArgs& a) {
@@ -770,16 +772,10 @@ __device__ void region_wall_forces(const ForceArgs& a, const double4& p, const d
 
 // hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615) with
 // surface energy, cohesion and rolling / twisting friction all zero
+// one particle (lane t of its G lanes)
 template <typename R, int LG>
-__global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
+__device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
     constexpr int G = 1 << LG;
-    // XCD-aware block order: each XCD's L2 holds the positions of one
-    // contiguous (spatially sorted) range of particles and its neighbors
-    const int64_t gt = int64_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-    const int64_t i = gt >> LG;
-    const int t = int(gt & (G - 1));
-    if (i >= a.n) return;
-    if (gated(a)) return;
     const double4 pi = a.pos[i];
     R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
     const uint32_t ti = a.type[i];
@@ -961,6 +957,55 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
         if (t == 0 && mask_out != mask_in) a.cmask[i] = mask_out;
     }
     if (t != 0) return;
+    if (a.use_walls)
+        wall_forces<R>(a, pi, &a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
+                       ty, tz);
+    if (a.use_gravity) {
+        const double m = a.mass[i];
+        fx += R(m * a.g[0]);
+        fy += R(m * a.g[1]);
+        fz += R(m * a.g[2]);
+    }
+    store_force<R>(a, i, fx, fy, fz, tx, ty, tz);
+}
+
+template <typename R, int LG>
+__global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
+    constexpr int G = 1 << LG;
+    // XCD-aware block order: each XCD's L2 holds the positions of one
+    // contiguous (spatially sorted) range of particles and its neighbors
+    const int64_t gt = int64_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    const int64_t i = gt >> LG;
+    const int t = int(gt & (G - 1));
+    if (i >= a.n) return;
+    if (gated(a)) return;
+    hertz_particle<R, LG>(a, i, t);
+}
+
+// The same pass split by row length (one lane per particle): the particles
+// whose row is not empty, listed at the last rebuild, run the pair kernel
+// above (grid-stride over the list); every other particle has no pair
+// state to read or write and only meets the walls and gravity, in a kernel
+// light enough to keep many wavefronts in flight. Each particle's
+// arithmetic is that of k_hertz, so the forces are the same bit for bit.
+template <typename R>
+__global__ void __launch_bounds__(256) k_hertz_listed(ForceArgs a) {
+    if (gated(a)) return;
+    const uint32_t nl = *a.listed_n;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += stride)
+        hertz_particle<R, 0>(a, int64_t(a.listed[k]), 0);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_hertz_unlisted(ForceArgs a) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= a.n) return;
+    if (gated(a)) return;
+    if (a.nnb[i] != 0u) return;  // a listed particle
+    const double4 pi = a.pos[i];
+    R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
+    const uint32_t ti = a.type[i];
     if (a.use_walls)
         wall_forces<R>(a, pi, &a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
                        ty, tz);
@@ -1426,7 +1471,10 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
                       c.prof_pending < int64_t(c.ev_pool.size() / 2);
     // the pass is one kernel except for cell tiles (staging) and region
     // walls (a second pass): the events go on that kernel, else around all
-    const bool one_kernel = !(c.model == 0 && c.list_tile) &&
+    // DEM at one lane per particle: the pass is split by row length
+    const bool split = c.model == 1 && c.listed_ok && c.list_valid && !c.list_injected &&
+                       c.list_lg == 0 && !c.ext_on && c.nsys == 0;
+    const bool one_kernel = !(c.model == 0 && c.list_tile) && !split &&
                             !(c.model == 1 && step >= 0 && c.walls.count > 0 && a.has_regions);
     if (prof) {
         e0 = c.ev_pool[2 * c.prof_pending];
@@ -1468,10 +1516,23 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
         // own pass follows the pair pass and adds gravity after them
         const bool regions = a.use_walls && a.has_regions;
         if (regions) a.use_gravity = 0;
-        if (mixed)
+        if (split) {
+            a.listed = c.listed.p;
+            a.listed_n = c.listed_n.p;
+            const int nb = blocks_for(c.n, 256);
+            const int nbl = nb < 2048 ? nb : 2048;
+            if (mixed) {
+                k_hertz_listed<float><<<nbl, 256, 0, c.stream>>>(a);
+                k_hertz_unlisted<float><<<nb, 256, 0, c.stream>>>(a);
+            } else {
+                k_hertz_listed<double><<<nbl, 256, 0, c.stream>>>(a);
+                k_hertz_unlisted<double><<<nb, 256, 0, c.stream>>>(a);
+            }
+        } else if (mixed) {
             launch_hertz<float>(c, a, k0, k1);
-        else
+        } else {
             launch_hertz<double>(c, a, k0, k1);
+        }
         if (regions) {
             ForceArgs b = a;
             b.regions_only = 1;

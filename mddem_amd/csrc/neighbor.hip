@@ -770,6 +770,22 @@ __global__ void __launch_bounds__(256) k_copy_image_rows(int64_t ghost_cap, int6
 // lower tag's frame, so it transfers unchanged). Only slots that were in
 // contact at the last force pass carry history (the reference prunes the
 // rest, contact.rs:586-589); their contact bits move with them.
+// The particles whose row is not empty, appended in wavefront batches (the
+// order is free: each particle's force is computed by one thread alone)
+__global__ void __launch_bounds__(256) k_list_rows(int64_t n, const uint32_t* __restrict__ nnb,
+                                                   uint32_t* __restrict__ listed,
+                                                   uint32_t* __restrict__ count) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const bool has = i < n && nnb[i] != 0u;
+    const uint64_t m = __ballot(has);
+    if (m == 0ull) return;
+    const int lead = __builtin_ctzll(m);
+    uint32_t base = 0u;
+    if (int(threadIdx.x & 63u) == lead) base = atomicAdd(count, uint32_t(__popcll(m)));
+    base = uint32_t(__shfl(int(base), lead, 64));
+    if (has) listed[base + lanes_before(m)] = uint32_t(i);
+}
+
 __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
                                 const uint32_t* __restrict__ nbr,
                                 const uint32_t* __restrict__ nnb,
@@ -1090,8 +1106,13 @@ void rebuild_attempt(Ctx& c) {
             n, c.npad, c.maxnb, c.lg, c.nbr.p, c.nnb.p, c.tag.p, c.order.p, c.npad_old,
             c.maxnb_old, c.lg_old, c.nbr_old.p, c.nnb_old.p, c.tag2.p, c.hist_old.p,
             c.cmask_old.p, c.hist.p, c.cmask.p, c.remap_have_old ? 1 : 0, c.hist_slots);
+        c.listed.resize(size_t(c.npad));
+        c.listed_n.resize(1);
+        fill_u32(c, c.listed_n.p, 0u, 1);
+        launch(c, k_list_rows, blocks_for(n, 256), 256, n, c.nnb.p, c.listed.p, c.listed_n.p);
         MDD_HIP(hipGetLastError());
     }
+    c.listed_ok = dem;
     c.list_valid = true;
     c.list_injected = false;
     c.list_lg = c.lg;

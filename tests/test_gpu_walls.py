@@ -207,3 +207,24 @@ def test_wall_runs_are_bitwise_deterministic():
         outs.append((s.state()["vel"], s.walls()["point"], s.walls()["force"]))
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+def test_one_lane_split_pass_matches_oracle(prec):
+    # at one lane per sphere the force pass runs the spheres with a nonempty
+    # row (listed at each rebuild) and the others (walls and gravity only) in
+    # two kernels; most of this cloud's spheres have no neighbor
+    w, o, s = _pair(EXTRAS["plain"], prec)
+    s.set_force_lanes(0)
+    for _ in range(3):
+        o.run(100)
+        s.run(100)
+        fw = o.walls()["force"]
+        assert np.abs(s.walls()["force"] - fw).max() <= 1e-6 * max(1.0, np.abs(fw).max())
+    ref = o.atoms()
+    st = s.state()
+    tol = 1e-9 if prec == hip.FP64 else 1e-4
+    dp = by_tag(st["tag"], st["pos"]) - by_tag(ref["tag"], ref["pos"])
+    assert np.abs(dp).max() < tol * 0.03
+    assert rel_err(by_tag(st["tag"], st["vel"]), by_tag(ref["tag"], ref["vel"])) < tol * 10
+    assert 0 < s.npairs() and s.thermo()["rebuilds"] == o.rebuilds
