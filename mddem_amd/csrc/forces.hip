@@ -1002,15 +1002,22 @@ __global__ void __launch_bounds__(256) k_hertz_unlisted(ForceArgs a) {
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (i >= a.n) return;
     if (gated(a)) return;
-    if (a.nnb[i] != 0u) return;  // a listed particle
+    // every load is issued before the row-length test: one memory latency
+    // per wavefront (a listed particle's loads are wasted)
+    const uint32_t nn = a.nnb[i];
     const double4 pi = a.pos[i];
-    R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
     const uint32_t ti = a.type[i];
+    const double mi = a.mass[i];
+    // (an empty statement that takes the loaded values keeps the compiler
+    // from sinking their loads below the test)
+    __asm__ volatile("" ::"v"(pi.x), "v"(pi.y), "v"(pi.z), "v"(pi.w), "v"(ti), "v"(mi));
+    if (nn != 0u) return;  // a listed particle
+    R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
     if (a.use_walls)
-        wall_forces<R>(a, pi, &a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
+        wall_forces<R>(a, pi, &a.vel[i], a.omega ? &a.omega[i] : nullptr, mi, ti, fx, fy, fz, tx,
                        ty, tz);
     if (a.use_gravity) {
-        const double m = a.mass[i];
+        const double m = mi;
         fx += R(m * a.g[0]);
         fy += R(m * a.g[1]);
         fz += R(m * a.g[2]);
