@@ -112,7 +112,9 @@ __device__ inline double nh_term(const IntArgs& a, double ke) {
 }
 
 // displacement_exceeded (neighbor.rs:564-601) + periodic box exit (651-666)
-__device__ inline void rebuild_check(const IntArgs& a, int64_t i, const double4& p) {
+// for position p and the position b at the last build
+__device__ inline void rebuild_check(const IntArgs& a, int64_t i, const double4& p,
+                                     const double4& b) {
     const BoxP* box = &a.box;
     double thr2 = a.thr2;
     if (a.sysp) {
@@ -120,7 +122,6 @@ __device__ inline void rebuild_check(const IntArgs& a, int64_t i, const double4&
         box = &S.box;
         thr2 = S.thr2;
     }
-    const double4 b = a.build_pos[i];
     double d[3] = {p.x - b.x, p.y - b.y, p.z - b.z};
     const double x[3] = {p.x, p.y, p.z};
     bool out = false;
@@ -218,12 +219,12 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
     if (NH) {
         const double ke = 0.5 * sum_partials(a, lds4);
         const double term = nh_term(a, ke);
-        double p = a.st->nh_p[a.q_in];
-        if (a.st->nh_kind[a.q_in] == 0) p += term;  // post half of the previous step
-        p += term;                                  // this step's pre half
-        scale = exp(-a.dt / 2.0 * p / a.q_mass);
+        double pxi = a.st->nh_p[a.q_in];
+        if (a.st->nh_kind[a.q_in] == 0) pxi += term;  // post half of the previous step
+        pxi += term;                                    // this step's pre half
+        scale = exp(-a.dt / 2.0 * pxi / a.q_mass);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
-            a.st->nh_p[a.q_out] = p;
+            a.st->nh_p[a.q_out] = pxi;
             a.st->nh_scale[a.q_out] = scale;
             a.st->nh_kind[a.q_out] = 0;
         }
@@ -249,7 +250,7 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
         load_t<F>(a, i, tx, ty, tz);
         rotate_initial(a, i, tx, ty, tz);
     }
-    if (a.check) rebuild_check(a, i, p);
+    if (a.check) rebuild_check(a, i, p, a.build_pos[i]);
 }
 
 // FinalIntegration + PostFinalIntegration of one step: Verlet kick,
@@ -309,11 +310,21 @@ __global__ void __launch_bounds__(kThreads) k_post_pre(IntArgs a) {
         if (a.wdyn && a.step >= 0) walls_advance(*a.walls, *a.wdyn, a.dt);
     }
     if (i >= a.n) return;
+    // every load before the first store (the pointers may alias as far as
+    // the compiler knows, so it would not move later loads above a store)
     double4 p = a.pos[i];
+    double4 v = make_double4(0.0, 0.0, 0.0, 0.0), w = v, b = v;
+    double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
     if (a.vv) {
-        double4 v = a.vel[i];
-        double fx, fy, fz;
+        v = a.vel[i];
         load_f<F>(a, i, fx, fy, fz);
+    }
+    if (a.rot) {
+        load_t<F>(a, i, tx, ty, tz);
+        w = a.omega[i];
+    }
+    if (a.check) b = a.build_pos[i];
+    if (a.vv) {
         const double h = 0.5 * a.dt * v.w;
         v.x += h * fx;
         v.y += h * fy;
@@ -323,9 +334,6 @@ __global__ void __launch_bounds__(kThreads) k_post_pre(IntArgs a) {
         a.pos[i] = p;
     }
     if (a.rot) {
-        double tx, ty, tz;
-        load_t<F>(a, i, tx, ty, tz);
-        double4 w = a.omega[i];
         const double ii = w.w;
         if (ii != 0.0) {
             const double dt = a.dt;
@@ -340,7 +348,7 @@ __global__ void __launch_bounds__(kThreads) k_post_pre(IntArgs a) {
             rotate_quat(a, i, w);
         }
     }
-    if (a.check) rebuild_check(a, i, p);
+    if (a.check) rebuild_check(a, i, p, b);
 }
 
 // KE partials of the current velocities, laid out like the post kernel's
