@@ -65,3 +65,22 @@ def test_small_ensemble_run_reports():
     # inelastic collisions only remove energy (Haff cooling); over 300 steps
     # many systems have had no collision yet
     assert rep["haff_tg_ratio_max"] <= 1.0 + 1e-9
+
+
+def test_batched_systems_at_one_lane_match_alone_runs():
+    # one lane per sphere: a system run alone splits its force pass into the
+    # spheres with neighbors and the rest; the batch runs one pass
+    systems = _systems()
+    s = ensemble.make_context(systems, hip.FP64, box_exit=True)
+    s.set_force_lanes(0)
+    s.run(150)
+    got = ensemble.system_states(s, len(systems))
+    for w, (p, v) in zip(systems, got):
+        a = hip.from_workload(w, hip.FP64, box_exit=True)
+        a.set_force_lanes(0)
+        a.run(150)
+        pa, va = _by_tag(a.state())
+        L = w.hi[0] - w.lo[0]
+        assert np.abs(_periodic_diff(p, pa, L)).max() < 1e-12 * L
+        assert rel_err(v, va) < 1e-9
+        a.close()
