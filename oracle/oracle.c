@@ -16,8 +16,11 @@
 #define ORC_MAX_MAT 64
 #define ORC_MAX_WALL 16
 #define ORC_MAX_PRIM 8
-/* dem_atom/src/lib.rs:63 */
+/* dem_atom/src/lib.rs:63 (overridable only so that the closed-form tests can
+ * show they reject a checker built with a wrong damping constant) */
+#ifndef SQRT_5_3
 #define SQRT_5_3 0.9128709291752768
+#endif
 /* dem_granular/src/lib.rs:93-108 */
 #define TANGENTIAL_EPSILON 1e-30
 #define LARGE_OVERLAP_WARN_THRESHOLD 0.90

@@ -15,7 +15,10 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_DEFAULT_LIB = os.path.join(_HERE, "liboracle.so")
+# MDDEM_ORACLE_LIB names another build of oracle.c (the tests build variants
+# with a planted mistake to show that their closed forms reject it)
+_LIB_PATH = os.environ.get("MDDEM_ORACLE_LIB", _DEFAULT_LIB)
 
 MODEL_LJ, MODEL_DEM = 0, 1
 HIST_REFERENCE, HIST_CANONICAL = 0, 1
@@ -31,7 +34,7 @@ _ip = C.POINTER(C.c_int)
 def build() -> str:
     """Compile liboracle.so with its Makefile (gcc, no GPU needed)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
-    return _LIB_PATH
+    return _DEFAULT_LIB
 
 
 _lib = None
