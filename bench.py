@@ -54,6 +54,17 @@ def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True, layout=Non
         s.set_force_lanes(layout)
         tune = False
     layout = 2 if layout is None else layout
+    # set-up, untimed and outside the warmup count: run until the rebuild
+    # graphs of both alternating buffer sets are captured and the rebuild
+    # interval has been seen twice, so that a short warmup does not leave
+    # graph capture or chunk-length learning inside the timed window
+    setup = 0
+    while setup < 600:
+        th = s.thermo()
+        if th["rebuilds"] >= 3 and s.launch_graph_stats()[1] >= (2 if graphs else 0):
+            break
+        s.run(20)
+        setup += 20
     options = (-1, 1, 2, 3) if w.model == "lj" else (1, 2, 3)
     trial = warmup // (len(options) + 1) if tune else 0
     if trial >= 100:
@@ -95,17 +106,24 @@ def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True, layout=Non
                temperature=th1["temperature"], pe=th1["pe"], ke=th1["ke"],
                layout=layout_name(layout), tile=layout < 0, rebuild_ms=rb_ms,
                kernel_name=kernel_name(layout, precision == hip.MIXED), rebuild_gap_ms=rb_gap_ms,
-               rebuilds_timed=rb_n)
+               rebuilds_timed=rb_n, setup_steps=setup)
     s.close()
     return res
 
 
-def run_dem(steps, warmup, n=4_000_000):
-    """DEM settling line: polydisperse glass spheres between plane walls under
-    gravity, mixed and all-FP64, the same steps timed for each."""
+def run_dem(steps, warmup, n=4_000_000, log_every=0):
+    """DEM line: the settling workload's 4M polydisperse glass spheres, walls
+    and gravity, started as a resting bed (every sphere in contact, shaken at
+    the config's velocities, U[-0.1, 0.1] m/s), mixed and all-FP64 with the
+    same warmup and timed steps. Reports the mean half-list pairs k and
+    contacts c per sphere over the timed window and the whole-step roofline
+    with B_ps = S + 4 k + 56 c (S = 216 B mixed, 264 B FP64)."""
     from mddem_amd import hip
     from mddem_amd import workloads as W
-    w = W.dem_settling(n=n)
+    t0 = time.perf_counter()
+    w = W.dem_packed_bed(n=n)
+    log(f"[bench] dem bed of {w.n} spheres built in {time.perf_counter() - t0:.1f} s "
+        f"(top at z = {w.pos[:, 2].max():.4f} m)")
     out = {}
     for name, prec in (("mixed", hip.MIXED), ("fp64", hip.FP64)):
         s = hip.from_workload(w, prec, box_exit=False)
@@ -114,32 +132,50 @@ def run_dem(steps, warmup, n=4_000_000):
         s.synchronize()
         th0 = s.thermo()
         s.set_profiling(5)
-        t0 = time.perf_counter()
-        s.run(steps)
-        s.synchronize()
-        t1 = time.perf_counter()
+        # the timed window in segments, so that k and c are averaged over it
+        seg = steps if log_every <= 0 else log_every
+        pairs, contacts, done, wall = [], [], 0, 0.0
+        while done < steps:
+            k = min(seg, steps - done)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.run(k)
+            s.synchronize()
+            wall += time.perf_counter() - t0
+            done += k
+            th = s.thermo()
+            pairs.append(th["npairs"])
+            contacts.append(th["ncontacts"])
         th1 = s.thermo()
         ms, _ = s.force_kernel_stats()
-        rate = w.n * steps / (t1 - t0)
-        # whole-step algorithmic bytes: 216 B (mixed) / 264 B (FP64) of state,
-        # 4 B per half-list pair, 56 B per contact with shear history
+        rb_ms, rb_gap_ms, rb_n = s.rebuild_stats()
+        rate = w.n * steps / wall
+        kbar = float(np.mean(pairs)) / w.n
+        cbar = float(np.mean(contacts)) / w.n
         state = 216.0 if prec == hip.MIXED else 264.0
-        bps = state + (4.0 * th1["npairs"] + 56.0 * th1["ncontacts"]) / w.n
+        bps = state + 4.0 * kbar + 56.0 * cbar
         out[name] = dict(value=round(rate, 1), step_roofline_frac=round(rate * bps / 8e12, 4),
-                         ms_per_step=round(1e3 * (t1 - t0) / steps, 4),
-                         force_kernel_us=round(ms * 1e3, 2), npairs=th1["npairs"],
+                         bytes_per_particle_step=round(bps, 1),
+                         ms_per_step=round(1e3 * wall / steps, 4),
+                         force_kernel_us=round(ms * 1e3, 2), k_half_pairs=round(kbar, 3),
+                         c_contacts=round(cbar, 3), npairs=th1["npairs"],
                          ncontacts=th1["ncontacts"],
-                         rebuilds=th1["rebuilds"] - th0["rebuilds"])
+                         rebuilds=th1["rebuilds"] - th0["rebuilds"],
+                         rebuild_us=round(rb_ms * 1e3, 1),
+                         idle_before_rebuild_us=round(rb_gap_ms * 1e3, 1))
         s.close()
         log(f"[bench] dem {name}: {out[name]['value']:.4g} particle-steps/s, "
-            f"{out[name]['ms_per_step']} ms/step, pairs {th1['npairs']}, "
-            f"contacts {th1['ncontacts']}")
+            f"{out[name]['ms_per_step']} ms/step, k {kbar:.3f} c {cbar:.3f}, "
+            f"{out[name]['rebuilds']} rebuilds ({rb_ms * 1e3:.0f} us each), "
+            f"force pass {ms * 1e3:.1f} us")
     out["mixed_over_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
-    out["workload"] = (f"DEM settling, {w.n} glass spheres r~U[0.8,1.2] mm, box 0.4x0.4x1.2 m "
-                       "periodic x/y, plane walls z=0 and z=1.2 m, gravity, dt=0.05 t_Rayleigh")
+    out["workload"] = (f"DEM settling spheres as a resting bed: {w.n} glass spheres "
+                       "r~U[0.8,1.2] mm deposited on z=0 (every sphere touching), box "
+                       "0.4x0.4x1.2 m periodic x/y, plane walls z=0 and z=1.2 m, gravity, "
+                       "v~U[-0.1,0.1] m/s, dt=0.05 t_Rayleigh(0.8 mm)")
     out["steps"] = steps
     out["warmup"] = warmup
-    return out
+    return out, w
 
 
 def force_kernel_bytes(n, npairs, force_bytes):
@@ -149,21 +185,46 @@ def force_kernel_bytes(n, npairs, force_bytes):
     return n * 24.0 + 4.0 * npairs + n * 3.0 * force_bytes
 
 
-def cpu_baseline(w, seconds_budget=12.0):
-    """The CPU checker (a C port of the reference path) on the host, 1 thread,
-    over a bounded number of steps of the same workload."""
+def _cpu_sample(args):
+    """One replica of the CPU checker: `seconds` of scheduled steps after one
+    untimed set-up step; returns (steps, wall seconds)."""
+    w, seconds = args
     from tests.helpers import oracle_from_workload  # the checker; CPU leg only
     o = oracle_from_workload(w)
-    o.step()  # setup + first rebuild, untimed
+    o.step()
     steps = 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds_budget:
+    while time.perf_counter() - t0 < seconds:
         o.run(5)
         steps += 5
-    dt = time.perf_counter() - t0
-    return dict(value=w.n * steps / dt, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"{steps} scheduled steps (after 1 untimed setup step) of the same "
-                       f"{w.n}-atom workload on 1 host thread")
+    return steps, time.perf_counter() - t0
+
+
+def cpu_baseline(w, seconds_budget=12.0, replicas=None):
+    """The CPU checker (a C port of the reference path) on the host: 1 thread
+    over a bounded number of steps of the same workload, then `replicas`
+    independent copies at once (one process each), the embarrassingly
+    parallel upper bound of the host. The checker keeps the reference's
+    rebuild rule (decide_rebuild: also when any atom leaves the periodic
+    box), which at this size rebuilds nearly every step."""
+    import multiprocessing as mp
+    nproc = os.cpu_count() or 1
+    steps, dt = _cpu_sample((w, seconds_budget))
+    out = dict(value=w.n * steps / dt, unit="particle-steps/s", cores=1, kind="port",
+               sample=f"{steps} scheduled steps (after 1 untimed setup step) of the same "
+                      f"{w.n}-atom workload on 1 host thread; rebuilds as the reference "
+                      "decides them (skin spent or an atom outside the periodic box)",
+               nproc=nproc)
+    if replicas is None:
+        replicas = min(nproc, 16)
+    if replicas > 1:
+        ctx = mp.get_context("fork")
+        with ctx.Pool(replicas) as pool:
+            res = pool.map(_cpu_sample, [(w, seconds_budget / 2)] * replicas)
+        out["replicas"] = dict(value=sum(w.n * st / t for st, t in res), cores=replicas,
+                               note="independent copies at once, one process each: an "
+                                    "embarrassingly parallel upper bound")
+    return out
 
 
 def traffic_from_profiles(name, kernel):
@@ -184,7 +245,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--ncell", type=int, default=40)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--fp64-steps", type=int, default=5000)
+    ap.add_argument("--fp64-steps", type=int, default=-1,
+                    help="timed steps of the all-FP64 leg (-1: the same as --steps)")
     ap.add_argument("--graphs", choices=["on", "off"], default="on",
                     help="issue neighbor rebuilds as captured HIP graphs")
     ap.add_argument("--layout", choices=["auto", "tile", "1", "2", "3"], default="2",
@@ -192,15 +254,20 @@ def main():
                          "2^k lanes per particle (default 4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--precision-report", type=int, default=2000,
+    ap.add_argument("--cpu-replicas", type=int, default=-1,
+                    help="concurrent CPU checker copies (-1: min(nproc, 16); 0/1: none)")
+    ap.add_argument("--precision-report", type=int, default=20000,
                     help="NVE steps of the mixed-vs-FP64 precision report (0 = skip)")
     ap.add_argument("--ensemble-systems", type=int, default=1024,
                     help="systems in the batched-ensemble measurement (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs in the serving-stream measurement (0 = skip)")
-    ap.add_argument("--dem-steps", type=int, default=500,
+    ap.add_argument("--dem-steps", type=int, default=6000,
                     help="timed steps of the DEM 4M line per precision (0 = skip)")
+    ap.add_argument("--dem-warmup", type=int, default=200)
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=(1000, 50000),
+                    help="step range of serving jobs (log-uniform)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU scaling is not "
@@ -224,10 +291,12 @@ def main():
         f"({r['rebuilds_timed']} timed)")
 
     fp64 = None
-    if prec == hip.MIXED and args.fp64_steps > 0:
-        r64 = run_gpu(w, hip.FP64, args.fp64_steps, args.warmup, layout=layout, graphs=graphs)
+    r64 = None
+    fp64_steps = args.steps if args.fp64_steps < 0 else args.fp64_steps
+    if prec == hip.MIXED and fp64_steps > 0:
+        r64 = run_gpu(w, hip.FP64, fp64_steps, args.warmup, layout=layout, graphs=graphs)
         fp64 = r64["n"] * r64["steps"] / r64["seconds"]
-        log(f"[bench] fp64 {fp64:.4g} particle-steps/s, force kernel "
+        log(f"[bench] fp64 {fp64:.4g} particle-steps/s, {r64['rebuilds']} rebuilds, force kernel "
             f"{r64['force_ms'] * 1e3:.1f} us")
 
     fbytes = 4 if prec == hip.MIXED else 8
@@ -249,7 +318,12 @@ def main():
 
     dem = None
     if args.dem_steps > 0:
-        dem = run_dem(args.dem_steps, 50)
+        dem, wdem = run_dem(args.dem_steps, args.dem_warmup, log_every=1000)
+        if args.precision_report > 0:
+            from mddem_amd import precision as P
+            dem["precision"] = dict(force_error=P.dem_force_errors(wdem))
+            log(f"[bench] dem precision: {json.dumps(dem['precision'])}")
+        del wdem
 
     ens = None
     if args.ensemble_systems > 0:
@@ -260,18 +334,20 @@ def main():
     serving = None
     if args.serve_jobs > 0:
         from mddem_amd import serve
-        jobs = serve.make_stream(args.serve_jobs, rate=30.0, steps=(1000, 5000))
+        lo_s, hi_s = args.serve_steps
+        jobs = serve.make_stream(args.serve_jobs, rate=30.0, steps=(lo_s, hi_s))
         for j in jobs:
             j.workload = serve.build_workload(j)
         serving = serve.Server(serve.default_budget(), slots=8).run(jobs)
         serving["stream"] = (f"{args.serve_jobs} jobs, Poisson 30/s, 70% granular gas 1k-200k, "
-                             "30% LJ FCC 4k-500k, steps log-uniform [1000, 5000], mixed")
+                             f"30% LJ FCC 4k-500k, steps log-uniform [{lo_s}, {hi_s}], mixed")
         log(f"[bench] serving: {json.dumps(serving)}")
 
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(w, args.cpu_seconds)
+        cpu = cpu_baseline(w, args.cpu_seconds,
+                           None if args.cpu_replicas < 0 else args.cpu_replicas)
         log(f"[bench] cpu {cpu['value']:.4g} particle-steps/s")
 
     out = {
@@ -294,8 +370,13 @@ def main():
                    "force_layout": r["layout"],
                    "rebuild_us": round(r["rebuild_ms"] * 1e3, 1),
                    "idle_before_rebuild_us": round(r["rebuild_gap_ms"] * 1e3, 1),
+                   "setup_steps_untimed": r["setup_steps"],
                    "pairs_per_atom_half": round(r["npairs"] / r["n"], 3)},
         "fp64_gpu_value": None if fp64 is None else round(fp64, 1),
+        "fp64_gpu": None if r64 is None else {
+            "steps": r64["steps"], "warmup": args.warmup, "rebuilds_in_timed_loop": r64["rebuilds"],
+            "force_kernel_us": round(r64["force_ms"] * 1e3, 2),
+            "rebuild_us": round(r64["rebuild_ms"] * 1e3, 1)},
         "mixed_over_fp64": None if fp64 is None else round(value / fp64, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -83,6 +83,15 @@ int mdd_set_neighbor(mdd_ctx* ctx, double skin_fraction, double bin_size, int ev
  * decide_rebuild does (neighbor.rs:651-666). 0: wrap lazily at displacement
  * rebuilds only (pair images stay valid between rebuilds). */
 int mdd_set_rebuild_on_box_exit(mdd_ctx* ctx, int enabled);
+/* How a contact's shear history crosses a neighbor rebuild. 0 (default): as
+ * ContactHistoryStore does (crates/dem_granular/src/tangential.rs:54-133,
+ * contact.rs:358-364, 575-589): the spring lives in the list of the particle
+ * that owns the pair in the half list and is found by partner tag, so it
+ * survives a rebuild (sort_atoms_by_bin, neighbor.rs:686-794) only when the
+ * same particle still owns the pair; when the pair moves to the partner's
+ * row the spring restarts from zero. 1: the spring is keyed by the pair and
+ * survives whichever side owns it. */
+int mdd_set_history_mode(mdd_ctx* ctx, int mode);
 /* Atom::dt; [run] dt or calculate_delta_time (crates/dem_atom/src/insert.rs:1392-1431) */
 int mdd_set_dt(mdd_ctx* ctx, double dt);
 /* [[dem.materials]] + MaterialTable::build_pair_tables (crates/dem_atom/src/lib.rs:389-515) */
@@ -257,6 +266,15 @@ int mdd_download_history7(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t*
                           double* hist7, int64_t* n);
 
 /* ---- observables (print_thermo, mddem_print/src/lib.rs:595-700) -------------- */
+/* VirialStress of the DEM contact force (mddem_core/src/virial.rs:19-120,
+ * registered by HertzMindlinContactPlugin, crates/dem_granular/src/contact.rs:
+ * 50-91, accumulated at contact.rs:338-345, 559-569, 962-970): the contact
+ * force pass of every step whose number is a multiple of `every` (and every
+ * standalone mdd_hertz_mindlin_contact_force call) also sums d (x) F over the
+ * pairs, d = x_j - x_i, F the force on i from j; mdd_compute_thermo reports
+ * the last such sum in mdd_thermo::virial. 0 (default): not computed.
+ * Single systems only. */
+int mdd_set_virial(mdd_ctx* ctx, int every);
 int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out);
 /* block until queued work is done */
 int mdd_synchronize(mdd_ctx* ctx);

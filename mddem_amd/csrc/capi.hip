@@ -395,6 +395,7 @@ void pack_upload(Ctx& c, int64_t n, const uint32_t* tag, const uint32_t* type, c
     c.list_injected = false;
     c.setup_done = false;
     c.nh_ready = false;
+    c.vir_valid = false;
     c.step = 0;
     c.rebuilds = 0;
     c.nghost = 0;
@@ -576,6 +577,20 @@ int mdd_set_neighbor(mdd_ctx* ctx, double skin_fraction, double bin_size, int ev
 
 int mdd_set_rebuild_on_box_exit(mdd_ctx* ctx, int enabled) {
     return guarded(ctx, [&](Ctx& c) { c.box_exit_rebuild = enabled ? 1 : 0; });
+}
+
+int mdd_set_virial(mdd_ctx* ctx, int every) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (every < 0) throw ApiError(MDD_ERR_INVALID, "virial interval must be >= 0");
+        c.virial_every = every;
+    });
+}
+
+int mdd_set_history_mode(mdd_ctx* ctx, int mode) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (mode != 0 && mode != 1) throw ApiError(MDD_ERR_INVALID, "history mode is 0 or 1");
+        c.hist_by_owner = mode == 0 ? 1 : 0;
+    });
 }
 
 int mdd_set_dt(mdd_ctx* ctx, double dt) {
@@ -1431,6 +1446,16 @@ int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out) {
                 int64_t nc = 0;
                 for (int64_t i = 0; i < c.n; i++) nc += __builtin_popcountll(mk[i]);
                 out->ncontacts = nc / 2;
+            }
+        }
+        if (c.model == MDD_MODEL_DEM && c.vir_valid && c.vir.p) {
+            // the contact virial of the last pass that computed it
+            MDD_HIP(hipStreamSynchronize(c.stream));
+            auto w = dl(c, c.vir, size_t(6) * size_t(c.npad));
+            for (int k = 0; k < 6; k++) {
+                double t = 0.0;
+                for (int64_t i = 0; i < c.n; i++) t += w[size_t(k) * size_t(c.npad) + size_t(i)];
+                out->virial[k] = t;
             }
         }
         if (with_energy && c.model == MDD_MODEL_LJ && c.list_valid) {

@@ -424,6 +424,12 @@ struct Ctx {
     DBuf<double> ke_total;          // NH kinetic energy sum for the next pre kernel
     DBuf<uint32_t> row_blk;         // per-block row maxima of the last build
     DBuf<double> vir;               // energy passes: per-particle virial, 6 x npad
+    // DEM contact virial: computed in the force pass of every step whose
+    // number is a multiple of virial_every (0: never); vir then holds the
+    // pass of step vir_step
+    int virial_every = 0;
+    bool vir_valid = false;
+    int64_t vir_step = -1;
 
     // ---- full neighbor list: entry k of particle i at row_addr(i, k, npad, lg)
     DBuf<uint32_t> nbr, nbr_old;
@@ -446,6 +452,10 @@ struct Ctx {
     // history values per contact: 3 (tangential spring) or, with the optional
     // models, 7 (spring, rolling displacement, twist angle)
     int hist_slots = 3;
+    // 1: a contact's history survives a rebuild only if the same particle
+    // still owns the pair (the reference's per-owner store); 0: it survives
+    // whichever side owns the pair (keyed by the pair alone)
+    int hist_by_owner = 1;
     DBuf<double> pe;              // per-particle LJ energy (thermo only)
 
     // ---- run state

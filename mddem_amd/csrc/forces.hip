@@ -73,6 +73,8 @@ struct ForceArgs {
     float* t32;
     double* pe;
     double* vir;  // energy passes: per-particle half share of sum d (x) F, 6 x npad
+                  // (LJ); DEM virial passes: the owned pairs' full share
+    int dem_virial;  // DEM: this pass writes vir
     Status* st;
     int step;
     const SysP* sysp;     // batched ensemble: per-system box and materials, or null
@@ -123,6 +125,18 @@ __device__ inline uint64_t group_or(uint64_t v) {
         v |= (uint64_t(hi) << 32) | lo;
     }
     return v;
+}
+
+// VirialStress::add_pair (mddem_core/src/virial.rs:59-67): d = x_j - x_i,
+// f = force on i from j; xx yy zz xy xz yz
+__device__ inline void virial_add(double w[6], double dx, double dy, double dz, double fx,
+                                  double fy, double fz) {
+    w[0] += dx * fx;
+    w[1] += dy * fy;
+    w[2] += dz * fz;
+    w[3] += dx * fy;
+    w[4] += dx * fz;
+    w[5] += dy * fz;
 }
 
 // partner (j) and own (i) periodic shift of a row entry, 64 B per entry
@@ -772,12 +786,16 @@ __device__ void region_wall_forces(const ForceArgs& a, const double4& p, const d
 
 // hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615) with
 // surface energy, cohesion and rolling / twisting friction all zero
-// one particle (lane t of its G lanes)
-template <typename R, int LG>
+// one particle (lane t of its G lanes). VIR: also the contact virial
+// (contact.rs:559-569): each particle sums d (x) F over the pairs it owns,
+// d = partner image - owner, F = -F_n + F_t the force on the owner, into
+// vir[q * npad + i] (xx yy zz xy xz yz).
+template <typename R, int LG, bool VIR>
 __device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
     constexpr int G = 1 << LG;
     const double4 pi = a.pos[i];
     R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
     const uint32_t ti = a.type[i];
     // box lengths and (batched ensemble) the system's single-material constants
     double Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
@@ -919,6 +937,18 @@ __device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
                 fty *= scale;
                 ftz *= scale;
             }
+            if constexpr (VIR) {
+                if (!rev) {
+                    const double vfx = double(-fnx + ftx), vfy = double(-fny + fty),
+                                 vfz = double(-fnz + ftz);
+                    w0 += dx * vfx;
+                    w1 += dy * vfy;
+                    w2 += dz * vfz;
+                    w3 += dx * vfy;
+                    w4 += dx * vfz;
+                    w5 += dy * vfz;
+                }
+            }
             if (rev) {
                 // partner: +F_n - F_t, torque (r2 n) x F_t
                 fx += fnx;
@@ -955,8 +985,25 @@ __device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
         tz = group_sum<G>(tz);
         mask_out = group_or<G>(mask_out);
         if (t == 0 && mask_out != mask_in) a.cmask[i] = mask_out;
+        if constexpr (VIR) {
+            w0 = group_sum<G>(w0);
+            w1 = group_sum<G>(w1);
+            w2 = group_sum<G>(w2);
+            w3 = group_sum<G>(w3);
+            w4 = group_sum<G>(w4);
+            w5 = group_sum<G>(w5);
+        }
     }
     if (t != 0) return;
+    if constexpr (VIR) {
+        const int64_t np = a.npad;
+        a.vir[i] = w0;
+        a.vir[np + i] = w1;
+        a.vir[2 * np + i] = w2;
+        a.vir[3 * np + i] = w3;
+        a.vir[4 * np + i] = w4;
+        a.vir[5 * np + i] = w5;
+    }
     if (a.use_walls)
         wall_forces<R>(a, pi, &a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
                        ty, tz);
@@ -969,7 +1016,7 @@ __device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
     store_force<R>(a, i, fx, fy, fz, tx, ty, tz);
 }
 
-template <typename R, int LG>
+template <typename R, int LG, bool VIR>
 __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
     constexpr int G = 1 << LG;
     // XCD-aware block order: each XCD's L2 holds the positions of one
@@ -979,7 +1026,7 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
     const int t = int(gt & (G - 1));
     if (i >= a.n) return;
     if (gated(a)) return;
-    hertz_particle<R, LG>(a, i, t);
+    hertz_particle<R, LG, VIR>(a, i, t);
 }
 
 // The same pass split by row length (one lane per particle): the particles
@@ -994,7 +1041,7 @@ __global__ void __launch_bounds__(256) k_hertz_listed(ForceArgs a) {
     const uint32_t nl = *a.listed_n;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += stride)
-        hertz_particle<R, 0>(a, int64_t(a.listed[k]), 0);
+        hertz_particle<R, 0, false>(a, int64_t(a.listed[k]), 0);
 }
 
 template <typename R>
@@ -1046,6 +1093,7 @@ __global__ void __launch_bounds__(256) k_contact_ext(ForceArgs a) {
     const double4 pi = a.pos[i];
     const double4 vi = a.vel[i];
     R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
+    double vw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // contact virial (owned pairs)
     const uint32_t ti = a.type[i];
     const double Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
     if (a.use_pairs) {
@@ -1170,7 +1218,11 @@ __global__ void __launch_bounds__(256) k_contact_ext(ForceArgs a) {
             fx += R(sn * (f_n * nx));
             fy += R(sn * (f_n * ny));
             fz += R(sn * (f_n * nz));
-            if (adhesion_only) continue;  // no tangential part, no history
+            if (adhesion_only) {  // no tangential part, no history
+                if (a.dem_virial && !rev)
+                    virial_add(vw, dx, dy, dz, -(f_n * nx), -(f_n * ny), -(f_n * nz));
+                continue;
+            }
             const double vtx = vrx - v_n * nx, vty = vry - v_n * ny, vtz = vrz - v_n * nz;
             const double sign = tagO < tagP ? 1.0 : -1.0;
             double h[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1206,6 +1258,9 @@ __global__ void __launch_bounds__(256) k_contact_ext(ForceArgs a) {
                 fty *= sc;
                 ftz *= sc;
             }
+            if (a.dem_virial && !rev)
+                virial_add(vw, dx, dy, dz, -(f_n * nx) + ftx, -(f_n * ny) + fty,
+                           -(f_n * nz) + ftz);
             // owner: +F_t, torque (r1 n) x F_t; partner: -F_t, torque (r2 n) x F_t
             const double rnx = rev ? r2nx : r1nx, rny = rev ? r2ny : r1ny, rnz = rev ? r2nz : r1nz;
             fx += R(-sn * ftx);
@@ -1300,6 +1355,12 @@ __global__ void __launch_bounds__(256) k_contact_ext(ForceArgs a) {
         tz = group_sum<G>(tz);
         mask_out = group_or<G>(mask_out);
         if (t == 0) a.cmask[i] = mask_out;
+        if (a.dem_virial) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) vw[q] = group_sum<G>(vw[q]);
+            if (t == 0)
+                for (int q = 0; q < 6; q++) a.vir[q * a.npad + i] = vw[q];
+        }
     }
     if (t != 0) return;
     if (a.use_walls)
@@ -1430,7 +1491,7 @@ void launch_lj(Ctx& c, const ForceArgs& a, hipEvent_t e0, hipEvent_t e1) {
     }
 }
 
-template <typename R>
+template <typename R, bool VIR>
 void launch_hertz(Ctx& c, const ForceArgs& a, hipEvent_t e0, hipEvent_t e1) {
     const int64_t threads = c.n << c.list_lg;
     const int nb = blocks_for(threads, 256);
@@ -1444,10 +1505,10 @@ void launch_hertz(Ctx& c, const ForceArgs& a, hipEvent_t e0, hipEvent_t e1) {
         return;
     }
     switch (c.list_lg) {
-        case 0: launch_force_kernel(c, k_hertz<R, 0>, nb, a, e0, e1); break;
-        case 1: launch_force_kernel(c, k_hertz<R, 1>, nb, a, e0, e1); break;
-        case 3: launch_force_kernel(c, k_hertz<R, 3>, nb, a, e0, e1); break;
-        default: launch_force_kernel(c, k_hertz<R, 2>, nb, a, e0, e1); break;
+        case 0: launch_force_kernel(c, k_hertz<R, 0, VIR>, nb, a, e0, e1); break;
+        case 1: launch_force_kernel(c, k_hertz<R, 1, VIR>, nb, a, e0, e1); break;
+        case 3: launch_force_kernel(c, k_hertz<R, 3, VIR>, nb, a, e0, e1); break;
+        default: launch_force_kernel(c, k_hertz<R, 2, VIR>, nb, a, e0, e1); break;
     }
 }
 
@@ -1481,8 +1542,19 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
     // DEM at one lane per particle: the pass is split by row length
     // (single systems: in batched ensembles the grid-stride pair kernel lost
     // to the one-kernel pass, 1.64e10 vs 1.72e10 particle-steps/s)
+    // DEM contact virial on this pass (VirialStress active: every step
+    // number that is a multiple of virial_every; standalone calls always)
+    const bool dem_vir = c.model == 1 && c.nsys == 0 && c.virial_every > 0 &&
+                         (step < 0 || step % c.virial_every == 0);
+    if (dem_vir) {
+        c.vir.resize(size_t(6) * c.npad);
+        a.vir = c.vir.p;
+        a.dem_virial = 1;
+        c.vir_step = step < 0 ? c.step : step;
+        c.vir_valid = true;
+    }
     const bool split = c.model == 1 && c.listed_ok && c.list_valid && !c.list_injected &&
-                       c.list_lg == 0 && !c.ext_on && c.nsys == 0;
+                       c.list_lg == 0 && !c.ext_on && c.nsys == 0 && !dem_vir;
     const bool one_kernel = !(c.model == 0 && c.list_tile) && !split &&
                             !(c.model == 1 && step >= 0 && c.walls.count > 0 && a.has_regions);
     if (prof) {
@@ -1537,10 +1609,15 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
                 k_hertz_listed<double><<<nbl, 256, 0, c.stream>>>(a);
                 k_hertz_unlisted<double><<<nb, 256, 0, c.stream>>>(a);
             }
+        } else if (dem_vir) {
+            if (mixed)
+                launch_hertz<float, true>(c, a, k0, k1);
+            else
+                launch_hertz<double, true>(c, a, k0, k1);
         } else if (mixed) {
-            launch_hertz<float>(c, a, k0, k1);
+            launch_hertz<float, false>(c, a, k0, k1);
         } else {
-            launch_hertz<double>(c, a, k0, k1);
+            launch_hertz<double, false>(c, a, k0, k1);
         }
         if (regions) {
             ForceArgs b = a;

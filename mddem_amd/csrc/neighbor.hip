@@ -769,7 +769,12 @@ __global__ void __launch_bounds__(256) k_copy_image_rows(int64_t ghost_cap, int6
 // by partner tag, tangential.rs:54-133, and the spring is stored in the
 // lower tag's frame, so it transfers unchanged). Only slots that were in
 // contact at the last force pass carry history (the reference prunes the
-// rest, contact.rs:586-589); their contact bits move with them.
+// rest, contact.rs:586-589); their contact bits move with them. With
+// by_owner set (the reference's behaviour) the pair must also sit in the
+// same particle's half-list row as before: the reference keeps a contact's
+// history in its owner's list only (contact.rs:358-364, 575-582), so when a
+// rebuild moves the pair into the partner's row the partner finds no entry
+// and the spring restarts from zero, and the old owner prunes its entry.
 // The particles whose row is not empty, appended in wavefront batches (the
 // order is free: each particle's force is computed by one thread alone)
 __global__ void __launch_bounds__(256) k_list_rows(int64_t n, const uint32_t* __restrict__ nnb,
@@ -797,7 +802,7 @@ __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
                                 const double* __restrict__ hist_old,
                                 const uint64_t* __restrict__ cmask_old,
                                 double* __restrict__ hist, uint64_t* __restrict__ cmask,
-                                int have_old, int nslot) {
+                                int have_old, int nslot, int by_owner) {
     int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (i >= n) return;
     const int64_t cs = int64_t(maxnb) * npad;
@@ -812,11 +817,14 @@ __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
     if (om) {
         for (int k = 0; k < cnt; k++) {
             const int64_t s = row_addr(i, k, npad, lg);
-            const uint32_t tp = tag[entry_idx(nbr[s])];
+            const uint32_t e = nbr[s];
+            const uint32_t tp = tag[entry_idx(e)];
             for (int ko = 0; ko < ocnt; ko++) {
                 if (!((om >> ko) & 1ull)) continue;
                 const int64_t so = row_addr(oi, ko, npad_old, lg_old);
-                if (tag_old[entry_idx(nbr_old[so])] == tp) {
+                const uint32_t eo = nbr_old[so];
+                if (tag_old[entry_idx(eo)] == tp) {
+                    if (by_owner && entry_rev(eo) != entry_rev(e)) break;  // owner changed
                     for (int q = 0; q < nslot; q++) hist[q * cs + s] = hist_old[q * cso + so];
                     nm |= 1ull << k;
                     break;
@@ -1105,7 +1113,8 @@ void rebuild_attempt(Ctx& c) {
         launch(c, k_remap_history, blocks_for(n, 256), 256,
             n, c.npad, c.maxnb, c.lg, c.nbr.p, c.nnb.p, c.tag.p, c.order.p, c.npad_old,
             c.maxnb_old, c.lg_old, c.nbr_old.p, c.nnb_old.p, c.tag2.p, c.hist_old.p,
-            c.cmask_old.p, c.hist.p, c.cmask.p, c.remap_have_old ? 1 : 0, c.hist_slots);
+            c.cmask_old.p, c.hist.p, c.cmask.p, c.remap_have_old ? 1 : 0, c.hist_slots,
+            c.hist_by_owner ? 1 : 0);
         c.listed.resize(size_t(c.npad));
         c.listed_n.resize(1);
         fill_u32(c, c.listed_n.p, 0u, 1);

@@ -123,6 +123,8 @@ SIGNATURES = {
     "mdd_set_neighbor": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                    C.c_int]),
     "mdd_set_rebuild_on_box_exit": (C.c_int, [_vp, C.c_int]),
+    "mdd_set_history_mode": (C.c_int, [_vp, C.c_int]),
+    "mdd_set_virial": (C.c_int, [_vp, C.c_int]),
     "mdd_set_dt": (C.c_int, [_vp, C.c_double]),
     "mdd_set_materials": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     "mdd_set_material_extras": (C.c_int, [_vp, C.c_int, _dp]),
@@ -251,6 +253,14 @@ class System:
 
     def set_rebuild_on_box_exit(self, enabled: bool):
         self._c(self.L.mdd_set_rebuild_on_box_exit(self.h, int(enabled)))
+
+    def set_virial(self, every: int):
+        """DEM contact virial on every `every`-th step's force pass (0: off)."""
+        self._c(self.L.mdd_set_virial(self.h, int(every)))
+
+    def set_history_mode(self, mode: int):
+        """0: history kept per owning particle (the reference), 1: per pair."""
+        self._c(self.L.mdd_set_history_mode(self.h, int(mode)))
 
     def set_dt(self, dt):
         self._c(self.L.mdd_set_dt(self.h, dt))
@@ -551,7 +561,8 @@ def device_memory(device: int = 0):
     return f.value, t.value
 
 
-def from_workload(w, precision=FP64, box_exit=True, sort_every=1000, device=0) -> System:
+def from_workload(w, precision=FP64, box_exit=True, sort_every=1000, device=0,
+                  history_mode=0) -> System:
     """Configure a System the way the reference plugin groups would
     (GranularDefaultPlugins / LJDefaultPlugins) for a workloads.Workload."""
     s = System(MODEL_DEM if w.model == "dem" else MODEL_LJ, precision, device)
@@ -562,6 +573,7 @@ def from_workload(w, precision=FP64, box_exit=True, sort_every=1000, device=0) -
     if w.model == "dem":
         m = w.materials
         s.set_materials(m["youngs"], m["poisson"], m["restitution"], m["friction"])
+        s.set_history_mode(history_mode)
         if w.gravity is not None:
             s.set_gravity(w.gravity)
         for p, nrm, mat in w.walls:

@@ -343,3 +343,142 @@ def dem_settling(n: int = 4_000_000, box=(0.4, 0.4, 1.2), rmin: float = 0.8e-3,
         omega=np.zeros((n, 3)), atype=np.zeros(n, np.uint32), dt=dt_factor * t_r,
         skin_fraction=1.1, bin_size=0.0, materials=glass_materials(),
         gravity=(0.0, 0.0, -9.81), walls=walls, steps=steps, density=dens)
+
+
+def _wrap2(d, L2):
+    return d - L2 * np.round(d / L2)
+
+
+def _deposited_bed(n, Lx, Ly, rmin, rmax, stream, overlap=5e-8, gap=1.2, look=4):
+    """A resting polydisperse bed of n spheres on the plane z = 0, periodic in
+    x and y, built layer by layer with no dynamics.
+
+    Layer k sits on the sites of a triangular lattice of pitch gap * 2 r_max
+    shifted by k hollows (A, B, C stacking). Each sphere is first placed
+    touching its three supporting spheres of the layer below (the upper
+    intersection of three spheres of radius r + r_j - overlap), or, where
+    that point does not exist or lies outside their reach, on its lattice
+    site resting on the highest supporter. Any sphere then closer than
+    r_i + r_j - 2 overlap to an earlier sphere of its layer or to a sphere of
+    the `look` layers below is raised until it rests on it, which can only
+    lift it; the loop ends when nothing is raised. Layer 0 rests on the
+    floor. Every contact therefore overlaps by `overlap` (to within
+    rounding) and no pair overlaps by more than 2 * overlap."""
+    a = 2.0 * rmax * gap
+    nx = int(Lx // a)
+    ny = int(Ly // (a * math.sqrt(3.0) / 2.0)) // 2 * 2
+    if nx < 3 or ny < 4:
+        raise ValueError("box too small for a deposited bed")
+    ax, hy = Lx / nx, Ly / ny
+    ii, rr = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    base = np.stack([(ii * ax + (rr % 2) * ax / 2.0).ravel(), (rr * hy).ravel()], axis=1)
+    m = base.shape[0]
+    L2 = np.array([Lx, Ly])
+    hollow = np.array([ax / 2.0, hy / 3.0])
+    from scipy.spatial import cKDTree
+    tree = cKDTree(base, boxsize=[Lx, Ly])
+    _, sup = tree.query(np.mod(base + hollow, L2), k=3)
+    smax = math.hypot(ax / 2.0, hy / 3.0)  # distance from a hollow to its supporters
+    reach = 2.0 * rmax + 2.0 * smax + 1e-12
+    # candidates of site s among the sites of the layer q below (q = 0: the
+    # earlier sites of its own layer); the stacking makes them the same for
+    # every layer
+    cand = []
+    for q in range(look + 1):
+        lists = tree.query_ball_point(np.mod(base + q * hollow, L2), reach)
+        width = max(len(x) for x in lists)
+        C = np.full((m, width), -1, np.int64)
+        for s, x in enumerate(lists):
+            x = [t for t in x if q > 0 or t < s]
+            C[s, :len(x)] = x
+        cand.append(C)
+    layers_p, layers_r = [], []
+    placed = 0
+    layer = 0
+    while placed < n:
+        xy = np.mod(base + layer * hollow, L2)
+        r = rmin + stream.uniform(m) * (rmax - rmin)
+        P = np.column_stack([xy, r - overlap])
+        if layer:
+            pp, pr = layers_p[-1], layers_r[-1]
+            rel = pp[sup].copy()
+            rel[:, :, :2] = _wrap2(rel[:, :, :2] - xy[:, None, :], L2)
+            d = r[:, None] + pr[sup] - overlap
+            p1, p2, p3 = rel[:, 0], rel[:, 1], rel[:, 2]
+            ex = p2 - p1
+            dd = np.linalg.norm(ex, axis=1)
+            ex /= dd[:, None]
+            t = p3 - p1
+            ti = np.sum(ex * t, axis=1)
+            ey = t - ti[:, None] * ex
+            tj = np.linalg.norm(ey, axis=1)
+            ey /= tj[:, None]
+            ez = np.cross(ex, ey)
+            ez *= np.sign(ez[:, 2:3])
+            x = (d[:, 0] ** 2 - d[:, 1] ** 2 + dd ** 2) / (2.0 * dd)
+            y = (d[:, 0] ** 2 - d[:, 2] ** 2 + ti ** 2 + tj ** 2) / (2.0 * tj) - (ti / tj) * x
+            z2 = d[:, 0] ** 2 - x ** 2 - y ** 2
+            sol = p1 + x[:, None] * ex + y[:, None] * ey + np.sqrt(np.maximum(z2, 0.0))[:, None] * ez
+            tri = ((z2 > 0) & np.all(sol[:, None, 2] > rel[:, :, 2], axis=1)
+                   & (np.hypot(sol[:, 0], sol[:, 1]) <= smax))
+            rho2 = np.sum(rel[:, :, :2] ** 2, axis=2)
+            P[:, 2] = np.max(rel[:, :, 2] + np.sqrt(np.maximum(d * d - rho2, 0.0)), axis=1)
+            P[tri] = np.column_stack([np.mod(xy[tri] + sol[tri, :2], L2), sol[tri, 2]])
+        rows = np.arange(m)
+        for _ in range(1000):
+            znew = P[rows, 2].copy()
+            for q in range(min(look, layer) + 1):
+                C = cand[q][rows]
+                Q, R = (P, r) if q == 0 else (layers_p[-q], layers_r[-q])
+                valid = C >= 0
+                Cc = np.where(valid, C, 0)
+                Qc = Q[Cc]
+                dv = P[rows, None, :] - Qc
+                dv[:, :, :2] = _wrap2(dv[:, :, :2], L2)
+                dk = r[rows, None] + R[Cc] - overlap
+                rho2 = dv[:, :, 0] ** 2 + dv[:, :, 1] ** 2
+                bad = valid & (rho2 + dv[:, :, 2] ** 2 < (dk - overlap) ** 2)
+                if bad.any():
+                    zr = np.where(bad, Qc[:, :, 2] + np.sqrt(np.maximum(dk * dk - rho2, 0.0)),
+                                  -np.inf)
+                    znew = np.maximum(znew, zr.max(axis=1))
+            moved = znew > P[rows, 2]
+            if not moved.any():
+                break
+            P[rows, 2] = znew
+            raised = np.zeros(m, bool)
+            raised[rows[moved]] = True
+            c0 = cand[0]
+            rows = np.nonzero(raised | ((c0 >= 0) & raised[np.where(c0 >= 0, c0, 0)]).any(axis=1))[0]
+        else:
+            raise RuntimeError("deposited bed did not settle its conflicts")
+        layers_p.append(P)
+        layers_r.append(r)
+        placed += m
+        layer += 1
+    return np.concatenate(layers_p)[:n], np.concatenate(layers_r)[:n]
+
+
+def dem_packed_bed(n: int = 4_000_000, box=(0.4, 0.4, 1.2), rmin: float = 0.8e-3,
+                   rmax: float = 1.2e-3, vmax: float = 0.1, dt_factor: float = 0.05,
+                   seed: int | None = None, steps: int = 50000) -> Workload:
+    """The settling workload's spheres, materials, walls and gravity, started
+    from a resting bed on the floor (a deposited packing, `_deposited_bed`)
+    instead of a dilute cloud: every sphere is in contact from the first
+    step. Velocities are uniform in [-vmax, vmax] per component."""
+    seed = SEED_BASE + 5 if seed is None else seed
+    st = Stream(seed)
+    lo, hi = np.zeros(3), np.asarray(box, dtype=np.float64)
+    pos, r = _deposited_bed(n, hi[0], hi[1], rmin, rmax, st)
+    if pos[:, 2].max() + rmax >= hi[2]:
+        raise ValueError("bed higher than the box")
+    vel = (st.uniform(3 * n).reshape(n, 3) * 2.0 - 1.0) * vmax
+    dens = GLASS["density"]
+    t_r = float(rayleigh_time(GLASS["youngs"], GLASS["poisson"], dens, rmin))
+    walls = [((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 0), ((0.0, 0.0, hi[2]), (0.0, 0.0, -1.0), 0)]
+    return Workload(
+        name=f"dem_bed_{n}", model="dem", lo=lo, hi=hi, periodic=(True, True, False),
+        pos=pos, vel=vel, mass=dem_mass(dens, r), cutoff_radius=r.copy(), radius=r,
+        omega=np.zeros((n, 3)), atype=np.zeros(n, np.uint32), dt=dt_factor * t_r,
+        skin_fraction=1.1, bin_size=0.0, materials=glass_materials(),
+        gravity=(0.0, 0.0, -9.81), walls=walls, steps=steps, density=dens)

@@ -122,6 +122,10 @@ struct orc_sys {
     double (*quat)[4], (*omega)[3], (*torque)[3];
     hist_list* hist;
     int history_mode;
+    /* VirialStress (mddem_core/src/virial.rs:19-68) of the last contact
+     * force pass, accumulated when active */
+    int virial_on;
+    double virial[6];
     cmap cm;
     swap_t swaps[6];
     int nswaps;
@@ -1707,6 +1711,21 @@ static void roll_twist(orc_sys* s, int i, int j, int k, double nx, double ny, do
     }
 }
 
+/* VirialStress::add_pair (mddem_core/src/virial.rs:59-67): d = pos[j] -
+ * pos[i], f = force on i from j */
+static void virial_add_pair(double w[6], double dx, double dy, double dz, double fx, double fy,
+                            double fz) {
+    w[0] += dx * fx;
+    w[1] += dy * fy;
+    w[2] += dz * fz;
+    w[3] += dx * fy;
+    w[4] += dx * fz;
+    w[5] += dy * fz;
+}
+
+void orc_set_virial(orc_sys* s, int on) { s->virial_on = on ? 1 : 0; }
+void orc_contact_virial(const orc_sys* s, double w[6]) { memcpy(w, s->virial, sizeof(s->virial)); }
+
 int orc_hertz_mindlin_contact_force(orc_sys* s) {
     /* hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615) or,
      * with contact_model "hooke", hooke_contact_force (contact.rs:622-990);
@@ -1723,6 +1742,8 @@ int orc_hertz_mindlin_contact_force(orc_sys* s) {
     } else {
         for (uint64_t k = 0; k < s->cm.cap; k++) s->cm.slots[k].active = 0;
     }
+    /* zero_virial_stress (virial.rs:112-120) when the virial is active */
+    if (s->virial_on) memset(s->virial, 0, sizeof(s->virial));
     for (int i = 0; i < nlocal; i++) {
         for (uint32_t m = s->offsets[i]; m < s->offsets[i + 1]; m++) {
             int j = (int)s->indices[m];
@@ -1833,8 +1854,12 @@ int orc_hertz_mindlin_contact_force(orc_sys* s) {
             s->force[j][1] += fny;
             s->force[j][2] += fnz;
             /* JKR adhesion-only: normal force only, no history touched
-             * (contact.rs:320-331: the entry is not marked active) */
-            if (jkr_adhesion_only) continue;
+             * (contact.rs:320-331: the entry is not marked active); the
+             * virial takes the normal force alone (contact.rs:338-345) */
+            if (jkr_adhesion_only) {
+                if (s->virial_on) virial_add_pair(s->virial, dx, dy, dz, -fnx, -fny, -fnz);
+                continue;
+            }
             double vtx = vrx - v_n * nx, vty = vry - v_n * ny, vtz = vrz - v_n * nz;
             uint32_t tag_i = s->tag[i], tag_j = s->tag[j];
             double sign = tag_i < tag_j ? 1.0 : -1.0;
@@ -1903,6 +1928,10 @@ int orc_hertz_mindlin_contact_force(orc_sys* s) {
             double rd[3] = {sign * stored[3], sign * stored[4], sign * stored[5]};
             double tw = sign * stored[6];
             roll_twist(s, i, j, kij, nx, ny, nz, f_n_mag, r_eff, rd, &tw);
+            /* virial: force on i from j = -F_n + F_t, newton (contact.rs:
+             * 559-569; Hooke 962-970) */
+            if (s->virial_on)
+                virial_add_pair(s->virial, dx, dy, dz, -fnx + ftx, -fny + fty, -fnz + ftz);
             double ns[HSLOTS] = {sign * sx,    sign * sy,    sign * sz, sign * rd[0],
                                  sign * rd[1], sign * rd[2], sign * tw};
             if (s->history_mode == ORC_HIST_REFERENCE) {
@@ -2314,6 +2343,36 @@ double orc_compute_ke(const orc_sys* s) {
         ke += s->mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     }
     return 0.5 * ke;
+}
+
+void orc_granular_temperature(const orc_sys* s, double out[6]) {
+    /* print_granular_temperature (dem_granular/src/granular_temp.rs:60-99):
+     * out = T_g, KE, |p|, p_x, p_y, p_z */
+    double mvx = 0.0, mvy = 0.0, mvz = 0.0, msum = 0.0;
+    for (int i = 0; i < s->nlocal; i++) {
+        mvx += s->mass[i] * s->vel[i][0];
+        mvy += s->mass[i] * s->vel[i][1];
+        mvz += s->mass[i] * s->vel[i][2];
+        msum += s->mass[i];
+    }
+    const double ax = mvx / msum, ay = mvy / msum, az = mvz / msum;
+    double vd = 0.0;
+    for (int i = 0; i < s->nlocal; i++) {
+        const double dx = s->vel[i][0] - ax, dy = s->vel[i][1] - ay, dz = s->vel[i][2] - az;
+        /* powi(2) is one multiplication */
+        vd += s->mass[i] * (dx * dx) + s->mass[i] * (dy * dy) + s->mass[i] * (dz * dz);
+    }
+    double ke = 0.0;
+    for (int i = 0; i < s->nlocal; i++) {
+        const double* v = s->vel[i];
+        ke += 0.5 * s->mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    out[0] = vd / (3.0 * msum);
+    out[1] = ke;
+    out[2] = sqrt(mvx * mvx + mvy * mvy + mvz * mvz);
+    out[3] = mvx;
+    out[4] = mvy;
+    out[5] = mvz;
 }
 
 void orc_setup_nose_hoover(orc_sys* s) {

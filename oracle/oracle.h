@@ -146,6 +146,11 @@ void orc_wall_contact_force(orc_sys* s);              /* dem_wall lib.rs:848-124
 void orc_apply_gravity(orc_sys* s);                   /* mddem_fixes lib.rs:756-761 */
 void orc_lj_force(orc_sys* s);                        /* md_lj lib.rs:488-633 */
 void orc_lj_virial(const orc_sys* s, double w[6]);   /* md_lj lib.rs:510-585 */
+/* VirialStress of the contact force (dem_granular/src/contact.rs:338-345,
+ * 559-569, 962-970; mddem_core/src/virial.rs:19-120): xx yy zz xy xz yz of
+ * the last contact pass, accumulated while on */
+void orc_set_virial(orc_sys* s, int on);
+void orc_contact_virial(const orc_sys* s, double w[6]);
 void orc_lj_tails(const orc_sys* s, double* e_tail, double* p_tail); /* md_lj 397-443 */
 double orc_pressure(const orc_sys* s, const double w[6], double p_tail); /* md_measure 605-651 */
 void orc_reverse_send_force(orc_sys* s);              /* comm.rs:579-656 */
@@ -157,6 +162,9 @@ void orc_setup_nose_hoover(orc_sys* s);               /* md_thermostat lib.rs:19
 void orc_nh_pre_initial(orc_sys* s);                  /* md_thermostat lib.rs:254-301 */
 void orc_nh_post_final(orc_sys* s);                   /* md_thermostat lib.rs:313-353 */
 double orc_compute_ke(const orc_sys* s);              /* atom.rs:489-502 */
+/* print_granular_temperature (dem_granular/src/granular_temp.rs:60-99):
+ * T_g = sum m |v - vbar|^2 / (3 M), KE, |p|, p_x, p_y, p_z */
+void orc_granular_temperature(const orc_sys* s, double out[6]);
 /* one full scheduled step in the reference phase order (schedule.dot); 0 ok */
 int orc_step(orc_sys* s);
 int orc_run(orc_sys* s, int64_t nsteps);

@@ -96,6 +96,9 @@ def lib():
             "orc_apply_gravity": (None, [vp]),
             "orc_lj_force": (None, [vp]),
             "orc_lj_virial": (None, [vp, _dp]),
+            "orc_set_virial": (None, [vp, C.c_int]),
+            "orc_granular_temperature": (None, [vp, _dp]),
+            "orc_contact_virial": (None, [vp, _dp]),
             "orc_lj_tails": (None, [vp, _dp, _dp]),
             "orc_pressure": (C.c_double, [vp, _dp, C.c_double]),
             "orc_reverse_send_force": (None, [vp]),
@@ -210,6 +213,22 @@ class Oracle:
 
     def set_lj(self, epsilon=1.0, sigma=1.0, cutoff=2.5):
         self.L.orc_set_lj(self.h, epsilon, sigma, cutoff)
+
+    def granular_temperature(self):
+        """print_granular_temperature's values: T_g, KE, |p| and p."""
+        w = np.zeros(6)
+        self.L.orc_granular_temperature(self.h, _ptr(w))
+        return dict(granular_temperature=w[0], ke=w[1], momentum_mag=w[2],
+                    momentum=w[3:6].copy())
+
+    def set_virial(self, on=True):
+        self.L.orc_set_virial(self.h, int(bool(on)))
+
+    def contact_virial(self):
+        """xx yy zz xy xz yz of the last contact force pass."""
+        w = np.zeros(6)
+        self.L.orc_contact_virial(self.h, _ptr(w))
+        return w
 
     def lj_virial(self):
         """W (xx, yy, zz, xy, xz, yz) over the current half list."""

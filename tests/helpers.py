@@ -7,7 +7,7 @@ import os
 
 import numpy as np
 
-from oracle.oracle import (HIST_CANONICAL, MODEL_DEM, MODEL_LJ, Oracle)
+from oracle.oracle import (HIST_REFERENCE, MODEL_DEM, MODEL_LJ, Oracle)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -17,7 +17,7 @@ def kats():
         return json.load(fh)
 
 
-def oracle_from_workload(w, history_mode=HIST_CANONICAL, sort_every=1000):
+def oracle_from_workload(w, history_mode=HIST_REFERENCE, sort_every=1000):
     o = Oracle(MODEL_DEM if w.model == "dem" else MODEL_LJ)
     o.set_box(w.lo, w.hi, w.periodic)
     o.set_neighbor(w.skin_fraction, w.bin_size, 0, True, sort_every, True)
@@ -90,3 +90,14 @@ def rel_err(a, b):
     b = np.asarray(b, np.float64)
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))
+
+
+def canon_history(a, b, spring):
+    """History entries as (lower tag, higher tag, value) sorted by the tag
+    pair: the checker's per-owner store lists (owner, partner), the HIP path
+    (lower, higher); the values are in the lower tag's frame in both."""
+    a = np.asarray(a, np.int64)
+    b = np.asarray(b, np.int64)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    k = np.lexsort((hi, lo))
+    return lo[k], hi[k], np.asarray(spring)[k]

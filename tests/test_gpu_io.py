@@ -89,3 +89,41 @@ def test_lj_restart_continuation(tmp_path):
     sa, sb = _by_tag(a), _by_tag(b)
     assert _min_image_err(sa["pos"], sb["pos"], w.hi - w.lo) < 1e-9
     assert rel_err(sb["vel"], sa["vel"]) < 1e-9
+
+
+def test_restart_file_runs_the_same_on_oracle_and_gpu(tmp_path):
+    # one RestartData file (bincode) read into the checker and into the HIP
+    # path (read_restart, mddem_print/src/lib.rs:1222-1246), then the same
+    # 300 steps on both: the file is the only state they share
+    from tests.helpers import oracle_from_workload
+    w = W.dem_gas(n=500)
+    a = hip.from_workload(w, hip.FP64, box_exit=True)
+    a.run(120)
+    for _ in range(40):
+        if a.thermo()["ncontacts"] == 0:
+            break
+        a.run(5)
+    step = a.thermo()["step"]
+    path = str(tmp_path / io.restart_file_name(step, 0, "bincode"))
+    io.write_restart(path, io.restart_from_system(a, step, w.dt, density=2500.0), "bincode")
+    data = io.read_restart(path)
+    b = hip.from_workload(w, hip.FP64, box_exit=True)
+    io.upload_restart(b, data)
+    dem = io.unpack_dem(data["atom_data_buffers"][0])
+    pos = np.stack([data["pos_x"], data["pos_y"], data["pos_z"]], axis=1)
+    vel = np.stack([data["vel_x"], data["vel_y"], data["vel_z"]], axis=1)
+    force = np.stack([data["force_x"], data["force_y"], data["force_z"]], axis=1)
+    o = oracle_from_workload(w)
+    o.set_atoms(pos, vel, np.asarray(data["mass"]), np.asarray(data["cutoff_radius"]),
+                radius=dem["radius"], omega=dem["omega"], tag=np.asarray(data["tag"]),
+                atype=np.asarray(data["atom_type"]))
+    o.set_forces(force, dem["torque"])
+    b.run(300)
+    o.run(300)
+    sb = _by_tag(b)
+    ref = o.atoms()
+    k = np.argsort(ref["tag"])
+    assert _min_image_err(ref["pos"][k], sb["pos"], w.hi - w.lo) < 1e-12
+    assert rel_err(sb["vel"], ref["vel"][k]) < 1e-9
+    assert rel_err(sb["omega"], ref["omega"][k]) < 1e-6
+    assert b.thermo()["rebuilds"] == o.rebuilds

@@ -13,7 +13,7 @@ import pytest
 from mddem_amd import hip
 from mddem_amd import workloads as W
 from oracle.oracle import HIST_CANONICAL, MODEL_DEM, MODEL_LJ, Oracle, dem_mass
-from tests.helpers import canonical_pairs, kats, oracle_from_workload, rel_err
+from tests.helpers import canon_history, canonical_pairs, kats, oracle_from_workload, rel_err
 
 pytestmark = pytest.mark.gpu
 K = kats()
@@ -273,6 +273,47 @@ def test_lj_nvt_trajectory_matches_oracle():
     th = s.thermo()
     assert abs(th["p_xi"] - o.p_xi) < 1e-9 * max(1.0, abs(o.p_xi))
     assert th["rebuilds"] == o.rebuilds
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_history_across_ownership_flips_matches_oracle(mode):
+    # a dense sheared gas run through rebuilds: spatial re-sorting moves pairs
+    # between the two spheres' half-list rows. Mode 0 is the reference's
+    # per-owner store (the spring restarts when the owner changes,
+    # contact.rs:358-364); mode 1 keys the spring by the pair.
+    from oracle.oracle import HIST_CANONICAL, HIST_REFERENCE
+    w = _dense_dem(n=1500, seed=21)
+    o = oracle_from_workload(w, HIST_REFERENCE if mode == 0 else HIST_CANONICAL)
+    s = hip.from_workload(w, hip.FP64, box_exit=True, history_mode=mode)
+    o.run(600)
+    s.run(600)
+    assert s.thermo()["rebuilds"] == o.rebuilds and o.rebuilds >= 3
+    a, b, spring = canon_history(*s.history())
+    oa, ob, ospring = canon_history(*o.history())
+    assert len(a) > 100
+    assert np.array_equal(a, oa) and np.array_equal(b, ob)
+    assert rel_err(spring, ospring) < 1e-9
+    st, ref = s.state(), o.atoms()
+    assert rel_err(by_tag(st["tag"], st["vel"]), by_tag(ref["tag"], ref["vel"])) < 1e-9
+
+
+def test_history_modes_differ_only_through_ownership_flips():
+    # the same run in both modes: the pair set is the same, and some springs
+    # differ (pairs whose owner changed restarted in mode 0)
+    w = _dense_dem(n=1500, seed=21)
+    out = []
+    for mode in (0, 1):
+        s = hip.from_workload(w, hip.FP64, box_exit=True, history_mode=mode)
+        s.run(600)
+        a, b, spring = s.history()
+        k = np.lexsort((b, a))
+        out.append((a[k], b[k], spring[k]))
+    same = {(x, y) for x, y in zip(out[0][0], out[0][1])} & {(x, y) for x, y in zip(out[1][0], out[1][1])}
+    assert len(same) > 100
+    d0 = {(x, y): v for x, y, v in zip(*out[0])}
+    d1 = {(x, y): v for x, y, v in zip(*out[1])}
+    differ = sum(1 for k in same if not np.array_equal(d0[k], d1[k]))
+    assert differ > 0
 
 
 def test_history_remap_is_bit_exact():

@@ -1,0 +1,157 @@
+"""The DEM settling physics (polydisperse Hertz-Mindlin with shear history,
+two plane walls, gravity, periodic x/y) on the HIP path against the checker,
+from a resting bed in which every sphere touches its neighbors: one force
+pass, then a run through several rebuilds; and the full 4M-sphere bed's
+size-independent properties. References: contact.rs:147-425,
+dem_wall/src/lib.rs:862-1002, mddem_fixes/src/lib.rs:756-761."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from mddem_amd import hip
+from mddem_amd import workloads as W
+from tests.helpers import canon_history, oracle_from_workload, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu case] {request.node.name}", flush=True)
+    sys.stdout.flush()
+    yield
+
+
+def bed(n=2500, vmax=0.5, spin=50.0, seed=None):
+    w = W.dem_packed_bed(n=n, box=(0.02, 0.02, 1.2), vmax=vmax, seed=seed)
+    w.omega = (W.Stream(77).uniform(3 * n).reshape(n, 3) * 2.0 - 1.0) * spin
+    return w
+
+
+def by_tag(tags, arr):
+    return np.asarray(arr)[np.argsort(tags)]
+
+
+def bed_stats(w):
+    from scipy.spatial import cKDTree
+    L = w.hi[:2]
+    p = w.pos
+    tr = cKDTree(np.column_stack([p[:, :2], p[:, 2] + 1.0]), boxsize=[L[0], L[1], 10.0])
+    pr = tr.query_pairs(2.0 * w.radius.max() * w.skin_fraction, output_type="ndarray")
+    d = p[pr[:, 0]] - p[pr[:, 1]]
+    d[:, :2] -= L * np.round(d[:, :2] / L)
+    dist = np.linalg.norm(d, axis=1)
+    s = w.radius[pr[:, 0]] + w.radius[pr[:, 1]]
+    return (np.sum(dist < w.skin_fraction * s), np.sum(dist < s), float((s - dist).max()))
+
+
+def test_bed_is_a_resting_contact_network():
+    # CPU-side facts the GPU cases rely on: most spheres touch, no overlap is
+    # deeper than twice the contact overlap, the bottom layer rests on z = 0
+    w = bed()
+    pairs, contacts, deepest = bed_stats(w)
+    assert contacts > 1.2 * w.n and pairs > contacts
+    assert 0 < deepest <= 1.0000001e-7
+    assert np.sum(w.pos[:, 2] < w.radius) > 30
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+def test_force_pass_matches_oracle(prec):
+    w = bed()
+    o = oracle_from_workload(w)
+    o.set_virial(True)
+    o.neighbor_setup()
+    o.pbc()
+    o.sort_atoms_by_bin()
+    o.borders()
+    o.bin_neighbor_list()
+    o.zero_all_forces()
+    o.hertz_mindlin_contact_force()
+    o.reverse_send_force()
+    o.wall_zero_force_accumulators()
+    o.wall_contact_force()
+    o.apply_gravity()
+    s = hip.from_workload(w, prec)
+    s.set_virial(1)
+    s.bin_neighbor_list()
+    s.hertz_mindlin_contact_force()
+    s.wall_zero_force_accumulators()
+    s.wall_contact_force()
+    s.apply_gravity()
+    ref = o.atoms()
+    st = s.state()
+    f, t = s.forces()
+    tol = 1e-12 if prec == hip.FP64 else 1e-5
+    assert s.thermo()["ncontacts"] > 1.2 * w.n
+    assert rel_err(by_tag(st["tag"], f), by_tag(ref["tag"], ref["force"])) < tol
+    assert rel_err(by_tag(st["tag"], t), by_tag(ref["tag"], ref["torque"])) < tol
+    # contact virial (contact.rs:559-569) over the same pairs
+    w_ref = o.contact_virial()
+    w_got = np.asarray(s.thermo()["virial"])
+    assert np.linalg.norm(w_got - w_ref) <= tol * np.linalg.norm(w_ref)
+    assert w_ref[:3].sum() < 0  # repulsive contacts
+    wf = s.walls()["force"]
+    owf = o.walls()["force"]
+    assert owf[0] > 0 and abs(wf[0] - owf[0]) <= 1e-9 * owf[0]
+    assert wf[1] == 0.0 and owf[1] == 0.0
+
+
+@pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])
+def test_run_through_rebuilds_matches_oracle(prec):
+    # 1.0 m/s shaking: the skin budget (0.1 r_min) is spent every few hundred
+    # steps, so 1500 steps cross several rebuilds with the shear history
+    # carried (both keep it per owning sphere, as the reference does)
+    w = bed(vmax=1.0)
+    o = oracle_from_workload(w)
+    s = hip.from_workload(w, prec, box_exit=True)
+    o.run(1500)
+    s.run(1500)
+    th = s.thermo()
+    assert th["rebuilds"] == o.rebuilds and o.rebuilds >= 3
+    assert th["ncontacts"] > w.n
+    ref = o.atoms()
+    st = s.state()
+    L = w.hi - w.lo
+    dp = by_tag(st["tag"], st["pos"]) - by_tag(ref["tag"], ref["pos"])
+    dp[:, :2] -= L[:2] * np.round(dp[:, :2] / L[:2])
+    if prec == hip.FP64:
+        assert np.abs(dp).max() < 1e-12
+        assert rel_err(by_tag(st["tag"], st["vel"]), by_tag(ref["tag"], ref["vel"])) < 1e-9
+        assert rel_err(by_tag(st["tag"], st["omega"]), by_tag(ref["tag"], ref["omega"])) < 1e-8
+    else:
+        assert np.abs(dp).max() < 1e-9
+        assert rel_err(by_tag(st["tag"], st["vel"]), by_tag(ref["tag"], ref["vel"])) < 1e-4
+        assert rel_err(by_tag(st["tag"], st["omega"]), by_tag(ref["tag"], ref["omega"])) < 1e-3
+    a, b, spring = canon_history(*s.history())
+    oa, ob, ospring = canon_history(*o.history())
+    assert np.array_equal(a, oa) and np.array_equal(b, ob)
+    tol = 1e-8 if prec == hip.FP64 else 1e-3
+    assert rel_err(spring, ospring) < tol
+
+
+@pytest.mark.timeout(600)
+def test_full_size_4m_bed_properties():
+    # the settling config's 4,000,000 spheres as a resting bed: the list
+    # holds pairs and contacts from the first step, x/y momentum is exchanged
+    # only between spheres (walls push along z, gravity pulls along z) and
+    # every value stays finite through a rebuild
+    w = W.dem_packed_bed()
+    print(f"[gpu case] bed built: n={w.n}", flush=True)
+    p0 = np.sum(w.mass[:, None] * w.vel, axis=0)
+    s = hip.from_workload(w, hip.MIXED, box_exit=False)
+    s.run(1)
+    th = s.thermo()
+    assert th["npairs"] > 2 * w.n and th["ncontacts"] > 1.5 * w.n
+    s.run(6000)
+    th = s.thermo()
+    assert th["rebuilds"] >= 1
+    assert th["ncontacts"] > 1.2 * w.n
+    p1 = np.asarray(th["momentum"])
+    scale = np.sum(w.mass) * 0.1
+    assert np.all(np.abs(p1[:2] - p0[:2]) < 1e-6 * scale)
+    st = s.state()
+    assert np.all(np.isfinite(st["pos"])) and np.all(np.isfinite(st["vel"]))
+    assert np.all(np.isfinite(st["omega"]))
+    assert st["pos"][:, 2].min() > 0.0 and st["pos"][:, 2].max() < 0.35

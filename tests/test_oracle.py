@@ -583,3 +583,26 @@ def test_granular_gas_fixture_is_reproduced():
     go, gp, gc = o.pairs()
     assert np.array_equal(canonical_pairs(go, gp, gc), ref["pairs"])
     assert len(ref["pairs"]) > 50 and len(ref["hist_lo"]) > 0
+
+
+def test_granular_temperature_hand_computed():
+    # print_granular_temperature (granular_temp.rs:60-99) for three particles
+    # of mass 1, 2, 3 moving along +x, +y, -z at unit speed: M = 6,
+    # p = (1, 2, -3), vbar = p / 6, sum m |v - vbar|^2 = sum m v^2 - M vbar^2
+    # = 6 - 14 / 6 = 11 / 3, so T_g = 11 / 54; KE = 3; |p| = sqrt(14)
+    o = Oracle(MODEL_DEM)
+    o.set_box((-1, -1, -1), (1, 1, 1), (False, False, False))
+    r = np.array([1e-3, 1.2e-3, 1.4e-3])
+    o.set_atoms([[0, 0, 0], [0.1, 0, 0], [0.2, 0, 0]],
+                [[1, 0, 0], [0, 1, 0], [0, 0, -1]], [1.0, 2.0, 3.0], r, radius=r)
+    g = o.granular_temperature()
+    assert g["granular_temperature"] == pytest.approx(11.0 / 54.0, rel=1e-15)
+    assert g["ke"] == pytest.approx(3.0, rel=1e-15)
+    assert g["momentum_mag"] == pytest.approx(math.sqrt(14.0), rel=1e-15)
+    assert list(g["momentum"]) == [1.0, 2.0, -3.0]
+    assert o.compute_ke() == pytest.approx(3.0, rel=1e-15)
+    # a uniform drift adds nothing to T_g
+    o.set_atoms([[0, 0, 0], [0.1, 0, 0], [0.2, 0, 0]],
+                [[6, 5, 5], [5, 6, 5], [5, 5, 4]], [1.0, 2.0, 3.0], r, radius=r)
+    assert o.granular_temperature()["granular_temperature"] == pytest.approx(11.0 / 54.0,
+                                                                            rel=1e-13)

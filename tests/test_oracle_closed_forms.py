@@ -253,3 +253,28 @@ def test_closed_forms_reject_a_checker_with_a_wrong_damping_constant(tmp_path):
         assert f"FAILED tests/test_oracle_closed_forms.py::{case}" in out, (case, out[-3000:])
     # the cases with no damping term still pass on the mutant
     assert "test_mass_and_material_closed_forms_agree_with_checker" not in out.split("short test summary")[-1]
+
+
+@pytest.mark.parametrize("owner", [0, 1])
+def test_contact_virial_of_one_pair(owner):
+    # VirialStress::add_pair(d, F_i) with d = x_j - x_i and F_i = -F_n + F_t,
+    # the force on the owner i (contact.rs:559-569, virial.rs:59-67): for one
+    # pair the tensor is d (x) F_i, the same whichever sphere owns the pair
+    n = np.array([0.6, 0.8, 0.0])
+    x1 = 1.999e-3 * n
+    args = ([0, 0, 0], x1.tolist(), 0.8e-3, 1.2e-3, [0.05, 0.02, -0.01], [-0.8, -0.5, 0.2],
+            [0, 0, 30.0], [10.0, -20.0, 5.0])
+    want = first_contact(*args, mat0=GLASS, mat1=STEEL)
+    w = np.outer(x1, want["f0"])  # d from sphere 0 to 1, force on sphere 0
+    want6 = np.array([w[0, 0], w[1, 1], w[2, 2], w[0, 1], w[0, 2], w[1, 2]])
+    o = oracle_pair(*args, mats=(GLASS, STEEL), types=(0, 1), owner=owner)
+    o.set_virial(True)
+    o.hertz_mindlin_contact_force()
+    got = o.contact_virial()
+    assert np.linalg.norm(got - want6) <= 1e-11 * np.linalg.norm(want6)
+    # the trace is -r F_n: repulsion gives a negative trace
+    assert got[:3].sum() == pytest.approx(-want["fn"] * np.linalg.norm(x1), rel=1e-11)
+    o.set_virial(False)
+    o.zero_all_forces()
+    o.hertz_mindlin_contact_force()
+    assert np.array_equal(o.contact_virial(), got)  # inactive: left as it was
