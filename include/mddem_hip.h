@@ -74,9 +74,12 @@ int mdd_version(void);
 /* Domain box and periodicity: [domain] section, Domain::update_derived
  * (crates/mddem_core/src/domain.rs:105-160) */
 int mdd_set_box(mdd_ctx* ctx, const double lo[3], const double hi[3], const int periodic[3]);
-/* [neighbor] section, NeighborConfig (crates/mddem_core/src/neighbor.rs:45-82);
- * every must be 0 (displacement rebuilds), newton selects nothing here (the
- * pair set is the newton half list; forces are gathered) */
+/* [neighbor] section, NeighborConfig (crates/mddem_core/src/neighbor.rs:45-82)
+ * with needs_rebuild's policies (neighbor.rs:605-627): every = 0 rebuilds on
+ * the displacement check; every > 0 also every `every` steps (check = 1) or
+ * only then (check = 0); sort_every forces a sorted rebuild every that many
+ * steps (sort_atoms_by_bin, neighbor.rs:686-794); newton selects nothing here
+ * (the pair set is the newton half list; forces are gathered) */
 int mdd_set_neighbor(mdd_ctx* ctx, double skin_fraction, double bin_size, int every,
                      int check, int sort_every, int newton);
 /* 1 (default): rebuild when any particle leaves the periodic box, as

@@ -161,8 +161,6 @@ void run_steps(Ctx& c, int64_t nsteps) {
         throw ApiError(MDD_ERR_UNSUPPORTED, "walls in a batched ensemble");
     if (c.nsys > 0 && c.ext_on)
         throw ApiError(MDD_ERR_UNSUPPORTED, "optional contact models in a batched ensemble");
-    if (c.every != 0)
-        throw ApiError(MDD_ERR_UNSUPPORTED, "only displacement rebuilds (every = 0) are supported");
     if (c.step + nsteps > INT_MAX - 2) throw ApiError(MDD_ERR_INVALID, "step count too large");
     if (!c.setup_done) mdd::neighbor_setup(c);
     if (c.nh_on && !c.nh_ready) setup_nose_hoover(c);
@@ -231,12 +229,18 @@ void run_steps(Ctx& c, int64_t nsteps) {
         if (c.status_host->overflow & 4u) mdd::rebuild_grow(c, *c.status_host);  // throws
         return c.status_host->abort_step;
     };
+    // needs_rebuild with every > 0 (neighbor.rs:605-627): steps_since_build
+    // is 0 after the build of step b and grows by one at each later step's
+    // check that keeps the list, so the count forces a rebuild at b + every + 1
+    auto count_due = [&](int64_t s) {
+        return c.every > 0 && s >= c.last_rebuild + int64_t(c.every) + 1;
+    };
     for (;;) {
         while (c.step < end) {
             const int64_t s = c.step + 1;
             const bool forced = !c.list_valid || c.list_injected || c.list_lg != c.lg ||
                                 c.list_tile != c.want_tile() ||
-                                (c.sort_every > 0 && s % c.sort_every == 0);
+                                (c.sort_every > 0 && s % c.sort_every == 0) || count_due(s);
             if (forced) {
                 // FullRebuild step: integrate, rebuild, force, integrate
                 first_half(int(s), false);
@@ -252,6 +256,7 @@ void run_steps(Ctx& c, int64_t nsteps) {
                 const int64_t next_sort = (c.step / c.sort_every + 1) * c.sort_every;
                 K = std::min<int64_t>(K, next_sort - 1 - c.step);
             }
+            if (c.every > 0) K = std::min<int64_t>(K, c.last_rebuild + c.every - c.step);
             if (K < 1) K = 1;
             for (int64_t k = 1; k <= K; k++) {
                 const int st = int(c.step + k);
@@ -562,8 +567,7 @@ int mdd_set_neighbor(mdd_ctx* ctx, double skin_fraction, double bin_size, int ev
                      int sort_every, int newton) {
     return guarded(ctx, [&](Ctx& c) {
         if (!(skin_fraction >= 1.0)) throw ApiError(MDD_ERR_INVALID, "skin_fraction must be >= 1");
-        if (every != 0)
-            throw ApiError(MDD_ERR_UNSUPPORTED, "only displacement rebuilds (every = 0)");
+        if (every < 0) throw ApiError(MDD_ERR_INVALID, "every must be >= 0");
         c.skin_fraction = skin_fraction;
         c.bin_min_size = bin_size;
         c.every = every;
@@ -1148,6 +1152,7 @@ int mdd_bin_neighbor_list(mdd_ctx* ctx) {
     return guarded(ctx, [&](Ctx& c) {
         need_state(c);
         mdd::rebuild(c);
+        c.last_rebuild = c.step;
     });
 }
 

@@ -445,7 +445,9 @@ IntArgs make_args(Ctx& c, int step, bool check) {
     a.vv = 0;
     a.check = check && c.list_valid && !c.list_injected ? 1 : 0;
     const double skin = (c.skin_fraction - 1.0) * c.min_skin;
-    a.thr2 = skin * skin;
+    // every > 0 without check: the step count alone decides (the periodic
+    // box exit still does, decide_rebuild, neighbor.rs:651-666)
+    a.thr2 = (c.every > 0 && !c.check) ? INFINITY : skin * skin;
     a.box = c.box;
     a.box_exit = c.box_exit_rebuild;
     a.st = c.status.p;

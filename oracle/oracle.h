@@ -22,7 +22,8 @@
  *  - the contact force restates both reference kernels: Hertz-Mindlin
  *    (contact.rs:103-615) and Hooke (contact.rs:622-990), chosen by the
  *    global contact model as the plugin does (contact.rs:65-90).
- *  - only `every = 0` (displacement-driven rebuilds) is supported.
+ *  - the rebuild policies of needs_rebuild (every, check) and the periodic
+ *    sort of sort_atoms_by_bin follow neighbor.rs:605-627, 686-794.
  */
 #ifndef MDDEM_ORACLE_H
 #define MDDEM_ORACLE_H

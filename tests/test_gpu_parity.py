@@ -493,3 +493,30 @@ def test_granular_gas_fixture(prec):
     k = np.lexsort((hi, lo))
     assert np.array_equal(lo[k], ref["hist_lo"]) and np.array_equal(hi[k], ref["hist_hi"])
     assert rel_err(spring[k], ref["hist_spring"]) < (1e-12 if prec == hip.FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("every,check,sort_every", [(7, True, 1000), (7, False, 1000),
+                                                    (0, True, 25)])
+def test_rebuild_policies_match_oracle(every, check, sort_every):
+    # needs_rebuild (neighbor.rs:605-627): a count of steps since the last
+    # build (alone or with the displacement check), and sort_atoms_by_bin's
+    # periodic sorted rebuild (neighbor.rs:686-794); the same rebuild steps
+    # give the same trajectory
+    w = _dense_dem(n=1200, seed=17)
+    o = oracle_from_workload(w, sort_every=sort_every)
+    o.set_neighbor(w.skin_fraction, w.bin_size, every, check, sort_every, True)
+    s = hip.from_workload(w, hip.FP64, box_exit=True, sort_every=sort_every)
+    s.set_neighbor(w.skin_fraction, w.bin_size, every, check, sort_every, True)
+    o.run(200)
+    s.run(200)
+    assert s.thermo()["rebuilds"] == o.rebuilds
+    if every:
+        assert o.rebuilds >= 200 // (every + 1)
+    else:
+        assert o.rebuilds >= 200 // sort_every
+    st, ref = s.state(), o.atoms()
+    assert rel_err(by_tag(st["tag"], st["vel"]), by_tag(ref["tag"], ref["vel"])) < 1e-9
+    a, b, spring = canon_history(*s.history())
+    oa, ob, ospring = canon_history(*o.history())
+    assert np.array_equal(a, oa) and np.array_equal(b, ob)
+    assert rel_err(spring, ospring) < 1e-9
