@@ -197,9 +197,10 @@ void run_steps(Ctx& c, int64_t nsteps) {
     // resume at the force pass of step ab after the device stopped there:
     // a rebuild request from the drift, or a rebuild that did not fit
     auto resume = [&](int ab) {
-        const bool retry = (c.status_host->overflow & 11u) != 0u;
+        const bool retry = (c.status_host->overflow & 15u) != 0u;
         // the rebuild's first kernel clears the abort
         if (retry) {
+            if (c.status_host->overflow & 4u) mdd::remove_outside(c);
             mdd::rebuild_grow(c, *c.status_host);
             if (++c.retries > 6) throw ApiError(MDD_ERR_CAPACITY, "neighbor rebuild did not fit");
         } else {
@@ -226,7 +227,6 @@ void run_steps(Ctx& c, int64_t nsteps) {
         sync_status(c);
         harvest_profile(c);
         check_device_error(c);
-        if (c.status_host->overflow & 4u) mdd::rebuild_grow(c, *c.status_host);  // throws
         return c.status_host->abort_step;
     };
     // needs_rebuild with every > 0 (neighbor.rs:605-627): steps_since_build

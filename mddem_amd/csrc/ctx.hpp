@@ -464,6 +464,7 @@ struct Ctx {
     DBuf<double> partial;           // per-block kinetic-energy partial sums
     DBuf<double> partial2;          // per-block thermo partial sums
     int64_t step = 0, rebuilds = 0;
+    int64_t removed = 0;  // particles removed at non-periodic faces (pbc)
     int64_t sort_counter = 0;
     int chunk = 8;
     // chunk sizing: step of the last rebuild and the smoothed number of steps
@@ -512,6 +513,8 @@ void fill_f32(Ctx& c, float* p, float v, int64_t n);
 // neighbor.hip
 void rebuild(Ctx& c);
 void rebuild_async(Ctx& c, int step, bool retry);
+// drop the particles a rebuild found outside a non-periodic face (pbc)
+void remove_outside(Ctx& c);
 void release_graphs(Ctx& c);
 bool rebuild_grow(Ctx& c, const Status& s);
 // forces.hip

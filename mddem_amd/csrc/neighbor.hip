@@ -880,43 +880,21 @@ __global__ void __launch_bounds__(256) k_rows_gate(int nblk, const uint32_t* __r
     }
 }
 
-// pbc, spatial sort of the particles, and the previous list kept as the
-// source of the shear-history remap
-void rebuild_prepare(Ctx& c) {
-    if (!c.setup_done) neighbor_setup(c);
-    const int64_t n = c.n;
-    const int64_t ncells = c.grid.ncells;
+
+void swap_alternates(Ctx& c);
+
+// Gather every per-particle array through `order` (n entries) into the
+// alternates, record the build positions, then swap: the arrays are in the
+// new order and the alternates keep the previous one (tag2: the previous
+// tags, the source of the shear-history remap).
+void permute_all(Ctx& c, const uint32_t* order, int64_t n) {
     const bool dem = c.model == 1;
     const bool mixed = c.precision == 1;
-    c.key.resize(n);
-    c.key2.resize(n);
-    c.order.resize(n);
-    c.cnt.resize(ncells);
-    c.start.resize(ncells + 1);
-    c.bin_start.resize(ncells + 1);
-    c.gstart.resize(ncells + 1);
-    c.gcnt.resize(n);
-    c.goff.resize(n + 1);
-    c.placed.resize(std::max<int64_t>(n, c.ghost_cap) + 1);
-    // scan scratch sized once for every scan of this rebuild, so no buffer is
-    // reallocated while kernels that use it are queued
-    c.scan_tmp.resize(size_t((std::max<int64_t>(ncells, n) + 2047) / 2048 + 2));
-
-    launch(c, k_rebuild_begin, blocks_for(ncells, 256) < 512 ? blocks_for(ncells, 256) : 512, 256,
-           c.status.p, c.cnt.p, ncells);
-    // pbc + keys
-    launch(c, k_wrap_key, blocks_for(n, 256), 256, n, c.pos.p, c.image.p, c.box, c.grid,
-                                                          c.key.p, c.status.p, c.sysp_or_null(),
-                                                          c.psys.p, c.cnt.p);
-    // sort_atoms_by_bin: stable counting sort of particles by bin
-    counting_sort(c, c.key.p, n, nullptr, ncells, c.cnt.p, c.start.p, c.placed.p, c.order.p,
-                  nullptr, true, nullptr, nullptr, true);
-    // permute every particle array into the alternates, then swap
     PermArgs a;
     std::memset(&a, 0, sizeof(a));  // padding too: the bytes are a launch signature
     a.n = n;
     a.npad = c.npad;
-    a.order = c.order.p;
+    a.order = order;
     a.pos = c.pos.p;
     a.vel = c.vel.p;
     a.omega = c.omega.p;
@@ -952,6 +930,13 @@ void rebuild_prepare(Ctx& c) {
     a.dem = dem;
     launch(c, k_permute, blocks_for(n, 256), 256, a);
     MDD_HIP(hipGetLastError());
+    swap_alternates(c);
+}
+
+// exchange every per-particle array with its alternate
+void swap_alternates(Ctx& c) {
+    const bool dem = c.model == 1;
+    const bool mixed = c.precision == 1;
     c.pos.swap(c.pos2);
     c.vel.swap(c.vel2);
     c.key.swap(c.key2);
@@ -971,6 +956,41 @@ void rebuild_prepare(Ctx& c) {
         c.omega.swap(c.omega2);
         c.quat.swap(c.quat2);
     }
+}
+
+// pbc, spatial sort of the particles, and the previous list kept as the
+// source of the shear-history remap
+void rebuild_prepare(Ctx& c) {
+    if (!c.setup_done) neighbor_setup(c);
+    const int64_t n = c.n;
+    const int64_t ncells = c.grid.ncells;
+    const bool dem = c.model == 1;
+    const bool mixed = c.precision == 1;
+    c.key.resize(n);
+    c.key2.resize(n);
+    c.order.resize(n);
+    c.cnt.resize(ncells);
+    c.start.resize(ncells + 1);
+    c.bin_start.resize(ncells + 1);
+    c.gstart.resize(ncells + 1);
+    c.gcnt.resize(n);
+    c.goff.resize(n + 1);
+    c.placed.resize(std::max<int64_t>(n, c.ghost_cap) + 1);
+    // scan scratch sized once for every scan of this rebuild, so no buffer is
+    // reallocated while kernels that use it are queued
+    c.scan_tmp.resize(size_t((std::max<int64_t>(ncells, n) + 2047) / 2048 + 2));
+
+    launch(c, k_rebuild_begin, blocks_for(ncells, 256) < 512 ? blocks_for(ncells, 256) : 512, 256,
+           c.status.p, c.cnt.p, ncells);
+    // pbc + keys
+    launch(c, k_wrap_key, blocks_for(n, 256), 256, n, c.pos.p, c.image.p, c.box, c.grid,
+                                                          c.key.p, c.status.p, c.sysp_or_null(),
+                                                          c.psys.p, c.cnt.p);
+    // sort_atoms_by_bin: stable counting sort of particles by bin
+    counting_sort(c, c.key.p, n, nullptr, ncells, c.cnt.p, c.start.p, c.placed.p, c.order.p,
+                  nullptr, true, nullptr, nullptr, true);
+    // permute every particle array into the alternates, then swap
+    permute_all(c, c.order.p, n);
     // the previous list becomes the remap source
     c.remap_have_old = c.list_valid && dem;
     c.nbr.swap(c.nbr_old);
@@ -1133,8 +1153,6 @@ void rebuild_attempt(Ctx& c) {
 // Capacities for the next attempt after a build that did not fit; false when
 // the status reports a complete build.
 bool rebuild_grow(Ctx& c, const Status& s) {
-    if (s.overflow & 4u)
-        throw ApiError(-5, "a particle left a non-periodic boundary (removal is not supported)");
     bool again = false;
     if (s.overflow & 1u) {
         c.ghost_cap = int64_t(s.ghost_total) + int64_t(s.ghost_total) / 4 + 1024;
@@ -1152,6 +1170,68 @@ bool rebuild_grow(Ctx& c, const Status& s) {
     return again;
 }
 
+namespace {
+__global__ void k_clear_exit(Status* st) { st->overflow &= ~4u; }
+}  // namespace
+
+// pbc's removal of the particles outside a non-periodic face (domain.rs:
+// 413-459), after a rebuild's wrap and sort saw them (status bit 4). The
+// reference walks the particles from the last index down and swap_removes
+// each one outside (the last particle takes its slot), then sorts by bin
+// (sort_atoms_by_bin, stable here); that order is rebuilt from the sort's
+// permutation on the host, and the survivors are gathered from the
+// pre-sort arrays in it. The history remap then reads the pre-sort rows
+// through the new order as after any sort. A rare event: done
+// synchronously, the build then repeats on the remaining particles.
+void remove_outside(Ctx& c) {
+    if (c.nsys > 0)
+        throw ApiError(-5, "a particle of a batched system left a non-periodic boundary");
+    MDD_HIP(hipStreamSynchronize(c.stream));
+    const int64_t n = c.n;
+    const int64_t ncells = c.grid.ncells;
+    const size_t un = static_cast<size_t>(n);
+    std::vector<double4> pos(un);
+    std::vector<uint32_t> key(un), order(un);
+    d2h(c, pos.data(), c.pos.p, sizeof(double4) * size_t(n));
+    d2h(c, key.data(), c.key.p, sizeof(uint32_t) * size_t(n));
+    d2h(c, order.data(), c.order.p, sizeof(uint32_t) * size_t(n));
+    std::vector<uint8_t> out(un, 0);
+    std::vector<uint32_t> key_pre(un);
+    for (int64_t k = 0; k < n; k++) {
+        const double x[3] = {pos[k].x, pos[k].y, pos[k].z};
+        bool o = false;
+        for (int d = 0; d < 3; d++)
+            if (!c.box.periodic[d] && (x[d] < c.box.lo[d] || x[d] >= c.box.hi[d])) o = true;
+        out[order[k]] = o ? 1 : 0;
+        key_pre[order[k]] = key[k];
+    }
+    // swap_remove from the last index down (the particle at the end has
+    // already been looked at and stays)
+    std::vector<uint32_t> cur(un);
+    for (int64_t i = 0; i < n; i++) cur[i] = uint32_t(i);
+    int64_t len = n;
+    for (int64_t i = n - 1; i >= 0; i--)
+        if (out[cur[i]]) cur[i] = cur[--len];
+    if (len == 0) throw ApiError(-1, "every particle left the box through a fixed boundary");
+    // stable counting sort of the survivors by bin
+    std::vector<uint32_t> start(static_cast<size_t>(ncells) + 1, 0u);
+    for (int64_t j = 0; j < len; j++) start[key_pre[cur[j]] + 1]++;
+    for (int64_t q = 0; q < ncells; q++) start[q + 1] += start[q];
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    std::vector<uint32_t> fin(static_cast<size_t>(len));
+    for (int64_t j = 0; j < len; j++) fin[fill[key_pre[cur[j]]]++] = cur[j];
+    // the pre-sort arrays back in place, then the survivors gathered
+    swap_alternates(c);
+    h2d(c, c.order.p, fin.data(), sizeof(uint32_t) * size_t(len));
+    h2d(c, c.start.p, start.data(), sizeof(uint32_t) * (size_t(ncells) + 1));
+    permute_all(c, c.order.p, len);
+    k_clear_exit<<<1, 1, 0, c.stream>>>(c.status.p);
+    MDD_HIP(hipGetLastError());
+    MDD_HIP(hipStreamSynchronize(c.stream));
+    c.n = len;
+    c.removed += n - len;
+}
+
 // full rebuild, checked on the host before it returns
 void rebuild(Ctx& c) {
     rebuild_prepare(c);
@@ -1163,6 +1243,11 @@ void rebuild(Ctx& c) {
         MDD_HIP(hipMemcpyAsync(c.status_host, c.status.p, sizeof(Status), hipMemcpyDeviceToHost,
                                c.stream));
         MDD_HIP(hipStreamSynchronize(c.stream));
+        if (c.status_host->overflow & 4u) {
+            remove_outside(c);
+            attempt--;
+            continue;
+        }
         fitted = !rebuild_grow(c, *c.status_host);
     }
     if (!fitted) throw ApiError(-4, "neighbor rebuild did not fit after retries");

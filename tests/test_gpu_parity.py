@@ -520,3 +520,32 @@ def test_rebuild_policies_match_oracle(every, check, sort_every):
     oa, ob, ospring = canon_history(*o.history())
     assert np.array_equal(a, oa) and np.array_equal(b, ob)
     assert rel_err(spring, ospring) < 1e-9
+
+
+def test_particles_leaving_a_fixed_boundary_are_removed_as_the_oracle_does():
+    # pbc (domain.rs:413-459): at a rebuild, a particle outside a non-periodic
+    # face is swap_removed (walking down from the last index), then the
+    # survivors are sorted by bin. Spheres in the outer 2 mm of z drift out
+    # through the two fixed faces over the run; dt = 1e-6 s so that they
+    # leave within it
+    w = W.dem_gas(n=400, box=0.025, seed=31)
+    w.periodic = (True, True, False)
+    w.dt = 1e-6
+    z = w.pos[:, 2]
+    w.vel[z < 0.002, 2] = -2.0
+    w.vel[z > 0.023, 2] = 2.0
+    o = oracle_from_workload(w)
+    s = hip.from_workload(w, hip.FP64, box_exit=True)
+    o.run(1500)
+    s.run(1500)
+    assert o.nlocal < w.n - 10
+    assert s.n == o.nlocal
+    st, ref = s.state(), o.atoms()
+    # the same survivors in the same (sorted) order
+    assert np.array_equal(st["tag"], ref["tag"])
+    assert np.abs(st["pos"] - ref["pos"]).max() < 1e-12
+    assert rel_err(st["vel"], ref["vel"]) < 1e-9
+    assert s.thermo()["rebuilds"] == o.rebuilds
+    a, b, spring = canon_history(*s.history())
+    oa, ob, ospring = canon_history(*o.history())
+    assert np.array_equal(a, oa) and np.array_equal(b, ob)
