@@ -48,6 +48,7 @@ class Job:
     done: float | None = None
     digest: dict | None = None
     error: str | None = None
+    final: dict | None = None
 
     @property
     def n(self) -> int:
@@ -104,12 +105,17 @@ def state_digest(s) -> dict:
                 graph_replays=replays, graph_captures=captures)
 
 
-def run_job(job: Job, precision) -> dict:
-    """One job on its own context; returns its digest."""
+def run_job(job: Job, precision, box_exit=False, keep_state=False) -> dict:
+    """One job on its own context; returns its digest (and, with keep_state,
+    its final tags, positions and velocities in the job's `final`)."""
     from mddem_amd import hip
-    s = hip.from_workload(job.workload, precision, box_exit=False)
+    s = hip.from_workload(job.workload, precision, box_exit=box_exit)
     try:
         s.run(job.steps)
+        if keep_state:
+            st = s.state()
+            job.final = dict(tag=st["tag"], pos=st["pos"], vel=st["vel"],
+                             rebuilds=s.thermo()["rebuilds"])
         return state_digest(s)
     finally:
         s.close()
@@ -129,11 +135,18 @@ def admit(queue, live_bytes, live_count, budget, slots):
 
 
 class Server:
-    def __init__(self, hbm_budget: int, slots: int = 8, precision=None):
+    """box_exit: rebuild when a particle leaves the periodic box (the
+    reference's rule) instead of wrapping lazily; keep_state: keep each
+    job's final state on the job (for comparisons)."""
+
+    def __init__(self, hbm_budget: int, slots: int = 8, precision=None, box_exit=False,
+                 keep_state=False):
         from mddem_amd import hip
         self.budget = int(hbm_budget)
         self.slots = int(slots)
         self.precision = hip.MIXED if precision is None else precision
+        self.box_exit = box_exit
+        self.keep_state = keep_state
         self.lock = threading.Condition()
         self.live_bytes = 0
         self.live = 0
@@ -141,7 +154,7 @@ class Server:
 
     def _worker(self, job: Job, t0: float):
         try:
-            job.digest = run_job(job, self.precision)
+            job.digest = run_job(job, self.precision, self.box_exit, self.keep_state)
         except Exception as e:  # reported per job; the loop goes on
             job.error = f"{type(e).__name__}: {e}"
         job.done = time.perf_counter() - t0

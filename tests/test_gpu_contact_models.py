@@ -12,7 +12,7 @@ import pytest
 
 from mddem_amd import hip
 from mddem_amd import workloads as W
-from tests.helpers import oracle_from_workload, rel_err
+from tests.helpers import canon_history, oracle_from_workload, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -69,8 +69,8 @@ def _pair(w, config, prec):
 
 
 def _canonical_history(a, b, h):
-    k = np.lexsort((b, a))
-    return a[k], b[k], h[k]
+    # (lower tag, higher tag) order: the checker lists (owner, partner)
+    return canon_history(a, b, h)
 
 
 @pytest.mark.parametrize("prec", [hip.FP64, hip.MIXED])

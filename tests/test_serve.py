@@ -80,3 +80,31 @@ def test_graph_rebuilds_with_uploads_and_copies_in_other_threads():
     for j, a in zip(jobs, alone):
         assert j.digest["sha256"] == a["sha256"], j.jid
     assert sum(j.digest["graph_replays"] for j in jobs) > 0
+
+
+@pytest.mark.gpu
+def test_jobs_served_in_the_mix_match_the_oracle():
+    # three jobs (two granular gases, one thermostatted LJ fluid) served
+    # concurrently in FP64 with the reference's rebuild rule; each final
+    # state against the CPU checker running that job's workload alone
+    from mddem_amd import hip
+    from tests.helpers import oracle_from_workload, rel_err
+    jobs = [serve.Job(0, "dem", 1500, 400, 0.0, dict(seed=21)),
+            serve.Job(1, "lj", 2048, 300, 0.0, dict(rho=0.85, temperature=1.0, seed=22)),
+            serve.Job(2, "dem", 2500, 300, 0.0, dict(seed=23))]
+    for j in jobs:
+        j.workload = serve.build_workload(j)
+    rep = serve.Server(hbm_budget=8 << 30, slots=4, precision=hip.FP64, box_exit=True,
+                       keep_state=True).run(jobs)
+    assert rep["failed"] == 0 and rep["max_concurrent"] >= 2
+    for j in jobs:
+        o = oracle_from_workload(j.workload)
+        o.run(j.steps)
+        ref = o.atoms()
+        a, b = np.argsort(j.final["tag"]), np.argsort(ref["tag"])
+        L = j.workload.hi - j.workload.lo
+        dp = j.final["pos"][a] - ref["pos"][b]
+        dp -= L * np.round(dp / L)
+        assert np.abs(dp).max() < 1e-12 * L.max() * 1e3, j.jid
+        assert rel_err(j.final["vel"][a], ref["vel"][b]) < 1e-9, j.jid
+        assert j.final["rebuilds"] == o.rebuilds, j.jid
