@@ -279,6 +279,16 @@ int mdd_download_history7(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t*
  * Single systems only. */
 int mdd_set_virial(mdd_ctx* ctx, int every);
 int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out);
+/* mdd_compute_thermo(ctx, 0, out) without any allocation. No counterpart in
+ * the reference: for callers that cannot allocate device or pinned memory
+ * (code under HIP graph capture, tight serving loops). Same kernels and the
+ * same summation order, so the same values bit for bit; pe, virial and
+ * pressure are left 0 (they need the energy pass). The per-block partial
+ * sums go to a static 2 MiB device array (one per GPU, held until the
+ * process ends), so at most 8,388,608 particles (MDD_ERR_INVALID beyond).
+ * Calls on one GPU share that array: this library serialises them within a
+ * process (each call returns only once its totals are on the host). */
+int mdd_compute_thermo_noalloc(mdd_ctx* ctx, mdd_thermo* out);
 /* block until queued work is done */
 int mdd_synchronize(mdd_ctx* ctx);
 /* Time every `every`-th scheduled pair-force launch with HIP events on the

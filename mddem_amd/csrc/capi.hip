@@ -520,6 +520,7 @@ int mdd_create(mdd_ctx** out, const mdd_params* p) {
         std::memset(cc.status_host, 0, sizeof(mdd::Status));
         k_status_init<<<1, 1, 0, cc.stream>>>(cc.status.p);
         MDD_HIP(hipStreamSynchronize(cc.stream));
+        mdd::init_thermo_static_scratch(cc.device);
     });
     if (rc != MDD_OK) {
         mdd_destroy(ctx);
@@ -1422,6 +1423,41 @@ int mdd_download_history7(mdd_ctx* ctx, int64_t cap, uint32_t* tag_lo, uint32_t*
     });
 }
 
+// the step observables from the device totals of thermo_reduce (sum m v^2,
+// sum m v, sum m, sum m |v - vbar|^2, half-list pairs, contact bits)
+static void thermo_fill(Ctx& c, const double s[8], mdd_thermo* out) {
+    sync_status(c);
+    out->step = c.step;
+    out->rebuilds = c.rebuilds;
+    out->nghost = c.nghost;
+    out->ke = 0.5 * s[0];
+    const double ndof = 3.0 * double(c.n) - 3.0;
+    out->temperature = ndof > 0 ? 2.0 * out->ke / ndof : 0.0;
+    out->momentum[0] = s[1];
+    out->momentum[1] = s[2];
+    out->momentum[2] = s[3];
+    out->granular_temperature = s[4] > 0 ? s[5] / (3.0 * s[4]) : 0.0;
+    out->p_xi = c.status_host->p_xi;
+    if (c.list_valid) {
+        out->npairs = int64_t(s[6]);
+        if (c.model == MDD_MODEL_DEM) out->ncontacts = int64_t(s[7]) / 2;
+    }
+}
+
+int mdd_compute_thermo_noalloc(mdd_ctx* ctx, mdd_thermo* out) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (!out) throw ApiError(MDD_ERR_INVALID, "null output");
+        if (c.systems_dirty)
+            throw ApiError(MDD_ERR_INVALID, "systems added since the last step: their first "
+                                            "set-up allocates (call mdd_run or mdd_compute_thermo)");
+        if (c.n <= 0) throw ApiError(MDD_ERR_INVALID, "no particles uploaded");
+        std::memset(out, 0, sizeof(*out));
+        double s[8];
+        mdd::thermo_reduce_static(c, s);
+        thermo_fill(c, s, out);
+    });
+}
+
 int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out) {
     return guarded(ctx, [&](Ctx& c) {
         need_state(c);
@@ -1429,30 +1465,7 @@ int mdd_compute_thermo(mdd_ctx* ctx, int with_energy, mdd_thermo* out) {
         std::memset(out, 0, sizeof(*out));
         double s[8];
         mdd::thermo_reduce(c, s);
-        sync_status(c);
-        out->step = c.step;
-        out->rebuilds = c.rebuilds;
-        out->nghost = c.nghost;
-        out->ke = 0.5 * s[0];
-        const double ndof = 3.0 * double(c.n) - 3.0;
-        out->temperature = ndof > 0 ? 2.0 * out->ke / ndof : 0.0;
-        out->momentum[0] = s[1];
-        out->momentum[1] = s[2];
-        out->momentum[2] = s[3];
-        out->granular_temperature = s[4] > 0 ? s[5] / (3.0 * s[4]) : 0.0;
-        out->p_xi = c.status_host->p_xi;
-        if (c.list_valid) {
-            auto h = dl(c, c.nhalf, size_t(c.n));
-            int64_t t = 0;
-            for (uint32_t v : h) t += v;
-            out->npairs = t;
-            if (c.model == MDD_MODEL_DEM) {
-                auto mk = dl(c, c.cmask, size_t(c.n));
-                int64_t nc = 0;
-                for (int64_t i = 0; i < c.n; i++) nc += __builtin_popcountll(mk[i]);
-                out->ncontacts = nc / 2;
-            }
-        }
+        thermo_fill(c, s, out);
         if (c.model == MDD_MODEL_DEM && c.vir_valid && c.vir.p) {
             // the contact virial of the last pass that computed it
             MDD_HIP(hipStreamSynchronize(c.stream));

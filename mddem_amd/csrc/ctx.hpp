@@ -550,7 +550,14 @@ void launch_nh_finish(Ctx& c, int parity);
 void launch_nh_seed(Ctx& c, int parity);
 void launch_nh_pre_standalone(Ctx& c);
 void launch_nh_post_standalone(Ctx& c);
+// totals of sum m v^2, sum m v (3), sum m, sum m |v - vbar|^2, half-list
+// pairs and contact bits: through a context buffer, or through the static
+// scratch of the context's device (allocates nothing; at most
+// thermo_static_max_particles() particles)
 void thermo_reduce(Ctx& c, double out[8]);
+void thermo_reduce_static(Ctx& c, double out[8]);
+void init_thermo_static_scratch(int device);
+int64_t thermo_static_max_particles();
 
 inline int blocks_for(int64_t n, int threads) {
     int64_t b = (n + threads - 1) / threads;

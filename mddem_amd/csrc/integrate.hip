@@ -20,6 +20,9 @@
 // momentum lives in two slots indexed by step parity: the pre kernel of step
 // s reads slot (s-1)&1 and its block 0 writes slot s&1, so no block reads a
 // value another block of the same launch is writing.
+#include <mutex>
+#include <string>
+
 #include "ctx.hpp"
 
 namespace mdd {
@@ -393,29 +396,81 @@ __global__ void __launch_bounds__(kThreads) k_nh_finish(IntArgs a, int q) {
     a.st->p_xi = p;
 }
 
-// per-block partial sums of the thermo quantities: sum m v^2, sum m v, sum m,
-// and (second pass, given vbar) sum m |v - vbar|^2
-__global__ void __launch_bounds__(kThreads) k_thermo(IntArgs a, double vbx, double vby,
-                                                     double vbz, double* out) {
+// Thermo reduction, all on the device: per-block partials of
+//   0 sum m v^2, 1-3 sum m v, 4 sum m, 5 sum m |v - vbar|^2,
+//   6 half-list pairs (sum of the row counts nhalf), 7 contact bits (cmask)
+// (kThermoCols per block), then one block sums each column in a fixed order
+// (thread t takes blocks t, t + 256, ... in turn, then a fixed tree), so the
+// totals do not depend on timing. Pass 1 leaves column 5 at 0 and the mean
+// velocity vbar = sum m v / sum m in the scratch; pass 2 fills column 5 from
+// it. Scratch layout in doubles: totals [0, 8), vbar [8, 11), partials from
+// kThermoPartOff, kThermoCols per block.
+constexpr int kThermoCols = 8;
+constexpr int kThermoPartOff = 16;
+
+__global__ void __launch_bounds__(kThreads) k_thermo(IntArgs a, const uint32_t* nhalf,
+                                                     const uint64_t* cmask, double* scratch,
+                                                     int pass) {
     __shared__ double lds4[4];
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    double q[6] = {0, 0, 0, 0, 0, 0};
-    if (i < a.n) {
-        const double4 v = a.vel[i];
-        const double m = a.mass[i];
-        q[0] = m * (v.x * v.x + v.y * v.y + v.z * v.z);
-        q[1] = m * v.x;
-        q[2] = m * v.y;
-        q[3] = m * v.z;
-        q[4] = m;
-        const double ux = v.x - vbx, uy = v.y - vby, uz = v.z - vbz;
-        q[5] = m * (ux * ux + uy * uy + uz * uz);
-    }
-    for (int k = 0; k < 6; k++) {
-        const double s = block_sum(q[k], lds4);
-        if (threadIdx.x == 0) out[blockIdx.x * 6 + k] = s;
+    double* out = scratch + kThermoPartOff + size_t(blockIdx.x) * kThermoCols;
+    if (pass == 1) {
+        double q[kThermoCols] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (i < a.n) {
+            const double4 v = a.vel[i];
+            const double m = a.mass[i];
+            q[0] = m * (v.x * v.x + v.y * v.y + v.z * v.z);
+            q[1] = m * v.x;
+            q[2] = m * v.y;
+            q[3] = m * v.z;
+            q[4] = m;
+            if (nhalf) q[6] = double(nhalf[i]);
+            if (cmask) q[7] = double(__popcll(cmask[i]));
+        }
+        for (int k = 0; k < kThermoCols; k++) {
+            const double s = block_sum(q[k], lds4);
+            if (threadIdx.x == 0) out[k] = s;
+        }
+    } else {
+        double q = 0.0;
+        if (i < a.n) {
+            const double4 v = a.vel[i];
+            const double m = a.mass[i];
+            const double ux = v.x - scratch[8], uy = v.y - scratch[9], uz = v.z - scratch[10];
+            q = m * (ux * ux + uy * uy + uz * uz);
+        }
+        const double s = block_sum(q, lds4);
+        if (threadIdx.x == 0) out[5] = s;
     }
 }
+
+// one block: column totals over nb block partials (pass 1: every column and
+// vbar; pass 2: column 5)
+__global__ void __launch_bounds__(kThreads) k_thermo_total(double* scratch, int nb, int pass) {
+    __shared__ double lds4[4];
+    const double* parts = scratch + kThermoPartOff;
+    for (int k = 0; k < kThermoCols; k++) {
+        if (pass == 2 && k != 5) continue;
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kThreads) s += parts[size_t(b) * kThermoCols + k];
+        s = block_sum(s, lds4);
+        if (threadIdx.x == 0) scratch[k] = s;
+    }
+    if (pass == 1 && threadIdx.x == 0) {
+        const double m = scratch[4];
+        scratch[8] = m > 0.0 ? scratch[1] / m : 0.0;
+        scratch[9] = m > 0.0 ? scratch[2] / m : 0.0;
+        scratch[10] = m > 0.0 ? scratch[3] / m : 0.0;
+    }
+}
+
+// The static scratch of mdd_compute_thermo_noalloc: totals, vbar and the
+// partials of at most kThermoStaticBlocks blocks (2 MiB), one copy per GPU
+// for the life of the process. Kernels receive its address as looked up on
+// each device (thermo_static_scratch); the symbol itself is never passed.
+constexpr int64_t kThermoStaticBlocks = 32768;
+static __device__ __align__(256) double scratch[kThermoPartOff +
+                                                kThermoCols * kThermoStaticBlocks];
 
 int post_blocks(const Ctx& c, bool nh) {
     const int nb = blocks_for(c.n, kThreads);
@@ -608,33 +663,68 @@ void launch_nh_post_standalone(Ctx& c) {
 }
 
 // out: [sum m v^2, sum m vx, sum m vy, sum m vz, sum m, sum m |v - vbar|^2]
-void thermo_reduce(Ctx& c, double out[8]) {
-    IntArgs a = make_args(c, -1, false);
+namespace {
+
+// both passes into `buf` (kThermoPartOff + kThermoCols * blocks doubles), the
+// 8 totals copied to `out` (host); nothing is allocated
+void thermo_into(Ctx& c, double* buf, double out[8]) {
+    // only the particle count, velocities and masses (make_args would size
+    // the step buffers)
+    IntArgs a{};
+    a.n = c.n;
+    a.npad = c.npad;
+    a.vel = c.vel.p;
+    a.mass = c.mass.p;
     const int nb = blocks_for(c.n, kThreads);
-    c.partial2.resize(size_t(nb) * 6);
-    std::vector<double> h(size_t(nb) * 6);
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int pass = 0; pass < 2; pass++) {
-        double vb[3] = {0, 0, 0};
-        if (pass == 1 && s[4] > 0) {
-            vb[0] = s[1] / s[4];
-            vb[1] = s[2] / s[4];
-            vb[2] = s[3] / s[4];
-        }
-        k_thermo<<<nb, kThreads, 0, c.stream>>>(a, vb[0], vb[1], vb[2], c.partial2.p);
-        MDD_HIP(hipGetLastError());
-        MDD_HIP(hipMemcpyAsync(h.data(), c.partial2.p, sizeof(double) * h.size(),
-                               hipMemcpyDeviceToHost, c.stream));
-        MDD_HIP(hipStreamSynchronize(c.stream));
-        double t[6] = {0, 0, 0, 0, 0, 0};
-        for (int b = 0; b < nb; b++)
-            for (int k = 0; k < 6; k++) t[k] += h[size_t(b) * 6 + k];
-        if (pass == 0)
-            for (int k = 0; k < 6; k++) s[k] = t[k];
-        else
-            s[5] = t[5];
+    const uint32_t* nhalf = c.list_valid && c.nhalf.p ? c.nhalf.p : nullptr;
+    const uint64_t* cmask = c.list_valid && c.model == 1 && c.cmask.p ? c.cmask.p : nullptr;
+    for (int pass = 1; pass <= 2; pass++) {
+        k_thermo<<<nb, kThreads, 0, c.stream>>>(a, nhalf, cmask, buf, pass);
+        k_thermo_total<<<1, kThreads, 0, c.stream>>>(buf, nb, pass);
     }
-    for (int k = 0; k < 6; k++) out[k] = s[k];
+    MDD_HIP(hipGetLastError());
+    MDD_HIP(hipMemcpyAsync(out, buf, sizeof(double) * 8, hipMemcpyDeviceToHost, c.stream));
+    MDD_HIP(hipStreamSynchronize(c.stream));
+}
+
+// address of `scratch` on each device, looked up once per device
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+}  // namespace
+
+void thermo_reduce(Ctx& c, double out[8]) {
+    const int nb = blocks_for(c.n, kThreads);
+    c.partial2.resize(size_t(kThermoPartOff) + size_t(nb) * kThermoCols);
+    thermo_into(c, c.partial2.p, out);
+}
+
+// Look up the static scratch's device address on `device` (made current by
+// the caller), once per device. Called when a context is created.
+void init_thermo_static_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw ApiError(-1, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    MDD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+}
+
+int64_t thermo_static_max_particles() { return kThermoStaticBlocks * kThreads; }
+
+// thermo_reduce through the static scratch of the context's device. Calls
+// are serialised within the process (the lock is held until the totals
+// are on the host), so concurrent calls on one GPU do not overlap.
+void thermo_reduce_static(Ctx& c, double out[8]) {
+    if (c.n > thermo_static_max_particles())
+        throw ApiError(-1, "mdd_compute_thermo_noalloc: " + std::to_string(c.n) +
+                               " particles exceed the static scratch limit of " +
+                               std::to_string(thermo_static_max_particles()));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    double* p = (c.device >= 0 && c.device < kMaxDevices) ? g_scratch_ptr[c.device] : nullptr;
+    if (!p) throw ApiError(-1, "static scratch not initialised on this device");
+    thermo_into(c, p, out);
 }
 
 }  // namespace mdd

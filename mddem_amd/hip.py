@@ -169,6 +169,7 @@ SIGNATURES = {
     "mdd_download_history": (C.c_int, [_vp, C.c_int64, _u32p, _u32p, _dp, _i64p]),
     "mdd_download_history7": (C.c_int, [_vp, C.c_int64, _u32p, _u32p, _dp, _i64p]),
     "mdd_compute_thermo": (C.c_int, [_vp, C.c_int, C.POINTER(Thermo)]),
+    "mdd_compute_thermo_noalloc": (C.c_int, [_vp, C.POINTER(Thermo)]),
     "mdd_synchronize": (C.c_int, [_vp]),
     "mdd_set_profiling": (C.c_int, [_vp, C.c_int]),
     "mdd_force_kernel_stats": (C.c_int, [_vp, _dp, _i64p]),
@@ -514,6 +515,13 @@ class System:
     def thermo(self, energy: bool = False) -> dict:
         t = Thermo()
         self._c(self.L.mdd_compute_thermo(self.h, int(energy), C.byref(t)))
+        return t.as_dict()
+
+    def thermo_noalloc(self) -> dict:
+        """mdd_compute_thermo_noalloc: the thermo values without any
+        allocation (static device scratch); no energy, virial or pressure."""
+        t = Thermo()
+        self._c(self.L.mdd_compute_thermo_noalloc(self.h, C.byref(t)))
         return t.as_dict()
 
     def set_profiling(self, every: int):
