@@ -111,7 +111,7 @@ def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True, layout=Non
     return res
 
 
-def run_dem(steps, warmup, n=4_000_000, log_every=0):
+def run_dem(steps, warmup, n=4_000_000, log_every=0, every=1000, tune_steps=200):
     """DEM line: the settling workload's 4M polydisperse glass spheres, walls
     and gravity, started as a resting bed (every sphere in contact, shaken at
     the config's velocities, U[-0.1, 0.1] m/s), mixed and all-FP64 with the
@@ -121,13 +121,34 @@ def run_dem(steps, warmup, n=4_000_000, log_every=0):
     from mddem_amd import hip
     from mddem_amd import workloads as W
     t0 = time.perf_counter()
-    w = W.dem_packed_bed(n=n)
+    w = W.dem_packed_bed(n=n, every=every)
     log(f"[bench] dem bed of {w.n} spheres built in {time.perf_counter() - t0:.1f} s "
         f"(top at z = {w.pos[:, 2].max():.4f} m)")
     out = {}
+    shape = None
     for name, prec in (("mixed", hip.MIXED), ("fp64", hip.FP64)):
         s = hip.from_workload(w, prec, box_exit=False)
-        s.set_force_lanes(0)  # one lane per sphere: rows are short here
+        if shape is None:
+            # launch shape of the force pass, tuned during the first leg's
+            # warmup: one lane per sphere (one kernel, or split by row
+            # length) or 2 / 4 lanes per sphere; each trial after a relayout
+            trials = []
+            for lg, split in ((0, 0), (0, 1), (1, -1), (2, -1)):
+                s.set_force_lanes(lg)
+                s.set_dem_split(split)
+                s.run(20)
+                s.synchronize()
+                t1 = time.perf_counter()
+                s.run(tune_steps)
+                s.synchronize()
+                per = (time.perf_counter() - t1) / tune_steps
+                trials.append((per, lg, split))
+                log(f"[bench]   dem tune: lanes {1 << lg}, split {split}: {per * 1e3:.4f} ms/step")
+            _, lg, split = min(trials)
+            shape = dict(lanes=1 << lg, split=split, trials=[
+                dict(lanes=1 << t[1], split=t[2], ms_per_step=round(t[0] * 1e3, 4)) for t in trials])
+        s.set_force_lanes({1: 0, 2: 1, 4: 2}[shape["lanes"]])
+        s.set_dem_split(shape["split"])
         s.run(warmup)
         s.synchronize()
         th0 = s.thermo()
@@ -169,10 +190,12 @@ def run_dem(steps, warmup, n=4_000_000, log_every=0):
             f"{out[name]['rebuilds']} rebuilds ({rb_ms * 1e3:.0f} us each), "
             f"force pass {ms * 1e3:.1f} us")
     out["mixed_over_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
-    out["workload"] = (f"DEM settling spheres as a resting bed: {w.n} glass spheres "
-                       "r~U[0.8,1.2] mm deposited on z=0 (every sphere touching), box "
-                       "0.4x0.4x1.2 m periodic x/y, plane walls z=0 and z=1.2 m, gravity, "
-                       "v~U[-0.1,0.1] m/s, dt=0.05 t_Rayleigh(0.8 mm)")
+    out["force_pass_shape"] = shape
+    out["workload"] = (f"DEM settling spheres as a bed at rest: {w.n} glass spheres "
+                       "r~U[0.8,1.2] mm deposited on z=0 (each resting on the spheres below, "
+                       "10 nm contact overlaps), box 0.4x0.4x1.2 m periodic x/y, plane walls "
+                       "z=0 and z=1.2 m, gravity, dt=0.05 t_Rayleigh(0.8 mm), "
+                       f"[neighbor] every={every} check=true")
     out["steps"] = steps
     out["warmup"] = warmup
     return out, w

@@ -95,6 +95,12 @@ int mdd_set_rebuild_on_box_exit(mdd_ctx* ctx, int enabled);
  * row the spring restarts from zero. 1: the spring is keyed by the pair and
  * survives whichever side owns it. */
 int mdd_set_history_mode(mdd_ctx* ctx, int mode);
+/* How the history is carried across a rebuild: 0 (default) the old list's
+ * contacts become records sorted by their 64-bit pair key (lower, higher
+ * particle index of the new order) and each new row slot finds its pair by a
+ * binary search; 1 each particle searches its own old row by partner tag.
+ * The results are identical. No reference counterpart: a method choice. */
+int mdd_set_history_remap(mdd_ctx* ctx, int method);
 /* Atom::dt; [run] dt or calculate_delta_time (crates/dem_atom/src/insert.rs:1392-1431) */
 int mdd_set_dt(mdd_ctx* ctx, double dt);
 /* [[dem.materials]] + MaterialTable::build_pair_tables (crates/dem_atom/src/lib.rs:389-515) */
@@ -307,6 +313,13 @@ int mdd_rebuild_stats(mdd_ctx* ctx, double* avg_ms, double* avg_gap_ms, int64_t*
 /* Free and total device memory of a device (hipMemGetInfo), for admission by
  * footprint. No reference counterpart: serving-loop support. */
 int mdd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
+/* DEM force pass at one lane per particle (log2 lanes 0): -1 (default)
+ * split by row length (particles with neighbors in the pair kernel, the
+ * rest in a walls-and-gravity kernel) when fewer than half the particles
+ * had neighbors at the last build the host has seen, else one kernel; 0
+ * always one kernel; 1 always split. Identical forces. No reference
+ * counterpart: launch-shape tuning. */
+int mdd_set_dem_split(mdd_ctx* ctx, int mode);
 /* Pair-force layout: log2 lanes per particle of the row kernels (0..3;
  * default 2), or -1 for the LJ cell-tile kernel (16-bit rows into each bin
  * cell's candidate list staged in LDS; LJ contexts only, others keep rows).

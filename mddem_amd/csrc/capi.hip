@@ -43,7 +43,8 @@ __global__ void k_status_init(mdd::Status* st) {
     st->ghost_total = 0u;
     st->overlap_count = 0;
     st->nh_kind[0] = st->nh_kind[1] = 1;
-    st->pad = 0;
+    st->hist_records = 0u;
+    st->listed = 0u;
     st->ke = 0.0;
     st->p_xi = 0.0;
     st->nh_p[0] = st->nh_p[1] = 0.0;
@@ -197,7 +198,7 @@ void run_steps(Ctx& c, int64_t nsteps) {
     // resume at the force pass of step ab after the device stopped there:
     // a rebuild request from the drift, or a rebuild that did not fit
     auto resume = [&](int ab) {
-        const bool retry = (c.status_host->overflow & 15u) != 0u;
+        const bool retry = (c.status_host->overflow & 31u) != 0u;
         // the rebuild's first kernel clears the abort
         if (retry) {
             if (c.status_host->overflow & 4u) mdd::remove_outside(c);
@@ -239,8 +240,7 @@ void run_steps(Ctx& c, int64_t nsteps) {
         while (c.step < end) {
             const int64_t s = c.step + 1;
             const bool forced = !c.list_valid || c.list_injected || c.list_lg != c.lg ||
-                                c.list_tile != c.want_tile() ||
-                                (c.sort_every > 0 && s % c.sort_every == 0) || count_due(s);
+                                c.list_tile != c.want_tile() || count_due(s);
             if (forced) {
                 // FullRebuild step: integrate, rebuild, force, integrate
                 first_half(int(s), false);
@@ -252,15 +252,13 @@ void run_steps(Ctx& c, int64_t nsteps) {
                 continue;
             }
             int64_t K = std::min<int64_t>(c.chunk, end - c.step);
-            if (c.sort_every > 0) {
-                const int64_t next_sort = (c.step / c.sort_every + 1) * c.sort_every;
-                K = std::min<int64_t>(K, next_sort - 1 - c.step);
-            }
             if (c.every > 0) K = std::min<int64_t>(K, c.last_rebuild + c.every - c.step);
             if (K < 1) K = 1;
             for (int64_t k = 1; k <= K; k++) {
                 const int st = int(c.step + k);
                 first_half(st, true);
+                // sort_atoms_by_bin's periodic sort: a rebuild unless still sorted
+                if (c.sort_every > 0 && st % c.sort_every == 0) mdd::launch_sort_check(c, st);
                 mdd::launch_pair_forces(c, st, false);
                 second_half(st);
             }
@@ -588,6 +586,20 @@ int mdd_set_virial(mdd_ctx* ctx, int every) {
     return guarded(ctx, [&](Ctx& c) {
         if (every < 0) throw ApiError(MDD_ERR_INVALID, "virial interval must be >= 0");
         c.virial_every = every;
+    });
+}
+
+int mdd_set_dem_split(mdd_ctx* ctx, int mode) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (mode < -1 || mode > 1) throw ApiError(MDD_ERR_INVALID, "split mode is -1, 0 or 1");
+        c.dem_split = mode;
+    });
+}
+
+int mdd_set_history_remap(mdd_ctx* ctx, int method) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (method != 0 && method != 1) throw ApiError(MDD_ERR_INVALID, "remap method is 0 or 1");
+        c.remap_sorted = method == 0;
     });
 }
 

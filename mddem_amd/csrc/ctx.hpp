@@ -223,13 +223,15 @@ struct Status {
     unsigned overflow;       // bit 0: image capacity, bit 1: neighbor row capacity,
                              // bit 2: particle outside a non-periodic face,
                              // bit 3: candidate list beyond 16-bit tile indices
+                             // bit 4: more old contacts than history records
     int max_row;             // longest full-list row at the last build
     int max_half;            // longest half-list row at the last build
     unsigned ghost_total;    // periodic images at the last build
     int overlap_count;       // large-overlap pairs in the current step
     int nh_kind[2];          // per step parity: 1 = nh_p is p_xi after a post
                              // half-update (or the initial p_xi), 0 = after a pre
-    int pad;
+    unsigned hist_records;   // contacts of the old list at the last history remap
+    unsigned listed;         // DEM: particles with a nonempty row at the last build
     double ke;               // kinetic energy at the last thermostat finish
     double p_xi;             // Nose-Hoover momentum at the last thermostat finish
     double nh_p[2];          // p_xi by step parity
@@ -372,6 +374,10 @@ struct Ctx {
     // count (the force pass at one lane per particle is split by them)
     DBuf<uint32_t> listed, listed_n;
     bool listed_ok = false;
+    // DEM force pass at one lane per particle: -1 split by row length when
+    // fewer than half the particles have neighbors (Status::listed of the
+    // last build the host has read), 0 one kernel, 1 always split
+    int dem_split = -1;
 
     // ---- particle arrays (current order)
     DBuf<double4> pos, vel, omega, quat;  // pos.w radius, vel.w 1/m, omega.w 1/I
@@ -456,6 +462,13 @@ struct Ctx {
     // still owns the pair (the reference's per-owner store); 0: it survives
     // whichever side owns the pair (keyed by the pair alone)
     int hist_by_owner = 1;
+    // history remap: by sorted pair keys (default) or by a search of each
+    // particle's old row
+    bool remap_sorted = true;
+    int64_t n_list_old = 0;   // particles of the list the remap reads
+    int64_t hk_cap = 0;       // capacity of the contact records
+    DBuf<uint32_t> hk_inv, hk_cnt, hk_off, hk_lo, hk_bcnt, hk_bstart;
+    DBuf<uint4> hk_rec, hk_rec2;
     DBuf<double> pe;              // per-particle LJ energy (thermo only)
 
     // ---- run state
@@ -515,6 +528,9 @@ void rebuild(Ctx& c);
 void rebuild_async(Ctx& c, int step, bool retry);
 // drop the particles a rebuild found outside a non-periodic face (pbc)
 void remove_outside(Ctx& c);
+// the periodic sort's test at `step`: a rebuild request unless the
+// particles are still in bin order
+void launch_sort_check(Ctx& c, int step);
 void release_graphs(Ctx& c);
 bool rebuild_grow(Ctx& c, const Status& s);
 // forces.hip

@@ -1553,8 +1553,14 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
         c.vir_step = step < 0 ? c.step : step;
         c.vir_valid = true;
     }
+    // the split pass pays off when most particles have no neighbors (a
+    // dilute cloud): the light kernel then covers most of them. In a dense
+    // packing every particle is listed and the one-kernel pass (XCD-aware
+    // block order, one thread per particle) is the faster one.
+    const bool sparse_rows = c.dem_split == 1 ||
+                             (c.dem_split < 0 && 2 * int64_t(c.status_host->listed) < c.n);
     const bool split = c.model == 1 && c.listed_ok && c.list_valid && !c.list_injected &&
-                       c.list_lg == 0 && !c.ext_on && c.nsys == 0 && !dem_vir;
+                       c.list_lg == 0 && !c.ext_on && c.nsys == 0 && !dem_vir && sparse_rows;
     const bool one_kernel = !(c.model == 0 && c.list_tile) && !split &&
                             !(c.model == 1 && step >= 0 && c.walls.count > 0 && a.has_regions);
     if (prof) {

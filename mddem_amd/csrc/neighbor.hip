@@ -791,6 +791,11 @@ __global__ void __launch_bounds__(256) k_list_rows(int64_t n, const uint32_t* __
     if (has) listed[base + lanes_before(m)] = uint32_t(i);
 }
 
+// the listed count into the status, where the host reads it
+__global__ void k_note_listed(const uint32_t* __restrict__ count, Status* st) {
+    st->listed = *count;
+}
+
 __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
                                 const uint32_t* __restrict__ nbr,
                                 const uint32_t* __restrict__ nnb,
@@ -831,6 +836,193 @@ __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
                 }
             }
         }
+    }
+    cmask[i] = nm;
+}
+
+// sort_atoms_by_bin's periodic sort (neighbor.rs:686-794) rebuilds only when
+// the stable sort would move a particle, i.e. when the bin keys of the
+// current positions are not non-decreasing in particle order: then the
+// step asks for a rebuild as a drift does
+__global__ void k_sort_check(int64_t n, const double4* __restrict__ pos, GridP g,
+                             const SysP* __restrict__ sysp, const uint32_t* __restrict__ psys,
+                             Status* st, int step) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i + 1 >= n) return;
+    auto key = [&](int64_t k) {
+        const double4 p = pos[k];
+        if (sysp) {
+            const SysP& S = sysp[psys[k]];
+            return uint64_t(S.cell_off) + cell_of(S.grid, p.x, p.y, p.z);
+        }
+        return uint64_t(cell_of(g, p.x, p.y, p.z));
+    };
+    if (key(i) > key(i + 1)) atomicMin(&st->abort_step, step);
+}
+
+// ---- shear-history remap by sorted pair keys ------------------------------
+// The old list's contacts become one record per pair (from its owner's
+// slot): the pair as particle indices of the NEW order, lo < hi, with the
+// new index of the old owner and the old history slot. Records are bucketed
+// by lo and sorted by hi inside each bucket, so (lo, hi) -- the 64-bit key
+// lo << 32 | hi -- is sorted over the whole table; every new row slot then
+// finds its pair's record by a binary search of bucket lo.
+struct HistRec {
+    uint32_t hi, owner, src, pad;
+};
+constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
+
+// new index of every old particle (kNoIndex for a removed one)
+__global__ void k_hk_invert(int64_t n, const uint32_t* __restrict__ order,
+                            uint32_t* __restrict__ new_of_old) {
+    const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (k < n) new_of_old[order[k]] = uint32_t(k);
+}
+
+// owned contact slots of every old particle
+__global__ void k_hk_count(int64_t n_old, int64_t npad_old, int maxnb_old, int lg_old,
+                           const uint32_t* __restrict__ nbr_old,
+                           const uint32_t* __restrict__ nnb_old,
+                           const uint64_t* __restrict__ cmask_old, uint32_t* __restrict__ cnt) {
+    const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (p >= n_old) return;
+    uint64_t m = cmask_old[p];
+    int len = int(nnb_old[p]);
+    if (len > maxnb_old) len = maxnb_old;
+    uint32_t c = 0;
+    while (m) {
+        const int k = __builtin_ctzll(m);
+        m &= m - 1ull;
+        if (k >= len) break;
+        if (!entry_rev(nbr_old[row_addr(p, k, npad_old, lg_old)])) c++;
+    }
+    cnt[p] = c;
+}
+
+// the records, at their old particle's offset; records of pairs with a
+// removed particle are marked with lo = kNoIndex. More records than the
+// capacity: status bit 16, the total in Status::hist_records.
+__global__ void k_hk_emit(int64_t n_old, int64_t npad_old, int maxnb_old, int lg_old,
+                          const uint32_t* __restrict__ nbr_old,
+                          const uint32_t* __restrict__ nnb_old,
+                          const uint64_t* __restrict__ cmask_old,
+                          const uint32_t* __restrict__ off, const uint32_t* __restrict__ new_of_old,
+                          uint32_t cap, uint32_t* __restrict__ lo_out,
+                          HistRec* __restrict__ rec_out, Status* st) {
+    const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (p == 0) {
+        const uint32_t total = off[n_old];
+        st->hist_records = total;
+        if (total > cap) atomicOr(&st->overflow, 16u);
+    }
+    if (p >= n_old) return;
+    uint64_t m = cmask_old[p];
+    int len = int(nnb_old[p]);
+    if (len > maxnb_old) len = maxnb_old;
+    uint32_t r = off[p];
+    const uint32_t ni = new_of_old[p];
+    while (m) {
+        const int k = __builtin_ctzll(m);
+        m &= m - 1ull;
+        if (k >= len) break;
+        const int64_t so = row_addr(p, k, npad_old, lg_old);
+        const uint32_t e = nbr_old[so];
+        if (entry_rev(e)) continue;
+        if (r < cap) {
+            const uint32_t nj = new_of_old[entry_idx(e)];
+            const bool gone = ni == kNoIndex || nj == kNoIndex;
+            lo_out[r] = gone ? kNoIndex : (ni < nj ? ni : nj);
+            HistRec h;
+            h.hi = ni < nj ? nj : ni;
+            h.owner = ni;
+            h.src = uint32_t(so);
+            h.pad = 0u;
+            rec_out[r] = h;
+        }
+        r++;
+    }
+}
+
+// records per bucket lo
+__global__ void k_hk_bucket_count(const uint32_t* __restrict__ total_dev, uint32_t cap,
+                                  const uint32_t* __restrict__ lo, uint32_t* __restrict__ cnt) {
+    const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    uint32_t tot = *total_dev;
+    if (tot > cap) tot = cap;
+    if (r >= int64_t(tot)) return;
+    const uint32_t b = lo[r];
+    if (b != kNoIndex) atomicAdd(&cnt[b], 1u);
+}
+
+// records into their buckets (any order inside a bucket: the sort follows)
+__global__ void k_hk_bucket_fill(const uint32_t* __restrict__ total_dev, uint32_t cap,
+                                 const uint32_t* __restrict__ lo,
+                                 const HistRec* __restrict__ rec,
+                                 const uint32_t* __restrict__ bstart, uint32_t* __restrict__ cursor,
+                                 HistRec* __restrict__ out) {
+    const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    uint32_t tot = *total_dev;
+    if (tot > cap) tot = cap;
+    if (r >= int64_t(tot)) return;
+    const uint32_t b = lo[r];
+    if (b == kNoIndex) return;
+    out[bstart[b] + atomicAdd(&cursor[b], 1u)] = rec[r];
+}
+
+// each bucket sorted by hi (insertion sort: a bucket holds the few contacts
+// of one particle)
+__global__ void k_hk_bucket_sort(int64_t n, const uint32_t* __restrict__ bstart,
+                                 HistRec* __restrict__ rec) {
+    const int64_t b = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (b >= n) return;
+    const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
+    for (uint32_t a = s0 + 1; a < s1; a++) {
+        const HistRec x = rec[a];
+        uint32_t t = a;
+        while (t > s0 && rec[t - 1].hi > x.hi) {
+            rec[t] = rec[t - 1];
+            t--;
+        }
+        rec[t] = x;
+    }
+}
+
+// every new row slot looks its pair (lo, hi) up: a binary search over hi in
+// bucket lo. A slot takes the record's history when the record exists and,
+// with by_owner (the reference's per-owner store), when the pair still has
+// the same owner; its contact bit is set.
+__global__ void k_remap_sorted(int64_t n, int64_t npad, int maxnb, int lg,
+                               const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nnb,
+                               const uint32_t* __restrict__ bstart,
+                               const HistRec* __restrict__ rec, int64_t cso,
+                               const double* __restrict__ hist_old, double* __restrict__ hist,
+                               uint64_t* __restrict__ cmask, int nslot, int by_owner) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const int64_t cs = int64_t(maxnb) * npad;
+    int cnt = int(nnb[i]);
+    if (cnt > maxnb) cnt = maxnb;
+    uint64_t nm = 0ull;
+    for (int k = 0; k < cnt; k++) {
+        const int64_t s = row_addr(i, k, npad, lg);
+        const uint32_t e = nbr[s];
+        const uint32_t j = entry_idx(e);
+        const uint32_t lo = uint32_t(i) < j ? uint32_t(i) : j;
+        const uint32_t hi = uint32_t(i) < j ? j : uint32_t(i);
+        uint32_t a = bstart[lo], b = bstart[lo + 1];
+        while (a < b) {  // first record with rec.hi >= hi
+            const uint32_t mid = (a + b) >> 1;
+            if (rec[mid].hi < hi)
+                a = mid + 1;
+            else
+                b = mid;
+        }
+        if (a >= bstart[lo + 1]) continue;
+        const HistRec h = rec[a];
+        if (h.hi != hi) continue;
+        if (by_owner && h.owner != (entry_rev(e) ? j : uint32_t(i))) continue;
+        for (int q = 0; q < nslot; q++) hist[q * cs + s] = hist_old[q * cso + h.src];
+        nm |= 1ull << k;
     }
     cmask[i] = nm;
 }
@@ -993,6 +1185,7 @@ void rebuild_prepare(Ctx& c) {
     permute_all(c, c.order.p, n);
     // the previous list becomes the remap source
     c.remap_have_old = c.list_valid && dem;
+    c.n_list_old = n;
     c.nbr.swap(c.nbr_old);
     c.nnb.swap(c.nnb_old);
     c.hist.swap(c.hist_old);
@@ -1129,16 +1322,65 @@ void rebuild_attempt(Ctx& c) {
         c.nmain.p, c.gnbr.p, c.nbr.p, tile ? c.xnbr.p : nullptr);
     MDD_HIP(hipGetLastError());
     c.list_tile = tile;
-    if (dem) {
+    if (dem && c.remap_sorted && c.remap_have_old) {
+        // the old contacts as records sorted by pair key, then every new
+        // row slot looks its pair up
+        const int64_t n_old = c.n_list_old;
+        if (c.hk_cap < 1024) c.hk_cap = std::max<int64_t>(1024, 4 * n_old);
+        c.hk_inv.resize(size_t(n_old));
+        c.hk_cnt.resize(size_t(n_old) + 1);
+        c.hk_off.resize(size_t(n_old) + 1);
+        c.hk_lo.resize(size_t(c.hk_cap));
+        c.hk_rec.resize(size_t(c.hk_cap));
+        c.hk_rec2.resize(size_t(c.hk_cap));
+        c.hk_bcnt.resize(size_t(n) + 1);
+        c.hk_bstart.resize(size_t(n) + 1);
+        HistRec* rec = reinterpret_cast<HistRec*>(c.hk_rec.p);
+        HistRec* rec2 = reinterpret_cast<HistRec*>(c.hk_rec2.p);
+        const uint32_t cap = uint32_t(c.hk_cap);
+        const uint32_t* total = c.hk_off.p + n_old;
+        fill_u32(c, c.hk_inv.p, kNoIndex, n_old);
+        launch(c, k_hk_invert, blocks_for(n, 256), 256, n, static_cast<const uint32_t*>(c.order.p),
+               c.hk_inv.p);
+        launch(c, k_hk_count, blocks_for(n_old, 256), 256, n_old, c.npad_old, c.maxnb_old,
+               c.lg_old, static_cast<const uint32_t*>(c.nbr_old.p),
+               static_cast<const uint32_t*>(c.nnb_old.p),
+               static_cast<const uint64_t*>(c.cmask_old.p), c.hk_cnt.p);
+        exclusive_scan(c, c.hk_cnt.p, c.hk_off.p, n_old);
+        launch(c, k_hk_emit, blocks_for(n_old, 256), 256, n_old, c.npad_old, c.maxnb_old,
+               c.lg_old, static_cast<const uint32_t*>(c.nbr_old.p),
+               static_cast<const uint32_t*>(c.nnb_old.p),
+               static_cast<const uint64_t*>(c.cmask_old.p),
+               static_cast<const uint32_t*>(c.hk_off.p), static_cast<const uint32_t*>(c.hk_inv.p),
+               cap, c.hk_lo.p, rec, c.status.p);
+        fill_u32(c, c.hk_bcnt.p, 0u, n + 1);
+        launch(c, k_hk_bucket_count, blocks_for(c.hk_cap, 256), 256, total, cap,
+               static_cast<const uint32_t*>(c.hk_lo.p), c.hk_bcnt.p);
+        exclusive_scan(c, c.hk_bcnt.p, c.hk_bstart.p, n);
+        fill_u32(c, c.hk_bcnt.p, 0u, n);
+        launch(c, k_hk_bucket_fill, blocks_for(c.hk_cap, 256), 256, total, cap,
+               static_cast<const uint32_t*>(c.hk_lo.p), static_cast<const HistRec*>(rec),
+               static_cast<const uint32_t*>(c.hk_bstart.p), c.hk_bcnt.p, rec2);
+        launch(c, k_hk_bucket_sort, blocks_for(n, 256), 256, n,
+               static_cast<const uint32_t*>(c.hk_bstart.p), rec2);
+        launch(c, k_remap_sorted, blocks_for(n, 256), 256, n, c.npad, c.maxnb, c.lg,
+               static_cast<const uint32_t*>(c.nbr.p), static_cast<const uint32_t*>(c.nnb.p),
+               static_cast<const uint32_t*>(c.hk_bstart.p), static_cast<const HistRec*>(rec2),
+               int64_t(c.maxnb_old) * c.npad_old, static_cast<const double*>(c.hist_old.p),
+               c.hist.p, c.cmask.p, c.hist_slots, c.hist_by_owner ? 1 : 0);
+    } else if (dem) {
         launch(c, k_remap_history, blocks_for(n, 256), 256,
             n, c.npad, c.maxnb, c.lg, c.nbr.p, c.nnb.p, c.tag.p, c.order.p, c.npad_old,
             c.maxnb_old, c.lg_old, c.nbr_old.p, c.nnb_old.p, c.tag2.p, c.hist_old.p,
             c.cmask_old.p, c.hist.p, c.cmask.p, c.remap_have_old ? 1 : 0, c.hist_slots,
             c.hist_by_owner ? 1 : 0);
+    }
+    if (dem) {
         c.listed.resize(size_t(c.npad));
         c.listed_n.resize(1);
         fill_u32(c, c.listed_n.p, 0u, 1);
         launch(c, k_list_rows, blocks_for(n, 256), 256, n, c.nnb.p, c.listed.p, c.listed_n.p);
+        launch(c, k_note_listed, 1, 1, static_cast<const uint32_t*>(c.listed_n.p), c.status.p);
         MDD_HIP(hipGetLastError());
     }
     c.listed_ok = dem;
@@ -1165,6 +1407,10 @@ bool rebuild_grow(Ctx& c, const Status& s) {
     }
     if (s.overflow & 8u) {
         c.tile_blocked = true;  // a candidate list beyond 16-bit indices
+        again = true;
+    }
+    if (s.overflow & 16u) {
+        c.hk_cap = int64_t(s.hist_records) + int64_t(s.hist_records) / 4 + 1024;
         again = true;
     }
     return again;
@@ -1324,6 +1570,13 @@ void release_graphs(Ctx& c) {
 // One rebuild. A first attempt (not a retry) is recorded and issued as a
 // graph: its ~30 launches reach the device at once instead of one host
 // launch at a time.
+void launch_sort_check(Ctx& c, int step) {
+    if (c.n < 2) return;
+    launch(c, k_sort_check, blocks_for(c.n, 256), 256, c.n, static_cast<const double4*>(c.pos.p),
+           c.grid, c.sysp_or_null(), static_cast<const uint32_t*>(c.psys.p), c.status.p, step);
+    MDD_HIP(hipGetLastError());
+}
+
 void rebuild_async(Ctx& c, int step, bool retry) {
     const bool timed = c.prof_every > 0 && c.rb_pending < int64_t(c.rb_ev.size() / 3);
     if (timed) {

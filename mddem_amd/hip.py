@@ -124,6 +124,8 @@ SIGNATURES = {
                                    C.c_int]),
     "mdd_set_rebuild_on_box_exit": (C.c_int, [_vp, C.c_int]),
     "mdd_set_history_mode": (C.c_int, [_vp, C.c_int]),
+    "mdd_set_history_remap": (C.c_int, [_vp, C.c_int]),
+    "mdd_set_dem_split": (C.c_int, [_vp, C.c_int]),
     "mdd_set_virial": (C.c_int, [_vp, C.c_int]),
     "mdd_set_dt": (C.c_int, [_vp, C.c_double]),
     "mdd_set_materials": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
@@ -258,6 +260,14 @@ class System:
     def set_virial(self, every: int):
         """DEM contact virial on every `every`-th step's force pass (0: off)."""
         self._c(self.L.mdd_set_virial(self.h, int(every)))
+
+    def set_dem_split(self, mode: int):
+        """DEM pass at one lane per particle: -1 auto, 0 one kernel, 1 split."""
+        self._c(self.L.mdd_set_dem_split(self.h, int(mode)))
+
+    def set_history_remap(self, method: int):
+        """0: sorted 64-bit pair keys (default), 1: search of each old row."""
+        self._c(self.L.mdd_set_history_remap(self.h, int(method)))
 
     def set_history_mode(self, mode: int):
         """0: history kept per owning particle (the reference), 1: per pair."""
@@ -575,7 +585,7 @@ def from_workload(w, precision=FP64, box_exit=True, sort_every=1000, device=0,
     (GranularDefaultPlugins / LJDefaultPlugins) for a workloads.Workload."""
     s = System(MODEL_DEM if w.model == "dem" else MODEL_LJ, precision, device)
     s.set_box(w.lo, w.hi, w.periodic)
-    s.set_neighbor(w.skin_fraction, w.bin_size, 0, True, sort_every, True)
+    s.set_neighbor(w.skin_fraction, w.bin_size, getattr(w, "every", 0), True, sort_every, True)
     s.set_rebuild_on_box_exit(box_exit)
     s.set_dt(w.dt)
     if w.model == "dem":

@@ -102,6 +102,7 @@ def nve_drift(w, steps=20000, samples=20, equilibrate=1000):
         s.set_nose_hoover(0.0, 1.0, enabled=False)
         s.set_integrators(True, False)
         s.set_forces(force)
+        s.bin_neighbor_list()  # the energy pass needs the list (the forces move with the sort)
         t, e = [], []
         per = max(1, steps // samples)
         for k in range(samples + 1):

@@ -96,6 +96,7 @@ class Workload:
     walls: list = field(default_factory=list)  # (point, normal, material)
     steps: int = 0
     density: float | None = None
+    every: int = 0                  # [neighbor] every (0: displacement rebuilds only)
 
     @property
     def n(self) -> int:
@@ -460,16 +461,19 @@ def _deposited_bed(n, Lx, Ly, rmin, rmax, stream, overlap=5e-8, gap=1.2, look=4)
 
 
 def dem_packed_bed(n: int = 4_000_000, box=(0.4, 0.4, 1.2), rmin: float = 0.8e-3,
-                   rmax: float = 1.2e-3, vmax: float = 0.1, dt_factor: float = 0.05,
-                   seed: int | None = None, steps: int = 50000) -> Workload:
+                   rmax: float = 1.2e-3, vmax: float = 0.0, overlap: float = 1e-8,
+                   dt_factor: float = 0.05, seed: int | None = None, steps: int = 50000,
+                   every: int = 0) -> Workload:
     """The settling workload's spheres, materials, walls and gravity, started
-    from a resting bed on the floor (a deposited packing, `_deposited_bed`)
-    instead of a dilute cloud: every sphere is in contact from the first
-    step. Velocities are uniform in [-vmax, vmax] per component."""
+    from a bed resting on the floor (a deposited packing, `_deposited_bed`)
+    instead of a dilute cloud: every sphere touches the spheres it rests on,
+    each contact overlapping by `overlap` (10 nm: about the weight of a
+    sphere, so the bed stays put). Velocities are uniform in [-vmax, vmax]
+    per component (0: at rest). every: the [neighbor] every of the runs."""
     seed = SEED_BASE + 5 if seed is None else seed
     st = Stream(seed)
     lo, hi = np.zeros(3), np.asarray(box, dtype=np.float64)
-    pos, r = _deposited_bed(n, hi[0], hi[1], rmin, rmax, st)
+    pos, r = _deposited_bed(n, hi[0], hi[1], rmin, rmax, st, overlap=overlap)
     if pos[:, 2].max() + rmax >= hi[2]:
         raise ValueError("bed higher than the box")
     vel = (st.uniform(3 * n).reshape(n, 3) * 2.0 - 1.0) * vmax
@@ -481,4 +485,4 @@ def dem_packed_bed(n: int = 4_000_000, box=(0.4, 0.4, 1.2), rmin: float = 0.8e-3
         pos=pos, vel=vel, mass=dem_mass(dens, r), cutoff_radius=r.copy(), radius=r,
         omega=np.zeros((n, 3)), atype=np.zeros(n, np.uint32), dt=dt_factor * t_r,
         skin_fraction=1.1, bin_size=0.0, materials=glass_materials(),
-        gravity=(0.0, 0.0, -9.81), walls=walls, steps=steps, density=dens)
+        gravity=(0.0, 0.0, -9.81), walls=walls, steps=steps, density=dens, every=every)

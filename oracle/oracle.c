@@ -122,6 +122,9 @@ struct orc_sys {
     double (*quat)[4], (*omega)[3], (*torque)[3];
     hist_list* hist;
     int history_mode;
+    /* 1: sort_atoms_by_bin permutes only on a rebuild step or a periodic
+     * sort (see orc_set_sort_only_on_rebuild) */
+    int sort_only_on_rebuild;
     /* VirialStress (mddem_core/src/virial.rs:19-68) of the last contact
      * force pass, accumulated when active */
     int virial_on;
@@ -970,6 +973,7 @@ void orc_wall_zero_force_accumulators(orc_sys* s) {
 }
 
 void orc_set_history_mode(orc_sys* s, int mode) { s->history_mode = mode; }
+void orc_set_sort_only_on_rebuild(orc_sys* s, int on) { s->sort_only_on_rebuild = on ? 1 : 0; }
 
 double orc_dem_mass(double density, double radius) {
     /* insert_single_particle (dem_atom/src/insert.rs:340) */
@@ -1408,6 +1412,12 @@ void orc_sort_atoms_by_bin(orc_sys* s) {
         return;
     }
     int rebuild_needed = needs_rebuild(s);
+    /* With every > 0, decide_rebuild has just counted this step, so
+     * needs_rebuild can turn true here while the step keeps its list: the
+     * reference then permutes the atoms under a list that still holds the
+     * old indices (neighbor.rs:715-717). sort_only_on_rebuild skips that
+     * permutation. */
+    if (s->sort_only_on_rebuild && s->state != ORC_STATE_FULL_REBUILD) rebuild_needed = 0;
     if (!periodic_sort && !rebuild_needed) return;
     double ibx = 1.0 / s->bin_size[0], iby = 1.0 / s->bin_size[1], ibz = 1.0 / s->bin_size[2];
     keyidx* ki = xrealloc(NULL, sizeof(keyidx) * n);
@@ -1595,6 +1605,7 @@ int64_t orc_npairs(const orc_sys* s) { return s->nidx; }
 int64_t orc_get_pairs(const orc_sys* s, uint32_t* owner_tag, uint32_t* partner_tag,
                       uint8_t* code) {
     int64_t k = 0;
+    if (!s->offsets) return 0;  /* no list built yet */
     for (int i = 0; i < s->nlocal; i++)
         for (uint32_t m = s->offsets[i]; m < s->offsets[i + 1]; m++, k++) {
             uint32_t j = s->indices[m];

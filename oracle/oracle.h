@@ -114,6 +114,12 @@ int orc_get_walls(const orc_sys* s, double* force_acc, double* point, double* ve
 void orc_wall_move(orc_sys* s);                        /* dem_wall lib.rs:766-815 */
 void orc_wall_zero_force_accumulators(orc_sys* s);     /* dem_wall lib.rs:821-834 */
 void orc_set_history_mode(orc_sys* s, int mode);
+/* 0 (default): sort_atoms_by_bin as the reference runs it. 1: its
+ * non-periodic permutation only on a rebuild step. With every > 0 the
+ * reference re-evaluates needs_rebuild after decide_rebuild has counted the
+ * step, so at the step before a count rebuild it permutes the atoms under
+ * the old list (neighbor.rs:639-674, 715-717); 1 leaves that step alone. */
+void orc_set_sort_only_on_rebuild(orc_sys* s, int on);
 
 /* ---- state ---------------------------------------------------------------
  * pos/vel/omega: n*3 row-major; omega may be NULL. radius may be NULL for LJ.

@@ -97,6 +97,7 @@ def lib():
             "orc_lj_force": (None, [vp]),
             "orc_lj_virial": (None, [vp, _dp]),
             "orc_set_virial": (None, [vp, C.c_int]),
+            "orc_set_sort_only_on_rebuild": (None, [vp, C.c_int]),
             "orc_granular_temperature": (None, [vp, _dp]),
             "orc_contact_virial": (None, [vp, _dp]),
             "orc_lj_tails": (None, [vp, _dp, _dp]),
@@ -300,6 +301,9 @@ class Oracle:
         acc, pt, vel = np.zeros(n), np.zeros((n, 3)), np.zeros((n, 3))
         self.L.orc_get_walls(self.h, _ptr(acc), _ptr(pt), _ptr(vel))
         return dict(force=acc, point=pt, velocity=vel)
+
+    def set_sort_only_on_rebuild(self, on=True):
+        self.L.orc_set_sort_only_on_rebuild(self.h, int(bool(on)))
 
     def set_history_mode(self, mode):
         self.L.orc_set_history_mode(self.h, mode)

@@ -20,7 +20,11 @@ def kats():
 def oracle_from_workload(w, history_mode=HIST_REFERENCE, sort_every=1000):
     o = Oracle(MODEL_DEM if w.model == "dem" else MODEL_LJ)
     o.set_box(w.lo, w.hi, w.periodic)
-    o.set_neighbor(w.skin_fraction, w.bin_size, 0, True, sort_every, True)
+    o.set_neighbor(w.skin_fraction, w.bin_size, getattr(w, "every", 0), True, sort_every, True)
+    # with every > 0 the reference permutes the atoms under the old list on
+    # the step before a count rebuild; the HIP path keeps the list and the
+    # order until the rebuild, and the checker is compared with that
+    o.set_sort_only_on_rebuild(getattr(w, "every", 0) > 0)
     o.set_dt(w.dt)
     if w.model == "dem":
         m = w.materials

@@ -606,3 +606,41 @@ def test_granular_temperature_hand_computed():
                 [[6, 5, 5], [5, 6, 5], [5, 5, 4]], [1.0, 2.0, 3.0], r, radius=r)
     assert o.granular_temperature()["granular_temperature"] == pytest.approx(11.0 / 54.0,
                                                                             rel=1e-13)
+
+
+def _pair_keys(o):
+    own, part, code = o.pairs()
+    return set(zip(np.minimum(own, part).tolist(), np.maximum(own, part).tolist()))
+
+
+def test_count_rebuild_policy_sort_without_rebuild():
+    # needs_rebuild with every > 0 (neighbor.rs:605-627): decide_rebuild counts
+    # the step before sort_atoms_by_bin asks needs_rebuild again, so at step
+    # b + every the reference permutes the atoms (when any moved to another
+    # bin) while the list keeps the old indices (neighbor.rs:715-717); the
+    # next step rebuilds. On such a step the list names other tag pairs than
+    # it was built for. set_sort_only_on_rebuild leaves those steps alone;
+    # with every = 0 it changes nothing.
+    from tests.test_gpu_parity import _dense_dem
+    w = _dense_dem(n=1200, seed=17)
+    stale = {}
+    for fix in (False, True):
+        o = oracle_from_workload(w)
+        o.set_neighbor(w.skin_fraction, w.bin_size, 7, True, 1000, True)
+        o.set_sort_only_on_rebuild(fix)
+        stale[fix] = []
+        for k in range(1, 61):
+            tags, pairs, r0 = o.atoms()["tag"].copy(), _pair_keys(o), o.rebuilds
+            o.step()
+            if o.rebuilds == r0 and not np.array_equal(tags, o.atoms()["tag"]):
+                stale[fix].append(k)
+                assert _pair_keys(o) != pairs
+    assert stale[False] and all(k % 8 == 0 for k in stale[False])  # b + 7, b = 1, 9, ...
+    assert stale[True] == []
+    runs = []
+    for fix in (False, True):
+        o = oracle_from_workload(w)
+        o.set_sort_only_on_rebuild(fix)
+        o.run(60)
+        runs.append(o.atoms()["vel"])
+    assert np.array_equal(runs[0], runs[1])
