@@ -250,6 +250,24 @@ def cpu_baseline(w, seconds_budget=12.0, replicas=None):
     return out
 
 
+def cpu_baseline_dem(w, seconds_budget=10.0):
+    """The CPU checker on the DEM line's workload (the full bed, same
+    neighbor settings) on 1 host thread: set-up and the first rebuild
+    untimed, then whole steps until the budget is spent."""
+    from tests.helpers import oracle_from_workload  # the checker; CPU leg only
+    o = oracle_from_workload(w)
+    o.step()
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds_budget:
+        o.run(1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return dict(value=w.n * steps / dt, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{steps} scheduled steps (after 1 untimed setup step) of the same "
+                       f"{w.n}-sphere bed on 1 host thread")
+
+
 def traffic_from_profiles(name, kernel):
     """Calibrated HBM bytes per launch of `kernel` from the committed counter
     summary for this workload, or None when it covers another kernel."""
@@ -346,6 +364,11 @@ def main():
             from mddem_amd import precision as P
             dem["precision"] = dict(force_error=P.dem_force_errors(wdem))
             log(f"[bench] dem precision: {json.dumps(dem['precision'])}")
+        if not args.no_cpu_baseline:
+            log("[bench] dem cpu baseline ...")
+            dem["cpu_baseline"] = cpu_baseline_dem(wdem, args.cpu_seconds)
+            dem["gpu_over_cpu"] = round(dem["mixed"]["value"] / dem["cpu_baseline"]["value"], 1)
+            log(f"[bench] dem cpu {dem['cpu_baseline']['value']:.4g} particle-steps/s")
         del wdem
 
     ens = None
