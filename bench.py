@@ -111,13 +111,15 @@ def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True, layout=Non
     return res
 
 
-def run_dem(steps, warmup, n=4_000_000, log_every=0, every=1000, tune_steps=200):
+def run_dem(steps, warmup, n=4_000_000, log_every=0, every=500, tune_steps=200):
     """DEM line: the settling workload's 4M polydisperse glass spheres, walls
-    and gravity, started as a resting bed (every sphere in contact, shaken at
-    the config's velocities, U[-0.1, 0.1] m/s), mixed and all-FP64 with the
-    same warmup and timed steps. Reports the mean half-list pairs k and
-    contacts c per sphere over the timed window and the whole-step roofline
-    with B_ps = S + 4 k + 56 c (S = 216 B mixed, 264 B FP64)."""
+    and gravity, started as a bed at rest (every sphere touching the spheres
+    it rests on), mixed and all-FP64 over the same window of the run: the
+    FP64 leg first runs, untimed, the steps the mixed leg spends choosing
+    the force pass's launch shape, then both take the same warmup and timed
+    steps. Reports the mean half-list pairs k and contacts c per sphere over
+    the timed window and the whole-step roofline with B_ps = S + 4 k + 56 c
+    (S = 216 B mixed, 264 B FP64)."""
     from mddem_amd import hip
     from mddem_amd import workloads as W
     t0 = time.perf_counter()
@@ -126,6 +128,7 @@ def run_dem(steps, warmup, n=4_000_000, log_every=0, every=1000, tune_steps=200)
         f"(top at z = {w.pos[:, 2].max():.4f} m)")
     out = {}
     shape = None
+    pre = 0
     for name, prec in (("mixed", hip.MIXED), ("fp64", hip.FP64)):
         s = hip.from_workload(w, prec, box_exit=False)
         if shape is None:
@@ -145,10 +148,13 @@ def run_dem(steps, warmup, n=4_000_000, log_every=0, every=1000, tune_steps=200)
                 trials.append((per, lg, split))
                 log(f"[bench]   dem tune: lanes {1 << lg}, split {split}: {per * 1e3:.4f} ms/step")
             _, lg, split = min(trials)
+            pre = s.thermo()["step"]
             shape = dict(lanes=1 << lg, split=split, trials=[
                 dict(lanes=1 << t[1], split=t[2], ms_per_step=round(t[0] * 1e3, 4)) for t in trials])
         s.set_force_lanes({1: 0, 2: 1, 4: 2}[shape["lanes"]])
         s.set_dem_split(shape["split"])
+        if s.thermo()["step"] < pre:
+            s.run(pre - s.thermo()["step"])
         s.run(warmup)
         s.synchronize()
         th0 = s.thermo()
@@ -193,19 +199,21 @@ def run_dem(steps, warmup, n=4_000_000, log_every=0, every=1000, tune_steps=200)
     out["force_pass_shape"] = shape
     out["workload"] = (f"DEM settling spheres as a bed at rest: {w.n} glass spheres "
                        "r~U[0.8,1.2] mm deposited on z=0 (each resting on the spheres below, "
-                       "10 nm contact overlaps), box 0.4x0.4x1.2 m periodic x/y, plane walls "
+                       "0.1 nm contact overlaps), box 0.4x0.4x1.2 m periodic x/y, plane walls "
                        "z=0 and z=1.2 m, gravity, dt=0.05 t_Rayleigh(0.8 mm), "
                        f"[neighbor] every={every} check=true")
     out["steps"] = steps
     out["warmup"] = warmup
+    out["steps_before_warmup"] = pre
     return out, w
 
 
-def force_kernel_bytes(n, npairs, force_bytes):
+def force_kernel_bytes(n, npairs, force_bytes, pos_bytes=24.0):
     """Algorithmic HBM bytes of one pair-force launch: positions read once
-    (24 B FP64), the half list streamed once (4 B per pair) and the force
-    written once (3 x 4 B mixed, 3 x 8 B FP64)."""
-    return n * 24.0 + 4.0 * npairs + n * 3.0 * force_bytes
+    (24 B FP64 x, y, z; 16 B for the mixed pass's fixed-point int4), the
+    half list streamed once (4 B per pair) and the force written once
+    (3 x 4 B mixed, 3 x 8 B FP64)."""
+    return n * pos_bytes + 4.0 * npairs + n * 3.0 * force_bytes
 
 
 def _cpu_sample(args):
@@ -304,9 +312,13 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs in the serving-stream measurement (0 = skip)")
-    ap.add_argument("--dem-steps", type=int, default=6000,
+    ap.add_argument("--dem-steps", type=int, default=3000,
                     help="timed steps of the DEM 4M line per precision (0 = skip)")
     ap.add_argument("--dem-warmup", type=int, default=200)
+    ap.add_argument("--dem-tune-steps", type=int, default=200,
+                    help="steps timed per launch shape of the DEM force pass")
+    ap.add_argument("--dem-every", type=int, default=500,
+                    help="[neighbor] every of the DEM line (rebuilds in the timed window)")
     ap.add_argument("--serve-steps", type=int, nargs=2, default=(1000, 50000),
                     help="step range of serving jobs (log-uniform)")
     args = ap.parse_args()
@@ -341,7 +353,9 @@ def main():
             f"{r64['force_ms'] * 1e3:.1f} us")
 
     fbytes = 4 if prec == hip.MIXED else 8
-    alg = force_kernel_bytes(r["n"], r["npairs"], fbytes)
+    # the mixed row pass gathers fixed-point positions (mdd_set_lj_positions)
+    pbytes = 16.0 if prec == hip.MIXED and not r["tile"] else 24.0
+    alg = force_kernel_bytes(r["n"], r["npairs"], fbytes, pbytes)
     achieved = alg / (r["force_ms"] * 1e-3) / 1e9 if r["force_ms"] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -359,7 +373,8 @@ def main():
 
     dem = None
     if args.dem_steps > 0:
-        dem, wdem = run_dem(args.dem_steps, args.dem_warmup, log_every=1000)
+        dem, wdem = run_dem(args.dem_steps, args.dem_warmup, log_every=500,
+                            every=args.dem_every, tune_steps=args.dem_tune_steps)
         if args.precision_report > 0:
             from mddem_amd import precision as P
             dem["precision"] = dict(force_error=P.dem_force_errors(wdem))

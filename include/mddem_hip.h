@@ -320,6 +320,22 @@ int mdd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
  * always one kernel; 1 always split. Identical forces. No reference
  * counterpart: launch-shape tuning. */
 int mdd_set_dem_split(mdd_ctx* ctx, int mode);
+/* DEM pair pass (single systems, rows of any lane count) in tile order:
+ * on (default) the particles are visited band of bin columns by band (16
+ * columns along y at each x, x after x), so that the partners gathered on
+ * one XCD stay in its L2; off: index order. Identical forces. Turning it
+ * on invalidates the list until the next rebuild builds the order. No
+ * reference counterpart: launch-order tuning. */
+int mdd_set_dem_tiles(mdd_ctx* ctx, int on);
+/* LJ pair pass of MDD_MIXED contexts (rows, single system, every box
+ * length at least twice the list cutoff): 1 (default) gathers fixed-point
+ * positions (x * 2^32 / L per axis, one 16-byte load per neighbor; 32-bit
+ * differences wrap to the minimum image) and computes the pair in FP32,
+ * deciding pairs within 1e-6 of the cutoff in FP64 from the same fixed-
+ * point differences; 0 gathers the FP64 positions and computes the
+ * geometry in FP64. No reference counterpart: precision/speed choice of
+ * the mixed path. */
+int mdd_set_lj_positions(mdd_ctx* ctx, int mode);
 /* Pair-force layout: log2 lanes per particle of the row kernels (0..3;
  * default 2), or -1 for the LJ cell-tile kernel (16-bit rows into each bin
  * cell's candidate list staged in LDS; LJ contexts only, others keep rows).

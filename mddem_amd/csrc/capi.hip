@@ -314,6 +314,7 @@ void pack_upload(Ctx& c, int64_t n, const uint32_t* tag, const uint32_t* type, c
     int64_t cap = std::max<int64_t>(n, c.cap);
     c.cap = cap;
     c.n = n;
+    c.xq_valid = false;
     c.npad = (cap + 63) / 64 * 64;
     const int64_t np = c.npad;
     c.pos.resize(np);
@@ -559,6 +560,7 @@ int mdd_set_box(mdd_ctx* ctx, const double lo[3], const double hi[3], const int 
         }
         c.setup_done = false;
         c.list_valid = false;
+        c.xq_valid = false;
     });
 }
 
@@ -593,6 +595,21 @@ int mdd_set_dem_split(mdd_ctx* ctx, int mode) {
     return guarded(ctx, [&](Ctx& c) {
         if (mode < -1 || mode > 1) throw ApiError(MDD_ERR_INVALID, "split mode is -1, 0 or 1");
         c.dem_split = mode;
+    });
+}
+
+int mdd_set_lj_positions(mdd_ctx* ctx, int mode) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (mode != 0 && mode != 1) throw ApiError(MDD_ERR_INVALID, "position mode is 0 or 1");
+        c.lj_fixed = mode == 1;
+        c.xq_valid = false;
+    });
+}
+
+int mdd_set_dem_tiles(mdd_ctx* ctx, int on) {
+    return guarded(ctx, [&](Ctx& c) {
+        c.tile_order = on != 0;
+        if (c.tile_order && !c.tile_ok) c.list_valid = false;  // the next pass rebuilds
     });
 }
 

@@ -103,6 +103,12 @@ struct BoxP {
     int periodic[3];
 };
 
+// A position in fixed point: x * 2^32 / L per axis, rounded to the nearest
+// integer and kept modulo 2^32 (qs[d] = 2^32 / L[d]).
+__host__ __device__ inline int quantize_axis(double x, double qs) {
+    return int(uint32_t(uint64_t(llrint(x * qs))));
+}
+
 struct GridP {
     double origin[3], inv_bs[3];
     int n[3];
@@ -378,6 +384,12 @@ struct Ctx {
     // fewer than half the particles have neighbors (Status::listed of the
     // last build the host has read), 0 one kernel, 1 always split
     int dem_split = -1;
+    // DEM pair pass in tile order (single systems): the particles listed
+    // tile by tile, a tile being kTileCols bin columns along y at one x,
+    // tiles x-major inside each band of columns; rebuilt with the list
+    DBuf<uint32_t> tile_list, tile_cnt, tile_off;
+    bool tile_order = true;  // mdd_set_dem_tiles
+    bool tile_ok = false;
 
     // ---- particle arrays (current order)
     DBuf<double4> pos, vel, omega, quat;  // pos.w radius, vel.w 1/m, omega.w 1/I
@@ -415,6 +427,21 @@ struct Ctx {
     // a candidate list outgrew 16-bit indices
     bool tile_enabled = false, tile_blocked = false, list_tile = false;
     bool want_tile() const { return model == 0 && nsys == 0 && tile_enabled && !tile_blocked; }
+    // LJ mixed row pass on fixed-point positions: each coordinate in units
+    // of L / 2^32 of its axis, so that the 32-bit difference of two
+    // particles wraps to the minimum image (every axis at least twice the
+    // list cutoff: the listed image is the nearest). The step kernels keep
+    // xq current while xq_valid; a rebuild or an upload moves the particles
+    // otherwise and clears it, and the next pair pass refills it.
+    DBuf<int4> xq;
+    bool lj_fixed = true;  // mdd_set_lj_positions
+    bool xq_valid = false;
+    bool xq_usable() const {
+        if (!(model == 0 && precision == 1 && nsys == 0 && lj_fixed && !list_tile)) return false;
+        for (int d = 0; d < 3; d++)
+            if (!(box.L[d] > 2.0 * max_cutoff)) return false;
+        return true;
+    }
     bool last_force_fused = false;  // the last scheduled force pass also integrated
     int64_t fused_parts = 0;        // its kinetic-energy partials
     // the final half step of step `step` is deferred into the next step's

@@ -79,6 +79,9 @@ struct ForceArgs {
     int step;
     const SysP* sysp;     // batched ensemble: per-system box and materials, or null
     const uint32_t* psys;
+    const int4* xq;            // LJ mixed: fixed-point positions, or null
+    float qf[3];               // their unit L / 2^32 per axis, FP32 and FP64
+    double qd[3];
     const uint32_t* listed;    // DEM: particles with a nonempty row (last rebuild)
     const uint32_t* listed_n;  // their count
 };
@@ -144,6 +147,70 @@ struct alignas(16) ShiftPair {
     double j[3], i[3], pad[2];
 };
 
+// Mixed LJ row of particle i (lane t of G) on fixed-point positions: one
+// 16-byte gather per neighbor; the 32-bit coordinate differences are the
+// minimum-image separation in units of L / 2^32, so image entries need no
+// shift. FP32 pair arithmetic; a pair within 1e-6 of the cutoff (the FP32
+// r^2 is good to ~3e-7) is decided on the FP64 r^2 of the same differences.
+template <int LG>
+__device__ inline void lj_row_fixed(const ForceArgs& a, int64_t i, int t) {
+    constexpr int G = 1 << LG;
+    constexpr int kB = MDD_LJ_BATCH;
+    int cnt = int(a.nnb[i]);
+    if (cnt > a.maxnb) cnt = a.maxnb;
+    const uint32_t* row = a.nbr + (i << LG) + t;
+    const int64_t stride = a.npad << LG;
+    const int4 qi = a.xq[i];
+    const float sx = a.qf[0], sy = a.qf[1], sz = a.qf[2];
+    const float cut2 = float(a.cut2), lj1 = float(a.lj1), lj2 = float(a.lj2);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    uint32_t en[kB];
+#pragma unroll
+    for (int u = 0; u < kB; u++) {
+        const int k = t + u * G;
+        en[u] = k < cnt ? row[int64_t(k >> LG) * stride] : 0u;
+    }
+    for (int kb = t; kb < cnt; kb += kB * G) {
+        int4 qj[kB];
+#pragma unroll
+        for (int u = 0; u < kB; u++) qj[u] = a.xq[entry_idx(en[u])];
+#pragma unroll
+        for (int u = 0; u < kB; u++) {
+            const int k = kb + (kB + u) * G;
+            en[u] = k < cnt ? row[int64_t(k >> LG) * stride] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kB; u++) {
+            const int ix = int(uint32_t(qj[u].x) - uint32_t(qi.x));
+            const int iy = int(uint32_t(qj[u].y) - uint32_t(qi.y));
+            const int iz = int(uint32_t(qj[u].z) - uint32_t(qi.z));
+            const float dx = float(ix) * sx, dy = float(iy) * sy, dz = float(iz) * sz;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const bool valid = kb + u * G < cnt;
+            bool in = valid && r2 < cut2;
+            if (fabsf(r2 - cut2) <= 1e-6f * cut2) {
+                const double ex = double(ix) * a.qd[0], ey = double(iy) * a.qd[1],
+                             ez = double(iz) * a.qd[2];
+                in = valid && fma(ex, ex, fma(ey, ey, ez * ez)) < a.cut2;
+            }
+            const float r2inv = __builtin_amdgcn_rcpf(r2);
+            const float r6inv = r2inv * r2inv * r2inv;
+            const float fpair = r2inv * r6inv * fmaf(lj1, r6inv, -lj2);
+            const float sc = in ? -fpair : 0.0f;
+            fx = fmaf(sc, dx, fx);
+            fy = fmaf(sc, dy, fy);
+            fz = fmaf(sc, dz, fz);
+        }
+    }
+    fx = group_sum<G>(fx);
+    fy = group_sum<G>(fy);
+    fz = group_sum<G>(fz);
+    if (t != 0) return;
+    a.f32[i] = fx;
+    a.f32[a.npad + i] = fy;
+    a.f32[2 * a.npad + i] = fz;
+}
+
 // lj_force (md_lj/src/lib.rs:488-633): for each neighbor inside the cutoff,
 // fpair = r^-2 r^-6 (lj1 r^-6 - lj2), owner gets fma(-fpair, d, f),
 // partner fma(fpair, d, f), d = partner image - owner.
@@ -186,6 +253,12 @@ __global__ void __launch_bounds__(256) k_lj(ForceArgs a) {
     __syncthreads();
     if (i >= a.n) return;
     if (gated(a)) return;
+    if constexpr (sizeof(R) == 4 && !ENERGY) {
+        if (a.xq) {
+            lj_row_fixed<LG>(a, i, t);
+            return;
+        }
+    }
     const double4 xi = a.pos[i];
     int cnt = int(a.nnb[i]);
     if (cnt > a.maxnb) cnt = a.maxnb;
@@ -1029,6 +1102,20 @@ __global__ void __launch_bounds__(256) k_hertz(ForceArgs a) {
     hertz_particle<R, LG, VIR>(a, i, t);
 }
 
+// The same pass over the particles in tile order (order[k], k < n): each
+// XCD works through a band of bin columns, so the partners it gathers stay
+// in its L2; each particle's arithmetic is that of k_hertz.
+template <typename R, int LG, bool VIR>
+__global__ void __launch_bounds__(256) k_hertz_tiles(ForceArgs a, const uint32_t* __restrict__ order) {
+    constexpr int G = 1 << LG;
+    const int64_t gt = int64_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    const int64_t k = gt >> LG;
+    const int t = int(gt & (G - 1));
+    if (k >= a.n) return;
+    if (gated(a)) return;
+    hertz_particle<R, LG, VIR>(a, int64_t(order[k]), t);
+}
+
 // The same pass split by row length (one lane per particle): the particles
 // whose row is not empty, listed at the last rebuild, run the pair kernel
 // above (grid-stride over the list); every other particle has no pair
@@ -1478,11 +1565,45 @@ void launch_force_kernel(Ctx& c, K kern, int nb, const ForceArgs& a, hipEvent_t 
     else
         kern<<<nb, 256, 0, c.stream>>>(a);
 }
+template <typename K>
+void launch_force_kernel(Ctx& c, K kern, int nb, const ForceArgs& a, hipEvent_t e0,
+                         hipEvent_t e1, const uint32_t* order) {
+    if (e0)
+        hipExtLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c.stream, e0, e1, 0, a, order);
+    else
+        kern<<<nb, 256, 0, c.stream>>>(a, order);
+}
+
+// fixed-point positions of every particle (x * qs per axis, modulo 2^32)
+__global__ void k_fill_xq(int64_t n, const double4* __restrict__ pos, int4* __restrict__ xq,
+                          double qx, double qy, double qz) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = pos[i];
+    xq[i] = make_int4(quantize_axis(p.x, qx), quantize_axis(p.y, qy), quantize_axis(p.z, qz), 0);
+}
 
 template <typename R, bool E>
-void launch_lj(Ctx& c, const ForceArgs& a, hipEvent_t e0, hipEvent_t e1) {
+void launch_lj(Ctx& c, const ForceArgs& a0, hipEvent_t e0, hipEvent_t e1) {
     const int64_t threads = c.n << c.list_lg;
     const int nb = blocks_for(threads, 256);
+    ForceArgs a = a0;
+    if (sizeof(R) == 4 && !E && c.xq_usable()) {
+        const double two32 = 4294967296.0;
+        if (!c.xq_valid) {
+            c.xq.resize(size_t(c.npad));
+            k_fill_xq<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(
+                c.n, static_cast<const double4*>(c.pos.p), c.xq.p, two32 / c.box.L[0],
+                two32 / c.box.L[1], two32 / c.box.L[2]);
+            MDD_HIP(hipGetLastError());
+            c.xq_valid = true;
+        }
+        a.xq = c.xq.p;
+        for (int d = 0; d < 3; d++) {
+            a.qd[d] = c.box.L[d] / two32;
+            a.qf[d] = float(a.qd[d]);
+        }
+    }
     switch (c.list_lg) {
         case 0: launch_force_kernel(c, k_lj<R, E, 0>, nb, a, e0, e1); break;
         case 1: launch_force_kernel(c, k_lj<R, E, 1>, nb, a, e0, e1); break;
@@ -1501,6 +1622,16 @@ void launch_hertz(Ctx& c, const ForceArgs& a, hipEvent_t e0, hipEvent_t e1) {
             case 1: launch_force_kernel(c, k_contact_ext<R, 1>, nb, a, e0, e1); break;
             case 3: launch_force_kernel(c, k_contact_ext<R, 3>, nb, a, e0, e1); break;
             default: launch_force_kernel(c, k_contact_ext<R, 2>, nb, a, e0, e1); break;
+        }
+        return;
+    }
+    if (c.tile_order && c.tile_ok && c.nsys == 0) {
+        const uint32_t* ord = c.tile_list.p;
+        switch (c.list_lg) {
+            case 0: launch_force_kernel(c, k_hertz_tiles<R, 0, VIR>, nb, a, e0, e1, ord); break;
+            case 1: launch_force_kernel(c, k_hertz_tiles<R, 1, VIR>, nb, a, e0, e1, ord); break;
+            case 3: launch_force_kernel(c, k_hertz_tiles<R, 3, VIR>, nb, a, e0, e1, ord); break;
+            default: launch_force_kernel(c, k_hertz_tiles<R, 2, VIR>, nb, a, e0, e1, ord); break;
         }
         return;
     }

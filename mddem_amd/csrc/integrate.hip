@@ -61,7 +61,14 @@ struct IntArgs {
     const uint32_t* psys;
     const WallsP* walls;    // walls (device copy) and their state, or null: scheduled
     WallsDyn* wdyn;         // steps move them (PreInitialIntegration) and clear the sums
+    int4* xq;               // fixed-point copy of the positions to keep current, or null
+    double qs[3];           // its scale 2^32 / L per axis
 };
+
+__device__ inline void store_xq(const IntArgs& a, int64_t i, const double4& p) {
+    a.xq[i] = make_int4(quantize_axis(p.x, a.qs[0]), quantize_axis(p.y, a.qs[1]),
+                        quantize_axis(p.z, a.qs[2]), 0);
+}
 
 template <typename F>
 __device__ inline void load_f(const IntArgs& a, int64_t i, double& fx, double& fy, double& fz) {
@@ -247,6 +254,7 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
         if (a.vv) kick_drift(a, v, p, fx, fy, fz);
         a.vel[i] = v;
         a.pos[i] = p;
+        if (a.xq) store_xq(a, i, p);
     }
     if (a.rot) {
         double tx, ty, tz;
@@ -335,6 +343,7 @@ __global__ void __launch_bounds__(kThreads) k_post_pre(IntArgs a) {
         kick_drift(a, v, p, fx, fy, fz);
         a.vel[i] = v;
         a.pos[i] = p;
+        if (a.xq) store_xq(a, i, p);
     }
     if (a.rot) {
         const double ii = w.w;
@@ -524,6 +533,12 @@ IntArgs make_args(Ctx& c, int step, bool check) {
     a.q_mass = c.nh_Q;
     a.sysp = c.sysp_or_null();
     a.psys = c.psys.p;
+    if (c.xq_usable() && c.xq_valid) {
+        a.xq = c.xq.p;
+        for (int d = 0; d < 3; d++) a.qs[d] = 4294967296.0 / c.box.L[d];
+    } else {
+        c.xq_valid = false;  // these launches may move particles without it
+    }
     return a;
 }
 

@@ -764,6 +764,77 @@ __global__ void __launch_bounds__(256) k_copy_image_rows(int64_t ghost_cap, int6
     }
 }
 
+// The particles whose row is not empty (the order is free: each particle's
+// force is computed by one thread alone). A workgroup lists kListPer x 256
+// particles into LDS and reserves their slots with one atomic: on the 4M
+// sphere bed a per-wavefront atomic on the one counter (62,500 of them,
+// serialized at the memory side) took 0.7 ms.
+constexpr int kListPer = 16;
+
+__global__ void __launch_bounds__(256) k_list_rows(int64_t n, const uint32_t* __restrict__ nnb,
+                                                   uint32_t* __restrict__ listed,
+                                                   uint32_t* __restrict__ count) {
+    __shared__ uint32_t buf[256 * kListPer];
+    __shared__ uint32_t wcount[4];
+    __shared__ uint32_t base_s;
+    const int w = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63u);
+    const int64_t b0 = int64_t(blockIdx.x) * (256 * kListPer);
+    uint32_t* mine = buf + w * 64 * kListPer;
+    uint32_t local = 0u;  // this wavefront's listed count (uniform)
+    for (int p = 0; p < kListPer; p++) {
+        const int64_t i = b0 + (int64_t(w) * kListPer + p) * 64 + lane;
+        const bool has = i < n && nnb[i] != 0u;
+        const uint64_t m = __ballot(has);
+        if (has) mine[local + lanes_before(m)] = uint32_t(i);
+        local += uint32_t(__popcll(m));
+    }
+    if (lane == 0) wcount[w] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        base_s = total ? atomicAdd(count, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = base_s;
+    for (int k = 0; k < w; k++) off += wcount[k];
+    for (uint32_t t = uint32_t(lane); t < local; t += 64u) listed[off + t] = mine[t];
+}
+
+// The DEM pair pass's particle order: a band of kTileCols bin columns
+// along y, x after x, band after band. Each (band, x) segment is the
+// particles of cells [x, y0 .. y1 - 1, all z], one contiguous range of the
+// bin-sorted particles; in flight on one XCD they and their neighbors span
+// about 13 x 18 columns instead of three whole x slabs.
+constexpr int kTileCols = 16;
+
+__global__ void k_tile_count(GridP g, const uint32_t* __restrict__ start,
+                             uint32_t* __restrict__ cnt, int64_t nseg) {
+    const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (s >= nseg) return;
+    const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int64_t band = s / nx, x = s - band * nx;
+    const int64_t y0 = band * kTileCols, y1 = y0 + kTileCols < ny ? y0 + kTileCols : ny;
+    cnt[s] = start[(x * ny + y1) * nz] - start[(x * ny + y0) * nz];
+}
+
+// one workgroup per segment
+__global__ void __launch_bounds__(256) k_tile_emit(GridP g, const uint32_t* __restrict__ start,
+                                                   const uint32_t* __restrict__ off,
+                                                   uint32_t* __restrict__ list) {
+    const int64_t s = blockIdx.x;
+    const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int64_t band = s / nx, x = s - band * nx;
+    const int64_t y0 = band * kTileCols;
+    const uint32_t lo = start[(x * ny + y0) * nz];
+    const uint32_t o = off[s], len = off[s + 1] - o;
+    for (uint32_t t = threadIdx.x; t < len; t += 256u) list[o + t] = lo + t;
+}
+
+// the listed count into the status, where the host reads it
+__global__ void k_note_listed(const uint32_t* __restrict__ count, Status* st) {
+    st->listed = *count;
+}
+
 // carry shear history across a rebuild: a new slot takes the old slot of the
 // same particle whose partner has the same tag (the reference keys history
 // by partner tag, tangential.rs:54-133, and the spring is stored in the
@@ -775,27 +846,6 @@ __global__ void __launch_bounds__(256) k_copy_image_rows(int64_t ghost_cap, int6
 // history in its owner's list only (contact.rs:358-364, 575-582), so when a
 // rebuild moves the pair into the partner's row the partner finds no entry
 // and the spring restarts from zero, and the old owner prunes its entry.
-// The particles whose row is not empty, appended in wavefront batches (the
-// order is free: each particle's force is computed by one thread alone)
-__global__ void __launch_bounds__(256) k_list_rows(int64_t n, const uint32_t* __restrict__ nnb,
-                                                   uint32_t* __restrict__ listed,
-                                                   uint32_t* __restrict__ count) {
-    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    const bool has = i < n && nnb[i] != 0u;
-    const uint64_t m = __ballot(has);
-    if (m == 0ull) return;
-    const int lead = __builtin_ctzll(m);
-    uint32_t base = 0u;
-    if (int(threadIdx.x & 63u) == lead) base = atomicAdd(count, uint32_t(__popcll(m)));
-    base = uint32_t(__shfl(int(base), lead, 64));
-    if (has) listed[base + lanes_before(m)] = uint32_t(i);
-}
-
-// the listed count into the status, where the host reads it
-__global__ void k_note_listed(const uint32_t* __restrict__ count, Status* st) {
-    st->listed = *count;
-}
-
 __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
                                 const uint32_t* __restrict__ nbr,
                                 const uint32_t* __restrict__ nnb,
@@ -829,7 +879,10 @@ __global__ void k_remap_history(int64_t n, int64_t npad, int maxnb, int lg,
                 const int64_t so = row_addr(oi, ko, npad_old, lg_old);
                 const uint32_t eo = nbr_old[so];
                 if (tag_old[entry_idx(eo)] == tp) {
-                    if (by_owner && entry_rev(eo) != entry_rev(e)) break;  // owner changed
+                    // by owner: only the old slot on the same side of the
+                    // pair (a pair the reference lists twice, through two
+                    // periodic images, has one slot on each side)
+                    if (by_owner && entry_rev(eo) != entry_rev(e)) continue;
                     for (int q = 0; q < nslot; q++) hist[q * cs + s] = hist_old[q * cso + so];
                     nm |= 1ull << k;
                     break;
@@ -969,8 +1022,14 @@ __global__ void k_hk_bucket_fill(const uint32_t* __restrict__ total_dev, uint32_
     out[bstart[b] + atomicAdd(&cursor[b], 1u)] = rec[r];
 }
 
-// each bucket sorted by hi (insertion sort: a bucket holds the few contacts
-// of one particle)
+// each bucket sorted by hi, then by old slot (insertion sort: a bucket
+// holds the few contacts of one particle). The old slot orders the records
+// of a pair listed twice (through two periodic images) whatever order the
+// fill's atomics left them in.
+__device__ inline bool rec_before(const HistRec& a, const HistRec& b) {
+    return a.hi < b.hi || (a.hi == b.hi && a.src < b.src);
+}
+
 __global__ void k_hk_bucket_sort(int64_t n, const uint32_t* __restrict__ bstart,
                                  HistRec* __restrict__ rec) {
     const int64_t b = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -979,7 +1038,7 @@ __global__ void k_hk_bucket_sort(int64_t n, const uint32_t* __restrict__ bstart,
     for (uint32_t a = s0 + 1; a < s1; a++) {
         const HistRec x = rec[a];
         uint32_t t = a;
-        while (t > s0 && rec[t - 1].hi > x.hi) {
+        while (t > s0 && rec_before(x, rec[t - 1])) {
             rec[t] = rec[t - 1];
             t--;
         }
@@ -1017,10 +1076,14 @@ __global__ void k_remap_sorted(int64_t n, int64_t npad, int maxnb, int lg,
             else
                 b = mid;
         }
-        if (a >= bstart[lo + 1]) continue;
+        // the first record of the pair (in old-slot order) whose owner is
+        // still the owner, with by_owner; else the first record
+        const uint32_t owner = entry_rev(e) ? j : uint32_t(i);
+        const uint32_t end = bstart[lo + 1];
+        for (; a < end && rec[a].hi == hi; a++)
+            if (!by_owner || rec[a].owner == owner) break;
+        if (a >= end || rec[a].hi != hi) continue;
         const HistRec h = rec[a];
-        if (h.hi != hi) continue;
-        if (by_owner && h.owner != (entry_rev(e) ? j : uint32_t(i))) continue;
         for (int q = 0; q < nslot; q++) hist[q * cs + s] = hist_old[q * cso + h.src];
         nm |= 1ull << k;
     }
@@ -1080,6 +1143,7 @@ void swap_alternates(Ctx& c);
 // new order and the alternates keep the previous one (tag2: the previous
 // tags, the source of the shear-history remap).
 void permute_all(Ctx& c, const uint32_t* order, int64_t n) {
+    c.xq_valid = false;
     const bool dem = c.model == 1;
     const bool mixed = c.precision == 1;
     PermArgs a;
@@ -1154,6 +1218,7 @@ void swap_alternates(Ctx& c) {
 // source of the shear-history remap
 void rebuild_prepare(Ctx& c) {
     if (!c.setup_done) neighbor_setup(c);
+    c.xq_valid = false;  // the wrap moves particles
     const int64_t n = c.n;
     const int64_t ncells = c.grid.ncells;
     const bool dem = c.model == 1;
@@ -1379,9 +1444,24 @@ void rebuild_attempt(Ctx& c) {
         c.listed.resize(size_t(c.npad));
         c.listed_n.resize(1);
         fill_u32(c, c.listed_n.p, 0u, 1);
-        launch(c, k_list_rows, blocks_for(n, 256), 256, n, c.nnb.p, c.listed.p, c.listed_n.p);
+        launch(c, k_list_rows, blocks_for(n, 256 * kListPer), 256, n, c.nnb.p, c.listed.p,
+               c.listed_n.p);
         launch(c, k_note_listed, 1, 1, static_cast<const uint32_t*>(c.listed_n.p), c.status.p);
         MDD_HIP(hipGetLastError());
+    }
+    c.tile_ok = false;
+    if (dem && c.nsys == 0 && c.tile_order && n > 0) {
+        const int64_t nseg = int64_t(c.grid.n[0]) * ((c.grid.n[1] + kTileCols - 1) / kTileCols);
+        c.tile_list.resize(size_t(n));
+        c.tile_cnt.resize(size_t(nseg) + 1);
+        c.tile_off.resize(size_t(nseg) + 1);
+        launch(c, k_tile_count, blocks_for(nseg, 256), 256, c.grid,
+               static_cast<const uint32_t*>(c.start.p), c.tile_cnt.p, nseg);
+        exclusive_scan(c, c.tile_cnt.p, c.tile_off.p, nseg);
+        launch(c, k_tile_emit, int(nseg), 256, c.grid, static_cast<const uint32_t*>(c.start.p),
+               static_cast<const uint32_t*>(c.tile_off.p), c.tile_list.p);
+        MDD_HIP(hipGetLastError());
+        c.tile_ok = true;
     }
     c.listed_ok = dem;
     c.list_valid = true;

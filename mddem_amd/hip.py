@@ -126,6 +126,8 @@ SIGNATURES = {
     "mdd_set_history_mode": (C.c_int, [_vp, C.c_int]),
     "mdd_set_history_remap": (C.c_int, [_vp, C.c_int]),
     "mdd_set_dem_split": (C.c_int, [_vp, C.c_int]),
+    "mdd_set_dem_tiles": (C.c_int, [_vp, C.c_int]),
+    "mdd_set_lj_positions": (C.c_int, [_vp, C.c_int]),
     "mdd_set_virial": (C.c_int, [_vp, C.c_int]),
     "mdd_set_dt": (C.c_int, [_vp, C.c_double]),
     "mdd_set_materials": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
@@ -264,6 +266,14 @@ class System:
     def set_dem_split(self, mode: int):
         """DEM pass at one lane per particle: -1 auto, 0 one kernel, 1 split."""
         self._c(self.L.mdd_set_dem_split(self.h, int(mode)))
+
+    def set_lj_positions(self, mode: int):
+        """LJ mixed pair pass: 1 fixed-point positions (default), 0 FP64 positions."""
+        self._c(self.L.mdd_set_lj_positions(self.h, int(mode)))
+
+    def set_dem_tiles(self, on: bool):
+        """DEM pair pass in tile order (bands of bin columns) or index order."""
+        self._c(self.L.mdd_set_dem_tiles(self.h, int(bool(on))))
 
     def set_history_remap(self, method: int):
         """0: sorted 64-bit pair keys (default), 1: search of each old row."""

@@ -461,15 +461,17 @@ def _deposited_bed(n, Lx, Ly, rmin, rmax, stream, overlap=5e-8, gap=1.2, look=4)
 
 
 def dem_packed_bed(n: int = 4_000_000, box=(0.4, 0.4, 1.2), rmin: float = 0.8e-3,
-                   rmax: float = 1.2e-3, vmax: float = 0.0, overlap: float = 1e-8,
+                   rmax: float = 1.2e-3, vmax: float = 0.0, overlap: float = 1e-10,
                    dt_factor: float = 0.05, seed: int | None = None, steps: int = 50000,
                    every: int = 0) -> Workload:
     """The settling workload's spheres, materials, walls and gravity, started
     from a bed resting on the floor (a deposited packing, `_deposited_bed`)
     instead of a dilute cloud: every sphere touches the spheres it rests on,
-    each contact overlapping by `overlap` (10 nm: about the weight of a
-    sphere, so the bed stays put). Velocities are uniform in [-vmax, vmax]
-    per component (0: at rest). every: the [neighbor] every of the runs."""
+    each contact overlapping by `overlap`. The default, 0.1 nm, carries
+    about 1/700 of a sphere's weight, so the bed starts touching and gravity
+    loads its contacts (a 10 nm overlap carries about 1.4 weights and the
+    bed springs apart). Velocities are uniform in [-vmax, vmax] per
+    component (0: at rest). every: the [neighbor] every of the runs."""
     seed = SEED_BASE + 5 if seed is None else seed
     st = Stream(seed)
     lo, hi = np.zeros(3), np.asarray(box, dtype=np.float64)
