@@ -29,6 +29,8 @@ extern "C" {
 #define MDD_ERR_CAPACITY (-4)    /* allocation failed */
 #define MDD_ERR_UNSUPPORTED (-5) /* configuration outside this path */
 #define MDD_ERR_NO_DEVICE (-6)   /* no usable gfx950 device */
+#define MDD_ERR_STALLED (-7)     /* a step asked for a rebuild again right after
+                                    its own rebuild (internal inconsistency) */
 
 enum { MDD_MODEL_LJ = 0, MDD_MODEL_DEM = 1 };
 enum { MDD_FP64 = 0, MDD_MIXED = 1 };

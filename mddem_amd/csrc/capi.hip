@@ -207,6 +207,12 @@ void run_steps(Ctx& c, int64_t nsteps) {
         } else {
             c.retries = 0;
         }
+        // a step that asks for a rebuild again right after its own rebuild
+        // (without a capacity retry) would never advance: stop with an error
+        if (!retry && ab == c.last_resume && ++c.repeats > 3)
+            throw ApiError(MDD_ERR_STALLED, "rebuild requested again at the same step");
+        if (ab != c.last_resume) c.repeats = 0;
+        c.last_resume = ab;
         mdd::rebuild_async(c, ab, retry);
         mdd::launch_pair_forces(c, ab, false);
         second_half(ab);

@@ -512,6 +512,8 @@ struct Ctx {
     int64_t last_rebuild = 0;
     double rb_interval = 0.0;
     int retries = 0;  // consecutive builds that did not fit
+    int64_t last_resume = -1;  // step of the last rebuild inside a run and
+    int repeats = 0;           // how many times in a row it was that step
 
     // ---- rebuild launch sequences captured as graphs
     LaunchLog rec;

@@ -1131,7 +1131,8 @@ __global__ void __launch_bounds__(256) k_rows_gate(int nblk, const uint32_t* __r
             ov |= 2u;
             st->overflow = ov;
         }
-        if (ov != 0u && step < st->abort_step) st->abort_step = step;
+        // only the bits that make the host repeat the build (0-4) stop the step
+        if ((ov & 31u) != 0u && step < st->abort_step) st->abort_step = step;
     }
 }
 

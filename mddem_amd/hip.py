@@ -24,7 +24,7 @@ MATERIAL_EXTRAS = ("rolling_friction", "twisting_friction", "cohesion_energy", "
                    "kn", "kt", "rolling_stiffness", "rolling_damping", "twisting_stiffness",
                    "twisting_damping")
 ERRORS = {-1: "invalid", -2: "hip", -3: "overlap", -4: "capacity", -5: "unsupported",
-          -6: "no gfx950 device"}
+          -6: "no gfx950 device", -7: "a step kept asking for a rebuild"}
 
 _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
