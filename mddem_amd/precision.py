@@ -154,10 +154,15 @@ def dem_force_errors(w, checkpoints=(0, 1, 10, 1000)):
         d = fm - f64
         frms = np.sqrt(np.mean(np.sum(f64 * f64, axis=1)))
         trms = np.sqrt(np.mean(np.sum(t64 * t64, axis=1)))
-        out.append(dict(step=k, e_F=float(np.linalg.norm(d) / np.linalg.norm(f64)),
-                        max_over_rms=float(np.max(np.linalg.norm(d, axis=1)) / frms),
-                        e_T=float(np.linalg.norm(tm - t64) / np.linalg.norm(t64)),
-                        torque_max_over_rms=float(np.max(np.linalg.norm(tm - t64, axis=1)) / trms)))
+        row = dict(step=k, e_F=float(np.linalg.norm(d) / np.linalg.norm(f64)),
+                   max_over_rms=float(np.max(np.linalg.norm(d, axis=1)) / frms))
+        # no torque at all (no spin and no stored springs yet): e_T undefined
+        if trms > 0.0:
+            row["e_T"] = float(np.linalg.norm(tm - t64) / np.linalg.norm(t64))
+            row["torque_max_over_rms"] = float(np.max(np.linalg.norm(tm - t64, axis=1)) / trms)
+        else:
+            row["e_T"] = None
+        out.append(row)
     ref.close()
     return out
 
