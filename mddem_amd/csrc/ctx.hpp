@@ -431,8 +431,8 @@ struct Ctx {
     // of L / 2^32 of its axis, so that the 32-bit difference of two
     // particles wraps to the minimum image (every axis at least twice the
     // list cutoff: the listed image is the nearest). The step kernels keep
-    // xq current while xq_valid; a rebuild or an upload moves the particles
-    // otherwise and clears it, and the next pair pass refills it.
+    // xq current while xq_valid and the rebuild's permutation rewrites it;
+    // an upload or a box change clears it and the next pair pass refills it.
     DBuf<int4> xq;
     bool lj_fixed = true;  // mdd_set_lj_positions
     bool xq_valid = false;

@@ -123,6 +123,8 @@ struct PermArgs {
     double *f64b, *t64b;
     const float *f32, *t32;
     float *f32b, *t32b;
+    int4* xq;  // LJ mixed: the fixed-point copy of the permuted positions, or null
+    double qs[3];
     int dem;
 };
 
@@ -135,6 +137,9 @@ __global__ void k_permute(PermArgs a) {
     double4 p = a.pos[o];
     a.pos2[k] = p;
     a.build_pos[k] = p;
+    if (a.xq)
+        a.xq[k] = make_int4(quantize_axis(p.x, a.qs[0]), quantize_axis(p.y, a.qs[1]),
+                            quantize_axis(p.z, a.qs[2]), 0);
     a.vel2[k] = a.vel[o];
     a.tag2[k] = a.tag[o];
     a.type2[k] = a.type[o];
@@ -201,14 +206,15 @@ __global__ void k_ghost_count(int64_t n, const double4* __restrict__ pos, BoxP b
     gcnt[i] = uint32_t((1 + lf[0] + hf[0]) * (1 + lf[1] + hf[1]) * (1 + lf[2] + hf[2]) - 1);
 }
 
-// periodic images of every particle (x shifted by exactly +-L per axis);
+// periodic images of every particle (x shifted by exactly +-L per axis),
+// counted per bin cell for the image sort (cnt zeroed by k_ghost_count);
 // when they do not all fit, none is written and the live count is 0
 __global__ void k_ghost_emit(int64_t n, const double4* __restrict__ pos, BoxP box, double gc,
                              GridP grid, const uint32_t* __restrict__ goff, int64_t ghost_cap,
                              uint32_t* __restrict__ groot, uint8_t* __restrict__ gcode,
                              double4* __restrict__ gpos, uint32_t* __restrict__ gkey,
                              uint32_t* ng_live, Status* st, const SysP* __restrict__ sysp,
-                             const uint32_t* __restrict__ psys) {
+                             const uint32_t* __restrict__ psys, uint32_t* __restrict__ cnt) {
     int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     const uint32_t total = goff[n];
     const bool fits = int64_t(total) <= ghost_cap;
@@ -245,10 +251,12 @@ __global__ void k_ghost_emit(int64_t n, const double4* __restrict__ pos, BoxP bo
                 const double gx = x[0] + image_shift(sx, box.L[0]);
                 const double gy = x[1] + image_shift(sy, box.L[1]);
                 const double gz = x[2] + image_shift(sz, box.L[2]);
+                const uint32_t key = uint32_t(coff + cell_of(grid, gx, gy, gz));
                 groot[g] = uint32_t(i);
                 gcode[g] = uint8_t(code_of(sx, sy, sz));
                 gpos[g] = make_double4(gx, gy, gz, 0.0);
-                gkey[g] = uint32_t(coff + cell_of(grid, gx, gy, gz));
+                gkey[g] = key;
+                atomicAdd(&cnt[key], 1u);  // the image sort's per-cell count
                 g++;
             }
         }
@@ -1144,7 +1152,6 @@ void swap_alternates(Ctx& c);
 // new order and the alternates keep the previous one (tag2: the previous
 // tags, the source of the shear-history remap).
 void permute_all(Ctx& c, const uint32_t* order, int64_t n) {
-    c.xq_valid = false;
     const bool dem = c.model == 1;
     const bool mixed = c.precision == 1;
     PermArgs a;
@@ -1185,9 +1192,17 @@ void permute_all(Ctx& c, const uint32_t* order, int64_t n) {
         a.t64b = dem ? c.t64b.p : nullptr;
     }
     a.dem = dem;
+    // the LJ mixed pass's fixed-point positions come with the permutation
+    const bool xq = c.xq_usable();
+    if (xq) {
+        c.xq.resize(size_t(c.npad));
+        a.xq = c.xq.p;
+        for (int d = 0; d < 3; d++) a.qs[d] = 4294967296.0 / c.box.L[d];
+    }
     launch(c, k_permute, blocks_for(n, 256), 256, a);
     MDD_HIP(hipGetLastError());
     swap_alternates(c);
+    c.xq_valid = xq;
 }
 
 // exchange every per-particle array with its alternate
@@ -1318,12 +1333,12 @@ void rebuild_attempt(Ctx& c) {
     launch(c, k_ghost_emit, blocks_for(n, 256), 256,
         n, c.pos.p, c.box, c.ghost_cutoff, c.grid, c.goff.p, c.ghost_cap, c.ghost_root.p,
         c.ghost_code.p, c.ghost_pos.p, c.ghost_key.p, ng_live, c.status.p, c.sysp_or_null(),
-        c.psys.p);
+        c.psys.p, c.cnt.p);
     // images binned after the particles of each cell (bin_neighbor_list
     // steps 1-2b: counting sort in index order, images have the larger indices)
     // bin_start = particle start + image start of every cell, from the same scan
     counting_sort(c, c.ghost_key.p, c.ghost_cap, ng_live, ncells, c.cnt.p, c.gstart.p, c.placed.p,
-                  c.ghost_order.p, nullptr, true, c.start.p, c.bin_start.p);
+                  c.ghost_order.p, nullptr, true, c.start.p, c.bin_start.p, true);
     launch(c, k_bin_fill, blocks_for(nall_max, 256), 256,
         n, c.ghost_cap, ng_live, c.key.p, c.start.p, c.gstart.p, c.bin_start.p, c.ghost_order.p,
         c.ghost_key.p, c.pos.p, c.cutr.p, c.ghost_pos.p, c.ghost_root.p, c.ghost_code.p,
