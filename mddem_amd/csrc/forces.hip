@@ -892,13 +892,23 @@ __device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
         const double* eeff_t = a.mat + 2 * nm * nm;
         const double* geff_t = a.mat + 3 * nm * nm;
         const double dt = a.dt;
+        // the next neighbor's position and the entry after it are loaded
+        // while this one is computed (entries past the row read entry and
+        // particle 0 and are not used); the partner's velocity and spin are
+        // read only for pairs in contact, so they are not fetched ahead
+        uint32_t e_n = t < cnt ? a.nbr[row_addr(i, t, a.npad, LG)] : 0u;
+        uint32_t e_nn = t + G < cnt ? a.nbr[row_addr(i, t + G, a.npad, LG)] : 0u;
+        double4 pj_n = a.pos[entry_idx(e_n)];
         for (int k = t; k < cnt; k += G) {
             const int64_t slot = row_addr(i, k, a.npad, LG);
-            const uint32_t e = a.nbr[slot];
+            const uint32_t e = e_n;
+            const double4 pj = pj_n;
+            e_n = e_nn;
+            pj_n = a.pos[entry_idx(e_n)];
+            e_nn = k + 2 * G < cnt ? a.nbr[row_addr(i, k + 2 * G, a.npad, LG)] : 0u;
             const uint32_t j = entry_idx(e);
             const int code = entry_code(e);
             const bool rev = entry_rev(e);
-            const double4 pj = a.pos[j];
             double shx, shy, shz;
             code_shifts(code, Lx, Ly, Lz, shx, shy, shz);
             // owner frame: d = partner image - owner; r1 owner radius
