@@ -329,6 +329,13 @@ int mdd_set_dem_split(mdd_ctx* ctx, int mode);
  * on invalidates the list until the next rebuild builds the order. No
  * reference counterpart: launch-order tuning. */
 int mdd_set_dem_tiles(mdd_ctx* ctx, int on);
+/* DEM pair pass at one lane per particle (plain Hertz-Mindlin, no virial
+ * on that step): on (default) each wavefront tests its rows' geometry
+ * first and queues the pairs in contact, then runs the contact arithmetic
+ * on 64 queued pairs at a time, one per lane; off: each lane runs its own
+ * row's contacts. Identical forces, torques and springs. No reference
+ * counterpart: launch-shape tuning. */
+int mdd_set_dem_wave(mdd_ctx* ctx, int on);
 /* LJ pair pass of MDD_MIXED contexts (rows, single system, every box
  * length at least twice the list cutoff): 1 (default) gathers fixed-point
  * positions (x * 2^32 / L per axis, one 16-byte load per neighbor; 32-bit

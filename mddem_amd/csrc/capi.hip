@@ -619,6 +619,10 @@ int mdd_set_dem_tiles(mdd_ctx* ctx, int on) {
     });
 }
 
+int mdd_set_dem_wave(mdd_ctx* ctx, int on) {
+    return guarded(ctx, [&](Ctx& c) { c.dem_wave = on != 0; });
+}
+
 int mdd_set_history_remap(mdd_ctx* ctx, int method) {
     return guarded(ctx, [&](Ctx& c) {
         if (method != 0 && method != 1) throw ApiError(MDD_ERR_INVALID, "remap method is 0 or 1");

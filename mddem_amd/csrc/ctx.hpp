@@ -390,6 +390,9 @@ struct Ctx {
     DBuf<uint32_t> tile_list, tile_cnt, tile_off;
     bool tile_order = true;  // mdd_set_dem_tiles
     bool tile_ok = false;
+    // DEM pair pass at one lane per particle with the pairs in contact
+    // compacted per wavefront (k_hertz_wave; mdd_set_dem_wave)
+    bool dem_wave = true;
 
     // ---- particle arrays (current order)
     DBuf<double4> pos, vel, omega, quat;  // pos.w radius, vel.w 1/m, omega.w 1/I

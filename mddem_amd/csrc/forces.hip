@@ -857,9 +857,225 @@ __device__ void region_wall_forces(const ForceArgs& a, const double4& p, const d
     }
 }
 
-// hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615) with
-// surface energy, cohesion and rolling / twisting friction all zero
-// one particle (lane t of its G lanes). VIR: also the contact virial
+// One row entry's pair geometry in the owner frame (the particle that
+// lists the pair without the rev flag): d = partner image - owner, r1 the
+// owner's radius. Returns whether the pair is in contact (delta > 0), with
+// the large-overlap warning raised for owned pairs (contact.rs:147-217).
+struct PairGeom {
+    double dx, dy, dz, distance, delta, r1, r2;
+};
+
+__device__ inline bool pair_geom(const ForceArgs& a, const double4& pi, const double4& pj,
+                                 uint32_t e, double Lx, double Ly, double Lz, PairGeom& g) {
+    const bool rev = entry_rev(e);
+    double shx, shy, shz;
+    code_shifts(entry_code(e), Lx, Ly, Lz, shx, shy, shz);
+    if (rev) {
+        g.dx = (pi.x + shx) - pj.x;
+        g.dy = (pi.y + shy) - pj.y;
+        g.dz = (pi.z + shz) - pj.z;
+        g.r1 = pj.w;
+        g.r2 = pi.w;
+    } else {
+        g.dx = (pj.x + shx) - pi.x;
+        g.dy = (pj.y + shy) - pi.y;
+        g.dz = (pj.z + shz) - pi.z;
+        g.r1 = pi.w;
+        g.r2 = pj.w;
+    }
+    const double dist_sq = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
+    const double sum_r = g.r1 + g.r2;
+    if (dist_sq >= sum_r * sum_r) return false;
+    g.distance = sqrt(dist_sq);
+    if (g.distance == 0.0) return false;
+    const double r_min = g.r1 < g.r2 ? g.r1 : g.r2;
+    g.delta = sum_r - g.distance;
+    if (0.5 * r_min < g.delta) g.delta = 0.5 * r_min;
+    if (g.delta > 0.0 && !rev && g.distance / sum_r < kLargeOverlap) {
+        int c = atomicAdd(&a.st->overlap_count, 1) + 1;
+        if (c > kMaxOverlapWarnings) {
+            atomicCAS(&a.st->error, 0, -3);
+            if (a.step >= 0) atomicMin(&a.st->abort_step, a.step);
+        }
+    }
+    return g.delta > 0.0;
+}
+
+// Hertz-Mindlin force of one pair in contact (contact.rs:218-424 with
+// surface energy, cohesion and rolling / twisting friction all zero), seen
+// from particle i of the row (entry e, rev: i is the partner): the normal
+// and tangential forces on the owner, the torque on i, and the shear spring
+// of the row slot, read when the slot held a contact (had) and written back.
+template <typename R>
+struct PairOut {
+    R fnx, fny, fnz, ftx, fty, ftz, tqx, tqy, tqz;
+};
+
+template <typename R>
+__device__ inline void contact_pair(const ForceArgs& a, const SysP* S, int64_t i, uint32_t ti,
+                                    uint32_t e, const PairGeom& g, int64_t slot, bool had,
+                                    PairOut<R>& o) {
+    const uint32_t j = entry_idx(e);
+    const bool rev = entry_rev(e);
+    const int nm = a.nm;
+    const double dx = g.dx, dy = g.dy, dz = g.dz, distance = g.distance, delta = g.delta;
+    const double r1 = g.r1, r2 = g.r2, sum_r = r1 + r2;
+    const double4 vi = a.vel[i], vj = a.vel[j];
+    const double4 wi = a.omega[i], wj = a.omega[j];
+    const uint32_t tagi = a.tag[i], tagj = a.tag[j];
+    const uint32_t tj = a.type[j];
+    // owner (O) / partner (P) quantities
+    const double4& vO = rev ? vj : vi;
+    const double4& vP = rev ? vi : vj;
+    const double4& wO = rev ? wj : wi;
+    const double4& wP = rev ? wi : wj;
+    const uint32_t tagO = rev ? tagj : tagi;
+    const uint32_t tagP = rev ? tagi : tagj;
+    const int mO = int(rev ? tj : ti), mP = int(rev ? ti : tj);
+    const int mij = mO * nm + mP;
+    const R r_eff = R(r1) * R(r2) / R(sum_r);
+    const R e_eff = R(S ? S->eeff : a.mat[2 * nm * nm + mij]);
+    const R g_eff = R(S ? S->geff : a.mat[3 * nm * nm + mij]);
+    const R beta = R(S ? S->beta : a.mat[mij]);
+    const R mu = R(S ? S->mu : a.mat[nm * nm + mij]);
+    const double inv_dist = 1.0 / distance;
+    const double nx = dx * inv_dist, ny = dy * inv_dist, nz = dz * inv_dist;
+    const R m_r = R(1.0) / (R(vO.w) + R(vP.w));
+    const R sdr = sqrt_r<R>(R(delta) * r_eff);
+    const R s_n = R(2.0) * e_eff * sdr;
+    const R k_n = R(4.0 / 3.0) * e_eff * sdr;
+    const R k_t = R(8.0) * g_eff * sdr;
+    // contact-point velocities v + w x (+-r n), FP64
+    const double r1nx = r1 * nx, r1ny = r1 * ny, r1nz = r1 * nz;
+    const double vcix = vO.x + (wO.y * r1nz - wO.z * r1ny);
+    const double vciy = vO.y + (wO.z * r1nx - wO.x * r1nz);
+    const double vciz = vO.z + (wO.x * r1ny - wO.y * r1nx);
+    const double r2nx = r2 * nx, r2ny = r2 * ny, r2nz = r2 * nz;
+    const double vcjx = vP.x + (-wP.y * r2nz + wP.z * r2ny);
+    const double vcjy = vP.y + (-wP.z * r2nx + wP.x * r2nz);
+    const double vcjz = vP.z + (-wP.x * r2ny + wP.y * r2nx);
+    const double vrx = vcjx - vcix, vry = vcjy - vciy, vrz = vcjz - vciz;
+    const double v_n = vrx * nx + vry * ny + vrz * nz;
+    const R f_diss_n = R(2.0) * beta * R(kSqrt53) * sqrt_r<R>(s_n * m_r) * R(v_n);
+    R f_n_mag = k_n * R(delta) - f_diss_n;
+    if (!(f_n_mag > R(0))) f_n_mag = R(0);
+    o.fnx = f_n_mag * R(nx);
+    o.fny = f_n_mag * R(ny);
+    o.fnz = f_n_mag * R(nz);
+    const double vtx = vrx - v_n * nx, vty = vry - v_n * ny, vtz = vrz - v_n * nz;
+    // shear spring in the lower tag's frame (tangential.rs:23-26)
+    const double sign = tagO < tagP ? 1.0 : -1.0;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (had) {
+        sx = sign * a.hist[slot];
+        sy = sign * a.hist[a.hist_cs + slot];
+        sz = sign * a.hist[2 * a.hist_cs + slot];
+    }
+    const double s_dot_n = sx * nx + sy * ny + sz * nz;
+    sx -= s_dot_n * nx;
+    sy -= s_dot_n * ny;
+    sz -= s_dot_n * nz;
+    sx += vtx * a.dt;
+    sy += vty * a.dt;
+    sz += vtz * a.dt;
+    const R s_mag = R(sqrt(sx * sx + sy * sy + sz * sz));
+    const R f_t_spring_mag = k_t * s_mag;
+    const R f_t_max = mu * fabs(f_n_mag);
+    if (f_t_spring_mag > f_t_max && f_t_spring_mag > R(kTangEps)) {
+        const R scale = f_t_max / f_t_spring_mag;
+        sx *= double(scale);
+        sy *= double(scale);
+        sz *= double(scale);
+    }
+    const R gamma_t = R(2.0) * R(kSqrt53) * beta * sqrt_r<R>(k_t * m_r);
+    R ftx = k_t * R(sx) - gamma_t * R(vtx);
+    R fty = k_t * R(sy) - gamma_t * R(vty);
+    R ftz = k_t * R(sz) - gamma_t * R(vtz);
+    const R f_t_mag = sqrt_r<R>(ftx * ftx + fty * fty + ftz * ftz);
+    if (f_t_mag > f_t_max && f_t_mag > R(kTangEps)) {
+        const R scale = f_t_max / f_t_mag;
+        ftx *= scale;
+        fty *= scale;
+        ftz *= scale;
+    }
+    o.ftx = ftx;
+    o.fty = fty;
+    o.ftz = ftz;
+    // torque on i: (r2 n) x F_t as the partner, (r1 n) x F_t as the owner
+    if (rev) {
+        o.tqx = R(r2ny) * ftz - R(r2nz) * fty;
+        o.tqy = R(r2nz) * ftx - R(r2nx) * ftz;
+        o.tqz = R(r2nx) * fty - R(r2ny) * ftx;
+    } else {
+        o.tqx = R(r1ny) * ftz - R(r1nz) * fty;
+        o.tqy = R(r1nz) * ftx - R(r1nx) * ftz;
+        o.tqz = R(r1nx) * fty - R(r1ny) * ftx;
+    }
+    a.hist[slot] = sign * sx;
+    a.hist[a.hist_cs + slot] = sign * sy;
+    a.hist[2 * a.hist_cs + slot] = sign * sz;
+}
+
+// adds one pair's forces to particle i's sums: the partner gets +F_n - F_t,
+// the owner -F_n + F_t
+template <typename R>
+__device__ inline void add_pair(bool rev, const PairOut<R>& o, R& fx, R& fy, R& fz, R& tx, R& ty,
+                                R& tz) {
+    if (rev) {
+        fx += o.fnx;
+        fy += o.fny;
+        fz += o.fnz;
+        fx -= o.ftx;
+        fy -= o.fty;
+        fz -= o.ftz;
+    } else {
+        fx -= o.fnx;
+        fy -= o.fny;
+        fz -= o.fnz;
+        fx += o.ftx;
+        fy += o.fty;
+        fz += o.ftz;
+    }
+    tx += o.tqx;
+    ty += o.tqy;
+    tz += o.tqz;
+}
+
+// the particle's box lengths: the context's, or its system's in a batched
+// ensemble (with the system's single-material constants)
+__device__ inline const SysP* particle_box(const ForceArgs& a, int64_t i, double& Lx, double& Ly,
+                                          double& Lz) {
+    Lx = a.L[0];
+    Ly = a.L[1];
+    Lz = a.L[2];
+    const SysP* S = a.sysp ? &a.sysp[a.psys[i]] : nullptr;
+    if (S) {
+        Lx = S->box.L[0];
+        Ly = S->box.L[1];
+        Lz = S->box.L[2];
+    }
+    return S;
+}
+
+// walls, gravity and the store of one particle's sums (lane t == 0)
+template <typename R>
+__device__ inline void finish_particle(const ForceArgs& a, int64_t i, const double4& pi, uint32_t ti,
+                                       R fx, R fy, R fz, R tx, R ty, R tz) {
+    if (a.use_walls)
+        wall_forces<R>(a, pi, &a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz,
+                       tx, ty, tz);
+    if (a.use_gravity) {
+        const double m = a.mass[i];
+        fx += R(m * a.g[0]);
+        fy += R(m * a.g[1]);
+        fz += R(m * a.g[2]);
+    }
+    store_force<R>(a, i, fx, fy, fz, tx, ty, tz);
+}
+
+// hertz_mindlin_contact_force (dem_granular/src/contact.rs:103-615) for
+// one particle (lane t of its G lanes): its row in order, each pair in
+// contact through contact_pair. VIR: also the contact virial
 // (contact.rs:559-569): each particle sums d (x) F over the pairs it owns,
 // d = partner image - owner, F = -F_n + F_t the force on the owner, into
 // vir[q * npad + i] (xx yy zz xy xz yz).
@@ -870,14 +1086,8 @@ __device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
     R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
     double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
     const uint32_t ti = a.type[i];
-    // box lengths and (batched ensemble) the system's single-material constants
-    double Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
-    const SysP* S = a.sysp ? &a.sysp[a.psys[i]] : nullptr;
-    if (S) {
-        Lx = S->box.L[0];
-        Ly = S->box.L[1];
-        Lz = S->box.L[2];
-    }
+    double Lx, Ly, Lz;
+    const SysP* S = particle_box(a, i, Lx, Ly, Lz);
     if (a.use_pairs) {
         int cnt = int(a.nnb[i]);
         if (cnt > a.maxnb) cnt = a.maxnb;
@@ -886,12 +1096,6 @@ __device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
         uint64_t mask_in = 0ull;
         if (cnt > 0) mask_in = a.cmask[i];
         uint64_t mask_out = 0ull;
-        const int nm = a.nm;
-        const double* beta_t = a.mat;
-        const double* mu_t = a.mat + nm * nm;
-        const double* eeff_t = a.mat + 2 * nm * nm;
-        const double* geff_t = a.mat + 3 * nm * nm;
-        const double dt = a.dt;
         // the next neighbor's position and the entry after it are loaded
         // while this one is computed (entries past the row read entry and
         // particle 0 and are not used); the partner's velocity and spin are
@@ -900,164 +1104,29 @@ __device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
         uint32_t e_nn = t + G < cnt ? a.nbr[row_addr(i, t + G, a.npad, LG)] : 0u;
         double4 pj_n = a.pos[entry_idx(e_n)];
         for (int k = t; k < cnt; k += G) {
-            const int64_t slot = row_addr(i, k, a.npad, LG);
             const uint32_t e = e_n;
             const double4 pj = pj_n;
             e_n = e_nn;
             pj_n = a.pos[entry_idx(e_n)];
             e_nn = k + 2 * G < cnt ? a.nbr[row_addr(i, k + 2 * G, a.npad, LG)] : 0u;
-            const uint32_t j = entry_idx(e);
-            const int code = entry_code(e);
-            const bool rev = entry_rev(e);
-            double shx, shy, shz;
-            code_shifts(code, Lx, Ly, Lz, shx, shy, shz);
-            // owner frame: d = partner image - owner; r1 owner radius
-            double dx, dy, dz, r1, r2;
-            if (rev) {
-                dx = (pi.x + shx) - pj.x;
-                dy = (pi.y + shy) - pj.y;
-                dz = (pi.z + shz) - pj.z;
-                r1 = pj.w;
-                r2 = pi.w;
-            } else {
-                dx = (pj.x + shx) - pi.x;
-                dy = (pj.y + shy) - pi.y;
-                dz = (pj.z + shz) - pi.z;
-                r1 = pi.w;
-                r2 = pj.w;
-            }
-            const double dist_sq = dx * dx + dy * dy + dz * dz;
-            const double sum_r = r1 + r2;
-            if (dist_sq >= sum_r * sum_r) continue;
-            const double distance = sqrt(dist_sq);
-            if (distance == 0.0) continue;
-            const double r_min = r1 < r2 ? r1 : r2;
-            double delta = sum_r - distance;
-            if (0.5 * r_min < delta) delta = 0.5 * r_min;
-            if (delta > 0.0 && !rev && distance / sum_r < kLargeOverlap) {
-                int c = atomicAdd(&a.st->overlap_count, 1) + 1;
-                if (c > kMaxOverlapWarnings) {
-                    atomicCAS(&a.st->error, 0, -3);
-                    if (a.step >= 0) atomicMin(&a.st->abort_step, a.step);
-                }
-            }
-            if (delta <= 0.0) continue;
-            // the particle's own velocity, spin and tag are read with the
-            // partner's, only for pairs in contact
-            const double4 vi = a.vel[i], vj = a.vel[j];
-            const double4 wi = a.omega[i], wj = a.omega[j];
-            const uint32_t tagi = a.tag[i], tagj = a.tag[j];
-            const uint32_t tj = a.type[j];
-            // owner (O) / partner (P) quantities
-            const double4& vO = rev ? vj : vi;
-            const double4& vP = rev ? vi : vj;
-            const double4& wO = rev ? wj : wi;
-            const double4& wP = rev ? wi : wj;
-            const uint32_t tagO = rev ? tagj : tagi;
-            const uint32_t tagP = rev ? tagi : tagj;
-            const int mO = int(rev ? tj : ti), mP = int(rev ? ti : tj);
-            const int mij = mO * nm + mP;
-            const R r_eff = R(r1) * R(r2) / R(sum_r);
-            const R e_eff = R(S ? S->eeff : eeff_t[mij]);
-            const R g_eff = R(S ? S->geff : geff_t[mij]);
-            const R beta = R(S ? S->beta : beta_t[mij]);
-            const R mu = R(S ? S->mu : mu_t[mij]);
-            const double inv_dist = 1.0 / distance;
-            const double nx = dx * inv_dist, ny = dy * inv_dist, nz = dz * inv_dist;
-            const R m_r = R(1.0) / (R(vO.w) + R(vP.w));
-            const R sdr = sqrt_r<R>(R(delta) * r_eff);
-            const R s_n = R(2.0) * e_eff * sdr;
-            const R k_n = R(4.0 / 3.0) * e_eff * sdr;
-            const R k_t = R(8.0) * g_eff * sdr;
-            // contact-point velocities v + w x (+-r n), FP64
-            const double r1nx = r1 * nx, r1ny = r1 * ny, r1nz = r1 * nz;
-            const double vcix = vO.x + (wO.y * r1nz - wO.z * r1ny);
-            const double vciy = vO.y + (wO.z * r1nx - wO.x * r1nz);
-            const double vciz = vO.z + (wO.x * r1ny - wO.y * r1nx);
-            const double r2nx = r2 * nx, r2ny = r2 * ny, r2nz = r2 * nz;
-            const double vcjx = vP.x + (-wP.y * r2nz + wP.z * r2ny);
-            const double vcjy = vP.y + (-wP.z * r2nx + wP.x * r2nz);
-            const double vcjz = vP.z + (-wP.x * r2ny + wP.y * r2nx);
-            const double vrx = vcjx - vcix, vry = vcjy - vciy, vrz = vcjz - vciz;
-            const double v_n = vrx * nx + vry * ny + vrz * nz;
-            const R f_diss_n = R(2.0) * beta * R(kSqrt53) * sqrt_r<R>(s_n * m_r) * R(v_n);
-            R f_n_mag = k_n * R(delta) - f_diss_n;
-            if (!(f_n_mag > R(0))) f_n_mag = R(0);
-            const R fnx = f_n_mag * R(nx), fny = f_n_mag * R(ny), fnz = f_n_mag * R(nz);
-            const double vtx = vrx - v_n * nx, vty = vry - v_n * ny, vtz = vrz - v_n * nz;
-            // shear spring in the lower tag's frame (tangential.rs:23-26)
-            const double sign = tagO < tagP ? 1.0 : -1.0;
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            if ((mask_in >> k) & 1ull) {
-                sx = sign * a.hist[slot];
-                sy = sign * a.hist[a.hist_cs + slot];
-                sz = sign * a.hist[2 * a.hist_cs + slot];
-            }
-            const double s_dot_n = sx * nx + sy * ny + sz * nz;
-            sx -= s_dot_n * nx;
-            sy -= s_dot_n * ny;
-            sz -= s_dot_n * nz;
-            sx += vtx * dt;
-            sy += vty * dt;
-            sz += vtz * dt;
-            const R s_mag = R(sqrt(sx * sx + sy * sy + sz * sz));
-            const R f_t_spring_mag = k_t * s_mag;
-            const R f_t_max = mu * fabs(f_n_mag);
-            if (f_t_spring_mag > f_t_max && f_t_spring_mag > R(kTangEps)) {
-                const R scale = f_t_max / f_t_spring_mag;
-                sx *= double(scale);
-                sy *= double(scale);
-                sz *= double(scale);
-            }
-            const R gamma_t = R(2.0) * R(kSqrt53) * beta * sqrt_r<R>(k_t * m_r);
-            R ftx = k_t * R(sx) - gamma_t * R(vtx);
-            R fty = k_t * R(sy) - gamma_t * R(vty);
-            R ftz = k_t * R(sz) - gamma_t * R(vtz);
-            const R f_t_mag = sqrt_r<R>(ftx * ftx + fty * fty + ftz * ftz);
-            if (f_t_mag > f_t_max && f_t_mag > R(kTangEps)) {
-                const R scale = f_t_max / f_t_mag;
-                ftx *= scale;
-                fty *= scale;
-                ftz *= scale;
-            }
+            PairGeom g;
+            if (!pair_geom(a, pi, pj, e, Lx, Ly, Lz, g)) continue;
+            PairOut<R> o;
+            contact_pair<R>(a, S, i, ti, e, g, row_addr(i, k, a.npad, LG), (mask_in >> k) & 1ull,
+                            o);
             if constexpr (VIR) {
-                if (!rev) {
-                    const double vfx = double(-fnx + ftx), vfy = double(-fny + fty),
-                                 vfz = double(-fnz + ftz);
-                    w0 += dx * vfx;
-                    w1 += dy * vfy;
-                    w2 += dz * vfz;
-                    w3 += dx * vfy;
-                    w4 += dx * vfz;
-                    w5 += dy * vfz;
+                if (!entry_rev(e)) {
+                    const double vfx = double(-o.fnx + o.ftx), vfy = double(-o.fny + o.fty),
+                                 vfz = double(-o.fnz + o.ftz);
+                    w0 += g.dx * vfx;
+                    w1 += g.dy * vfy;
+                    w2 += g.dz * vfz;
+                    w3 += g.dx * vfy;
+                    w4 += g.dx * vfz;
+                    w5 += g.dy * vfz;
                 }
             }
-            if (rev) {
-                // partner: +F_n - F_t, torque (r2 n) x F_t
-                fx += fnx;
-                fy += fny;
-                fz += fnz;
-                fx -= ftx;
-                fy -= fty;
-                fz -= ftz;
-                tx += R(r2ny) * ftz - R(r2nz) * fty;
-                ty += R(r2nz) * ftx - R(r2nx) * ftz;
-                tz += R(r2nx) * fty - R(r2ny) * ftx;
-            } else {
-                // owner: -F_n + F_t, torque (r1 n) x F_t
-                fx -= fnx;
-                fy -= fny;
-                fz -= fnz;
-                fx += ftx;
-                fy += fty;
-                fz += ftz;
-                tx += R(r1ny) * ftz - R(r1nz) * fty;
-                ty += R(r1nz) * ftx - R(r1nx) * ftz;
-                tz += R(r1nx) * fty - R(r1ny) * ftx;
-            }
-            a.hist[slot] = sign * sx;
-            a.hist[a.hist_cs + slot] = sign * sy;
-            a.hist[2 * a.hist_cs + slot] = sign * sz;
+            add_pair<R>(entry_rev(e), o, fx, fy, fz, tx, ty, tz);
             mask_out |= 1ull << k;
         }
         fx = group_sum<G>(fx);
@@ -1087,16 +1156,7 @@ __device__ inline void hertz_particle(const ForceArgs& a, int64_t i, int t) {
         a.vir[4 * np + i] = w4;
         a.vir[5 * np + i] = w5;
     }
-    if (a.use_walls)
-        wall_forces<R>(a, pi, &a.vel[i], a.omega ? &a.omega[i] : nullptr, a.mass[i], ti, fx, fy, fz, tx,
-                       ty, tz);
-    if (a.use_gravity) {
-        const double m = a.mass[i];
-        fx += R(m * a.g[0]);
-        fy += R(m * a.g[1]);
-        fz += R(m * a.g[2]);
-    }
-    store_force<R>(a, i, fx, fy, fz, tx, ty, tz);
+    finish_particle<R>(a, i, pi, ti, fx, fy, fz, tx, ty, tz);
 }
 
 template <typename R, int LG, bool VIR>
@@ -1124,6 +1184,172 @@ __global__ void __launch_bounds__(256) k_hertz_tiles(ForceArgs a, const uint32_t
     if (k >= a.n) return;
     if (gated(a)) return;
     hertz_particle<R, LG, VIR>(a, int64_t(order[k]), t);
+}
+
+// The same pass with the pairs in contact compacted per wavefront (one lane
+// per particle, no virial). In hertz_particle a wavefront runs the contact
+// arithmetic whenever any lane's pair at that row position is in contact,
+// with every other lane idle (its pair is apart, or its row has ended).
+// Here the lanes walk their rows together testing the geometry only, and
+// queue each pair in contact (geometry, entry, row position, lane) in LDS;
+// each time 64 are queued, every lane runs the contact arithmetic of one of
+// them, and then each particle adds the results of its own queued pairs in
+// row order. A pair's arithmetic (contact_pair) and the order of each
+// particle's sums are those of hertz_particle, so forces, torques and
+// springs are the same bit for bit.
+constexpr int kWaveQ = 64;
+
+struct WaveQueue {
+    // per queued pair: dx dy dz distance delta r1 r2; the contact
+    // arithmetic overwrites slots 0..63 with its 9 results (the last two in
+    // res), while slots 64..127 hold the pairs queued after the 64th
+    double g[7][2 * kWaveQ];
+    double res[2][kWaveQ];
+    uint32_t e[2 * kWaveQ];
+    uint32_t lk[2 * kWaveQ];  // lane | row position << 8 | spring held << 16
+};
+
+// orders the wavefront's LDS accesses on either side (lanes read what other
+// lanes of the same wavefront wrote)
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// contact arithmetic of the queued pairs 0..nq-1 (nq <= 64), then each lane
+// adds those it queued (bits of own) to its sums
+template <typename R>
+__device__ inline void wave_contacts(const ForceArgs& a, WaveQueue& q, int nq, uint32_t my_i,
+                                     uint64_t own, R& fx, R& fy, R& fz, R& tx, R& ty, R& tz) {
+    const int lane = int(threadIdx.x & 63);
+    wave_lds_sync();
+    const uint32_t lk = q.lk[lane];
+    const uint32_t io = uint32_t(__shfl(int(my_i), int(lk & 63u), 64));
+    if (lane < nq) {
+        const uint32_t e = q.e[lane];
+        PairGeom g;
+        g.dx = q.g[0][lane];
+        g.dy = q.g[1][lane];
+        g.dz = q.g[2][lane];
+        g.distance = q.g[3][lane];
+        g.delta = q.g[4][lane];
+        g.r1 = q.g[5][lane];
+        g.r2 = q.g[6][lane];
+        const SysP* S = a.sysp ? &a.sysp[a.psys[io]] : nullptr;
+        PairOut<R> o;
+        contact_pair<R>(a, S, io, a.type[io], e, g, row_addr(io, int((lk >> 8) & 255u), a.npad, 0),
+                        ((lk >> 16) & 1u) != 0u, o);
+        q.g[0][lane] = double(o.fnx);
+        q.g[1][lane] = double(o.fny);
+        q.g[2][lane] = double(o.fnz);
+        q.g[3][lane] = double(o.ftx);
+        q.g[4][lane] = double(o.fty);
+        q.g[5][lane] = double(o.ftz);
+        q.g[6][lane] = double(o.tqx);
+        q.res[0][lane] = double(o.tqy);
+        q.res[1][lane] = double(o.tqz);
+    }
+    wave_lds_sync();
+    while (own != 0ull) {
+        const int p = __builtin_ctzll(own);
+        own &= own - 1ull;
+        PairOut<R> o;
+        o.fnx = R(q.g[0][p]);
+        o.fny = R(q.g[1][p]);
+        o.fnz = R(q.g[2][p]);
+        o.ftx = R(q.g[3][p]);
+        o.fty = R(q.g[4][p]);
+        o.ftz = R(q.g[5][p]);
+        o.tqx = R(q.g[6][p]);
+        o.tqy = R(q.res[0][p]);
+        o.tqz = R(q.res[1][p]);
+        add_pair<R>(entry_rev(q.e[p]), o, fx, fy, fz, tx, ty, tz);
+    }
+    wave_lds_sync();
+}
+
+// order: tile order (particle order[k] for k < n), or null for XCD-aware
+// block order over the particles
+template <typename R>
+__global__ void __launch_bounds__(256) k_hertz_wave(ForceArgs a, const uint32_t* __restrict__ order) {
+    __shared__ WaveQueue queues[4];
+    WaveQueue& q = queues[threadIdx.x >> 6];
+    const int lane = int(threadIdx.x & 63);
+    if (gated(a)) return;
+    const int64_t k = int64_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    const bool live = k < a.n;
+    const int64_t i = live ? (order ? int64_t(order[k]) : k) : 0;
+    const double4 pi = a.pos[i];
+    R fx = R(0), fy = R(0), fz = R(0), tx = R(0), ty = R(0), tz = R(0);
+    const uint32_t ti = a.type[i];
+    double Lx, Ly, Lz;
+    particle_box(a, i, Lx, Ly, Lz);
+    if (a.use_pairs) {
+        int cnt = live ? int(a.nnb[i]) : 0;
+        if (cnt > a.maxnb) cnt = a.maxnb;
+        uint64_t mask_in = 0ull;
+        if (cnt > 0) mask_in = a.cmask[i];
+        uint64_t mask_out = 0ull;
+        uint32_t e_n = 0 < cnt ? a.nbr[row_addr(i, 0, a.npad, 0)] : 0u;
+        uint32_t e_nn = 1 < cnt ? a.nbr[row_addr(i, 1, a.npad, 0)] : 0u;
+        double4 pj_n = a.pos[entry_idx(e_n)];
+        // queued pairs: nq (wavefront-uniform); this lane's among them at
+        // the bits of own_lo (slots 0..63) and own_hi (slots 64..127)
+        int nq = 0;
+        uint64_t own_lo = 0ull, own_hi = 0ull;
+        for (int t = 0; __ballot(t < cnt) != 0ull; t++) {
+            bool in = false;
+            uint32_t e = 0u;
+            PairGeom g;
+            if (t < cnt) {
+                e = e_n;
+                const double4 pj = pj_n;
+                e_n = e_nn;
+                pj_n = a.pos[entry_idx(e_n)];
+                e_nn = t + 2 < cnt ? a.nbr[row_addr(i, t + 2, a.npad, 0)] : 0u;
+                in = pair_geom(a, pi, pj, e, Lx, Ly, Lz, g);
+            }
+            const uint64_t b = __ballot(in);
+            if (b == 0ull) continue;
+            if (in) {
+                const int p = nq + int(lanes_before(b));
+                q.g[0][p] = g.dx;
+                q.g[1][p] = g.dy;
+                q.g[2][p] = g.dz;
+                q.g[3][p] = g.distance;
+                q.g[4][p] = g.delta;
+                q.g[5][p] = g.r1;
+                q.g[6][p] = g.r2;
+                q.e[p] = e;
+                q.lk[p] = uint32_t(lane) | (uint32_t(t) << 8) | (uint32_t((mask_in >> t) & 1ull) << 16);
+                if (p < kWaveQ)
+                    own_lo |= 1ull << p;
+                else
+                    own_hi |= 1ull << (p - kWaveQ);
+                mask_out |= 1ull << t;
+            }
+            nq += __popcll(b);
+            if (nq >= kWaveQ) {
+                wave_contacts<R>(a, q, kWaveQ, uint32_t(i), own_lo, fx, fy, fz, tx, ty, tz);
+                // the pairs queued past the 64th move to the front
+                const int rest = nq - kWaveQ;
+                if (lane < rest) {
+#pragma unroll
+                    for (int c = 0; c < 7; c++) q.g[c][lane] = q.g[c][kWaveQ + lane];
+                    q.e[lane] = q.e[kWaveQ + lane];
+                    q.lk[lane] = q.lk[kWaveQ + lane];
+                }
+                wave_lds_sync();
+                own_lo = own_hi;
+                own_hi = 0ull;
+                nq = rest;
+            }
+        }
+        if (nq > 0) wave_contacts<R>(a, q, nq, uint32_t(i), own_lo, fx, fy, fz, tx, ty, tz);
+        if (live && mask_out != mask_in) a.cmask[i] = mask_out;
+    }
+    if (live) finish_particle<R>(a, i, pi, ti, fx, fy, fz, tx, ty, tz);
 }
 
 // The same pass split by row length (one lane per particle): the particles
@@ -1635,7 +1861,13 @@ void launch_hertz(Ctx& c, const ForceArgs& a, hipEvent_t e0, hipEvent_t e1) {
         }
         return;
     }
-    if (c.tile_order && c.tile_ok && c.nsys == 0) {
+    const bool tiles = c.tile_order && c.tile_ok && c.nsys == 0;
+    if (c.dem_wave && c.list_lg == 0 && !VIR) {
+        launch_force_kernel(c, k_hertz_wave<R>, nb, a, e0, e1,
+                            tiles ? static_cast<const uint32_t*>(c.tile_list.p) : nullptr);
+        return;
+    }
+    if (tiles) {
         const uint32_t* ord = c.tile_list.p;
         switch (c.list_lg) {
             case 0: launch_force_kernel(c, k_hertz_tiles<R, 0, VIR>, nb, a, e0, e1, ord); break;

@@ -190,6 +190,32 @@ def test_tile_order_gives_the_same_run(lg):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("prec,tiles", [(hip.MIXED, True), (hip.MIXED, False), (hip.FP64, True)])
+def test_compacted_contacts_give_the_same_run(prec, tiles):
+    # one lane per sphere: each wavefront queues its pairs in contact and runs
+    # their arithmetic 64 at a time, then every sphere adds its own results
+    # in row order; the pair arithmetic and each sphere's order of sums are
+    # those of the per-lane loop, so the runs agree bit for bit
+    w = W.dem_packed_bed(n=20000, box=(0.1, 0.1, 1.2), every=100)
+    out = []
+    for wave in (False, True):
+        s = hip.from_workload(w, prec, box_exit=True)
+        s.set_dem_tiles(tiles)
+        s.set_force_lanes(0)
+        s.set_dem_split(0)
+        s.set_dem_wave(wave)
+        s.run(250)
+        st = s.state()
+        f, t = s.forces()
+        a, b, spring = canon_history(*s.history())
+        out.append((st["pos"], st["vel"], st["omega"], f, t, a, b, spring))
+        th = s.thermo()
+        assert th["rebuilds"] >= 2 and th["ncontacts"] > w.n
+        s.close()
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+
+
 def test_pairs_listed_twice_keep_their_springs_deterministically():
     # spheres of this bed that sit on the periodic boundary (np.mod rounds a
     # few to y = L) give pairs the reference lists twice, once through each
