@@ -1291,9 +1291,15 @@ __global__ void __launch_bounds__(256) k_hertz_wave(ForceArgs a, const uint32_t*
         uint64_t mask_in = 0ull;
         if (cnt > 0) mask_in = a.cmask[i];
         uint64_t mask_out = 0ull;
+        // the partner positions of the next row steps are loaded ahead: two
+        // steps in FP32 (160 VGPRs, still 3 wavefronts per SIMD), one in
+        // FP64, whose registers allow only one at 3 wavefronts
+        constexpr bool kTwoAhead = std::is_same<R, float>::value;
         uint32_t e_n = 0 < cnt ? a.nbr[row_addr(i, 0, a.npad, 0)] : 0u;
         uint32_t e_nn = 1 < cnt ? a.nbr[row_addr(i, 1, a.npad, 0)] : 0u;
+        uint32_t e_n3 = kTwoAhead && 2 < cnt ? a.nbr[row_addr(i, 2, a.npad, 0)] : 0u;
         double4 pj_n = a.pos[entry_idx(e_n)];
+        double4 pj_nn = kTwoAhead ? a.pos[entry_idx(e_nn)] : pj_n;
         // queued pairs: nq (wavefront-uniform); this lane's among them at
         // the bits of own_lo (slots 0..63) and own_hi (slots 64..127)
         int nq = 0;
@@ -1306,8 +1312,15 @@ __global__ void __launch_bounds__(256) k_hertz_wave(ForceArgs a, const uint32_t*
                 e = e_n;
                 const double4 pj = pj_n;
                 e_n = e_nn;
-                pj_n = a.pos[entry_idx(e_n)];
-                e_nn = t + 2 < cnt ? a.nbr[row_addr(i, t + 2, a.npad, 0)] : 0u;
+                if constexpr (kTwoAhead) {
+                    pj_n = pj_nn;
+                    e_nn = e_n3;
+                    pj_nn = a.pos[entry_idx(e_nn)];
+                    e_n3 = t + 3 < cnt ? a.nbr[row_addr(i, t + 3, a.npad, 0)] : 0u;
+                } else {
+                    pj_n = a.pos[entry_idx(e_n)];
+                    e_nn = t + 2 < cnt ? a.nbr[row_addr(i, t + 2, a.npad, 0)] : 0u;
+                }
                 in = pair_geom(a, pi, pj, e, Lx, Ly, Lz, g);
             }
             const uint64_t b = __ballot(in);

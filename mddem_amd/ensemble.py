@@ -28,12 +28,14 @@ from mddem_amd import hip
 from mddem_amd import workloads as W
 
 CLASSES = ((200, 1000), (1000, 5000), (5000, 20001))
-LANE_OPTIONS = (0, 1, 2)       # log2 lanes per sphere
+# force pass shapes: (log2 lanes per sphere, contacts queued per wavefront);
+# the queue only applies at one lane per sphere
+SHAPES = ((0, True), (0, False), (1, False), (2, False))
 BIN_FACTORS = (1.0, 1.5)       # bin size / pair cutoff
 
 
 def make_context(systems, precision=hip.MIXED, box_exit=False, bin_size=None,
-                 lanes_log2=None) -> hip.System:
+                 lanes_log2=None, wave=None) -> hip.System:
     """One batched context holding `systems` (workloads.gas_system)."""
     w0 = systems[0]
     s = hip.System(hip.MODEL_DEM, precision)
@@ -43,6 +45,8 @@ def make_context(systems, precision=hip.MIXED, box_exit=False, bin_size=None,
     s.set_dt(w0.dt)
     if lanes_log2 is not None:
         s.set_force_lanes(lanes_log2)
+    if wave is not None:
+        s.set_dem_wave(wave)
     for w in systems:
         m = w.materials
         s.add_system(w.lo, w.hi, m["youngs"][0], m["poisson"][0], m["restitution"][0],
@@ -70,12 +74,12 @@ def granular_temperature(vel, mass):
 
 
 def tune_class(systems, precision, trial_steps=60):
-    """Lanes per sphere and bin factor with the fastest trial step."""
+    """Force pass shape and bin factor with the fastest trial step."""
     cut = 2.0 * float(systems[0].cutoff_radius[0]) * systems[0].skin_fraction
     best = None
     for bf in BIN_FACTORS:
-        for lg in LANE_OPTIONS:
-            s = make_context(systems, precision, bin_size=bf * cut, lanes_log2=lg)
+        for lg, wave in SHAPES:
+            s = make_context(systems, precision, bin_size=bf * cut, lanes_log2=lg, wave=wave)
             s.run(10)
             s.synchronize()
             t0 = time.perf_counter()
@@ -84,8 +88,8 @@ def tune_class(systems, precision, trial_steps=60):
             dt = time.perf_counter() - t0
             s.close()
             if best is None or dt < best[0]:
-                best = (dt, lg, bf)
-    return dict(lanes_log2=best[1], bin_factor=best[2], trial_s=best[0])
+                best = (dt, lg, wave, bf)
+    return dict(lanes_log2=best[1], wave=best[2], bin_factor=best[3], trial_s=best[0])
 
 
 def run(nsys=1024, steps=5000, segment=100, precision=hip.MIXED, tune=True, seed=None, log=None):
@@ -98,14 +102,16 @@ def run(nsys=1024, steps=5000, segment=100, precision=hip.MIXED, tune=True, seed
     plan = []
     for idx in classes:
         sub = [systems[k] for k in idx]
-        cfg = tune_class(sub, precision) if tune else dict(lanes_log2=2, bin_factor=1.0)
+        cfg = tune_class(sub, precision) if tune else dict(lanes_log2=2, wave=False,
+                                                            bin_factor=1.0)
         cut = 2.0 * float(sub[0].cutoff_radius[0]) * sub[0].skin_fraction
         plan.append((idx, sub, cfg, cut))
         if log:
             log(f"[ensemble] class n in [{min(w.n for w in sub)}, {max(w.n for w in sub)}]: "
                 f"{len(sub)} systems, {sum(w.n for w in sub)} spheres, tuned {cfg}")
     ctxs = [make_context(sub, precision, bin_size=cfg["bin_factor"] * cut,
-                         lanes_log2=cfg["lanes_log2"]) for idx, sub, cfg, cut in plan]
+                         lanes_log2=cfg["lanes_log2"], wave=cfg["wave"])
+            for idx, sub, cfg, cut in plan]
     for s in ctxs:
         s.run(1)  # setup and first rebuild, untimed
         s.synchronize()
