@@ -223,7 +223,12 @@ __global__ void k_ghost_emit(int64_t n, const double4* __restrict__ pos, BoxP bo
         *ng_live = fits ? total : 0u;
         if (!fits) atomicOr(&st->overflow, 1u);
     }
-    if (!fits || i >= n) return;
+    if (!fits) return;
+    // every lane stays to the end: the per-cell counts are added once per
+    // run of equal cells among the wavefront's images of one shift (the
+    // particles are in cell order, so such images are too)
+    const bool live = i < n;
+    if (!live) i = 0;
     int64_t coff = 0;
     if (sysp) {
         const SysP& S = sysp[psys[i]];
@@ -236,28 +241,37 @@ __global__ void k_ghost_emit(int64_t n, const double4* __restrict__ pos, BoxP bo
     double x[3] = {p.x, p.y, p.z};
     int lf[3], hf[3];
     image_flags(box, gc, x, lf, hf);
-    uint32_t g = goff[i];
+    uint32_t g = live ? goff[i] : 0u;
     const int sv[3] = {0, 1, -1};
     for (int a = 0; a < 3; a++) {
         const int sx = sv[a];
-        if ((sx == 1 && !lf[0]) || (sx == -1 && !hf[0])) continue;
+        const bool ha = live && !((sx == 1 && !lf[0]) || (sx == -1 && !hf[0]));
+        if (__ballot(ha) == 0ull) continue;
         for (int b = 0; b < 3; b++) {
             const int sy = sv[b];
-            if ((sy == 1 && !lf[1]) || (sy == -1 && !hf[1])) continue;
+            const bool hb = ha && !((sy == 1 && !lf[1]) || (sy == -1 && !hf[1]));
+            if (__ballot(hb) == 0ull) continue;
             for (int cz = 0; cz < 3; cz++) {
                 const int sz = sv[cz];
-                if ((sz == 1 && !lf[2]) || (sz == -1 && !hf[2])) continue;
                 if (sx == 0 && sy == 0 && sz == 0) continue;
-                const double gx = x[0] + image_shift(sx, box.L[0]);
-                const double gy = x[1] + image_shift(sy, box.L[1]);
-                const double gz = x[2] + image_shift(sz, box.L[2]);
-                const uint32_t key = uint32_t(coff + cell_of(grid, gx, gy, gz));
-                groot[g] = uint32_t(i);
-                gcode[g] = uint8_t(code_of(sx, sy, sz));
-                gpos[g] = make_double4(gx, gy, gz, 0.0);
-                gkey[g] = key;
-                atomicAdd(&cnt[key], 1u);  // the image sort's per-cell count
-                g++;
+                const bool has = hb && !((sz == 1 && !lf[2]) || (sz == -1 && !hf[2]));
+                if (__ballot(has) == 0ull) continue;
+                uint32_t key = 0u;
+                if (has) {
+                    const double gx = x[0] + image_shift(sx, box.L[0]);
+                    const double gy = x[1] + image_shift(sy, box.L[1]);
+                    const double gz = x[2] + image_shift(sz, box.L[2]);
+                    key = uint32_t(coff + cell_of(grid, gx, gy, gz));
+                    groot[g] = uint32_t(i);
+                    gcode[g] = uint8_t(code_of(sx, sy, sz));
+                    gpos[g] = make_double4(gx, gy, gz, 0.0);
+                    gkey[g] = key;
+                    g++;
+                }
+                // the image sort's per-cell count
+                int h, t;
+                wave_runs(key, has, h, t);
+                if (has && int(threadIdx.x & 63) == t) atomicAdd(&cnt[key], uint32_t(t - h + 1));
             }
         }
     }
