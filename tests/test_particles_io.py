@@ -168,3 +168,24 @@ def test_rust_exponent_format_and_granular_temp_line():
     assert io.rust_exp(-2.5e-7, 10) == "-2.5000000000e-7"
     line = io.granular_temp_line(100, 1.466980e-7, 0.04, 0.5, (3.0, 4.0, 0.0))
     assert line == "100 1.466980e-5 4.0000000000e-2 5.0000000000e-1 5.0000000000e0"
+
+
+def test_granular_temp_lines_of_the_reference_hopper_run():
+    # GranularTemp.txt as the reference's hopper example wrote it
+    # (examples/hopper/data/GranularTemp.txt, kept as a fixture): every one of
+    # its 1000 lines is reproduced by the formatter from the values it holds,
+    # the step, the time in {:.6e} and T_g, KE and |p| in {:.10e}
+    # (granular_temp.rs:128-133), from 0.000000e0 to values near 1e-31
+    with open(os.path.join(GOLDEN, "hopper_GranularTemp.txt")) as fh:
+        lines = fh.read().splitlines()
+    assert len(lines) == 1000
+    exps = set()
+    for line in lines:
+        tok = line.split()
+        assert len(tok) == 5
+        step = int(tok[0])
+        t, tg, ke, p = (float(x) for x in tok[1:])
+        dt = t / step if step else 1.0
+        assert io.granular_temp_line(step, dt, tg, ke, (p, 0.0, 0.0)) == line
+        exps.update(x.split("e")[1] for x in tok[1:])
+    assert {"0", "-3", "-31"} <= exps
