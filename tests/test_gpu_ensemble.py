@@ -28,35 +28,48 @@ def _periodic_diff(a, b, L):
     return d - L * np.round(d / L)
 
 
+def _collided(w, v):
+    # spheres whose velocity changed: only contacts change one in these gases
+    return int(np.sum(np.linalg.norm(v - w.vel, axis=1) > 1e-12 * np.abs(w.vel).max()))
+
+
 def test_batched_systems_match_alone_runs():
+    # 4000 steps: the denser gases have had collisions by then (a 100-step
+    # window has none)
     systems = _systems()
     s = ensemble.make_context(systems, hip.FP64, box_exit=True)
     assert s.nsystems() == len(systems)
-    s.run(150)
+    s.run(4000)
     got = ensemble.system_states(s, len(systems))
+    hits = 0
     for w, (p, v) in zip(systems, got):
         a = hip.from_workload(w, hip.FP64, box_exit=True)
-        a.run(150)
+        a.run(4000)
         pa, va = _by_tag(a.state())
         L = w.hi[0] - w.lo[0]
         assert np.abs(_periodic_diff(p, pa, L)).max() < 1e-12 * L
         assert rel_err(v, va) < 1e-9
+        hits += _collided(w, v)
         a.close()
+    assert hits > 20
 
 
 def test_batched_system_matches_cpu_checker():
-    systems = _systems()[:2]
+    systems = [_systems()[k] for k in (1, 3)]
     s = ensemble.make_context(systems, hip.FP64, box_exit=True)
-    s.run(100)
+    s.run(4000)
     got = ensemble.system_states(s, len(systems))
+    hits = 0
     for w, (p, v) in zip(systems, got):
         o = oracle_from_workload(w)
-        o.run(100)
+        o.run(4000)
         ref = o.atoms()
         order = np.argsort(ref["tag"])
         L = w.hi[0] - w.lo[0]
         assert np.abs(_periodic_diff(p, ref["pos"][order], L)).max() < 1e-12 * L
         assert rel_err(v, ref["vel"][order]) < 1e-9
+        hits += _collided(w, v)
+    assert hits > 10
 
 
 def test_small_ensemble_run_reports():
@@ -84,3 +97,25 @@ def test_batched_systems_at_one_lane_match_alone_runs():
         assert np.abs(_periodic_diff(p, pa, L)).max() < 1e-12 * L
         assert rel_err(v, va) < 1e-9
         a.close()
+
+
+def test_batched_contact_queue_gives_the_same_run():
+    # one lane per sphere in a batch: the wavefront's pairs in contact queued
+    # and computed 64 at a time (each sphere's system materials looked up by
+    # the queued pair's owner) against each lane's own loop, bit for bit
+    systems = _systems()
+    speed0 = np.sort(np.concatenate([np.linalg.norm(w.vel, axis=1) for w in systems]))
+    out = []
+    for wave in (False, True):
+        s = ensemble.make_context(systems, hip.MIXED, box_exit=True, lanes_log2=0, wave=wave)
+        s.run(4000)
+        st = s.state()
+        f, t = s.forces()
+        out.append((st["pos"], st["vel"], st["omega"], f, t))
+        s.close()
+    # collisions happened (a few dozen spheres by 4000 steps): only contacts
+    # change a speed in these gases
+    speed = np.sort(np.linalg.norm(out[0][1], axis=1))
+    assert np.sum(np.abs(speed - speed0) > 1e-9 * speed0.max()) > 10
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)

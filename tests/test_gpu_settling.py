@@ -245,3 +245,29 @@ def test_pairs_listed_twice_keep_their_springs_deterministically():
     for r in runs[1:]:
         for x, y in zip(runs[0], r):
             assert np.array_equal(x, y)
+
+
+def test_contact_queue_with_virial_steps_gives_the_same_run():
+    # with the contact virial on every third step those steps run the
+    # per-lane kernel (the queue has no virial), the others the queue: the
+    # springs and contact bits pass between the two kernels, and the run is
+    # the per-lane run bit for bit
+    w = bed(n=2500, every=100)
+    out = []
+    for wave in (0, 1):
+        s = hip.from_workload(w, hip.MIXED, box_exit=True)
+        s.set_force_lanes(0)
+        s.set_dem_split(0)
+        s.set_dem_wave(wave)
+        s.set_virial(3)
+        s.run(301)
+        st = s.state()
+        f, t = s.forces()
+        a, b, spring = canon_history(*s.history())
+        th = s.thermo()
+        out.append((st["pos"], st["vel"], st["omega"], f, t, a, b, spring,
+                    np.asarray(th["virial"])))
+        assert th["ncontacts"] > w.n // 5
+        s.close()
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
