@@ -89,9 +89,10 @@ def test_jobs_served_in_the_mix_match_the_oracle():
     # state against the CPU checker running that job's workload alone
     from mddem_amd import hip
     from tests.helpers import oracle_from_workload, rel_err
-    jobs = [serve.Job(0, "dem", 1500, 400, 0.0, dict(seed=21)),
+    # the gases run 3000 steps, by which about 50 spheres of each have collided
+    jobs = [serve.Job(0, "dem", 1500, 3000, 0.0, dict(seed=21)),
             serve.Job(1, "lj", 2048, 300, 0.0, dict(rho=0.85, temperature=1.0, seed=22)),
-            serve.Job(2, "dem", 2500, 300, 0.0, dict(seed=23))]
+            serve.Job(2, "dem", 2500, 3000, 0.0, dict(seed=23))]
     for j in jobs:
         j.workload = serve.build_workload(j)
     rep = serve.Server(hbm_budget=8 << 30, slots=4, precision=hip.FP64, box_exit=True,
@@ -108,3 +109,6 @@ def test_jobs_served_in_the_mix_match_the_oracle():
         assert np.abs(dp).max() < 1e-12 * L.max() * 1e3, j.jid
         assert rel_err(j.final["vel"][a], ref["vel"][b]) < 1e-9, j.jid
         assert j.final["rebuilds"] == o.rebuilds, j.jid
+        if j.kind == "dem":
+            v0 = j.workload.vel[np.argsort(j.workload.tag)]
+            assert np.sum(np.linalg.norm(j.final["vel"][a] - v0, axis=1) > 1e-12) > 20, j.jid
