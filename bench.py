@@ -32,10 +32,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_name(lg, mixed):
+def kernel_name(lg, mixed, fused=False):
     """Name of the pair-force kernel a layout runs, as the profiler shows it."""
     r = "float" if mixed else "double"
-    return f"k_lj_tile<{r},false,true>" if lg < 0 else f"k_lj<{r},false,{lg}>"
+    if lg < 0:
+        return f"k_lj_tile<{r},false,true>"
+    return f"k_lj_post<{lg}>" if fused else f"k_lj<{r},false,{lg}>"
+
+
+def post_fused(w, precision, lg):
+    """Whether the library runs the thermostatted final half step inside the
+    row pass (mdd_set_lj_post_fused, on by default): mixed Nose-Hoover LJ rows
+    of at most 4 lanes, at most 262,144 particles."""
+    from mddem_amd import hip
+    return (w.model == "lj" and precision == hip.MIXED and w.nose_hoover is not None
+            and 0 <= lg <= 2 and w.n <= 262144)
 
 
 def layout_name(lg):
@@ -105,7 +116,9 @@ def run_gpu(w, precision, steps, warmup, profile_every=10, tune=True, layout=Non
                npairs=th1["npairs"], force_ms=avg_ms, force_launches=launches,
                temperature=th1["temperature"], pe=th1["pe"], ke=th1["ke"],
                layout=layout_name(layout), tile=layout < 0, rebuild_ms=rb_ms,
-               kernel_name=kernel_name(layout, precision == hip.MIXED), rebuild_gap_ms=rb_gap_ms,
+               kernel_name=kernel_name(layout, precision == hip.MIXED,
+                                       post_fused(w, precision, layout)),
+               fused=post_fused(w, precision, layout), rebuild_gap_ms=rb_gap_ms,
                rebuilds_timed=rb_n, setup_steps=setup)
     s.close()
     return res
@@ -356,11 +369,16 @@ def main():
     # the mixed row pass gathers fixed-point positions (mdd_set_lj_positions)
     pbytes = 16.0 if prec == hip.MIXED and not r["tile"] else 24.0
     alg = force_kernel_bytes(r["n"], r["npairs"], fbytes, pbytes)
+    if r["fused"]:
+        # the final half step in the same launch: velocity read and written,
+        # mass read (the force it kicks with is the one the pass computed)
+        alg += r["n"] * (32.0 + 32.0 + 8.0)
     achieved = alg / (r["force_ms"] * 1e-3) / 1e9 if r["force_ms"] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles(w.name, r["kernel_name"]),
                 "kernel": ("k_lj_tile (pair force + final half step)" if r["tile"]
+                           else "k_lj_post (pair force + final half step)" if r["fused"]
                            else "k_lj (pair force)"), "bytes_per_launch": alg,
                 "avg_launch_us": round(r["force_ms"] * 1e3, 2)}
 

@@ -336,6 +336,13 @@ int mdd_set_dem_tiles(mdd_ctx* ctx, int on);
  * row's contacts. Identical forces, torques and springs. No reference
  * counterpart: launch-shape tuning. */
 int mdd_set_dem_wave(mdd_ctx* ctx, int on);
+/* Mixed LJ with the Nose-Hoover thermostat (single systems of at most
+ * 262,144 particles, rows of 1, 2 or 4 lanes): on (default) each scheduled
+ * step's final half (Verlet kick, thermostat rescale, kinetic-energy
+ * partials) runs inside the row pass, right after each particle's force;
+ * off: in its own kernel. Identical results. No reference counterpart:
+ * launch-shape tuning. */
+int mdd_set_lj_post_fused(mdd_ctx* ctx, int on);
 /* LJ pair pass of MDD_MIXED contexts (rows, single system, every box
  * length at least twice the list cutoff): 1 (default) gathers fixed-point
  * positions (x * 2^32 / L per axis, one 16-byte load per neighbor; 32-bit

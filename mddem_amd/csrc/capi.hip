@@ -619,6 +619,10 @@ int mdd_set_dem_tiles(mdd_ctx* ctx, int on) {
     });
 }
 
+int mdd_set_lj_post_fused(mdd_ctx* ctx, int on) {
+    return guarded(ctx, [&](Ctx& c) { c.lj_post_fused = on != 0; });
+}
+
 int mdd_set_dem_wave(mdd_ctx* ctx, int on) {
     return guarded(ctx, [&](Ctx& c) { c.dem_wave = on != 0; });
 }

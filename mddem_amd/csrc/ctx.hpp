@@ -49,6 +49,8 @@ constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
 constexpr uint32_t kRevBit = 1u << 31;
 constexpr int kCodeNone = 13;
 constexpr int64_t kMaxParticles = int64_t(1) << kIdxBits;
+// kinetic-energy partials of a thermostatted post kernel (one per block)
+constexpr int kMaxKePartials = 1024;
 
 __host__ __device__ inline uint32_t enc_entry(uint32_t idx, int code, bool rev) {
     return idx | (uint32_t(code) << kIdxBits) | (rev ? kRevBit : 0u);
@@ -446,7 +448,12 @@ struct Ctx {
         return true;
     }
     bool last_force_fused = false;  // the last scheduled force pass also integrated
-    int64_t fused_parts = 0;        // its kinetic-energy partials
+    int64_t fused_parts = 0;        // its kinetic-energy partials (cell tiles)
+    // the mixed LJ row pass with the thermostatted final half fused in
+    // (mdd_set_lj_post_fused); fused_rows: the last pass was that one, and
+    // left the post kernel's partials
+    bool lj_post_fused = true;
+    bool fused_rows = false;
     // the final half step of step `step` is deferred into the next step's
     // first pass (runs without a thermostat)
     bool post_pending = false;

@@ -153,7 +153,8 @@ struct alignas(16) ShiftPair {
 // shift. FP32 pair arithmetic; a pair within 1e-6 of the cutoff (the FP32
 // r^2 is good to ~3e-7) is decided on the FP64 r^2 of the same differences.
 template <int LG>
-__device__ inline void lj_row_fixed(const ForceArgs& a, int64_t i, int t) {
+__device__ inline void lj_row_fixed_force(const ForceArgs& a, int64_t i, int t, float& fx_out,
+                                          float& fy_out, float& fz_out) {
     constexpr int G = 1 << LG;
     constexpr int kB = MDD_LJ_BATCH;
     int cnt = int(a.nnb[i]);
@@ -202,13 +203,77 @@ __device__ inline void lj_row_fixed(const ForceArgs& a, int64_t i, int t) {
             fz = fmaf(sc, dz, fz);
         }
     }
-    fx = group_sum<G>(fx);
-    fy = group_sum<G>(fy);
-    fz = group_sum<G>(fz);
+    fx_out = group_sum<G>(fx);
+    fy_out = group_sum<G>(fy);
+    fz_out = group_sum<G>(fz);
+}
+
+template <int LG>
+__device__ inline void lj_row_fixed(const ForceArgs& a, int64_t i, int t) {
+    float fx, fy, fz;
+    lj_row_fixed_force<LG>(a, i, t, fx, fy, fz);
     if (t != 0) return;
     a.f32[i] = fx;
     a.f32[a.npad + i] = fy;
     a.f32[2 * a.npad + i] = fz;
+}
+
+// The thermostatted final half step (k_post of integrate.hip) fused into
+// the mixed LJ row pass: each particle's kick with its new force and this
+// step's thermostat rescale right after the force, and the kinetic-energy
+// partial of each 256 particles summed exactly as k_post sums it (one
+// particle per thread, xor butterfly per wavefront, the four wavefront sums
+// in order), so the next first-half kernel reads the same partials bit for
+// bit. One workgroup per 256 particles, 1 << LG lanes each.
+struct PostArgs {
+    double4* vel;
+    const double* mass;
+    double* partial;
+    int q;  // thermostat slot of this step
+    double dt;
+};
+
+template <int LG>
+__global__ void __launch_bounds__(256 << LG) k_lj_post(ForceArgs a, PostArgs p) {
+    constexpr int G = 1 << LG;
+    __shared__ double ke_lds[256];
+    __shared__ double lds4[4];
+    if (gated(a)) return;
+    const int64_t pb = xcd_block(blockIdx.x, gridDim.x);
+    const int local = int(threadIdx.x >> LG);
+    const int t = int(threadIdx.x & (G - 1));
+    const int64_t i = pb * 256 + local;
+    double ke = 0.0;
+    if (i < a.n) {
+        float fx, fy, fz;
+        lj_row_fixed_force<LG>(a, i, t, fx, fy, fz);
+        if (t == 0) {
+            a.f32[i] = fx;
+            a.f32[a.npad + i] = fy;
+            a.f32[2 * a.npad + i] = fz;
+            double4 v = p.vel[i];
+            const double h = 0.5 * p.dt * v.w;
+            v.x += h * double(fx);
+            v.y += h * double(fy);
+            v.z += h * double(fz);
+            const double scale = a.st->nh_scale[p.q];
+            v.x *= scale;
+            v.y *= scale;
+            v.z *= scale;
+            p.vel[i] = v;
+            ke = p.mass[i] * (v.x * v.x + v.y * v.y + v.z * v.z);
+        }
+    }
+    if (t == 0) ke_lds[local] = ke;
+    __syncthreads();
+    if (threadIdx.x < 256u) {
+        double v = ke_lds[threadIdx.x];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63u) == 0u) lds4[threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) p.partial[pb] = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
 }
 
 // lj_force (md_lj/src/lib.rs:488-633): for each neighbor inside the cutoff,
@@ -1832,33 +1897,71 @@ __global__ void k_fill_xq(int64_t n, const double4* __restrict__ pos, int4* __re
     xq[i] = make_int4(quantize_axis(p.x, qx), quantize_axis(p.y, qy), quantize_axis(p.z, qz), 0);
 }
 
+// the fixed-point positions of the mixed LJ pass (filled when stale) and
+// their units, into the pass's arguments
+void lj_fixed_setup(Ctx& c, ForceArgs& a) {
+    const double two32 = 4294967296.0;
+    if (!c.xq_valid) {
+        c.xq.resize(size_t(c.npad));
+        k_fill_xq<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(
+            c.n, static_cast<const double4*>(c.pos.p), c.xq.p, two32 / c.box.L[0],
+            two32 / c.box.L[1], two32 / c.box.L[2]);
+        MDD_HIP(hipGetLastError());
+        c.xq_valid = true;
+    }
+    a.xq = c.xq.p;
+    for (int d = 0; d < 3; d++) {
+        a.qd[d] = c.box.L[d] / two32;
+        a.qf[d] = float(a.qd[d]);
+    }
+}
+
 template <typename R, bool E>
 void launch_lj(Ctx& c, const ForceArgs& a0, hipEvent_t e0, hipEvent_t e1) {
     const int64_t threads = c.n << c.list_lg;
     const int nb = blocks_for(threads, 256);
     ForceArgs a = a0;
-    if (sizeof(R) == 4 && !E && c.xq_usable()) {
-        const double two32 = 4294967296.0;
-        if (!c.xq_valid) {
-            c.xq.resize(size_t(c.npad));
-            k_fill_xq<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(
-                c.n, static_cast<const double4*>(c.pos.p), c.xq.p, two32 / c.box.L[0],
-                two32 / c.box.L[1], two32 / c.box.L[2]);
-            MDD_HIP(hipGetLastError());
-            c.xq_valid = true;
-        }
-        a.xq = c.xq.p;
-        for (int d = 0; d < 3; d++) {
-            a.qd[d] = c.box.L[d] / two32;
-            a.qf[d] = float(a.qd[d]);
-        }
-    }
+    if (sizeof(R) == 4 && !E && c.xq_usable()) lj_fixed_setup(c, a);
     switch (c.list_lg) {
         case 0: launch_force_kernel(c, k_lj<R, E, 0>, nb, a, e0, e1); break;
         case 1: launch_force_kernel(c, k_lj<R, E, 1>, nb, a, e0, e1); break;
         case 3: launch_force_kernel(c, k_lj<R, E, 3>, nb, a, e0, e1); break;
         default: launch_force_kernel(c, k_lj<R, E, 2>, nb, a, e0, e1); break;
     }
+}
+
+// The mixed LJ row pass with the thermostatted final half step fused in
+// (k_lj_post), for a scheduled step of a single Nose-Hoover system whose
+// post kernel would use one block per 256 particles: returns false (and
+// launches nothing) otherwise.
+bool launch_lj_post(Ctx& c, const ForceArgs& a0, hipEvent_t e0, hipEvent_t e1, int step) {
+    const int64_t nb = blocks_for(c.n, 256);
+    if (!c.lj_post_fused || step < 0 || !c.nh_on || c.vv_on || c.rot_on || c.nsys != 0 ||
+        c.list_lg > 2 || nb > kMaxKePartials || !c.xq_usable() || c.list_tile)
+        return false;
+    ForceArgs a = a0;
+    lj_fixed_setup(c, a);
+    c.partial.resize(kMaxKePartials);
+    PostArgs p;
+    p.vel = c.vel.p;
+    p.mass = c.mass.p;
+    p.partial = c.partial.p;
+    p.q = step & 1;
+    p.dt = c.dt;
+    const dim3 block(256u << c.list_lg);
+    auto go = [&](auto kern) {
+        if (e0)
+            hipExtLaunchKernelGGL(kern, dim3(int(nb)), block, 0, c.stream, e0, e1, 0, a, p);
+        else
+            kern<<<dim3(int(nb)), block, 0, c.stream>>>(a, p);
+    };
+    switch (c.list_lg) {
+        case 0: go(k_lj_post<0>); break;
+        case 1: go(k_lj_post<1>); break;
+        default: go(k_lj_post<2>); break;
+    }
+    MDD_HIP(hipGetLastError());
+    return true;
 }
 
 template <typename R, bool VIR>
@@ -1956,6 +2059,7 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
     }
     hipEvent_t k0 = one_kernel ? e0 : nullptr, k1 = one_kernel ? e1 : nullptr;
     c.last_force_fused = false;
+    c.fused_rows = false;
     if (c.model == 0 && c.list_tile) {
         if (energy) {
             c.pe.resize(c.npad);
@@ -1975,6 +2079,9 @@ void launch_pair_forces(Ctx& c, int step, bool energy) {
                 launch_lj<float, true>(c, a, k0, k1);
             else
                 launch_lj<double, true>(c, a, k0, k1);
+        } else if (mixed && launch_lj_post(c, a, k0, k1, step)) {
+            c.last_force_fused = true;
+            c.fused_rows = true;
         } else {
             if (mixed)
                 launch_lj<float, false>(c, a, k0, k1);

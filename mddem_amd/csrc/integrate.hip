@@ -30,7 +30,7 @@ namespace mdd {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMaxPartials = 1024;  // blocks of a thermostatted post kernel
+constexpr int kMaxPartials = kMaxKePartials;  // blocks of a thermostatted post kernel
 
 struct IntArgs {
     int64_t n, npad;
@@ -592,8 +592,9 @@ void launch_post_pre(Ctx& c, int step, bool check_rebuild) {
 // scheduled post kernel of a step; its thermostat half uses slot step & 1
 void launch_post(Ctx& c, int step) {
     if (c.last_force_fused) {
-        // the force pass integrated already; sum its KE partials
-        if (c.nh_on) launch_ke_reduce(c, step, c.fused_parts);
+        // the force pass integrated already; cell tiles leave per-cell
+        // partials, summed here; fused rows leave the post kernel's own
+        if (c.nh_on && !c.fused_rows) launch_ke_reduce(c, step, c.fused_parts);
         return;
     }
     IntArgs a = make_args(c, step, false);

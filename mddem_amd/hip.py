@@ -128,6 +128,7 @@ SIGNATURES = {
     "mdd_set_dem_split": (C.c_int, [_vp, C.c_int]),
     "mdd_set_dem_tiles": (C.c_int, [_vp, C.c_int]),
     "mdd_set_dem_wave": (C.c_int, [_vp, C.c_int]),
+    "mdd_set_lj_post_fused": (C.c_int, [_vp, C.c_int]),
     "mdd_set_lj_positions": (C.c_int, [_vp, C.c_int]),
     "mdd_set_virial": (C.c_int, [_vp, C.c_int]),
     "mdd_set_dt": (C.c_int, [_vp, C.c_double]),
@@ -280,6 +281,11 @@ class System:
         """DEM pair pass at one lane per particle: contacts compacted per wavefront (on) or
         each lane's own contacts (off)."""
         self._c(self.L.mdd_set_dem_wave(self.h, int(bool(on))))
+
+    def set_lj_post_fused(self, on: bool):
+        """Mixed thermostatted LJ: the final half step inside the row pass (on) or in its
+        own kernel (off)."""
+        self._c(self.L.mdd_set_lj_post_fused(self.h, int(bool(on))))
 
     def set_history_remap(self, method: int):
         """0: sorted 64-bit pair keys (default), 1: search of each old row."""

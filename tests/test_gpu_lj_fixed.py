@@ -93,3 +93,25 @@ def test_melted_state_forces_within_the_mixed_budget():
     e_fixed = rel_err(out["fixed"], out["fp64"])
     e_geom = rel_err(out["geom64"], out["fp64"])
     assert e_fixed < 3e-6 and e_geom < 3e-6, (e_fixed, e_geom)
+
+
+@pytest.mark.parametrize("lg", [0, 2])
+def test_fused_final_half_gives_the_same_run(lg):
+    # thermostatted mixed LJ: the final half step inside the row pass (kick,
+    # rescale, kinetic-energy partials summed as the post kernel sums them)
+    # against the post kernel, bit for bit through rebuilds
+    w = W.lj_fcc(ncell=10)
+    out = []
+    for fused in (False, True):
+        s = hip.from_workload(w, hip.MIXED, box_exit=True)
+        s.set_force_lanes(lg)
+        s.set_lj_post_fused(fused)
+        s.run(500)
+        st = s.state()
+        f, _ = s.forces()
+        th = s.thermo()
+        out.append((st["pos"], st["vel"], f, np.array([th["p_xi"], th["ke"]])))
+        assert th["rebuilds"] >= 5
+        s.close()
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
