@@ -109,11 +109,34 @@ __device__ inline double block_sum(double v, double* lds4) {
     return (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
 }
 
-// total of the KE partials (sum m v^2), identical in every block
-__device__ inline double sum_partials(const IntArgs& a, double* lds4) {
+// the KE partials (sum m v^2) thread t adds: t, t + 256, ... (at most
+// kMaxPartials in all), loaded together so that one memory latency covers them
+constexpr int kPartsPerThread = kMaxPartials / kThreads;
+static_assert(kPartsPerThread * kThreads == kMaxPartials, "partials per thread");
+
+__device__ inline void load_partials(const IntArgs& a, double (&v)[kPartsPerThread]) {
+#pragma unroll
+    for (int k = 0; k < kPartsPerThread; k++) {
+        const int b = threadIdx.x + k * kThreads;
+        v[k] = b < a.nparts ? a.partial[b] : 0.0;
+    }
+}
+
+// total of the loaded partials, identical in every block (each thread adds
+// its own in index order, then block_sum)
+__device__ inline double sum_loaded(const IntArgs& a, const double (&v)[kPartsPerThread],
+                                    double* lds4) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < a.nparts; b += kThreads) s += a.partial[b];
+#pragma unroll
+    for (int k = 0; k < kPartsPerThread; k++)
+        if (threadIdx.x + k * kThreads < a.nparts) s += v[k];
     return block_sum(s, lds4);
+}
+
+__device__ inline double sum_partials(const IntArgs& a, double* lds4) {
+    double v[kPartsPerThread];
+    load_partials(a, v);
+    return sum_loaded(a, v, lds4);
 }
 
 // p_xi half update (md_thermostat/src/lib.rs:266 and 352)
@@ -219,6 +242,21 @@ template <typename F, bool NH>
 __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
     __shared__ double lds4[4];
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    // the particle's loads and the KE partials are issued before the first
+    // store and the reduction's barriers, so their latencies overlap
+    const bool live = i < a.n;
+    double4 v = make_double4(0.0, 0.0, 0.0, 0.0), p = v, b = v;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (live) {
+        p = a.pos[i];
+        if (NH || a.vv) {
+            v = a.vel[i];
+            load_f<F>(a, i, fx, fy, fz);
+        }
+        if (a.check) b = a.build_pos[i];
+    }
+    double part[kPartsPerThread];
+    if (NH) load_partials(a, part);
     // uniform over the launch: only earlier kernels lower abort_step below step
     if (a.step >= 0 && a.st->abort_step < a.step) return;
     if (i == 0) {
@@ -227,7 +265,7 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
     }
     double scale = 1.0;
     if (NH) {
-        const double ke = 0.5 * sum_partials(a, lds4);
+        const double ke = 0.5 * sum_loaded(a, part, lds4);
         const double term = nh_term(a, ke);
         double pxi = a.st->nh_p[a.q_in];
         if (a.st->nh_kind[a.q_in] == 0) pxi += term;  // post half of the previous step
@@ -239,12 +277,8 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
             a.st->nh_kind[a.q_out] = 0;
         }
     }
-    if (i >= a.n) return;
-    double4 v = a.vel[i];
-    double4 p = a.pos[i];
+    if (!live) return;
     if (NH || a.vv) {
-        double fx, fy, fz;
-        load_f<F>(a, i, fx, fy, fz);
         if (NH) {
             v.x *= scale;
             v.y *= scale;
@@ -261,7 +295,7 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
         load_t<F>(a, i, tx, ty, tz);
         rotate_initial(a, i, tx, ty, tz);
     }
-    if (a.check) rebuild_check(a, i, p, a.build_pos[i]);
+    if (a.check) rebuild_check(a, i, p, b);
 }
 
 // FinalIntegration + PostFinalIntegration of one step: Verlet kick,
