@@ -57,7 +57,7 @@ constexpr int kEventPool = 2048;
 // a chunk after a rebuild runs to this multiple of the smoothed rebuild
 // interval, plus one step
 #ifndef MDD_CHUNK_OVERSHOOT
-#define MDD_CHUNK_OVERSHOOT 1.0
+#define MDD_CHUNK_OVERSHOOT 0.85
 #endif
 constexpr double kChunkOvershoot = MDD_CHUNK_OVERSHOOT;
 
@@ -217,9 +217,10 @@ void run_steps(Ctx& c, int64_t nsteps) {
         mdd::launch_pair_forces(c, ab, false);
         second_half(ab);
         if (!retry) {
-            // the next chunk ends a little past the expected next rebuild:
-            // a few idle launches cost less than a chunk that ends too early
-            // (one more host round trip)
+            // the next chunk ends near the expected next rebuild (0.85 of
+            // the interval, plus one step): launches after a rebuild request
+            // return at once but still cost about 2 us each, and an early
+            // end costs one more host round trip
             const int64_t ran = int64_t(ab) - c.last_rebuild;
             if (ran > 0) {
                 c.rb_interval = c.rb_interval > 0.0 ? 0.5 * c.rb_interval + 0.5 * double(ran)
