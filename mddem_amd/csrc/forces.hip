@@ -164,6 +164,7 @@ __device__ inline void lj_row_fixed_force(const ForceArgs& a, int64_t i, int t, 
     const int4 qi = a.xq[i];
     const float sx = a.qf[0], sy = a.qf[1], sz = a.qf[2];
     const float cut2 = float(a.cut2), lj1 = float(a.lj1), lj2 = float(a.lj2);
+    const float cut2_eps = 1e-6f * cut2;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     uint32_t en[kB];
 #pragma unroll
@@ -188,8 +189,10 @@ __device__ inline void lj_row_fixed_force(const ForceArgs& a, int64_t i, int t, 
             const float dx = float(ix) * sx, dy = float(iy) * sy, dz = float(iz) * sz;
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             const bool valid = kb + u * G < cnt;
-            bool in = valid && r2 < cut2;
-            if (fabsf(r2 - cut2) <= 1e-6f * cut2) {
+            // the sign of the rounded difference is the sign of r2 - cut2
+            const float dr = r2 - cut2;
+            bool in = valid && dr < 0.0f;
+            if (__builtin_expect(fabsf(dr) <= cut2_eps, 0)) {
                 const double ex = double(ix) * a.qd[0], ey = double(iy) * a.qd[1],
                              ez = double(iz) * a.qd[2];
                 in = valid && fma(ex, ex, fma(ey, ey, ez * ez)) < a.cut2;
