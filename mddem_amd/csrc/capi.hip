@@ -390,6 +390,7 @@ void pack_upload(Ctx& c, int64_t n, const uint32_t* tag, const uint32_t* type, c
     h2d(c, c.cutr.p, cutr, sizeof(double) * n);
     dzero(c, c.image.p, sizeof(int4) * np);
     h2d(c, c.build_pos.p, hp.data(), sizeof(double4) * n);
+    c.bxq_valid = false;
     if (mixed)
         dzero(c, c.f32.p, sizeof(float) * 3 * np);
     else

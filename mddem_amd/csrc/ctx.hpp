@@ -441,6 +441,13 @@ struct Ctx {
     DBuf<int4> xq;
     bool lj_fixed = true;  // mdd_set_lj_positions
     bool xq_valid = false;
+    // the same fixed-point form of the build positions, written by the
+    // permutation beside xq: the first half step's displacement check reads
+    // 16 B instead of the 32 B double4 and goes back to the double4 only
+    // near the threshold. Valid from such a permutation until an upload
+    // rewrites the build positions.
+    DBuf<int4> bxq;
+    bool bxq_valid = false;
     bool xq_usable() const {
         if (!(model == 0 && precision == 1 && nsys == 0 && lj_fixed && !list_tile)) return false;
         for (int d = 0; d < 3; d++)

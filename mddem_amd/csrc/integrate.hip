@@ -63,11 +63,18 @@ struct IntArgs {
     WallsDyn* wdyn;         // steps move them (PreInitialIntegration) and clear the sums
     int4* xq;               // fixed-point copy of the positions to keep current, or null
     double qs[3];           // its scale 2^32 / L per axis
+    const int4* bxq;        // with xq on a periodic box: fixed-point build positions, or null
+    double qu[3];           // L / 2^32 per axis
+    double thr_band;        // |d2 - thr2| up to this: the double4 build position decides
 };
 
+__device__ inline int4 quantize_pos(const IntArgs& a, const double4& p) {
+    return make_int4(quantize_axis(p.x, a.qs[0]), quantize_axis(p.y, a.qs[1]),
+                     quantize_axis(p.z, a.qs[2]), 0);
+}
+
 __device__ inline void store_xq(const IntArgs& a, int64_t i, const double4& p) {
-    a.xq[i] = make_int4(quantize_axis(p.x, a.qs[0]), quantize_axis(p.y, a.qs[1]),
-                        quantize_axis(p.z, a.qs[2]), 0);
+    a.xq[i] = quantize_pos(a, p);
 }
 
 template <typename F>
@@ -173,6 +180,32 @@ __device__ inline void rebuild_check(const IntArgs& a, int64_t i, const double4&
         atomicMin(&a.st->abort_step, a.step);
 }
 
+// rebuild_check from the fixed-point position and build position (every
+// axis periodic). The int32 differences are the minimum image, and each
+// coordinate is within one quantum of its double value, so d2 is within
+// thr_band of rebuild_check's d2 near the threshold; outside the band both
+// decide alike, inside it rebuild_check decides from the double4.
+__device__ inline void rebuild_check_fixed(const IntArgs& a, int64_t i, const double4& p,
+                                           const int4& bq) {
+    const int4 q = quantize_pos(a, p);
+    const double dx = double(int(uint32_t(q.x) - uint32_t(bq.x))) * a.qu[0];
+    const double dy = double(int(uint32_t(q.y) - uint32_t(bq.y))) * a.qu[1];
+    const double dz = double(int(uint32_t(q.z) - uint32_t(bq.z))) * a.qu[2];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (fabs(d2 - a.thr2) <= a.thr_band) {
+        rebuild_check(a, i, p, a.build_pos[i]);
+        return;
+    }
+    bool out = false;
+    if (a.box_exit) {
+        const double x[3] = {p.x, p.y, p.z};
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (x[k] < a.box.lo[k] || x[k] >= a.box.hi[k]) out = true;
+    }
+    if (d2 > a.thr2 || out) atomicMin(&a.st->abort_step, a.step);
+}
+
 // orientation update of initial_rotation with the kicked omega w
 // (rotational.rs:70-83)
 __device__ inline void rotate_quat(const IntArgs& a, int64_t i, const double4& w) {
@@ -246,6 +279,7 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
     // store and the reduction's barriers, so their latencies overlap
     const bool live = i < a.n;
     double4 v = make_double4(0.0, 0.0, 0.0, 0.0), p = v, b = v;
+    int4 bq = make_int4(0, 0, 0, 0);
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (live) {
         p = a.pos[i];
@@ -253,7 +287,12 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
             v = a.vel[i];
             load_f<F>(a, i, fx, fy, fz);
         }
-        if (a.check) b = a.build_pos[i];
+        if (a.check) {
+            if (a.bxq)
+                bq = a.bxq[i];
+            else
+                b = a.build_pos[i];
+        }
     }
     double part[kPartsPerThread];
     if (NH) load_partials(a, part);
@@ -295,7 +334,12 @@ __global__ void __launch_bounds__(kThreads) k_pre(IntArgs a) {
         load_t<F>(a, i, tx, ty, tz);
         rotate_initial(a, i, tx, ty, tz);
     }
-    if (a.check) rebuild_check(a, i, p, b);
+    if (a.check) {
+        if (a.bxq)
+            rebuild_check_fixed(a, i, p, bq);
+        else
+            rebuild_check(a, i, p, b);
+    }
 }
 
 // FinalIntegration + PostFinalIntegration of one step: Verlet kick,
@@ -570,6 +614,19 @@ IntArgs make_args(Ctx& c, int step, bool check) {
     if (c.xq_usable() && c.xq_valid) {
         a.xq = c.xq.p;
         for (int d = 0; d < 3; d++) a.qs[d] = 4294967296.0 / c.box.L[d];
+        const bool periodic = c.box.periodic[0] && c.box.periodic[1] && c.box.periodic[2];
+        if (c.bxq_valid && periodic && std::isfinite(a.thr2)) {
+            // each coordinate differs from its double value by at most one
+            // quantum u, so |d2 - d2_double| <= 2 sqrt(3) u |d| + 3 u^2;
+            // the band holds that at |d| = thr with a margin of more than 2
+            double u = 0.0;
+            for (int d = 0; d < 3; d++) {
+                a.qu[d] = c.box.L[d] / 4294967296.0;
+                u = std::max(u, a.qu[d]);
+            }
+            a.bxq = c.bxq.p;
+            a.thr_band = 8.0 * u * (std::sqrt(a.thr2) + u) + 1e-12 * a.thr2;
+        }
     } else {
         c.xq_valid = false;  // these launches may move particles without it
     }

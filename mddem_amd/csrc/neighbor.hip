@@ -123,7 +123,8 @@ struct PermArgs {
     double *f64b, *t64b;
     const float *f32, *t32;
     float *f32b, *t32b;
-    int4* xq;  // LJ mixed: the fixed-point copy of the permuted positions, or null
+    int4* xq;   // LJ mixed: the fixed-point copy of the permuted positions, or null
+    int4* bxq;  // with xq: the same values as the fixed-point build positions
     double qs[3];
     int dem;
 };
@@ -137,9 +138,12 @@ __global__ void k_permute(PermArgs a) {
     double4 p = a.pos[o];
     a.pos2[k] = p;
     a.build_pos[k] = p;
-    if (a.xq)
-        a.xq[k] = make_int4(quantize_axis(p.x, a.qs[0]), quantize_axis(p.y, a.qs[1]),
-                            quantize_axis(p.z, a.qs[2]), 0);
+    if (a.xq) {
+        const int4 q = make_int4(quantize_axis(p.x, a.qs[0]), quantize_axis(p.y, a.qs[1]),
+                                 quantize_axis(p.z, a.qs[2]), 0);
+        a.xq[k] = q;
+        a.bxq[k] = q;
+    }
     a.vel2[k] = a.vel[o];
     a.tag2[k] = a.tag[o];
     a.type2[k] = a.type[o];
@@ -1210,13 +1214,16 @@ void permute_all(Ctx& c, const uint32_t* order, int64_t n) {
     const bool xq = c.xq_usable();
     if (xq) {
         c.xq.resize(size_t(c.npad));
+        c.bxq.resize(size_t(c.npad));
         a.xq = c.xq.p;
+        a.bxq = c.bxq.p;
         for (int d = 0; d < 3; d++) a.qs[d] = 4294967296.0 / c.box.L[d];
     }
     launch(c, k_permute, blocks_for(n, 256), 256, a);
     MDD_HIP(hipGetLastError());
     swap_alternates(c);
     c.xq_valid = xq;
+    c.bxq_valid = xq;
 }
 
 // exchange every per-particle array with its alternate
