@@ -1,7 +1,9 @@
 """The LJ mixed pair pass on fixed-point positions (x * 2^32 / L per axis,
 32-bit differences wrapping to the minimum image): pairs at the cutoff are
 decided as in FP64, the periodic images need no shift, and the forces of a
-melted state stay within the mixed budget of the FP64 pass."""
+melted state stay within the mixed budget of the FP64 pass. The displacement
+check on fixed-point build positions decides as the double check does, also
+at the threshold (neighbor.rs:564-601)."""
 import sys
 
 import numpy as np
@@ -115,3 +117,45 @@ def test_fused_final_half_gives_the_same_run(lg):
         s.close()
     for x, y in zip(out[0], out[1]):
         assert np.array_equal(x, y)
+
+
+def _threshold_walkers(mode):
+    # 8 particles 10 apart (no pairs) moving along the axes at |v| = 0.5:
+    # with dt = 0.005 each drifts 0.0025 per step, so its displacement
+    # reaches the 0.15 threshold ((1.12 - 1) x 1.25) at every 60th step to
+    # within a few ulp, inside the band where the fixed-point check reads
+    # the double4 build position. The thermostat's target is the state's
+    # own temperature (2 KE / ndof = 2 / 21 exactly times 21), so p_xi
+    # stays 0, the scale is 1 and the velocities keep their exact values.
+    w = W.lj_fcc(ncell=12)
+    c = np.array([[x, y, z] for x in (5.0, 15.0) for y in (5.0, 15.0) for z in (5.0, 15.0)])
+    v = np.zeros((8, 3))
+    for k in range(8):
+        v[k, k % 3] = 0.5 if k % 2 == 0 else -0.5
+    w.pos, w.vel = c, v
+    w.mass = np.ones(8)
+    w.cutoff_radius = np.full(8, 1.25)
+    ndof = 3 * 8 - 3
+    T = 2.0 / ndof
+    assert ndof * T == 2.0
+    w.nose_hoover = (T, 1.0)
+    s = hip.from_workload(w, hip.MIXED)
+    s.set_lj_positions(mode)
+    s.run(200)
+    st = s.state()
+    rb = s.thermo()["rebuilds"]
+    s.close()
+    return st, rb
+
+
+def test_displacement_check_at_the_threshold_is_the_double_check():
+    # the fixed-point build positions (mode 1) against the double4 check
+    # (mode 0): with no pairs the two runs are the same arithmetic, so the
+    # rebuild steps and the states agree bit for bit
+    st1, rb1 = _threshold_walkers(1)
+    st0, rb0 = _threshold_walkers(0)
+    assert rb1 == rb0 >= 3
+    assert np.array_equal(st1["pos"], st0["pos"])
+    assert np.array_equal(st1["vel"], st0["vel"])
+    assert np.all(np.abs(st1["vel"]) <= 0.5)
+    assert np.array_equal(np.sort(np.abs(st1["vel"]).sum(axis=1)), np.full(8, 0.5))
